@@ -1,0 +1,204 @@
+"""Benchmark of the batched FWI gradient on one MI355X.
+
+A step is one FWI gradient over the whole shot batch of the 1024x320
+workload (dx=dz=10 m, dt=0.8 ms, nt=4000, 64 shots, 1024 receivers, 20-cell
+CPML, free surface): model upload, forward propagation of every shot,
+L2 misfit and residual on the device (observed data resident in HBM),
+exact-adjoint back-propagation with source reconstruction and FP64 imaging,
+gradient copied back.  Synthetic seeded layered/faulted model.
+
+Prints ONE JSON line (rank 0):
+  value       shot-gradients/s (whole job)
+  roofline    dominant kernel's algorithmic bytes / HIP-event duration vs 8 TB/s
+  cpu_baseline  the FP32 OpenMP oracle (a restatement of the reference CPU
+              path) on a bounded sample of the same workload, on this host
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "fwi2dacoustic-mpi-lbfgs_amd")
+sys.path.insert(0, PKG)
+
+HBM_PEAK = 8.0e12          # MI355X datasheet HBM3E bandwidth, B/s
+BYTES_PER_CELL_UPDATE = 16  # read p^n, p^{n-1}, ekp; write p^{n+1} (FP32)
+PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "traffic_1024x320_64shots.json")
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--shots", type=int, default=64)
+    ap.add_argument("--nt", type=int, default=4000)
+    ap.add_argument("--tile-z", type=int, default=0)
+    ap.add_argument("--group", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-single", action="store_true", help="skip the one-shot-per-launch comparison")
+    ap.add_argument("--cpu-nt", type=int, default=1500, help="time steps of the CPU sample shot")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device copy (HBM counter calibration for rocprofv3 --pmc)")
+    return ap.parse_args()
+
+
+def cpu_baseline(g, vel, shot, nt_sample):
+    """FP32 oracle (OpenMP, the reference's unfused loop structure) on one
+    shot of the workload with nt reduced to nt_sample: rate in cell-updates/s
+    converted to shot-gradients/s of the full nt."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle
+    from fwihip.workloads import Shot
+    o = Oracle(np.float32)
+    s = Shot(nt=nt_sample, isx=shot.isx, isz=shot.isz, wav=shot.wav[:nt_sample], irx=shot.irx, irz=shot.irz)
+    obs = np.zeros((s.nr, nt_sample), np.float32)
+    t0 = time.perf_counter()
+    o.gradient(g, vel, s, obs)
+    dt = time.perf_counter() - t0
+    cells = (g.pnx * g.pnz * 2 + g.nx * g.nz) * nt_sample
+    rate = cells / dt
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return rate, dt, threads
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); "
+              "the shot batch would shard across GPUs with one gradient all-reduce, not measured here.",
+              file=sys.stderr)
+        sys.exit(2)
+    if args.copy_calibration:
+        import torch
+        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4, "launches": 5}))
+        return
+    import numpy as np
+    from fwihip import lib
+    from fwihip.workloads import config2, triangle_smooth
+
+    g, vel, shots = config2(nshots=args.shots, nt=args.nt)
+    start = triangle_smooth(vel, 25)
+    nr = max(s.nr for s in shots)
+    ctx = lib.Context(g, max_shots=len(shots), max_nt=args.nt, max_nr=nr, tile_z=args.tile_z,
+                      shots_per_group=args.group)
+    ctx.set_model(vel)
+    ctx.set_shots(shots)
+    obs = ctx.forward()
+    ctx.set_observed(obs)  # observed data resident in HBM from here on
+
+    def step():
+        ctx.set_model(start)
+        ctx.forward(want_traces=False)
+        J = ctx.misfit_l2()
+        grad, _ = ctx.adjoint()
+        return J, grad
+
+    for _ in range(args.warmup):
+        step()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        J, grad = step()
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = wall / args.steps * 1e3
+    nshots = len(shots)
+    cells_shot = (g.pnx * g.pnz * 2 + g.nx * g.nz) * args.nt
+    sg_per_s = nshots * args.steps / wall
+    gcell = cells_shot * nshots * args.steps / wall / 1e9
+
+    # per-kernel durations (HIP events on the context stream), one extra step
+    ctx.set_timing(True)
+    step()
+    kern = {}
+    names = ("k_fwd_step", "k_adj_step", "k_rec_step")
+    cells_launch = (g.pnx * g.pnz * nshots, g.pnx * g.pnz * nshots, g.nx * g.nz * nshots)
+    for w, name in enumerate(names):
+        ms, n = ctx.kernel_time(w)
+        avg = ms / max(n, 1) * 1e-3
+        byts = BYTES_PER_CELL_UPDATE * cells_launch[w]
+        kern[name] = dict(launches=n, avg_us=avg * 1e6, algorithmic_bytes=byts,
+                          achieved_GBps=byts / avg / 1e9 if avg > 0 else None)
+    ctx.set_timing(False)
+    dom = max(names, key=lambda k: kern[k]["avg_us"] * kern[k]["launches"])
+    traffic = None
+    if os.path.exists(PROFILE_TRAFFIC):
+        try:
+            traffic = json.load(open(PROFILE_TRAFFIC)).get(dom)
+        except Exception:
+            traffic = None
+    roof = dict(bound="hbm", kernel=dom, achieved=kern[dom]["achieved_GBps"], peak=HBM_PEAK / 1e9, unit="GB/s",
+                frac=kern[dom]["achieved_GBps"] / (HBM_PEAK / 1e9), traffic=traffic)
+
+    single = None
+    if not args.no_single:
+        # the plain one-shot-per-launch GPU form, on the first 4 shots
+        sub = shots[:4]
+        c1 = lib.Context(g, max_shots=1, max_nt=args.nt, max_nr=nr, tile_z=args.tile_z)
+        c1.set_model(vel)
+        t1 = 0.0
+        for s in sub:
+            c1.set_shots([s])
+            o1 = c1.forward()
+            c1.set_observed(o1)
+            c1.set_model(start)
+            c1.synchronize()
+            ts = time.perf_counter()
+            c1.forward(want_traces=False)
+            c1.misfit_l2()
+            c1.adjoint()
+            c1.synchronize()
+            t1 += time.perf_counter() - ts
+            c1.set_model(vel)
+        single = len(sub) / t1
+        c1.close()
+
+    cpu = None
+    if not args.no_cpu:
+        rate, dt, threads = cpu_baseline(g, start, shots[0], args.cpu_nt)
+        cpu = dict(value=rate / cells_shot, unit="shot-gradients/s", cores=threads, kind="port",
+                   sample=f"1 shot of this workload with nt={args.cpu_nt} instead of {args.nt} "
+                          f"({dt:.1f} s, FP32 oracle, OpenMP): {rate / 1e9:.3f} Gcell-updates/s, "
+                          "scaled to the full-nt shot-gradient")
+
+    out = {
+        "metric": "shot-gradients/s & Gcell-updates/s at 1024x320, nt=4000; % HBM peak",
+        "value": sg_per_s,
+        "unit": "shot-gradients/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded layered/faulted model, Ricker shots, observed data from the true model)",
+        "config": {"workload": f"batched FWI gradient {g.nx}x{g.nz} dx=dz={g.dx:g} m dt={g.dt * 1e3:g} ms "
+                               f"nt={args.nt}, {nshots} shots x {nr} receivers, npml={g.npml}, free surface",
+                   "nx": g.nx, "nz": g.nz, "nt": args.nt, "nshots": nshots, "nr": nr, "npml": g.npml},
+        "gcell_updates_per_s": gcell,
+        "hbm_frac_whole_job": gcell * 1e9 * BYTES_PER_CELL_UPDATE / HBM_PEAK,
+        "misfit": J,
+        "roofline": roof,
+        "kernels": kern,
+        "one_shot_per_launch_shot_gradients_per_s": single,
+        "batched_over_single": (sg_per_s / single) if single else None,
+        "cpu_baseline": cpu,
+        "gpu_over_cpu": (sg_per_s / cpu["value"]) if cpu else None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,53 @@
+"""Builds libfwi_hip.so in-tree (gfx950) from csrc/.
+
+    python fwi2dacoustic-mpi-lbfgs_amd/build.py
+
+Device code: hipcc --offload-arch=gfx950, -ffp-contract=off (every
+expression rounds as written, like the C reference) and packed FP32 ops
+disabled.  The C host driver (grad4_1shot.c) is compiled with gcc.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libfwi_hip.so")
+INC = os.path.join(os.path.dirname(HERE), "include")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("FWI_HIP_ARCH", "gfx950")
+HIP_FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-ffp-contract=off",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wall", "-Wno-unused-variable", "-Wno-unused-function", f"-I{INC}"]
+C_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-Wall", f"-I{INC}"]
+
+
+def run(cmd):
+    print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build(verbose=True):
+    os.makedirs(BUILD, exist_ok=True)
+    objs = []
+    for src in ("fwi_kernels.hip", "fwi_hip.cpp"):
+        obj = os.path.join(BUILD, src + ".o")
+        run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    for src in ("grad4_1shot.c",):
+        path = os.path.join(CSRC, src)
+        if os.path.exists(path):
+            obj = os.path.join(BUILD, src + ".o")
+            run(["gcc", *C_FLAGS, "-c", path, "-o", obj])
+            objs.append(obj)
+    tmp = LIB + ".tmp"
+    run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

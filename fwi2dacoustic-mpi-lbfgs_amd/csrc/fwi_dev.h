@@ -1,0 +1,92 @@
+// fwi_dev.h -- launch parameter blocks shared by the HIP kernels
+// (fwi_kernels.hip) and the C-ABI host layer (fwi_hip.cpp).
+//
+// Device layout of one batch (all FP32 unless noted):
+//   f-grid fields  [shot][pnz][ld]    ld = 64 * ceil(pnx / 64), padding zero
+//   p^k of a shot lives in buffer P[k & 1]; the update writes p^{k+1} over
+//   p^{k-1} (two buffers, in place).  The adjoint mu^m lives in M[m & 1].
+//   CPML memory (forward psi, adjoint chi), per shot:
+//     xh  [2][pnz][2*(npml+1)]  half (u) nodes, left strip then right strip
+//     xd  [2][pnz][2*npml]      p nodes of the x strips
+//     zh  [2][npml+1][ld]       half (v) nodes of the bottom strip
+//     zd  [2][npml][ld]         p nodes of the bottom strip
+//   ping-pong buffer index = step & 1 (read old, write new).
+//   frame strips [shot][nt][6nx+6nz]: rows {0,1,2,nz-1,nz-2,nz-3} x nx, then
+//   nz x cols {0,1,2,nx-1,nx-2,nx-3} (the bc_ud / bc_lr order of
+//   write_bdr, fwi_2d.c:893-910), one block per step.
+//   traces: time-major [shot][nt][nr] while propagating.
+#pragma once
+#include <stdint.h>
+
+#define FWI_TX 64      // x cells per wave (one lane per column)
+#define FWI_MAXSG 4    // shots (waves) per workgroup
+
+struct FwiShotDev {
+    int nt;          // time steps of this shot
+    int isx, isz;    // source cell on the f-grid (x already offset by npml)
+    int nr;          // receivers
+    long long src_off;    // into srcf: wav[it] * iarea, nt entries
+    long long trace_off;  // into syn/res (time-major nt x nr)
+    long long strip_off;  // into pstrip/xstrip (nt x sl)
+    int rrow_off;         // into rrow (pnz entries: receiver-row slot or -1)
+    int pad_;
+};
+
+struct FwiGeomDev {
+    int nx, nz, npml, pnx, pnz, ld;
+    int tz;              // tile height in rows (>= 8)
+    int ntx, ntz;        // f-grid tiles
+    int bntx, bntz;      // physical-grid tiles
+    int sl;              // strip length per step = 6 nx + 6 nz
+    float c0x, c1x, ix, c0z, c1z, iz, iz840, iz24;
+};
+
+struct FwiBatchDev {
+    int nshots, sg, ngroups;
+    int pad_;
+    const FwiShotDev *shots;
+    long long fs;        // field elements per shot (pnz * ld)
+    long long xh_s, xd_s, zh_s, zd_s;   // per-shot stride of the CPML arrays
+    long long xh_b, xd_b, zh_b, zd_b;   // ping-pong buffer stride
+    const float *ekp;    // [pnz][ld]
+    const float *tab;    // 6 x (2 npml + 1): a0x b0x a1x b1x a1z b1z
+    const float *srcf;
+    const int *rrow;
+    const int *rmap;     // [slot][ld]
+};
+
+struct FwiFwdArgs {
+    FwiGeomDev g;
+    FwiBatchDev b;
+    float *P0, *P1;
+    float *xh, *xd, *zh, *zd;
+    float *syn;
+    float *pstrip, *xstrip;
+};
+
+struct FwiAdjArgs {
+    FwiGeomDev g;
+    FwiBatchDev b;
+    float *M0, *M1;
+    float *xh, *xd, *zh, *zd;
+    const float *res;
+};
+
+struct FwiRecArgs {
+    FwiGeomDev g;
+    FwiBatchDev b;
+    float *P0, *P1;
+    const float *M0, *M1;
+    const float *pstrip, *xstrip;
+    double *gacc;        // [ngroups][nz][nx]
+};
+
+// per-step kernels with the reference's loop structure (unit parity)
+struct FwiRefArgs {
+    int pnx, pnz, npml;
+    float c0x, c1x, ix, c0z, c1z, iz, iz840, iz24;
+    const float *p;
+    float *u, *v, *ux, *vz;
+    float *u0, *u1, *v1, *ux0, *ux1, *vz1;
+    const float *tab;
+};

@@ -1,0 +1,794 @@
+// fwi_hip.cpp -- host side of the C ABI declared in include/fwi_hip.h.
+//
+// Owns the device buffers of a batch of shots, prepares the launch
+// parameter blocks of fwi_kernels.hip and sequences the time loops.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "fwi_dev.h"
+#include "../../include/fwi_hip.h"
+
+extern "C" {
+hipError_t fwi_launch_fwd(const FwiFwdArgs *a, int k, int nblocks, hipStream_t st);
+hipError_t fwi_launch_adj(const FwiAdjArgs *a, int k, int nblocks, hipStream_t st);
+hipError_t fwi_launch_rec(const FwiRecArgs *a, int k, int nblocks, hipStream_t st);
+hipError_t fwi_launch_fill_f32(float *p, long long n, float v, hipStream_t st);
+hipError_t fwi_launch_fill_f64(double *p, long long n, double v, hipStream_t st);
+hipError_t fwi_launch_transpose(const float *in, float *out, int rows, int cols, hipStream_t st);
+hipError_t fwi_launch_misfit_l2(const float *syn, const float *obs, float *res, long long n, double *partial,
+                                int nblocks, hipStream_t st);
+hipError_t fwi_launch_grad_finalize(const double *gacc, int ngroups, long long n, const float *vel, double *gk,
+                                    double *grad, hipStream_t st);
+}
+
+extern "C" {
+hipError_t fwi_launch_ref_phase(const FwiRefArgs *a, int phase, hipStream_t st);
+hipError_t fwi_launch_ref_time_integral(float *pnew, const float *pnow, const float *pold, const float *pxx,
+                                        const float *pzz, const float *ekp, long long n, hipStream_t st);
+hipError_t fwi_launch_ref_write_bdr(float *ud, float *lr, const float *pf, int nx, int nz, int npml, hipStream_t st);
+hipError_t fwi_launch_ref_read_bdr(const float *ud, const float *lr, float *pb, int nx, int nz, hipStream_t st);
+}
+
+static char g_err[512] = "";
+
+struct fwi_hip_ctx {
+    fwi_hip_desc d;
+    FwiGeomDev g;
+    hipStream_t st = nullptr;
+    bool own_stream = false;
+    char err[512] = "";
+    size_t bytes = 0;
+    std::vector<void *> allocs;
+
+    // model
+    float *ekp = nullptr, *vel = nullptr;
+    bool have_model = false;
+    // wavefields and CPML memory
+    float *P0 = nullptr, *P1 = nullptr, *M0 = nullptr, *M1 = nullptr;
+    float *xh = nullptr, *xd = nullptr, *zh = nullptr, *zd = nullptr;
+    long long fs = 0, xh_b = 0, xd_b = 0, zh_b = 0, zd_b = 0;
+    float *tab = nullptr;
+    // frames and traces
+    float *pstrip = nullptr, *xstrip = nullptr;
+    float *syn_tm = nullptr, *syn = nullptr, *obs = nullptr, *res = nullptr, *res_tm = nullptr;
+    double *gacc = nullptr, *gk = nullptr, *grad = nullptr, *partial = nullptr;
+    int npartial = 1024;
+    // batch description
+    int nshots = 0, kmax = 0, sg = 1, ngroups = 0;
+    long long ntrace = 0;
+    std::vector<FwiShotDev> shots_h;
+    std::vector<int> nt_h, nr_h;
+    FwiShotDev *shots_d = nullptr;
+    float *srcf = nullptr;
+    long long srcf_cap = 0;
+    int *rrow = nullptr;
+    int *rmap = nullptr;
+    long long rmap_cap = 0;
+    bool have_fwd = false, have_obs = false, have_res = false;
+    // timing
+    bool timing = false;
+    double t_ms[3] = {0, 0, 0};
+    long long t_n[3] = {0, 0, 0};
+    std::vector<hipEvent_t> ev;
+};
+
+static int set_err(fwi_hip_ctx *c, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (c) snprintf(c->err, sizeof(c->err), "%s", buf);
+    snprintf(g_err, sizeof(g_err), "%s", buf);
+    return code;
+}
+
+#define HIPCHK(c, call)                                                                              \
+    do {                                                                                             \
+        hipError_t e_ = (call);                                                                      \
+        if (e_ != hipSuccess)                                                                        \
+            return set_err((c), FWI_HIP_EHIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                           __FILE__, __LINE__);                                                      \
+    } while (0)
+
+static int dalloc(fwi_hip_ctx *c, void **p, size_t bytes)
+{
+    *p = nullptr;
+    if (bytes == 0) bytes = 256;
+    if (c->d.hbm_budget && c->bytes + bytes > c->d.hbm_budget)
+        return set_err(c, FWI_HIP_ENOMEM, "HBM budget exceeded: %zu + %zu > %zu bytes", c->bytes, bytes,
+                       c->d.hbm_budget);
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return set_err(c, FWI_HIP_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    c->bytes += bytes;
+    c->allocs.push_back(*p);
+    return FWI_HIP_OK;
+}
+
+static void dfree(fwi_hip_ctx *c, void *p)
+{
+    if (!p) return;
+    auto it = std::find(c->allocs.begin(), c->allocs.end(), p);
+    if (it != c->allocs.end()) c->allocs.erase(it);
+    (void)hipFree(p);
+}
+
+// --------------------------------------------------------------------------
+// reference arithmetic on the host
+
+// cpml_coe, fwi_2d.c:110-137: float storage, double where C promotes
+extern "C" int fwi_hip_cpml_coe(float *a0, float *b0, int npml, float dx, float dt, float maxF, float maxV,
+                                float coefR)
+{
+    if (!a0 || !b0 || npml < 1 || !(dx > 0) || !(dt > 0)) return set_err(nullptr, FWI_HIP_EINVAL, "cpml_coe: bad args");
+    const float ndx = dx * 0.5;
+    const int npower = 2; /* pml_base.dim at fwi_2d.c:257 */
+    const double PI_REF = 3.141592654; /* the PI fwi_2d.c compiles with (fwi_sys.h) */
+    float thickness_pml = npml * dx, PIf0 = PI_REF * maxF, slope = PIf0 / thickness_pml;
+    float d0 = -(npower + 1) * maxV * log(coefR) / (2.0 * thickness_pml);
+    for (int i = 0; i < 2 * npml + 1; i++) {
+        float d = (1.0 - i * ndx / thickness_pml);
+        d *= d;
+        d *= d0;
+        float alpha = slope * (i * ndx);
+        b0[i] = exp(-(d + alpha) * dt);
+        a0[i] = d / (d + alpha) * (b0[i] - 1.0);
+    }
+    return FWI_HIP_OK;
+}
+
+struct Coe {
+    float c0x, c1x, ix, c0z, c1z, iz, iz840, iz24;
+};
+
+// fwi_fdcoe, fwi_2d.c:302-316
+static Coe fd_coe(float dx, float dz)
+{
+    Coe c;
+    c.ix = 1.0 / dx;
+    c.iz = 1.0 / dz;
+    c.c0x = 0.1129042e1 * c.ix;
+    c.c1x = -0.4301412e-1 * c.ix;
+    c.c0z = 0.1129042e1 * c.iz;
+    c.c1z = -0.4301412e-1 * c.iz;
+    c.iz840 = c.iz / 840.0;
+    c.iz24 = c.iz / 24.0;
+    return c;
+}
+
+// the six CPML rows used by the kernels: a0x b0x a1x b1x a1z b1z (mirror fwi_2d.c:275-283)
+static void cpml_tables(const fwi_hip_grid &g, std::vector<float> &tab)
+{
+    const int T = 2 * g.npml + 1;
+    std::vector<float> ax(T), bx(T), az(T), bz(T);
+    fwi_hip_cpml_coe(ax.data(), bx.data(), g.npml, g.dx, g.dt, g.max_f, g.max_v, g.coef_r);
+    fwi_hip_cpml_coe(az.data(), bz.data(), g.npml, g.dz, g.dt, g.max_f, g.max_v, g.coef_r);
+    tab.assign(6 * T, 0.f);
+    for (int i = 0; i < T; i++) {
+        tab[0 * T + i] = ax[i];
+        tab[1 * T + i] = bx[i];
+        tab[2 * T + i] = ax[2 * g.npml - i];
+        tab[3 * T + i] = bx[2 * g.npml - i];
+        tab[4 * T + i] = az[2 * g.npml - i];
+        tab[5 * T + i] = bz[2 * g.npml - i];
+    }
+}
+
+static int check_grid(fwi_hip_ctx *c, const fwi_hip_grid &g)
+{
+    if (g.nx < 8 || g.nz < 8) return set_err(c, FWI_HIP_EINVAL, "grid %dx%d too small (need nx, nz >= 8)", g.nx, g.nz);
+    if (g.npml < 2) return set_err(c, FWI_HIP_EINVAL, "npml=%d: need at least 2 CPML cells", g.npml);
+    if (!(g.dx > 0) || !(g.dz > 0) || !(g.dt > 0)) return set_err(c, FWI_HIP_EINVAL, "dx, dz, dt must be > 0");
+    if (!(g.max_v > 0) || !(g.coef_r > 0) || !(g.coef_r < 1) || !(g.max_f >= 0))
+        return set_err(c, FWI_HIP_EINVAL, "CPML meta: need max_v > 0, 0 < coef_r < 1, max_f >= 0");
+    return FWI_HIP_OK;
+}
+
+// --------------------------------------------------------------------------
+extern "C" const char *fwi_hip_last_error(const fwi_hip_ctx *c) { return c ? c->err : g_err; }
+extern "C" size_t fwi_hip_bytes(const fwi_hip_ctx *c) { return c ? c->bytes : 0; }
+
+extern "C" void fwi_hip_destroy(fwi_hip_ctx *c)
+{
+    if (!c) return;
+    if (c->st) (void)hipStreamSynchronize(c->st);
+    for (void *p : c->allocs) (void)hipFree(p);
+    for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
+    if (c->own_stream && c->st) (void)hipStreamDestroy(c->st);
+    delete c;
+}
+
+extern "C" int fwi_hip_create(const fwi_hip_desc *desc, fwi_hip_ctx **out)
+{
+    if (!desc || !out) return set_err(nullptr, FWI_HIP_EINVAL, "fwi_hip_create: NULL argument");
+    *out = nullptr;
+    fwi_hip_ctx *c = new fwi_hip_ctx();
+    c->d = *desc;
+    const fwi_hip_grid &gr = desc->grid;
+    int rc = check_grid(c, gr);
+    if (!rc && (desc->max_shots < 1 || desc->max_nt < 1 || desc->max_nr < 0))
+        rc = set_err(c, FWI_HIP_EINVAL, "max_shots, max_nt must be >= 1 and max_nr >= 0");
+    if (!rc && desc->tile_z != 0 && desc->tile_z < 8) rc = set_err(c, FWI_HIP_EINVAL, "tile_z must be 0 or >= 8");
+    if (!rc && (desc->shots_per_group < 0 || desc->shots_per_group > FWI_MAXSG))
+        rc = set_err(c, FWI_HIP_EINVAL, "shots_per_group must be 0..%d", FWI_MAXSG);
+    if (rc) {
+        snprintf(g_err, sizeof(g_err), "%s", c->err);
+        delete c;
+        return rc;
+    }
+    FwiGeomDev &g = c->g;
+    memset(&g, 0, sizeof(g));
+    g.nx = gr.nx;
+    g.nz = gr.nz;
+    g.npml = gr.npml;
+    g.pnx = gr.nx + 2 * gr.npml;
+    g.pnz = gr.nz + gr.npml;
+    g.ld = ((g.pnx + FWI_TX - 1) / FWI_TX) * FWI_TX;
+    g.ntx = g.ld / FWI_TX;
+    int tz = desc->tile_z;
+    if (tz == 0) {
+        const int nt_z = std::max(1, (g.pnz + 39) / 40);
+        tz = std::max(8, (g.pnz + nt_z - 1) / nt_z);
+    }
+    g.tz = tz;
+    g.ntz = (g.pnz + tz - 1) / tz;
+    g.bntx = (g.nx + FWI_TX - 1) / FWI_TX;
+    g.bntz = (g.nz + tz - 1) / tz;
+    g.sl = 6 * g.nx + 6 * g.nz;
+    Coe co = fd_coe(gr.dx, gr.dz);
+    g.c0x = co.c0x; g.c1x = co.c1x; g.ix = co.ix;
+    g.c0z = co.c0z; g.c1z = co.c1z; g.iz = co.iz;
+    g.iz840 = co.iz840; g.iz24 = co.iz24;
+    c->sg = desc->shots_per_group ? desc->shots_per_group : std::min(FWI_MAXSG, desc->max_shots);
+
+    if (desc->stream) {
+        c->st = (hipStream_t)desc->stream;
+    } else {
+        hipError_t e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            rc = set_err(c, FWI_HIP_EHIP, "hipStreamCreate: %s", hipGetErrorString(e));
+            delete c;
+            return rc;
+        }
+        c->own_stream = true;
+    }
+
+    const long long S = desc->max_shots, NT = desc->max_nt, NR = desc->max_nr;
+    const int n1 = g.npml + 1;
+    c->fs = (long long)g.pnz * g.ld;
+    c->xh_b = (long long)g.pnz * 2 * n1;
+    c->xd_b = (long long)g.pnz * 2 * g.npml;
+    c->zh_b = (long long)n1 * g.ld;
+    c->zd_b = (long long)g.npml * g.ld;
+    const int maxgroups = (int)((S + c->sg - 1) / c->sg);
+    struct A { void **p; size_t bytes; } plan[] = {
+        {(void **)&c->ekp, sizeof(float) * c->fs},
+        {(void **)&c->vel, sizeof(float) * gr.nx * gr.nz},
+        {(void **)&c->P0, sizeof(float) * S * c->fs},
+        {(void **)&c->P1, sizeof(float) * S * c->fs},
+        {(void **)&c->M0, sizeof(float) * S * c->fs},
+        {(void **)&c->M1, sizeof(float) * S * c->fs},
+        {(void **)&c->xh, sizeof(float) * S * 2 * c->xh_b},
+        {(void **)&c->xd, sizeof(float) * S * 2 * c->xd_b},
+        {(void **)&c->zh, sizeof(float) * S * 2 * c->zh_b},
+        {(void **)&c->zd, sizeof(float) * S * 2 * c->zd_b},
+        {(void **)&c->tab, sizeof(float) * 6 * (2 * g.npml + 1)},
+        {(void **)&c->pstrip, sizeof(float) * S * NT * g.sl},
+        {(void **)&c->xstrip, sizeof(float) * S * NT * g.sl},
+        {(void **)&c->syn_tm, sizeof(float) * S * NT * NR},
+        {(void **)&c->syn, sizeof(float) * S * NT * NR},
+        {(void **)&c->obs, sizeof(float) * S * NT * NR},
+        {(void **)&c->res, sizeof(float) * S * NT * NR},
+        {(void **)&c->res_tm, sizeof(float) * S * NT * NR},
+        {(void **)&c->gacc, sizeof(double) * maxgroups * gr.nx * gr.nz},
+        {(void **)&c->gk, sizeof(double) * gr.nx * gr.nz},
+        {(void **)&c->grad, sizeof(double) * gr.nx * gr.nz},
+        {(void **)&c->partial, sizeof(double) * c->npartial},
+        {(void **)&c->shots_d, sizeof(FwiShotDev) * S},
+        {(void **)&c->rrow, sizeof(int) * S * g.pnz},
+    };
+    for (auto &a : plan) {
+        rc = dalloc(c, a.p, a.bytes);
+        if (rc) {
+            char keep[512];
+            snprintf(keep, sizeof(keep), "%s", c->err);
+            fwi_hip_destroy(c);
+            snprintf(g_err, sizeof(g_err), "%s", keep);
+            return rc;
+        }
+    }
+    // zero every field once so that the row padding stays zero for good
+    long long nz_fields[] = {c->fs, S * c->fs, S * c->fs, S * c->fs, S * c->fs};
+    float *fields[] = {c->ekp, c->P0, c->P1, c->M0, c->M1};
+    for (int i = 0; i < 5; i++) {
+        hipError_t e = fwi_launch_fill_f32(fields[i], nz_fields[i], 0.f, c->st);
+        if (e != hipSuccess) {
+            rc = set_err(c, FWI_HIP_EHIP, "fill: %s", hipGetErrorString(e));
+            fwi_hip_destroy(c);
+            return rc;
+        }
+    }
+    std::vector<float> tab;
+    cpml_tables(gr, tab);
+    hipError_t e = hipMemcpyAsync(c->tab, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    if (e != hipSuccess) {
+        rc = set_err(c, FWI_HIP_EHIP, "table upload: %s", hipGetErrorString(e));
+        fwi_hip_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return FWI_HIP_OK;
+}
+
+// model_ext (fwi_2d.c:67-107) of kp = vel*vel*dt*dt into the pitched f-grid
+extern "C" int fwi_hip_set_model(fwi_hip_ctx *c, const float *vel)
+{
+    if (!c || !vel) return set_err(c, FWI_HIP_EINVAL, "set_model: NULL argument");
+    const FwiGeomDev &g = c->g;
+    const float dt = c->d.grid.dt;
+    const int nx = g.nx, nz = g.nz, npml = g.npml;
+    for (long long i = 0; i < (long long)nx * nz; i++)
+        if (!(vel[i] > 0) || !isfinite(vel[i]))
+            return set_err(c, FWI_HIP_EINVAL, "set_model: velocity must be finite and > 0 (cell %lld)", i);
+    std::vector<float> kp((size_t)nx * nz), ekp((size_t)c->fs, 0.f);
+    for (long long i = 0; i < (long long)nx * nz; i++) kp[i] = vel[i] * vel[i] * dt * dt;
+    for (int z = 0; z < g.pnz; z++) {
+        const int zs = std::min(z, nz - 1);
+        for (int x = 0; x < g.pnx; x++) {
+            const int xs = std::min(std::max(x - npml, 0), nx - 1);
+            ekp[(size_t)z * g.ld + x] = kp[(size_t)zs * nx + xs];
+        }
+    }
+    HIPCHK(c, hipMemcpyAsync(c->ekp, ekp.data(), sizeof(float) * ekp.size(), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->vel, vel, sizeof(float) * nx * nz, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->have_model = true;
+    return FWI_HIP_OK;
+}
+
+extern "C" int fwi_hip_set_shots(fwi_hip_ctx *c, int nshots, const fwi_hip_shot *shots)
+{
+    if (!c || !shots) return set_err(c, FWI_HIP_EINVAL, "set_shots: NULL argument");
+    if (nshots < 1 || nshots > c->d.max_shots)
+        return set_err(c, FWI_HIP_EINVAL, "set_shots: nshots=%d outside 1..%d", nshots, c->d.max_shots);
+    const FwiGeomDev &g = c->g;
+    std::vector<FwiShotDev> sd(nshots);
+    std::vector<int> rrow((size_t)nshots * g.pnz, -1);
+    std::vector<int> rmap;
+    std::vector<float> srcf;
+    const float iarea = 1.0 / (c->d.grid.dx * c->d.grid.dz); /* fwi_2d.c:216 */
+    long long toff = 0, soff = 0;
+    int slots = 0, kmax = 0;
+    c->nt_h.assign(nshots, 0);
+    c->nr_h.assign(nshots, 0);
+    for (int s = 0; s < nshots; s++) {
+        const fwi_hip_shot &sh = shots[s];
+        if (sh.nt < 1 || sh.nt > c->d.max_nt)
+            return set_err(c, FWI_HIP_EINVAL, "shot %d: nt=%d outside 1..%d", s, sh.nt, c->d.max_nt);
+        if (sh.nr < 0 || sh.nr > c->d.max_nr)
+            return set_err(c, FWI_HIP_EINVAL, "shot %d: nr=%d outside 0..%d", s, sh.nr, c->d.max_nr);
+        if (!sh.wav || (sh.nr > 0 && (!sh.irx || !sh.irz)))
+            return set_err(c, FWI_HIP_EINVAL, "shot %d: NULL wavelet or receiver arrays", s);
+        if (sh.isx < 0 || sh.isx >= g.nx || sh.isz < 0 || sh.isz >= g.nz)
+            return set_err(c, FWI_HIP_EINVAL, "shot %d: source cell (%d,%d) outside the %dx%d grid", s, sh.isx,
+                           sh.isz, g.nx, g.nz);
+        FwiShotDev &d = sd[s];
+        d.nt = sh.nt;
+        d.isx = sh.isx + g.npml;
+        d.isz = sh.isz;
+        d.nr = sh.nr;
+        d.src_off = (long long)srcf.size();
+        for (int k = 0; k < sh.nt; k++) srcf.push_back(sh.wav[k] * iarea);
+        d.trace_off = toff;
+        toff += (long long)sh.nr * sh.nt;
+        d.strip_off = soff;
+        soff += (long long)sh.nt * g.sl;
+        d.rrow_off = s * g.pnz;
+        std::map<int, int> rowslot;
+        for (int r = 0; r < sh.nr; r++) {
+            const int x = sh.irx[r], z = sh.irz[r];
+            if (x < 0 || x >= g.nx || z < 0 || z >= g.nz)
+                return set_err(c, FWI_HIP_EINVAL, "shot %d receiver %d: cell (%d,%d) outside the grid", s, r, x, z);
+            auto it = rowslot.find(z);
+            int slot;
+            if (it == rowslot.end()) {
+                slot = slots++;
+                rowslot[z] = slot;
+                rrow[(size_t)s * g.pnz + z] = slot;
+                rmap.resize((size_t)slots * g.ld, -1);
+            } else {
+                slot = it->second;
+            }
+            int &cell = rmap[(size_t)slot * g.ld + x + g.npml];
+            if (cell >= 0)
+                return set_err(c, FWI_HIP_EINVAL, "shot %d: receivers %d and %d share cell (%d,%d)", s, cell, r, x, z);
+            cell = r;
+        }
+        c->nt_h[s] = sh.nt;
+        c->nr_h[s] = sh.nr;
+        kmax = std::max(kmax, sh.nt);
+    }
+    if (toff > (long long)c->d.max_shots * c->d.max_nt * c->d.max_nr)
+        return set_err(c, FWI_HIP_EINVAL, "set_shots: traces exceed the context capacity");
+    // (re)allocate the per-batch tables
+    if ((long long)srcf.size() > c->srcf_cap) {
+        dfree(c, c->srcf);
+        c->srcf = nullptr;
+        c->srcf_cap = 0;
+        int rc = dalloc(c, (void **)&c->srcf, sizeof(float) * srcf.size());
+        if (rc) return rc;
+        c->srcf_cap = (long long)srcf.size();
+    }
+    if (rmap.empty()) rmap.assign(g.ld, -1);
+    if ((long long)rmap.size() > c->rmap_cap) {
+        dfree(c, c->rmap);
+        c->rmap = nullptr;
+        c->rmap_cap = 0;
+        int rc = dalloc(c, (void **)&c->rmap, sizeof(int) * rmap.size());
+        if (rc) return rc;
+        c->rmap_cap = (long long)rmap.size();
+    }
+    HIPCHK(c, hipMemcpyAsync(c->srcf, srcf.data(), sizeof(float) * srcf.size(), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->rmap, rmap.data(), sizeof(int) * rmap.size(), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->rrow, rrow.data(), sizeof(int) * rrow.size(), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->shots_d, sd.data(), sizeof(FwiShotDev) * nshots, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->shots_h = sd;
+    c->nshots = nshots;
+    c->kmax = kmax;
+    c->ntrace = toff;
+    c->ngroups = (nshots + c->sg - 1) / c->sg;
+    c->have_fwd = c->have_obs = c->have_res = false;
+    return FWI_HIP_OK;
+}
+
+static FwiBatchDev batch_dev(fwi_hip_ctx *c)
+{
+    FwiBatchDev b;
+    memset(&b, 0, sizeof(b));
+    b.nshots = c->nshots;
+    b.sg = c->sg;
+    b.ngroups = c->ngroups;
+    b.shots = c->shots_d;
+    b.fs = c->fs;
+    b.xh_b = c->xh_b;
+    b.xd_b = c->xd_b;
+    b.zh_b = c->zh_b;
+    b.zd_b = c->zd_b;
+    b.xh_s = 2 * c->xh_b;
+    b.xd_s = 2 * c->xd_b;
+    b.zh_s = 2 * c->zh_b;
+    b.zd_s = 2 * c->zd_b;
+    b.ekp = c->ekp;
+    b.tab = c->tab;
+    b.srcf = c->srcf;
+    b.rrow = c->rrow;
+    b.rmap = c->rmap;
+    return b;
+}
+
+static int zero_cpml(fwi_hip_ctx *c)
+{
+    const long long S = c->nshots;
+    HIPCHK(c, fwi_launch_fill_f32(c->xh, S * 2 * c->xh_b, 0.f, c->st));
+    HIPCHK(c, fwi_launch_fill_f32(c->xd, S * 2 * c->xd_b, 0.f, c->st));
+    HIPCHK(c, fwi_launch_fill_f32(c->zh, S * 2 * c->zh_b, 0.f, c->st));
+    HIPCHK(c, fwi_launch_fill_f32(c->zd, S * 2 * c->zd_b, 0.f, c->st));
+    return FWI_HIP_OK;
+}
+
+// timing helpers: an event pair around a launch when timing is on
+static int tick(fwi_hip_ctx *c, hipEvent_t *e)
+{
+    *e = nullptr;
+    if (!c->timing) return FWI_HIP_OK;
+    HIPCHK(c, hipEventCreate(e));
+    c->ev.push_back(*e);
+    HIPCHK(c, hipEventRecord(*e, c->st));
+    return FWI_HIP_OK;
+}
+
+static int collect_timing(fwi_hip_ctx *c, std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> &marks)
+{
+    if (marks.empty()) return FWI_HIP_OK;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (auto &m : marks) {
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, m.second.first, m.second.second));
+        c->t_ms[m.first] += ms;
+        c->t_n[m.first] += 1;
+    }
+    for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
+    c->ev.clear();
+    marks.clear();
+    return FWI_HIP_OK;
+}
+
+extern "C" int fwi_hip_forward(fwi_hip_ctx *c, float *traces)
+{
+    if (!c) return set_err(c, FWI_HIP_EINVAL, "forward: NULL context");
+    if (!c->have_model || c->nshots == 0) return set_err(c, FWI_HIP_ESTATE, "forward: set_model and set_shots first");
+    const FwiGeomDev &g = c->g;
+    const long long S = c->nshots;
+    HIPCHK(c, fwi_launch_fill_f32(c->P0, S * c->fs, 0.f, c->st));
+    HIPCHK(c, fwi_launch_fill_f32(c->P1, S * c->fs, 0.f, c->st));
+    int rc = zero_cpml(c);
+    if (rc) return rc;
+    FwiFwdArgs a;
+    memset(&a, 0, sizeof(a));
+    a.g = g;
+    a.b = batch_dev(c);
+    a.P0 = c->P0;
+    a.P1 = c->P1;
+    a.xh = c->xh;
+    a.xd = c->xd;
+    a.zh = c->zh;
+    a.zd = c->zd;
+    a.syn = c->syn_tm;
+    a.pstrip = c->pstrip;
+    a.xstrip = c->xstrip;
+    const int nblocks = g.ntx * g.ntz * c->ngroups;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
+    for (int k = 0; k < c->kmax; k++) {
+        hipEvent_t e0, e1;
+        if ((rc = tick(c, &e0))) return rc;
+        HIPCHK(c, fwi_launch_fwd(&a, k, nblocks, c->st));
+        if ((rc = tick(c, &e1))) return rc;
+        if (c->timing) marks.push_back({0, {e0, e1}});
+        if (marks.size() >= 512 && (rc = collect_timing(c, marks))) return rc;
+    }
+    if ((rc = collect_timing(c, marks))) return rc;
+    for (int s = 0; s < c->nshots; s++) {
+        const FwiShotDev &d = c->shots_h[s];
+        if (d.nr == 0) continue;
+        HIPCHK(c, fwi_launch_transpose(c->syn_tm + d.trace_off, c->syn + d.trace_off, d.nt, d.nr, c->st));
+    }
+    if (traces)
+        HIPCHK(c, hipMemcpyAsync(traces, c->syn, sizeof(float) * c->ntrace, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->have_fwd = true;
+    c->have_res = false;
+    return FWI_HIP_OK;
+}
+
+extern "C" int fwi_hip_set_observed(fwi_hip_ctx *c, const float *obs)
+{
+    if (!c || !obs) return set_err(c, FWI_HIP_EINVAL, "set_observed: NULL argument");
+    if (c->nshots == 0) return set_err(c, FWI_HIP_ESTATE, "set_observed: set_shots first");
+    HIPCHK(c, hipMemcpyAsync(c->obs, obs, sizeof(float) * c->ntrace, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->have_obs = true;
+    return FWI_HIP_OK;
+}
+
+extern "C" int fwi_hip_misfit_l2(fwi_hip_ctx *c, double *misfit)
+{
+    if (!c || !misfit) return set_err(c, FWI_HIP_EINVAL, "misfit_l2: NULL argument");
+    if (!c->have_fwd || !c->have_obs) return set_err(c, FWI_HIP_ESTATE, "misfit_l2: forward and set_observed first");
+    const int nb = c->npartial;
+    HIPCHK(c, fwi_launch_misfit_l2(c->syn, c->obs, c->res, c->ntrace, c->partial, nb, c->st));
+    std::vector<double> part(nb);
+    HIPCHK(c, hipMemcpyAsync(part.data(), c->partial, sizeof(double) * nb, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    double J = 0.0;
+    for (double v : part) J += v;
+    *misfit = J;
+    c->have_res = true;
+    return FWI_HIP_OK;
+}
+
+extern "C" int fwi_hip_set_residual(fwi_hip_ctx *c, const float *res)
+{
+    if (!c || !res) return set_err(c, FWI_HIP_EINVAL, "set_residual: NULL argument");
+    if (c->nshots == 0) return set_err(c, FWI_HIP_ESTATE, "set_residual: set_shots first");
+    HIPCHK(c, hipMemcpyAsync(c->res, res, sizeof(float) * c->ntrace, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->have_res = true;
+    return FWI_HIP_OK;
+}
+
+extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
+{
+    if (!c) return set_err(c, FWI_HIP_EINVAL, "adjoint: NULL context");
+    if (!c->have_fwd || !c->have_res)
+        return set_err(c, FWI_HIP_ESTATE, "adjoint: forward and a residual (misfit_l2 or set_residual) first");
+    const FwiGeomDev &g = c->g;
+    const long long S = c->nshots, nb = (long long)g.nx * g.nz;
+    HIPCHK(c, fwi_launch_fill_f32(c->M0, S * c->fs, 0.f, c->st));
+    HIPCHK(c, fwi_launch_fill_f32(c->M1, S * c->fs, 0.f, c->st));
+    HIPCHK(c, fwi_launch_fill_f64(c->gacc, (long long)c->ngroups * nb, 0.0, c->st));
+    int rc = zero_cpml(c);
+    if (rc) return rc;
+    for (int s = 0; s < c->nshots; s++) {
+        const FwiShotDev &d = c->shots_h[s];
+        if (d.nr == 0) continue;
+        HIPCHK(c, fwi_launch_transpose(c->res + d.trace_off, c->res_tm + d.trace_off, d.nr, d.nt, c->st));
+    }
+    FwiAdjArgs a;
+    memset(&a, 0, sizeof(a));
+    a.g = g;
+    a.b = batch_dev(c);
+    a.M0 = c->M0;
+    a.M1 = c->M1;
+    a.xh = c->xh;
+    a.xd = c->xd;
+    a.zh = c->zh;
+    a.zd = c->zd;
+    a.res = c->res_tm;
+    FwiRecArgs r;
+    memset(&r, 0, sizeof(r));
+    r.g = g;
+    r.b = a.b;
+    r.P0 = c->P0;
+    r.P1 = c->P1;
+    r.M0 = c->M0;
+    r.M1 = c->M1;
+    r.pstrip = c->pstrip;
+    r.xstrip = c->xstrip;
+    r.gacc = c->gacc;
+    const int nb_adj = g.ntx * g.ntz * c->ngroups;
+    const int nb_rec = g.bntx * g.bntz * c->ngroups;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
+    for (int k = c->kmax - 1; k >= 0; k--) {
+        hipEvent_t e0, e1, e2;
+        if ((rc = tick(c, &e0))) return rc;
+        HIPCHK(c, fwi_launch_adj(&a, k, nb_adj, c->st));
+        if ((rc = tick(c, &e1))) return rc;
+        HIPCHK(c, fwi_launch_rec(&r, k, nb_rec, c->st));
+        if ((rc = tick(c, &e2))) return rc;
+        if (c->timing) {
+            marks.push_back({1, {e0, e1}});
+            marks.push_back({2, {e1, e2}});
+        }
+        if (marks.size() >= 512 && (rc = collect_timing(c, marks))) return rc;
+    }
+    if ((rc = collect_timing(c, marks))) return rc;
+    HIPCHK(c, fwi_launch_grad_finalize(c->gacc, c->ngroups, nb, c->vel, c->gk, c->grad, c->st));
+    if (grad) HIPCHK(c, hipMemcpyAsync(grad, c->grad, sizeof(double) * nb, hipMemcpyDeviceToHost, c->st));
+    if (gk) HIPCHK(c, hipMemcpyAsync(gk, c->gk, sizeof(double) * nb, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return FWI_HIP_OK;
+}
+
+extern "C" int fwi_hip_synchronize(fwi_hip_ctx *c)
+{
+    if (!c) return set_err(c, FWI_HIP_EINVAL, "synchronize: NULL context");
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return FWI_HIP_OK;
+}
+
+extern "C" int fwi_hip_set_timing(fwi_hip_ctx *c, int on)
+{
+    if (!c) return set_err(c, FWI_HIP_EINVAL, "set_timing: NULL context");
+    c->timing = on != 0;
+    for (int i = 0; i < 3; i++) {
+        c->t_ms[i] = 0;
+        c->t_n[i] = 0;
+    }
+    return FWI_HIP_OK;
+}
+
+extern "C" int fwi_hip_kernel_time(fwi_hip_ctx *c, int which, double *ms, long long *launches)
+{
+    if (!c || which < 0 || which > 2) return set_err(c, FWI_HIP_EINVAL, "kernel_time: bad argument");
+    if (ms) *ms = c->t_ms[which];
+    if (launches) *launches = c->t_n[which];
+    return FWI_HIP_OK;
+}
+
+extern "C" int fwi_hip_read_state(fwi_hip_ctx *c, int what, int shot, float *out)
+{
+    if (!c || !out || shot < 0 || shot >= c->nshots || what < 0 || what > 5)
+        return set_err(c, FWI_HIP_EINVAL, "read_state: bad argument");
+    const FwiGeomDev &g = c->g;
+    if (what <= 3) {
+        float *base[] = {c->P0, c->P1, c->M0, c->M1};
+        const float *src = base[what] + (long long)shot * c->fs;
+        HIPCHK(c, hipMemcpy2DAsync(out, sizeof(float) * g.pnx, src, sizeof(float) * g.ld, sizeof(float) * g.pnx,
+                                   g.pnz, hipMemcpyDeviceToHost, c->st));
+    } else {
+        const FwiShotDev &d = c->shots_h[shot];
+        const float *src = (what == 4 ? c->pstrip : c->xstrip) + d.strip_off;
+        HIPCHK(c, hipMemcpyAsync(out, src, sizeof(float) * (size_t)d.nt * g.sl, hipMemcpyDeviceToHost, c->st));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return FWI_HIP_OK;
+}
+
+// --------------------------------------------------------------------------
+// per-step entry points (caller-owned device buffers)
+
+static int ref_args(const fwi_hip_grid *g, FwiRefArgs &a, int pnx, int pnz)
+{
+    if (!g) return set_err(nullptr, FWI_HIP_EINVAL, "NULL grid");
+    int rc = check_grid(nullptr, *g);
+    if (rc) return rc;
+    memset(&a, 0, sizeof(a));
+    Coe co = fd_coe(g->dx, g->dz);
+    a.pnx = pnx;
+    a.pnz = pnz;
+    a.npml = g->npml;
+    a.c0x = co.c0x; a.c1x = co.c1x; a.ix = co.ix;
+    a.c0z = co.c0z; a.c1z = co.c1z; a.iz = co.iz;
+    a.iz840 = co.iz840; a.iz24 = co.iz24;
+    return FWI_HIP_OK;
+}
+
+#define HIPCHK0(call)                                                                                      \
+    do {                                                                                                   \
+        hipError_t e_ = (call);                                                                            \
+        if (e_ != hipSuccess)                                                                              \
+            return set_err(nullptr, FWI_HIP_EHIP, "%s failed: %s", #call, hipGetErrorString(e_));          \
+    } while (0)
+
+extern "C" int fwi_hip_op_pml(const fwi_hip_grid *g, const float *p, float *u, float *v, float *ux, float *vz,
+                              float *u0, float *u1, float *v1, float *ux0, float *ux1, float *vz1, void *stream)
+{
+    FwiRefArgs a;
+    int rc = ref_args(g, a, g ? g->nx + 2 * g->npml : 0, g ? g->nz + g->npml : 0);
+    if (rc) return rc;
+    if (!p || !u || !v || !ux || !vz || !u0 || !u1 || !v1 || !ux0 || !ux1 || !vz1)
+        return set_err(nullptr, FWI_HIP_EINVAL, "op_pml: NULL buffer");
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<float> tab;
+    cpml_tables(*g, tab);
+    float *dtab = nullptr;
+    HIPCHK0(hipMalloc(&dtab, sizeof(float) * tab.size()));
+    hipError_t e = hipMemcpy(dtab, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice);
+    a.p = p; a.u = u; a.v = v; a.ux = ux; a.vz = vz;
+    a.u0 = u0; a.u1 = u1; a.v1 = v1; a.ux0 = ux0; a.ux1 = ux1; a.vz1 = vz1;
+    a.tab = dtab;
+    for (int ph = 0; ph < 12 && e == hipSuccess; ph++) e = fwi_launch_ref_phase(&a, ph, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(dtab);
+    if (e != hipSuccess) return set_err(nullptr, FWI_HIP_EHIP, "op_pml: %s", hipGetErrorString(e));
+    return FWI_HIP_OK;
+}
+
+extern "C" int fwi_hip_op_interior(const fwi_hip_grid *g, const float *p, float *u, float *v, float *ux, float *vz,
+                                   void *stream)
+{
+    FwiRefArgs a;
+    int rc = ref_args(g, a, g ? g->nx : 0, g ? g->nz : 0);
+    if (rc) return rc;
+    if (!p || !u || !v || !ux || !vz) return set_err(nullptr, FWI_HIP_EINVAL, "op_interior: NULL buffer");
+    a.p = p; a.u = u; a.v = v; a.ux = ux; a.vz = vz;
+    for (int ph = 20; ph < 24; ph++) HIPCHK0(fwi_launch_ref_phase(&a, ph, (hipStream_t)stream));
+    return FWI_HIP_OK;
+}
+
+extern "C" int fwi_hip_time_integral(float *pnew, const float *pnow, const float *pold, const float *pxx,
+                                     const float *pzz, const float *ekp, size_t m, size_t n, void *stream)
+{
+    if (!pnew || !pnow || !pold || !pxx || !pzz || !ekp) return set_err(nullptr, FWI_HIP_EINVAL, "time_integral: NULL");
+    HIPCHK0(fwi_launch_ref_time_integral(pnew, pnow, pold, pxx, pzz, ekp, (long long)(m * n), (hipStream_t)stream));
+    return FWI_HIP_OK;
+}
+
+extern "C" int fwi_hip_write_bdr(const fwi_hip_grid *g, float *ud, float *lr, const float *pf, void *stream)
+{
+    if (!g || !ud || !lr || !pf) return set_err(nullptr, FWI_HIP_EINVAL, "write_bdr: NULL");
+    HIPCHK0(fwi_launch_ref_write_bdr(ud, lr, pf, g->nx, g->nz, g->npml, (hipStream_t)stream));
+    return FWI_HIP_OK;
+}
+
+extern "C" int fwi_hip_read_bdr(const fwi_hip_grid *g, const float *ud, const float *lr, float *pb, void *stream)
+{
+    if (!g || !ud || !lr || !pb) return set_err(nullptr, FWI_HIP_EINVAL, "read_bdr: NULL");
+    HIPCHK0(fwi_launch_ref_read_bdr(ud, lr, pb, g->nx, g->nz, (hipStream_t)stream));
+    return FWI_HIP_OK;
+}

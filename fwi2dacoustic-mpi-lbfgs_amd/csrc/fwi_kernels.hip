@@ -1,0 +1,915 @@
+// fwi_kernels.hip -- wave-equation core for gfx950 (MI355X).
+//
+// Three production kernels, one launch per time step for every shot of the
+// batch (grid = tiles x shot groups):
+//   k_fwd_step   a_lap_opt_2d_pml + time_integral + source + receivers +
+//                frame save, fused (fwi_2d.c:761-880, 882-910)
+//   k_adj_step   exact transpose of the forward step over the time history,
+//                with the residual injected at the receivers
+//   k_rec_step   source-field reconstruction on the physical grid
+//                (a_lap_opt_2d + time_integral + read_bdr, fwi_2d.c:722-759,
+//                866-880, 940-957) fused with the zero-lag imaging condition,
+//                summed over the shots of a workgroup in FP64
+//
+// Tiling: a wave owns 64 adjacent columns (one lane per column) and marches
+// down `tz` rows, keeping p (or mu), v and the x-derivative result of the
+// last rows in registers.  The x-derivatives of a row go through LDS with a
+// 3-cell halo each side; the z-derivatives come from the register window.
+// Each workgroup holds up to 4 waves = 4 shots on the same tile.
+//
+// Arithmetic follows the reference expression by expression (built with
+// -ffp-contract=off): float stencils, the free-surface rows and the leapfrog
+// sum evaluated in double as C evaluates the reference's double literals.
+#include <hip/hip_runtime.h>
+#include "fwi_dev.h"
+
+#define TX FWI_TX
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// CPML table rows: 0 a0x, 1 b0x, 2 a1x, 3 b1x, 4 a1z, 5 b1z; T = 2 npml + 1
+#define TB(row, i) tab[(row) * T + (i)]
+
+// frame position of physical cell (bz, bx) inside one step of the strips
+__device__ __forceinline__ int frame_pos(int bz, int bx, int nx, int nz)
+{
+    if (bz < 3) return bz * nx + bx;
+    if (bz >= nz - 3) return (3 + nz - 1 - bz) * nx + bx;
+    if (bx < 3) return 6 * nx + bz * 6 + bx;
+    return 6 * nx + bz * 6 + 3 + (nx - 1 - bx);
+}
+
+// ============================================================================
+// forward: x chain of one row (u nodes, CPML on u, ux nodes, CPML on ux)
+// ============================================================================
+
+// u at half node j (between p[j-1] and p[j]); pi = LDS index of p[j]
+__device__ __forceinline__ float fwd_u_node(const FwiGeomDev &g, const float *tab, const float *Ps,
+                                            int j, int pi, int r, const float *xh_o, float *xh_n,
+                                            int x0)
+{
+    const int pnx = g.pnx, npml = g.npml, T = 2 * npml + 1, n1 = npml + 1;
+    float u;
+    if (j <= 0 || j >= pnx) u = 0.f;
+    else if (j == 1 || j == pnx - 1) u = g.ix * (Ps[pi] - Ps[pi - 1]);
+    else u = g.c0x * (Ps[pi] - Ps[pi - 1]) + g.c1x * (Ps[pi + 1] - Ps[pi - 2]);
+    const bool own = (j >= x0) && (j < x0 + TX);
+    if (j >= 0 && j <= npml) {
+        float ps = xh_o[r * 2 * n1 + j];
+        ps = TB(1, 2 * j) * ps + TB(0, 2 * j) * u;
+        u = u + ps;
+        if (own) xh_n[r * 2 * n1 + j] = ps;
+    }
+    if (j >= pnx - npml && j <= pnx) {
+        const int q = j - (pnx - npml);
+        float ps = xh_o[r * 2 * n1 + n1 + q];
+        ps = TB(3, 2 * q) * ps + TB(2, 2 * q) * u;
+        u = u + ps;
+        if (own) xh_n[r * 2 * n1 + n1 + q] = ps;
+    }
+    return u;
+}
+
+// ux at p node i; ui = LDS index of u[i]
+__device__ __forceinline__ float fwd_ux_node(const FwiGeomDev &g, const float *tab, const float *Us,
+                                             int i, int ui, int r, const float *xd_o, float *xd_n)
+{
+    const int pnx = g.pnx, npml = g.npml, T = 2 * npml + 1;
+    if (i >= pnx) return 0.f;
+    float ux;
+    if (i == 0 || i == pnx - 1) ux = (Us[ui + 1] - Us[ui]) * g.ix;
+    else ux = g.c0x * (Us[ui + 1] - Us[ui]) + g.c1x * (Us[ui + 2] - Us[ui - 1]);
+    if (i < npml) {
+        float ps = xd_o[r * 2 * npml + i];
+        ps = TB(1, 2 * i + 1) * ps + TB(0, 2 * i + 1) * ux;
+        ux = ux + ps;
+        xd_n[r * 2 * npml + i] = ps;
+    } else if (i >= pnx - npml) {
+        const int q = i - (pnx - npml);
+        float ps = xd_o[r * 2 * npml + npml + q];
+        ps = TB(3, 2 * q + 1) * ps + TB(2, 2 * q + 1) * ux;
+        ux = ux + ps;
+        xd_n[r * 2 * npml + npml + q] = ps;
+    }
+    return ux;
+}
+
+template <bool FAST>
+__device__ __forceinline__ float fwd_xrow(const FwiGeomDev &g, const float *tab, const float *prow,
+                                          float pself, float *Ps, float *Us, int x0, int lane, int r,
+                                          const float *xh_o, float *xh_n, const float *xd_o, float *xd_n)
+{
+    wave_sync();
+    Ps[lane + 3] = pself;
+    if (lane < 3) {
+        const int q = x0 - 3 + lane;
+        Ps[lane] = (q >= 0) ? prow[q] : 0.f;
+    } else if (lane < 6) {
+        const int q = x0 + 61 + lane;
+        Ps[lane + 64] = (q < g.pnx) ? prow[q] : 0.f;
+    }
+    wave_sync();
+    if (FAST) {
+        Us[lane] = g.c0x * (Ps[lane + 2] - Ps[lane + 1]) + g.c1x * (Ps[lane + 3] - Ps[lane]);
+        if (lane < 3)
+            Us[64 + lane] = g.c0x * (Ps[lane + 66] - Ps[lane + 65]) + g.c1x * (Ps[lane + 67] - Ps[lane + 64]);
+    } else {
+        Us[lane] = fwd_u_node(g, tab, Ps, x0 - 1 + lane, lane + 2, r, xh_o, xh_n, x0);
+        if (lane < 3) Us[64 + lane] = fwd_u_node(g, tab, Ps, x0 + 63 + lane, lane + 66, r, xh_o, xh_n, x0);
+    }
+    wave_sync();
+    if (FAST) return g.c0x * (Us[lane + 2] - Us[lane + 1]) + g.c1x * (Us[lane + 3] - Us[lane]);
+    return fwd_ux_node(g, tab, Us, x0 + lane, lane + 1, r, xd_o, xd_n);
+}
+
+__device__ __forceinline__ float v_row0(float p0, float p1, float p2, float p3, float iz840)
+{
+    return (float)((3675.0 * p0 - 1225.0 * p1 + 441.0 * p2 - 75.0 * p3) * (double)iz840);
+}
+__device__ __forceinline__ float v_row1(float p0, float p1, float p2, float p3, float iz840)
+{
+    return (float)((-1085.0 * p0 + 1015.0 * p1 - 63.0 * p2 + 5.0 * p3) * (double)iz840);
+}
+__device__ __forceinline__ float vz_row0(float v0, float v1, float v2, float v3, float v4, float iz24)
+{
+    return (float)((-22. * v0 + 17. * v1 + 9. * v2 - 5. * v3 + v4) * (double)iz24);
+}
+
+// leapfrog, fwi_2d.c:878: 2.0*pnow - pold + ekp*(pxx+pzz), in double, rounded once
+__device__ __forceinline__ float leapfrog(float pnow, float pold, float k_times_x)
+{
+    return (float)(2.0 * (double)pnow - (double)pold + (double)k_times_x);
+}
+
+__global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_fwd_step(FwiFwdArgs A, int k)
+{
+    const FwiGeomDev &g = A.g;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int ntiles = g.ntx * g.ntz;
+    const int grp = blockIdx.x / ntiles, t = blockIdx.x - grp * ntiles;
+    const int tzi = t / g.ntx, txi = t - tzi * g.ntx;
+    const int s = grp * A.b.sg + wv;
+    if (s >= A.b.nshots) return;
+    const FwiShotDev sh = A.b.shots[s];
+    if (k >= sh.nt) return;
+
+    __shared__ float lds[FWI_MAXSG][2][72];
+    float *Ps = lds[wv][0], *Us = lds[wv][1];
+
+    const int npml = g.npml, pnx = g.pnx, pnz = g.pnz, ld = g.ld, T = 2 * npml + 1;
+    const float *tab = A.b.tab;
+    const int x0 = txi * TX, x = x0 + lane;
+    const int z0 = tzi * g.tz, z1 = min(z0 + g.tz, pnz);
+    const long long so = (long long)s * A.b.fs;
+    const int cur = k & 1, nxt = cur ^ 1;
+    const float *pnow = (cur ? A.P1 : A.P0) + so;
+    float *pold = (cur ? A.P0 : A.P1) + so;
+    const float *ekp = A.b.ekp;
+    const float *xh_o = A.xh + s * A.b.xh_s + cur * A.b.xh_b;
+    float *xh_n = A.xh + s * A.b.xh_s + nxt * A.b.xh_b;
+    const float *xd_o = A.xd + s * A.b.xd_s + cur * A.b.xd_b;
+    float *xd_n = A.xd + s * A.b.xd_s + nxt * A.b.xd_b;
+    const float *zh_o = A.zh + s * A.b.zh_s + cur * A.b.zh_b;
+    float *zh_n = A.zh + s * A.b.zh_s + nxt * A.b.zh_b;
+    const float *zd_o = A.zd + s * A.b.zd_s + cur * A.b.zd_b;
+    float *zd_n = A.zd + s * A.b.zd_s + nxt * A.b.zd_b;
+    const bool fastx = (x0 >= npml + 3) && (x0 + 66 <= pnx - npml - 1);
+    const int bx = x - npml, nx = g.nx, nz = g.nz;
+    const bool in_bx = (bx >= 0) && (bx < nx);
+
+    float p0w = 0.f, p1w = 0.f, p2w = 0.f, p3w = 0.f;  // p[r-3..r]
+    float v0w = 0.f, v1w = 0.f, v2w = 0.f, v3w = 0.f;  // v[r-4..r-1]
+    float q0 = 0.f, q1 = 0.f, q2 = 0.f, q3 = 0.f;      // ux[r-3..r]
+
+    for (int r = z0 - 3; r <= z1 + 2; ++r) {
+        float pr = 0.f;
+        if (r >= 0 && r < pnz) pr = pnow[(long long)r * ld + x];
+        float uxr = 0.f;
+        if (r >= z0 && r < z1) {
+            const float *prow = pnow + (long long)r * ld;
+            uxr = fastx ? fwd_xrow<true>(g, tab, prow, pr, Ps, Us, x0, lane, r, xh_o, xh_n, xd_o, xd_n)
+                        : fwd_xrow<false>(g, tab, prow, pr, Ps, Us, x0, lane, r, xh_o, xh_n, xd_o, xd_n);
+        }
+        p0w = p1w; p1w = p2w; p2w = p3w; p3w = pr;
+        q0 = q1; q1 = q2; q2 = q3; q3 = uxr;
+
+        // v at half node j = r - 1: window holds p[j-2..j+1]
+        const int j = r - 1;
+        float vj = 0.f;
+        if (j >= 0 && j < pnz) {
+            if (j == 0) vj = v_row0(p2w, p3w, pnow[2 * ld + x], pnow[3 * ld + x], g.iz840);
+            else if (j == 1) vj = v_row1(p1w, p2w, p3w, pnow[3 * ld + x], g.iz840);
+            else if (j == pnz - 1) vj = (p2w - p1w) * g.iz;
+            else vj = g.c0z * (p2w - p1w) + g.c1z * (p3w - p0w);
+            if (j >= pnz - npml) {
+                const int q = j - (pnz - npml);
+                float ps = zh_o[q * ld + x];
+                ps = TB(5, 2 * q) * ps + TB(4, 2 * q) * vj;
+                vj = vj + ps;
+                if (j >= z0 && j < z1) zh_n[q * ld + x] = ps;
+            }
+        }
+        v0w = v1w; v1w = v2w; v2w = v3w; v3w = vj;
+
+        const int i = r - 3;  // output row: window v[i-1..i+2], p[i] = p0w, ux[i] = q0
+        if (i >= z0 && i < z1) {
+            float vz;
+            if (i == 0) {
+                const float p4 = pnow[4 * ld + x], p5 = pnow[5 * ld + x];
+                const float v3 = g.c0z * (p3w - p2w) + g.c1z * (p4 - p1w);
+                const float v4 = g.c0z * (p4 - p3w) + g.c1z * (p5 - p2w);
+                vz = vz_row0(v1w, v2w, v3w, v3, v4, g.iz24);
+            } else if (i == pnz - 1) {
+                vz = (v2w - v1w) * g.iz;
+            } else {
+                vz = g.c0z * (v2w - v1w) + g.c1z * (v3w - v0w);
+            }
+            if (i >= pnz - npml) {
+                const int q = i - (pnz - npml);
+                float ps = zd_o[q * ld + x];
+                ps = TB(5, 2 * q + 1) * ps + TB(4, 2 * q + 1) * vz;
+                vz = vz + ps;
+                zd_n[q * ld + x] = ps;
+            }
+            float ux = q0;
+            if (i == sh.isz && x == sh.isx) ux = ux + A.b.srcf[sh.src_off + k];
+            const float X = ux + vz;
+            const long long o = (long long)i * ld + x;
+            const float pnew = leapfrog(p0w, pold[o], ekp[o] * X);
+            if (x < pnx) pold[o] = pnew;
+            const int slot = A.b.rrow[sh.rrow_off + i];
+            if (slot >= 0) {
+                const int ri = A.b.rmap[slot * ld + x];
+                if (ri >= 0) A.syn[sh.trace_off + (long long)k * sh.nr + ri] = pnew;
+            }
+            if (in_bx && i < nz && (i < 3 || i >= nz - 3 || bx < 3 || bx >= nx - 3)) {
+                const long long so2 = sh.strip_off + (long long)k * g.sl;
+                if (i < 3 || i >= nz - 3) {
+                    const int pos = (i < 3) ? i * nx + bx : (3 + nz - 1 - i) * nx + bx;
+                    A.pstrip[so2 + pos] = pnew;
+                    A.xstrip[so2 + pos] = X;
+                }
+                if (bx < 3 || bx >= nx - 3) {
+                    const int pos = 6 * nx + i * 6 + ((bx < 3) ? bx : 3 + (nx - 1 - bx));
+                    A.pstrip[so2 + pos] = pnew;
+                    A.xstrip[so2 + pos] = X;
+                }
+            }
+        }
+    }
+}
+
+// ============================================================================
+// adjoint: exact transpose, x chain
+// ============================================================================
+
+// C^T of the ux-node CPML (backward in time) applied to w at p node q
+__device__ __forceinline__ float adj_filt_d(const FwiGeomDev &g, const float *tab, int q, float w, int r,
+                                            const float *xd_o, float *xd_n, bool own)
+{
+    const int pnx = g.pnx, npml = g.npml, T = 2 * npml + 1;
+    if (q >= 0 && q < npml) {
+        float c = xd_o[r * 2 * npml + q];
+        c = TB(1, 2 * q + 1) * c + TB(0, 2 * q + 1) * w;
+        w = w + c;
+        if (own) xd_n[r * 2 * npml + q] = c;
+    } else if (q >= pnx - npml && q < pnx) {
+        const int m = q - (pnx - npml);
+        float c = xd_o[r * 2 * npml + npml + m];
+        c = TB(3, 2 * m + 1) * c + TB(2, 2 * m + 1) * w;
+        w = w + c;
+        if (own) xd_n[r * 2 * npml + npml + m] = c;
+    }
+    return w;
+}
+
+// rows of the second x derivative (p node i <- u nodes): taps u[i+1],u[i],u[i+2],u[i-1]
+__device__ __forceinline__ float hxA(const FwiGeomDev &g, int i)
+{ return (i < 0 || i >= g.pnx) ? 0.f : ((i == 0 || i == g.pnx - 1) ? g.ix : g.c0x); }
+__device__ __forceinline__ float hxB(const FwiGeomDev &g, int i)
+{ return (i < 0 || i >= g.pnx) ? 0.f : ((i == 0 || i == g.pnx - 1) ? -g.ix : -g.c0x); }
+__device__ __forceinline__ float hxC(const FwiGeomDev &g, int i)
+{ return (i < 1 || i >= g.pnx - 1) ? 0.f : g.c1x; }
+__device__ __forceinline__ float hxD(const FwiGeomDev &g, int i)
+{ return (i < 1 || i >= g.pnx - 1) ? 0.f : -g.c1x; }
+// rows of the first x derivative (u node j <- p nodes): taps p[j],p[j-1],p[j+1],p[j-2]
+__device__ __forceinline__ float gxE(const FwiGeomDev &g, int j)
+{ return (j <= 0 || j >= g.pnx) ? 0.f : ((j == 1 || j == g.pnx - 1) ? g.ix : g.c0x); }
+__device__ __forceinline__ float gxF(const FwiGeomDev &g, int j)
+{ return (j <= 0 || j >= g.pnx) ? 0.f : ((j == 1 || j == g.pnx - 1) ? -g.ix : -g.c0x); }
+__device__ __forceinline__ float gxG(const FwiGeomDev &g, int j)
+{ return (j <= 1 || j >= g.pnx - 1) ? 0.f : g.c1x; }
+__device__ __forceinline__ float gxH(const FwiGeomDev &g, int j)
+{ return (j <= 1 || j >= g.pnx - 1) ? 0.f : -g.c1x; }
+
+// H^T gather at u node j (wi = LDS index of w'[j]) followed by C^T of the u-node CPML
+__device__ __forceinline__ float adj_u_node(const FwiGeomDev &g, const float *tab, const float *Ws, int j,
+                                            int wi, int r, const float *xh_o, float *xh_n, int x0)
+{
+    const int pnx = g.pnx, npml = g.npml, T = 2 * npml + 1, n1 = npml + 1;
+    if (j < 0 || j > pnx) return 0.f;
+    float u = hxA(g, j - 1) * Ws[wi - 1] + hxB(g, j) * Ws[wi] + hxC(g, j - 2) * Ws[wi - 2] + hxD(g, j + 1) * Ws[wi + 1];
+    const bool own = (j >= x0) && (j < x0 + TX);
+    if (j <= npml) {
+        float c = xh_o[r * 2 * n1 + j];
+        c = TB(1, 2 * j) * c + TB(0, 2 * j) * u;
+        u = u + c;
+        if (own) xh_n[r * 2 * n1 + j] = c;
+    }
+    if (j >= pnx - npml) {
+        const int q = j - (pnx - npml);
+        float c = xh_o[r * 2 * n1 + n1 + q];
+        c = TB(3, 2 * q) * c + TB(2, 2 * q) * u;
+        u = u + c;
+        if (own) xh_n[r * 2 * n1 + n1 + q] = c;
+    }
+    return u;
+}
+
+template <bool FAST>
+__device__ __forceinline__ float adj_xrow(const FwiGeomDev &g, const float *tab, const float *wrow, float wself,
+                                          float *Ws, float *Us, int x0, int lane, int r,
+                                          const float *xh_o, float *xh_n, const float *xd_o, float *xd_n)
+{
+    wave_sync();
+    float wm = wself;
+    if (!FAST) wm = adj_filt_d(g, tab, x0 + lane, wm, r, xd_o, xd_n, x0 + lane < g.pnx);
+    Ws[lane + 3] = wm;
+    if (lane < 6) {
+        const int q = (lane < 3) ? x0 - 3 + lane : x0 + 61 + lane;
+        const int idx = (lane < 3) ? lane : lane + 64;
+        float wh = (q >= 0 && q < g.pnx) ? wrow[q] : 0.f;
+        if (!FAST) wh = adj_filt_d(g, tab, q, wh, r, xd_o, xd_n, false);
+        Ws[idx] = wh;
+    }
+    wave_sync();
+    if (FAST) {
+        Us[lane] = g.c0x * (Ws[lane + 1] - Ws[lane + 2]) + g.c1x * (Ws[lane] - Ws[lane + 3]);
+        if (lane < 3)
+            Us[64 + lane] = g.c0x * (Ws[lane + 65] - Ws[lane + 66]) + g.c1x * (Ws[lane + 64] - Ws[lane + 67]);
+    } else {
+        Us[lane] = adj_u_node(g, tab, Ws, x0 - 1 + lane, lane + 2, r, xh_o, xh_n, x0);
+        if (lane < 3) Us[64 + lane] = adj_u_node(g, tab, Ws, x0 + 63 + lane, lane + 66, r, xh_o, xh_n, x0);
+    }
+    wave_sync();
+    if (FAST) return g.c0x * (Us[lane + 1] - Us[lane + 2]) + g.c1x * (Us[lane] - Us[lane + 3]);
+    const int i = x0 + lane, ui = lane + 1;
+    if (i >= g.pnx) return 0.f;
+    return gxE(g, i) * Us[ui] + gxF(g, i + 1) * Us[ui + 1] + gxG(g, i - 1) * Us[ui - 1] + gxH(g, i + 2) * Us[ui + 2];
+}
+
+__global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_adj_step(FwiAdjArgs A, int k)
+{
+    const FwiGeomDev &g = A.g;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int ntiles = g.ntx * g.ntz;
+    const int grp = blockIdx.x / ntiles, t = blockIdx.x - grp * ntiles;
+    const int tzi = t / g.ntx, txi = t - tzi * g.ntx;
+    const int s = grp * A.b.sg + wv;
+    if (s >= A.b.nshots) return;
+    const FwiShotDev sh = A.b.shots[s];
+    if (k >= sh.nt) return;
+
+    __shared__ float lds[FWI_MAXSG][2][72];
+    float *Ws = lds[wv][0], *Us = lds[wv][1];
+
+    const int npml = g.npml, pnx = g.pnx, pnz = g.pnz, ld = g.ld, T = 2 * npml + 1;
+    const float *tab = A.b.tab;
+    const int x0 = txi * TX, x = x0 + lane;
+    const int z0 = tzi * g.tz, z1 = min(z0 + g.tz, pnz);
+    const long long so = (long long)s * A.b.fs;
+    // mu^{k+2} in M[k&1] (read), mu^{k+3} in M[(k+1)&1] overwritten by mu^{k+1}
+    const int cur = k & 1, prv = cur ^ 1;
+    const float *mnow = (cur ? A.M1 : A.M0) + so;
+    float *mold = (cur ? A.M0 : A.M1) + so;
+    const float *ekp = A.b.ekp;
+    // chi after step k goes to buffer (k & 1); it was read from buffer ((k+1) & 1)
+    const float *xh_o = A.xh + s * A.b.xh_s + prv * A.b.xh_b;
+    float *xh_n = A.xh + s * A.b.xh_s + cur * A.b.xh_b;
+    const float *xd_o = A.xd + s * A.b.xd_s + prv * A.b.xd_b;
+    float *xd_n = A.xd + s * A.b.xd_s + cur * A.b.xd_b;
+    const float *zh_o = A.zh + s * A.b.zh_s + prv * A.b.zh_b;
+    float *zh_n = A.zh + s * A.b.zh_s + cur * A.b.zh_b;
+    const float *zd_o = A.zd + s * A.b.zd_s + prv * A.b.zd_b;
+    float *zd_n = A.zd + s * A.b.zd_s + cur * A.b.zd_b;
+    const bool fastx = (x0 >= npml + 3) && (x0 + 66 <= pnx - npml - 1);
+    const float c0z = g.c0z, c1z = g.c1z, izf = g.iz;
+    // transposed free-surface rows
+    const double i24 = (double)g.iz24, i840 = (double)g.iz840;
+    const float hz0[5] = {(float)(-22. * i24), (float)(17. * i24), (float)(9. * i24), (float)(-5. * i24),
+                          (float)(1. * i24)};
+    const float gz0[4] = {(float)(3675.0 * i840), (float)(-1225.0 * i840), (float)(441.0 * i840),
+                          (float)(-75.0 * i840)};
+    const float gz1[4] = {(float)(-1085.0 * i840), (float)(1015.0 * i840), (float)(-63.0 * i840),
+                          (float)(5.0 * i840)};
+
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;  // w'_z[r-3..r] (z-filtered mu^{k+2})
+    float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f;  // raw mu^{k+2}[r-3..r]
+    float b0 = 0.f, b1 = 0.f, b2 = 0.f, b3 = 0.f;  // v~'[r-4..r-1]
+    float q0 = 0.f, q1 = 0.f, q2 = 0.f, q3 = 0.f;  // out_x[r-3..r]
+    float wz0 = 0.f, tv0 = 0.f, tv1 = 0.f;          // top rows held by the first tile
+
+    for (int r = z0 - 3; r <= z1 + 2; ++r) {
+        float wr = 0.f;
+        if (r >= 0 && r < pnz) wr = mnow[(long long)r * ld + x];
+        float oxr = 0.f;
+        if (r >= z0 && r < z1) {
+            const float *wrow = mnow + (long long)r * ld;
+            oxr = fastx ? adj_xrow<true>(g, tab, wrow, wr, Ws, Us, x0, lane, r, xh_o, xh_n, xd_o, xd_n)
+                        : adj_xrow<false>(g, tab, wrow, wr, Ws, Us, x0, lane, r, xh_o, xh_n, xd_o, xd_n);
+        }
+        // z chain: C^T of the vz-node CPML
+        float wz = wr;
+        if (r >= pnz - npml && r < pnz) {
+            const int q = r - (pnz - npml);
+            float c = zd_o[q * ld + x];
+            c = TB(5, 2 * q + 1) * c + TB(4, 2 * q + 1) * wz;
+            wz = wz + c;
+            if (r >= z0 && r < z1) zd_n[q * ld + x] = c;
+        }
+        if (r == 0) wz0 = wz;
+        a0 = a1; a1 = a2; a2 = a3; a3 = wz;
+        m0 = m1; m1 = m2; m2 = m3; m3 = wr;
+        q0 = q1; q1 = q2; q2 = q3; q3 = oxr;
+
+        // v~ at half node j = r - 1 from w'[j-2..j+1] = a0..a3  (H_z^T)
+        const int j = r - 1;
+        float tv = 0.f;
+        if (j >= 0 && j <= pnz) {
+            if (j > 4 && j + 1 <= pnz - 2) {
+                tv = c0z * (a1 - a2) + c1z * (a0 - a3);
+            } else {
+                // interior rows i in [1, pnz-1): taps v[i+1]=c0, v[i]=-c0, v[i+2]=c1, v[i-1]=-c1
+                if (j - 1 >= 1 && j - 1 < pnz - 1) tv += c0z * a1;
+                if (j >= 1 && j < pnz - 1) tv -= c0z * a2;
+                if (j - 2 >= 1 && j - 2 < pnz - 1) tv += c1z * a0;
+                if (j + 1 >= 1 && j + 1 < pnz - 1) tv -= c1z * a3;
+                if (j <= 4) tv += hz0[j] * wz0;
+                if (j == pnz) tv += izf * a1;
+                if (j == pnz - 1) tv -= izf * a2;
+            }
+            if (j >= pnz - npml && j < pnz) {
+                const int q = j - (pnz - npml);
+                float c = zh_o[q * ld + x];
+                c = TB(5, 2 * q) * c + TB(4, 2 * q) * tv;
+                tv = tv + c;
+                if (j >= z0 && j < z1) zh_n[q * ld + x] = c;
+            }
+        }
+        if (j == 0) tv0 = tv;
+        if (j == 1) tv1 = tv;
+        b0 = b1; b1 = b2; b2 = b3; b3 = tv;
+
+        const int i = r - 3;  // output row: v~'[i-1..i+2] = b0..b3
+        if (i >= z0 && i < z1) {
+            float oz = 0.f;
+            if (i > 3 && i - 1 >= 2 && i + 2 < pnz - 1) {
+                oz = c0z * (b1 - b2) + c1z * (b0 - b3);
+            } else {
+                // rows j in [2, pnz-1): taps p[j]=c0, p[j-1]=-c0, p[j+1]=c1, p[j-2]=-c1
+                if (i >= 2 && i < pnz - 1) oz += c0z * b1;
+                if (i + 1 >= 2 && i + 1 < pnz - 1) oz -= c0z * b2;
+                if (i - 1 >= 2 && i - 1 < pnz - 1) oz += c1z * b0;
+                if (i + 2 >= 2 && i + 2 < pnz - 1) oz -= c1z * b3;
+                if (i <= 3) oz += gz0[i] * tv0 + gz1[i] * tv1;
+                if (i == pnz - 1) oz += izf * b1;
+                if (i == pnz - 2) oz -= izf * b2;
+            }
+            float out = q0 + oz;
+            const int slot = A.b.rrow[sh.rrow_off + i];
+            if (slot >= 0) {
+                const int ri = A.b.rmap[slot * ld + x];
+                if (ri >= 0) out = out + A.res[sh.trace_off + (long long)k * sh.nr + ri];
+            }
+            const long long o = (long long)i * ld + x;
+            const float mnew = leapfrog(m0, mold[o], ekp[o] * out);
+            if (x < pnx) mold[o] = mnew;
+        }
+    }
+}
+
+// ============================================================================
+// reconstruction (physical grid) + zero-lag imaging
+// ============================================================================
+__global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_rec_step(FwiRecArgs A, int k)
+{
+    const FwiGeomDev &g = A.g;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int ntiles = g.bntx * g.bntz;
+    const int grp = blockIdx.x / ntiles, t = blockIdx.x - grp * ntiles;
+    const int tzi = t / g.bntx, txi = t - tzi * g.bntx;
+    const int s = grp * A.b.sg + wv;
+    bool active = s < A.b.nshots;
+    FwiShotDev sh;
+    if (active) {
+        sh = A.b.shots[s];
+        active = k < sh.nt;
+    }
+
+    __shared__ float lds[FWI_MAXSG][2][72];
+    __shared__ double red[FWI_MAXSG][TX];
+    float *Ps = lds[wv][0], *Us = lds[wv][1];
+
+    const int npml = g.npml, nx = g.nx, nz = g.nz, ld = g.ld, sg = A.b.sg;
+    const int bx0 = txi * TX, bx = bx0 + lane, fx = bx + npml;
+    const int z0 = tzi * g.tz, z1 = min(z0 + g.tz, nz);
+    const bool colok = bx < nx;
+    const long long so = (long long)(active ? s : 0) * A.b.fs;
+    const int cur = k & 1;
+    const float *pn = (cur ? A.P1 : A.P0) + so;  // p^k
+    float *pp = (cur ? A.P0 : A.P1) + so;        // p^{k+1} -> p^{k-1}
+    const float *mu = (cur ? A.M0 : A.M1) + so;  // mu^{k+1}
+    const float *ekp = A.b.ekp;
+    const float c0x = g.c0x, c1x = g.c1x, c0z = g.c0z, c1z = g.c1z;
+    double *gacc = A.gacc + (long long)grp * nx * nz;
+
+    float p0w = 0.f, p1w = 0.f, p2w = 0.f, p3w = 0.f;
+    float v0w = 0.f, v1w = 0.f, v2w = 0.f, v3w = 0.f;
+    float q0 = 0.f, q1 = 0.f, q2 = 0.f, q3 = 0.f;
+
+    for (int r = z0 - 3; r <= z1 + 2; ++r) {
+        float pr = 0.f;
+        if (active && r >= 0 && r < nz && colok) pr = pn[(long long)r * ld + fx];
+        float uxr = 0.f;
+        if (active && r >= z0 && r < z1 && r >= 3 && r < nz - 3) {
+            const float *prow = pn + (long long)r * ld + npml;  // physical column 0
+            wave_sync();
+            Ps[lane + 3] = pr;
+            if (lane < 3) {
+                const int q = bx0 - 3 + lane;
+                Ps[lane] = (q >= 0) ? prow[q] : 0.f;
+            } else if (lane < 6) {
+                const int q = bx0 + 61 + lane;
+                Ps[lane + 64] = (q < nx) ? prow[q] : 0.f;
+            }
+            wave_sync();
+            Us[lane] = c0x * (Ps[lane + 2] - Ps[lane + 1]) + c1x * (Ps[lane + 3] - Ps[lane]);
+            if (lane < 3)
+                Us[64 + lane] = c0x * (Ps[lane + 66] - Ps[lane + 65]) + c1x * (Ps[lane + 67] - Ps[lane + 64]);
+            wave_sync();
+            uxr = c0x * (Us[lane + 2] - Us[lane + 1]) + c1x * (Us[lane + 3] - Us[lane]);
+        }
+        p0w = p1w; p1w = p2w; p2w = p3w; p3w = pr;
+        q0 = q1; q1 = q2; q2 = q3; q3 = uxr;
+        const float vj = c0z * (p2w - p1w) + c1z * (p3w - p0w);  // v[r-1]; used only where all taps are inside
+        v0w = v1w; v1w = v2w; v2w = v3w; v3w = vj;
+
+        const int i = r - 3;
+        if (i >= z0 && i < z1) {
+            double img = 0.0;
+            if (active && colok) {
+                const long long so2 = sh.strip_off + (long long)k * g.sl;
+                const bool inner = (i >= 3 && i < nz - 3 && bx >= 3 && bx < nx - 3);
+                float X;
+                if (inner) {
+                    float ux = q0;
+                    if (i == sh.isz && fx == sh.isx) ux = ux + A.b.srcf[sh.src_off + k];
+                    const float vz = c0z * (v2w - v1w) + c1z * (v3w - v0w);
+                    X = ux + vz;
+                } else {
+                    X = A.xstrip[so2 + frame_pos(i, bx, nx, nz)];
+                }
+                const long long o = (long long)i * ld + fx;
+                img = (double)mu[o] * (double)X;
+                if (k >= 1) {
+                    float pm;
+                    if (inner) pm = leapfrog(p0w, pp[o], ekp[o] * X);
+                    else pm = (k >= 2) ? A.pstrip[sh.strip_off + (long long)(k - 2) * g.sl + frame_pos(i, bx, nx, nz)]
+                                       : 0.f;
+                    pp[o] = pm;
+                }
+            }
+            red[wv][lane] = img;
+            __syncthreads();
+            if (wv == 0 && colok) {
+                double acc = 0.0;
+                for (int w = 0; w < sg; ++w) acc += red[w][lane];
+                gacc[(long long)i * nx + bx] += acc;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// ============================================================================
+// small kernels
+// ============================================================================
+__global__ void k_fill_f32(float *p, long long n, float v)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_fill_f64(double *p, long long n, double v)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// time-major [nt][nr] <-> trace-major [nr][nt] for one shot
+__global__ void k_transpose(const float *in, float *out, int rows, int cols)
+{
+    __shared__ float tile[32][33];
+    const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+    for (int yy = threadIdx.y; yy < 32; yy += blockDim.y) {
+        const int r = by + yy, c = bx + threadIdx.x;
+        if (r < rows && c < cols) tile[yy][threadIdx.x] = in[(long long)r * cols + c];
+    }
+    __syncthreads();
+    for (int yy = threadIdx.y; yy < 32; yy += blockDim.y) {
+        const int r = bx + yy, c = by + threadIdx.x;  // out is cols x rows
+        if (r < cols && c < rows) out[(long long)r * rows + c] = tile[threadIdx.x][yy];
+    }
+}
+
+// res = syn - obs; partial sums of 1/2 res^2 in double, one per block
+__global__ void k_misfit_l2(const float *syn, const float *obs, float *res, long long n, double *partial)
+{
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const float rr = syn[i] - obs[i];
+        res[i] = rr;
+        acc += 0.5 * (double)rr * (double)rr;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// grad = 2 * sum_groups gacc / vel
+__global__ void k_grad_finalize(const double *gacc, int ngroups, long long n, const float *vel, double *gk,
+                                double *grad)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        double acc = 0.0;
+        for (int gi = 0; gi < ngroups; ++gi) acc += gacc[(long long)gi * n + i];
+        gk[i] = acc;
+        grad[i] = 2.0 * acc / (double)vel[i];
+    }
+}
+
+// ============================================================================
+// per-step kernels with the reference's loop structure (unit parity)
+// p[pnz][pnx], u[pnz][pnx+1], v[pnz+1][pnx], ux/vz[pnz][pnx], dense rows.
+// ============================================================================
+
+// phase = index of the reference loop in a_lap_opt_2d_pml (0..11, fwi_2d.c:775-862)
+// or 20..23 for a_lap_opt_2d (fwi_2d.c:735-756)
+__global__ void k_ref_phase(FwiRefArgs a, int phase)
+{
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x, iz = blockIdx.y * blockDim.y + threadIdx.y;
+    const int pnx = a.pnx, pnz = a.pnz, npml = a.npml, T = 2 * npml + 1;
+    const int unx = pnx + 1, unz = pnz, vnx = pnx, vnz = pnz + 1, n1 = npml + 1;
+    const float *p = a.p, *tab = a.tab;
+    float *u = a.u, *v = a.v, *ux = a.ux, *vz = a.vz;
+#define P_(z, x) p[(z) * pnx + (x)]
+#define U_(z, x) u[(z) * unx + (x)]
+#define V_(z, x) v[(z) * vnx + (x)]
+#define UX_(z, x) ux[(z) * pnx + (x)]
+#define VZ_(z, x) vz[(z) * pnx + (x)]
+    switch (phase) {
+    case 0:
+        if (iz < unz && ix >= 2 && ix < unx - 2)
+            U_(iz, ix) = a.c0x * (P_(iz, ix) - P_(iz, ix - 1)) + a.c1x * (P_(iz, ix + 1) - P_(iz, ix - 2));
+        break;
+    case 1:
+        if (iz < unz && ix == 0) {
+            U_(iz, 1) = a.ix * (P_(iz, 1) - P_(iz, 0));
+            U_(iz, unx - 2) = a.ix * (P_(iz, pnx - 1) - P_(iz, pnx - 2));
+        }
+        break;
+    case 2:
+        if (iz < unz && ix < npml + 1) {
+            a.u0[iz * n1 + ix] = TB(1, 2 * ix) * a.u0[iz * n1 + ix] + TB(0, 2 * ix) * U_(iz, ix);
+            U_(iz, ix) += a.u0[iz * n1 + ix];
+            a.u1[iz * n1 + ix] = TB(3, 2 * ix) * a.u1[iz * n1 + ix] + TB(2, 2 * ix) * U_(iz, ix + unx - npml - 1);
+            U_(iz, ix + unx - npml - 1) += a.u1[iz * n1 + ix];
+        }
+        break;
+    case 3:
+        if (iz >= 2 && iz < vnz - 2 && ix < vnx)
+            V_(iz, ix) = a.c0z * (P_(iz, ix) - P_(iz - 1, ix)) + a.c1z * (P_(iz + 1, ix) - P_(iz - 2, ix));
+        break;
+    case 4:
+        if (iz == 0 && ix < vnx) {
+            V_(0, ix) = v_row0(P_(0, ix), P_(1, ix), P_(2, ix), P_(3, ix), a.iz840);
+            V_(1, ix) = v_row1(P_(0, ix), P_(1, ix), P_(2, ix), P_(3, ix), a.iz840);
+            V_(vnz - 2, ix) = (P_(pnz - 1, ix) - P_(pnz - 2, ix)) * a.iz;
+        }
+        break;
+    case 5:
+        if (iz < npml + 1 && ix < vnx) {
+            a.v1[iz * vnx + ix] = TB(5, 2 * iz) * a.v1[iz * vnx + ix] + TB(4, 2 * iz) * V_(iz + vnz - npml - 1, ix);
+            V_(iz + vnz - npml - 1, ix) += a.v1[iz * vnx + ix];
+        }
+        break;
+    case 6:
+        if (iz < pnz && ix >= 1 && ix < pnx - 1)
+            UX_(iz, ix) = a.c0x * (U_(iz, ix + 1) - U_(iz, ix)) + a.c1x * (U_(iz, ix + 2) - U_(iz, ix - 1));
+        break;
+    case 7:
+        if (iz < pnz && ix == 0) {
+            UX_(iz, 0) = (U_(iz, 1) - U_(iz, 0)) * a.ix;
+            UX_(iz, pnx - 1) = (U_(iz, unx - 1) - U_(iz, unx - 2)) * a.ix;
+        }
+        break;
+    case 8:
+        if (iz < pnz && ix < npml) {
+            a.ux0[iz * npml + ix] = TB(1, 2 * ix + 1) * a.ux0[iz * npml + ix] + TB(0, 2 * ix + 1) * UX_(iz, ix);
+            UX_(iz, ix) += a.ux0[iz * npml + ix];
+            a.ux1[iz * npml + ix] = TB(3, 2 * ix + 1) * a.ux1[iz * npml + ix] + TB(2, 2 * ix + 1) * UX_(iz, ix + pnx - npml);
+            UX_(iz, ix + pnx - npml) += a.ux1[iz * npml + ix];
+        }
+        break;
+    case 9:
+        if (iz == 0 && ix < pnx) {
+            VZ_(0, ix) = vz_row0(V_(0, ix), V_(1, ix), V_(2, ix), V_(3, ix), V_(4, ix), a.iz24);
+            VZ_(pnz - 1, ix) = (V_(vnz - 1, ix) - V_(vnz - 2, ix)) * a.iz;
+        }
+        break;
+    case 10:
+        if (iz >= 1 && iz < pnz - 1 && ix < pnx)
+            VZ_(iz, ix) = a.c0z * (V_(iz + 1, ix) - V_(iz, ix)) + a.c1z * (V_(iz + 2, ix) - V_(iz - 1, ix));
+        break;
+    case 11:
+        if (iz < npml && ix < pnx) {
+            a.vz1[iz * pnx + ix] = TB(5, 2 * iz + 1) * a.vz1[iz * pnx + ix] + TB(4, 2 * iz + 1) * VZ_(iz + pnz - npml, ix);
+            VZ_(iz + pnz - npml, ix) += a.vz1[iz * pnx + ix];
+        }
+        break;
+    case 20:
+        if (iz >= 2 && iz < unz - 2 && ix >= 2 && ix < unx - 2)
+            U_(iz, ix) = a.c0x * (P_(iz, ix) - P_(iz, ix - 1)) + a.c1x * (P_(iz, ix + 1) - P_(iz, ix - 2));
+        break;
+    case 21:
+        if (iz >= 2 && iz < vnz - 2 && ix >= 2 && ix < vnx - 2)
+            V_(iz, ix) = a.c0z * (P_(iz, ix) - P_(iz - 1, ix)) + a.c1z * (P_(iz + 1, ix) - P_(iz - 2, ix));
+        break;
+    case 22:
+        if (iz >= 3 && iz < pnz - 3 && ix >= 3 && ix < pnx - 3)
+            UX_(iz, ix) = a.c0x * (U_(iz, ix + 1) - U_(iz, ix)) + a.c1x * (U_(iz, ix + 2) - U_(iz, ix - 1));
+        break;
+    case 23:
+        if (iz >= 3 && iz < pnz - 3 && ix >= 3 && ix < pnx - 3)
+            VZ_(iz, ix) = a.c0z * (V_(iz + 1, ix) - V_(iz, ix)) + a.c1z * (V_(iz + 2, ix) - V_(iz - 1, ix));
+        break;
+    default:
+        break;
+    }
+#undef P_
+#undef U_
+#undef V_
+#undef UX_
+#undef VZ_
+}
+
+__global__ void k_ref_time_integral(float *pnew, const float *pnow, const float *pold, const float *pxx,
+                                    const float *pzz, const float *ekp, long long n)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        pnew[i] = leapfrog(pnow[i], pold[i], ekp[i] * (pxx[i] + pzz[i]));
+}
+
+// write_bdr / read_bdr for one step (fwi_2d.c:891-910, 940-957)
+__global__ void k_ref_write_bdr(float *ud, float *lr, const float *pf, int nx, int nz, int npml)
+{
+    const int pnx_f = nx + 2 * npml;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < 3 * nx) {
+        const int i = t / nx, j = t - i * nx;
+        ud[i * nx + j] = pf[i * pnx_f + j + npml];
+        ud[(i + 3) * nx + j] = pf[(nz - i - 1) * pnx_f + j + npml];
+    }
+    if (t < 3 * nz) {
+        const int i = t / 3, j = t - i * 3;
+        lr[i * 6 + j] = pf[i * pnx_f + j + npml];
+        lr[i * 6 + j + 3] = pf[i * pnx_f + nx + npml - j - 1];
+    }
+}
+
+__global__ void k_ref_read_bdr_ud(const float *ud, float *pb, int nx, int nz)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < 3 * nx) {
+        const int i = t / nx, j = t - i * nx;
+        pb[i * nx + j] = ud[i * nx + j];
+        pb[(nz - i - 1) * nx + j] = ud[(i + 3) * nx + j];
+    }
+}
+
+__global__ void k_ref_read_bdr_lr(const float *lr, float *pb, int nx, int nz)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < 3 * nz) {
+        const int i = t / 3, j = t - i * 3;
+        pb[i * nx + j] = lr[i * 6 + j];
+        pb[i * nx + nx - j - 1] = lr[i * 6 + j + 3];
+    }
+}
+
+// ============================================================================
+// host-side launchers (C linkage inside the library)
+// ============================================================================
+extern "C" {
+
+hipError_t fwi_launch_fwd(const FwiFwdArgs *a, int k, int nblocks, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fwd_step, dim3(nblocks), dim3(FWI_TX * a->b.sg), 0, st, *a, k);
+    return hipGetLastError();
+}
+
+hipError_t fwi_launch_adj(const FwiAdjArgs *a, int k, int nblocks, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_adj_step, dim3(nblocks), dim3(FWI_TX * a->b.sg), 0, st, *a, k);
+    return hipGetLastError();
+}
+
+hipError_t fwi_launch_rec(const FwiRecArgs *a, int k, int nblocks, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_rec_step, dim3(nblocks), dim3(FWI_TX * a->b.sg), 0, st, *a, k);
+    return hipGetLastError();
+}
+
+hipError_t fwi_launch_fill_f32(float *p, long long n, float v, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    long long nb = (n + 255) / 256;
+    if (nb > 65536) nb = 65536;
+    hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)nb), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t fwi_launch_fill_f64(double *p, long long n, double v, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    long long nb = (n + 255) / 256;
+    if (nb > 65536) nb = 65536;
+    hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)nb), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t fwi_launch_transpose(const float *in, float *out, int rows, int cols, hipStream_t st)
+{
+    dim3 grid((cols + 31) / 32, (rows + 31) / 32);
+    hipLaunchKernelGGL(k_transpose, grid, dim3(32, 8), 0, st, in, out, rows, cols);
+    return hipGetLastError();
+}
+
+hipError_t fwi_launch_misfit_l2(const float *syn, const float *obs, float *res, long long n, double *partial,
+                                int nblocks, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_misfit_l2, dim3(nblocks), dim3(256), 0, st, syn, obs, res, n, partial);
+    return hipGetLastError();
+}
+
+hipError_t fwi_launch_grad_finalize(const double *gacc, int ngroups, long long n, const float *vel, double *gk,
+                                    double *grad, hipStream_t st)
+{
+    long long nb = (n + 255) / 256;
+    if (nb > 65536) nb = 65536;
+    hipLaunchKernelGGL(k_grad_finalize, dim3((unsigned)nb), dim3(256), 0, st, gacc, ngroups, n, vel, gk, grad);
+    return hipGetLastError();
+}
+
+hipError_t fwi_launch_ref_phase(const FwiRefArgs *a, int phase, hipStream_t st)
+{
+    dim3 blk(64, 4);
+    dim3 grid((a->pnx + 2 + 63) / 64, (a->pnz + 2 + 3) / 4);
+    hipLaunchKernelGGL(k_ref_phase, grid, blk, 0, st, *a, phase);
+    return hipGetLastError();
+}
+
+hipError_t fwi_launch_ref_time_integral(float *pnew, const float *pnow, const float *pold, const float *pxx,
+                                        const float *pzz, const float *ekp, long long n, hipStream_t st)
+{
+    long long nb = (n + 255) / 256;
+    if (nb > 65536) nb = 65536;
+    hipLaunchKernelGGL(k_ref_time_integral, dim3((unsigned)nb), dim3(256), 0, st, pnew, pnow, pold, pxx, pzz, ekp, n);
+    return hipGetLastError();
+}
+
+hipError_t fwi_launch_ref_write_bdr(float *ud, float *lr, const float *pf, int nx, int nz, int npml, hipStream_t st)
+{
+    const int n = 3 * (nx > nz ? nx : nz);
+    hipLaunchKernelGGL(k_ref_write_bdr, dim3((n + 255) / 256), dim3(256), 0, st, ud, lr, pf, nx, nz, npml);
+    return hipGetLastError();
+}
+
+hipError_t fwi_launch_ref_read_bdr(const float *ud, const float *lr, float *pb, int nx, int nz, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_ref_read_bdr_ud, dim3((3 * nx + 255) / 256), dim3(256), 0, st, ud, pb, nx, nz);
+    hipLaunchKernelGGL(k_ref_read_bdr_lr, dim3((3 * nz + 255) / 256), dim3(256), 0, st, lr, pb, nx, nz);
+    return hipGetLastError();
+}
+
+}  // extern "C"

@@ -1,0 +1,248 @@
+"""ctypes binding of libfwi_hip.so (include/fwi_hip.h).
+
+The library is the compute path; this module only marshals arguments.  It
+fails loudly when the library has not been built: there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG, "libfwi_hip.so")
+
+OK, EINVAL, ENOMEM, EHIP, ESTATE = 0, 1, 2, 3, 4
+
+# CPML meta of the reference (fwi_sys.h: MAX_F, MAX_V, COEF_R)
+MAX_F, MAX_V, COEF_R = 15.0, 6000.0, 0.001
+
+EXPORTS = (
+    "fwi_hip_create", "fwi_hip_destroy", "fwi_hip_last_error", "fwi_hip_bytes", "fwi_hip_set_model",
+    "fwi_hip_set_shots", "fwi_hip_forward", "fwi_hip_set_observed", "fwi_hip_misfit_l2",
+    "fwi_hip_set_residual", "fwi_hip_adjoint", "fwi_hip_synchronize", "fwi_hip_set_timing",
+    "fwi_hip_kernel_time", "fwi_hip_read_state", "fwi_hip_op_pml", "fwi_hip_op_interior",
+    "fwi_hip_time_integral", "fwi_hip_write_bdr", "fwi_hip_read_bdr", "fwi_hip_cpml_coe",
+)
+
+
+class FwiError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"fwi_hip error {code}: {msg}")
+        self.code = code
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("nz", ctypes.c_int), ("npml", ctypes.c_int),
+                ("dx", ctypes.c_float), ("dz", ctypes.c_float), ("dt", ctypes.c_float),
+                ("max_f", ctypes.c_float), ("max_v", ctypes.c_float), ("coef_r", ctypes.c_float)]
+
+
+class Desc(ctypes.Structure):
+    _fields_ = [("grid", Grid), ("max_shots", ctypes.c_int), ("max_nt", ctypes.c_int),
+                ("max_nr", ctypes.c_int), ("tile_z", ctypes.c_int), ("shots_per_group", ctypes.c_int),
+                ("hbm_budget", ctypes.c_size_t), ("stream", ctypes.c_void_p)]
+
+
+class ShotC(ctypes.Structure):
+    _fields_ = [("nt", ctypes.c_int), ("isx", ctypes.c_int), ("isz", ctypes.c_int), ("nr", ctypes.c_int),
+                ("irx", ctypes.c_void_p), ("irz", ctypes.c_void_p), ("wav", ctypes.c_void_p)]
+
+
+_lib = None
+
+
+def load():
+    """Load the library (raises if it is missing: the HIP path is the product)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(f"{LIB_PATH} is not built: run __graft_entry__.build() "
+                                    "or python fwi2dacoustic-mpi-lbfgs_amd/build.py")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name in EXPORTS:
+            fn = getattr(lib, name)
+            fn.restype = ctypes.c_int
+        lib.fwi_hip_last_error.restype = ctypes.c_char_p
+        lib.fwi_hip_bytes.restype = ctypes.c_size_t
+        lib.fwi_hip_destroy.restype = None
+        _lib = lib
+    return _lib
+
+
+def make_grid(g, max_f=MAX_F, max_v=MAX_V, coef_r=COEF_R):
+    return Grid(g.nx, g.nz, g.npml, g.dx, g.dz, g.dt, max_f, max_v, coef_r)
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else ctypes.c_void_p(0)
+
+
+class Context:
+    """One fwi_hip context: a batch of shots on one model."""
+
+    def __init__(self, geom, max_shots, max_nt, max_nr, tile_z=0, shots_per_group=0, hbm_budget=0):
+        self.lib = load()
+        self.geom = geom
+        d = Desc(make_grid(geom), max_shots, max_nt, max_nr, tile_z, shots_per_group, hbm_budget, None)
+        h = ctypes.c_void_p()
+        rc = self.lib.fwi_hip_create(ctypes.byref(d), ctypes.byref(h))
+        if rc:
+            raise FwiError(rc, self.lib.fwi_hip_last_error(None).decode())
+        self.h = h
+        self._keep = []
+        self.shots = []
+
+    def _chk(self, rc):
+        if rc:
+            raise FwiError(rc, self.lib.fwi_hip_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.fwi_hip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def bytes(self):
+        return self.lib.fwi_hip_bytes(self.h)
+
+    def set_model(self, vel):
+        vel = np.ascontiguousarray(vel, np.float32)
+        assert vel.shape == (self.geom.nz, self.geom.nx)
+        self._chk(self.lib.fwi_hip_set_model(self.h, _p(vel)))
+
+    def set_shots(self, shots):
+        arr = (ShotC * len(shots))()
+        keep = []
+        for i, s in enumerate(shots):
+            irx = np.ascontiguousarray(s.irx, np.int32)
+            irz = np.ascontiguousarray(s.irz, np.int32)
+            wav = np.ascontiguousarray(s.wav, np.float32)
+            keep += [irx, irz, wav]
+            arr[i] = ShotC(s.nt, s.isx, s.isz, len(irx), irx.ctypes.data, irz.ctypes.data, wav.ctypes.data)
+        self._chk(self.lib.fwi_hip_set_shots(self.h, len(shots), arr))
+        self.shots = list(shots)
+
+    def ntrace(self):
+        return sum(s.nr * s.nt for s in self.shots)
+
+    def split(self, packed):
+        out, o = [], 0
+        for s in self.shots:
+            out.append(packed[o:o + s.nr * s.nt].reshape(s.nr, s.nt))
+            o += s.nr * s.nt
+        return out
+
+    def pack(self, traces):
+        return np.ascontiguousarray(np.concatenate([np.asarray(t, np.float32).ravel() for t in traces]))
+
+    def forward(self, want_traces=True):
+        out = np.empty(self.ntrace(), np.float32) if want_traces else None
+        self._chk(self.lib.fwi_hip_forward(self.h, _p(out)))
+        return out
+
+    def set_observed(self, obs_packed):
+        obs = np.ascontiguousarray(obs_packed, np.float32)
+        assert obs.size == self.ntrace()
+        self._chk(self.lib.fwi_hip_set_observed(self.h, _p(obs)))
+
+    def misfit_l2(self):
+        J = ctypes.c_double()
+        self._chk(self.lib.fwi_hip_misfit_l2(self.h, ctypes.byref(J)))
+        return J.value
+
+    def set_residual(self, res_packed):
+        res = np.ascontiguousarray(res_packed, np.float32)
+        assert res.size == self.ntrace()
+        self._chk(self.lib.fwi_hip_set_residual(self.h, _p(res)))
+
+    def adjoint(self, want=True):
+        g = self.geom
+        grad = np.empty((g.nz, g.nx), np.float64) if want else None
+        gk = np.empty((g.nz, g.nx), np.float64) if want else None
+        self._chk(self.lib.fwi_hip_adjoint(self.h, _p(grad), _p(gk)))
+        return grad, gk
+
+    def synchronize(self):
+        self._chk(self.lib.fwi_hip_synchronize(self.h))
+
+    def set_timing(self, on):
+        self._chk(self.lib.fwi_hip_set_timing(self.h, int(on)))
+
+    def kernel_time(self, which):
+        ms = ctypes.c_double()
+        n = ctypes.c_longlong()
+        self._chk(self.lib.fwi_hip_kernel_time(self.h, which, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def read_state(self, what, shot):
+        g = self.geom
+        if what <= 3:
+            out = np.empty((g.pnz, g.pnx), np.float32)
+        else:
+            out = np.empty((self.shots[shot].nt, 6 * g.nx + 6 * g.nz), np.float32)
+        self._chk(self.lib.fwi_hip_read_state(self.h, what, shot, _p(out)))
+        return out
+
+
+def cpml_coe(npml, dx, dt, max_f=MAX_F, max_v=MAX_V, coef_r=COEF_R):
+    lib = load()
+    a = np.zeros(2 * npml + 1, np.float32)
+    b = np.zeros(2 * npml + 1, np.float32)
+    rc = lib.fwi_hip_cpml_coe(_p(a), _p(b), npml, ctypes.c_float(dx), ctypes.c_float(dt), ctypes.c_float(max_f),
+                              ctypes.c_float(max_v), ctypes.c_float(coef_r))
+    if rc:
+        raise FwiError(rc, lib.fwi_hip_last_error(None).decode())
+    return a, b
+
+
+def _dp(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def op_pml(geom, p, u, v, ux, vz, u0, u1, v1, ux0, ux1, vz1):
+    """a_lap_opt_2d_pml on torch device tensors (in place)."""
+    lib = load()
+    gr = make_grid(geom)
+    rc = lib.fwi_hip_op_pml(ctypes.byref(gr), *[_dp(t) for t in (p, u, v, ux, vz, u0, u1, v1, ux0, ux1, vz1)],
+                            ctypes.c_void_p(0))
+    if rc:
+        raise FwiError(rc, lib.fwi_hip_last_error(None).decode())
+
+
+def op_interior(geom, p, u, v, ux, vz):
+    lib = load()
+    gr = make_grid(geom)
+    rc = lib.fwi_hip_op_interior(ctypes.byref(gr), *[_dp(t) for t in (p, u, v, ux, vz)], ctypes.c_void_p(0))
+    if rc:
+        raise FwiError(rc, lib.fwi_hip_last_error(None).decode())
+
+
+def time_integral(pnew, pnow, pold, pxx, pzz, ekp):
+    lib = load()
+    m, n = pnow.shape
+    rc = lib.fwi_hip_time_integral(*[_dp(t) for t in (pnew, pnow, pold, pxx, pzz, ekp)], ctypes.c_size_t(m),
+                                   ctypes.c_size_t(n), ctypes.c_void_p(0))
+    if rc:
+        raise FwiError(rc, lib.fwi_hip_last_error(None).decode())
+
+
+def write_bdr(geom, ud, lr, pf):
+    lib = load()
+    gr = make_grid(geom)
+    rc = lib.fwi_hip_write_bdr(ctypes.byref(gr), _dp(ud), _dp(lr), _dp(pf), ctypes.c_void_p(0))
+    if rc:
+        raise FwiError(rc, lib.fwi_hip_last_error(None).decode())
+
+
+def read_bdr(geom, ud, lr, pb):
+    lib = load()
+    gr = make_grid(geom)
+    rc = lib.fwi_hip_read_bdr(ctypes.byref(gr), _dp(ud), _dp(lr), _dp(pb), ctypes.c_void_p(0))
+    if rc:
+        raise FwiError(rc, lib.fwi_hip_last_error(None).decode())

@@ -1,0 +1,142 @@
+/*
+ * fwi_hip.h -- C ABI of the MI355X (gfx950) wave-equation core.
+ *
+ * Plain C: pointers, sizes and ints only.  Every call returns 0 on success
+ * and a FWI_HIP_E* code otherwise (fwi_hip_last_error() has the message);
+ * nothing calls exit().  One context per thread; a context owns its device
+ * buffers and runs on one HIP stream.
+ *
+ * Layouts are the reference's: 2-D fields row-major with x fastest
+ * (alloc.c:3-12), physical grid nz x nx, propagation grid (nz+npml) x
+ * (nx+2 npml) under the free surface (fwi_2d.c:148-165).
+ *
+ * Replaces, in the reference:
+ *   the body of grad4_1shot (declared header.h:283, called MBotOpt.c:55,99,
+ *   never defined): fwi_hip_forward + misfit + fwi_hip_adjoint below, and
+ *   grad4_1shot itself in fwi_plan.h;
+ *   the per-step kernels a_lap_opt_2d_pml (fwi_2d.c:761), a_lap_opt_2d
+ *   (fwi_2d.c:722), time_integral (fwi_2d.c:866), write_bdr / read_bdr
+ *   (fwi_2d.c:882,919) and cpml_coe (fwi_2d.c:110): the fwi_hip_op_* /
+ *   fwi_hip_*_bdr / fwi_hip_cpml_coe entry points, on device buffers.
+ */
+#ifndef FWI_HIP_H
+#define FWI_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FWI_HIP_OK 0
+#define FWI_HIP_EINVAL 1  /* bad argument or shape */
+#define FWI_HIP_ENOMEM 2  /* allocation failed or the HBM budget is exceeded */
+#define FWI_HIP_EHIP 3    /* a HIP runtime call failed */
+#define FWI_HIP_ESTATE 4  /* call out of order (e.g. adjoint before forward) */
+
+typedef struct fwi_hip_ctx fwi_hip_ctx;
+
+/* Grid of one model.  Replaces fwi_base (header.h:21-27) and the CPML meta
+ * pml_base (header.h:84-89, set at fwi_2d.c:257). Free surface on top.     */
+typedef struct {
+    int nx, nz;     /* physical grid (fwi_base.nx, .nz)                     */
+    int npml;       /* CPML cells left, right and below (fwi_base.npml)    */
+    float dx, dz, dt;
+    float max_f, max_v, coef_r; /* MAX_F, MAX_V, COEF_R (fwi_sys.h)         */
+} fwi_hip_grid;
+
+typedef struct {
+    fwi_hip_grid grid;
+    int max_shots;       /* batch capacity (one MPI rank per shot in the reference) */
+    int max_nt;          /* longest shot (fwi_base.nt)                              */
+    int max_nr;          /* most receivers in a shot (fwi_msf.nr)                   */
+    int tile_z;          /* rows per wave tile, >= 8 (0 = automatic)                */
+    int shots_per_group; /* shots per workgroup 1..4 (0 = automatic)                */
+    size_t hbm_budget;   /* bytes the context may allocate (0 = no limit)           */
+    void *stream;        /* hipStream_t to run on (NULL = the context makes one)    */
+} fwi_hip_desc;
+
+/* One shot.  Replaces fwi_sou (header.h:109-117): cells as computed at
+ * fwi_2d.c:363-371 (physical grid), wavelet already scaled (fwi_2d.c:353). */
+typedef struct {
+    int nt;
+    int isx, isz;
+    int nr;
+    const int *irx, *irz;
+    const float *wav;
+} fwi_hip_shot;
+
+int fwi_hip_create(const fwi_hip_desc *desc, fwi_hip_ctx **out);
+void fwi_hip_destroy(fwi_hip_ctx *ctx);
+const char *fwi_hip_last_error(const fwi_hip_ctx *ctx); /* ctx may be NULL */
+size_t fwi_hip_bytes(const fwi_hip_ctx *ctx);           /* device bytes held */
+
+/* Velocity model vel[nz][nx] (m/s, host).  ekp = vel^2 dt^2 extended into
+ * the CPML by edge replication (model_ext, fwi_2d.c:67-107).                */
+int fwi_hip_set_model(fwi_hip_ctx *ctx, const float *vel);
+
+/* The shots of the next forward/adjoint pair (receiver cells must be unique
+ * within a shot).  Traces of a batch are packed shot after shot, each
+ * [nr][nt] with time contiguous (fwi_msf.syn, header.h:66).                 */
+int fwi_hip_set_shots(fwi_hip_ctx *ctx, int nshots, const fwi_hip_shot *shots);
+
+/* Forward propagation of every shot in one launch per time step.  Receiver
+ * traces stay on the device; copied to `traces` (host) when non-NULL.       */
+int fwi_hip_forward(fwi_hip_ctx *ctx, float *traces);
+
+/* Observed traces (host, packed like the synthetics), kept on the device. */
+int fwi_hip_set_observed(fwi_hip_ctx *ctx, const float *obs);
+/* L2 misfit on the device: residual = syn - obs, *misfit = 1/2 sum res^2
+ * accumulated in FP64 (msf_pool.h:1 declares L2norm without a body).      */
+int fwi_hip_misfit_l2(fwi_hip_ctx *ctx, double *misfit);
+/* Adjoint source computed elsewhere (host misfit pool), packed like obs. */
+int fwi_hip_set_residual(fwi_hip_ctx *ctx, const float *res);
+
+/* Back-propagation of the residual (exact transpose of the forward),
+ * source-field reconstruction from the saved frames, zero-lag imaging
+ * summed over all shots in FP64.  grad[nz][nx] = dJ/dvel, gk[nz][nx] =
+ * ekp * dJ/dekp; either may be NULL (then it stays on the device).         */
+int fwi_hip_adjoint(fwi_hip_ctx *ctx, double *grad, double *gk);
+
+/* Wait for all work of the context. */
+int fwi_hip_synchronize(fwi_hip_ctx *ctx);
+
+/* Per-kernel timing with HIP events on the context stream (0 = off).
+ * fwi_hip_kernel_time returns the summed milliseconds and launch count of
+ * kernel `which` (0 forward step, 1 adjoint step, 2 reconstruction step)
+ * since timing was switched on.                                            */
+int fwi_hip_set_timing(fwi_hip_ctx *ctx, int on);
+int fwi_hip_kernel_time(fwi_hip_ctx *ctx, int which, double *ms, long long *launches);
+
+/* Device state for inspection (host copy, dense rows):
+ *   what 0/1: p buffer 0/1 [pnz][pnx]   2/3: adjoint buffer 0/1 [pnz][pnx]
+ *   4/5: boundary / operator frames of the shot [nt][6 nx + 6 nz]          */
+int fwi_hip_read_state(fwi_hip_ctx *ctx, int what, int shot, float *out);
+
+/* ---- per-step entry points on caller-owned device buffers (dense rows) --
+ * Shapes follow fwi_init (fwi_2d.c:230-296) for the f-grid p[pnz][pnx],
+ * u[pnz][pnx+1], v[pnz+1][pnx], ux/vz[pnz][pnx], CPML memory
+ * u0,u1[pnz][npml+1], v1[npml+1][pnx], ux0,ux1[pnz][npml], vz1[npml][pnx].
+ * stream: hipStream_t or NULL (default stream).  They return after
+ * enqueueing (except fwi_hip_op_pml's table upload, which is synchronous). */
+
+/* a_lap_opt_2d_pml, fwi_2d.c:761-864 (free surface) */
+int fwi_hip_op_pml(const fwi_hip_grid *g, const float *p, float *u, float *v, float *ux, float *vz,
+                   float *u0, float *u1, float *v1, float *ux0, float *ux1, float *vz1, void *stream);
+/* a_lap_opt_2d, fwi_2d.c:722-759, on the physical grid p[nz][nx] */
+int fwi_hip_op_interior(const fwi_hip_grid *g, const float *p, float *u, float *v, float *ux, float *vz,
+                        void *stream);
+/* time_integral, fwi_2d.c:866-880 */
+int fwi_hip_time_integral(float *pnew, const float *pnow, const float *pold, const float *pxx,
+                          const float *pzz, const float *ekp, size_t m, size_t n, void *stream);
+/* one step of write_bdr (fwi_2d.c:891-910): ud[6][nx], lr[nz][6] from the f-grid field */
+int fwi_hip_write_bdr(const fwi_hip_grid *g, float *ud, float *lr, const float *pf, void *stream);
+/* one step of read_bdr (fwi_2d.c:940-957) into a physical-grid field pb[nz][nx] */
+int fwi_hip_read_bdr(const fwi_hip_grid *g, const float *ud, const float *lr, float *pb, void *stream);
+/* cpml_coe, fwi_2d.c:110-137 (host): a[2 npml + 1], b[2 npml + 1] */
+int fwi_hip_cpml_coe(float *a, float *b, int npml, float dx, float dt, float max_f, float max_v, float coef_r);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,154 @@
+/*
+ * fwi_plan.h -- the reference's plan structs and its per-shot gradient entry
+ * point, for C host code shaped like MBotOpt.c.
+ *
+ * The struct layouts are those of the reference's header.h:21-185 (member
+ * order, types and constness kept, because MBotOpt.c-style callers pass
+ * these structs by pointer): fwi_base, fwi_shape, fwi_specs, fwi_model,
+ * fwi_msf, fwi_state, pml_base, pml_profile, fwi_fdcoe, fwi_sou, fwi_grad,
+ * fwi_files_path, fwi_plan.  Only the GSL/FFTW/OpenMP includes of
+ * header.h:7-14 are gone: nothing here needs them.
+ *
+ * grad4_1shot is declared at header.h:283 and called at MBotOpt.c:55,99 but
+ * has no body in the reference.  This library provides it on top of
+ * fwi_hip.h.  What it reads and writes:
+ *   in:  grid->base (nx, nz, npml, nt, dx, dz, dt; free_surface must be 1),
+ *        model->md[nz][nx] = velocity (m/s), sou->ixyz_sr (source then
+ *        receiver cells, fwi_2d.c:363-371), sou->wav0[nt] (the scaled
+ *        wavelet fwi_init reads, fwi_2d.c:352-353), msf->nr, msf->nt,
+ *        msf->jump, msf->obs0[nr][nt0] (the observed gather fwi_init reads,
+ *        fwi_2d.c:342; the host conditioning steps the reference applies to
+ *        it are not run here);
+ *   out: msf->syn[nr][nt0] (synthetics every jump steps), msf->val[0] =
+ *        msf->val[1] = 1/2 sum (syn - obs)^2 (L2 misfit, FP64 sum),
+ *        grad->body[nz*nx] = dJ/dmd (exact gradient of the discrete misfit,
+ *        the CPML extension of the model held fixed).
+ * The iteration number k is accepted and unused, as in MBotOpt.c.
+ */
+#ifndef FWI_PLAN_H
+#define FWI_PLAN_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    const size_t dim, mm, npml, nx, nz, nt, nt_mem, free_surface;
+    const float dx, dz, dt;
+} fwi_base;
+
+typedef struct {
+    size_t mnx, mnz;
+    size_t pnx, pnz;
+    size_t unx, unz;
+    size_t vnx, vnz;
+} fwi_shape;
+
+typedef struct {
+    fwi_base *base;
+    fwi_shape *f, *b;
+} fwi_specs;
+
+typedef struct {
+    float **md, **kp, **ekp, *btry, alpha_btry;
+    int *ibtry;
+} fwi_model;
+
+typedef struct {
+    const size_t nt, nr, jump, rectx, rectt, times;
+    const int alpha_fbt0, alpha_fbt1;
+    const float dr, dt, os0, os1, alpha_msf;
+    float **obs0, **obs, **obs_t, **syn, **syn0, **syn_t, *fbt, *val;
+    int ios0, ios1, *ifbt;
+} fwi_msf;
+
+typedef struct {
+    float **pold, **pnow, **pnew, **u, **v, **ux, **vz;
+} fwi_state;
+
+typedef struct {
+    const size_t dim;
+    const float maxF, maxV, coefR;
+} pml_base;
+
+typedef struct {
+    int npml;
+    float *a0[2], *b0[2], *a1[2], *b1[2];
+    float **u0, **u1, **v0, **v1, **ux0, **ux1, **vz0, **vz1;
+} pml_profile;
+
+typedef struct {
+    float c0x, c1x, ix;
+    float c0z, c1z, iz;
+    float iz840, iz24;
+} fwi_fdcoe;
+
+typedef struct {
+    float iarea, *xyz_sr;
+    int *ixyz_sr;
+    float *wav0, *wav, **res;
+} fwi_sou;
+
+typedef struct {
+    size_t id;
+    const float end_t, start_t, alpha_grad, perc;
+    const size_t jump;
+    const size_t rectx1, rectz1, repeat1;
+    const size_t rectx2, rectz2, repeat2;
+    float *body;
+} fwi_grad;
+
+typedef struct {
+    char *common;
+    char *shots;
+    char *fbt;
+    char *xyz_sr;
+    char *init_mdl;
+    char *btry;
+    char *wavelet;
+} fwi_files_path;
+
+typedef struct {
+    fwi_specs *grid;
+    fwi_model *model;
+    fwi_state *fw;
+    fwi_state *bw;
+    fwi_sou *sou;
+    fwi_msf *msf;
+    pml_profile *pml;
+    fwi_fdcoe *coe;
+    fwi_files_path *path;
+    fwi_grad *grad;
+    struct {
+        char path_bd_wr[300];
+        float ***bc_ud, ***bc_lr;
+        int offset;
+    };
+} fwi_plan;
+
+/* header.h:283 -- one shot: forward, L2 misfit, adjoint, gradient. */
+void grad4_1shot(fwi_plan *plan, int k);
+
+/* All shots of an iteration in one batched run (replaces the one-rank-per-
+ * shot loop of MBotOpt.c:51).  The plans share the model of plans[0];
+ * each plan gets its own syn and val; the gradient summed over the shots
+ * goes to plans[0]->grad->body.  Returns 0 or a FWI_HIP_E* code. */
+int grad_all_shots(fwi_plan **plans, int nshots, int k);
+
+/* Message of the last failure of grad4_1shot / grad_all_shots. */
+const char *grad4_last_error(void);
+
+/* A plan holding exactly what grad4_1shot reads and writes, with the
+ * reference's row-pointer arrays (alloc.c:3-48) and shapes (fwi_2d.c:148-
+ * 165, 179-222, 380): for callers without fwi_init's input files.
+ * nt0 = (nt - 1) / jump + 1 samples per trace (fwi_2d.c:195).  Free with
+ * fwi_plan_free. */
+fwi_plan *fwi_plan_alloc(int nx, int nz, int npml, int nt, int jump, int nr, float dx, float dz, float dt);
+void fwi_plan_free(fwi_plan *plan);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,247 @@
+"""HIP path against the oracle, through the C ABI (needs an MI355X).
+
+Tolerances: seismograms rel-L2 <= 1e-5 and gradients
+rel-L2 <= 1e-4 against the FP32 oracle; both also reported against FP64.
+The per-step reference-structured kernels are compared element by element.
+"""
+import numpy as np
+import pytest
+
+from oracle import Oracle
+from fwihip.workloads import Geometry, Shot, ricker, config1, layered_faulted, surface_shots
+
+pytestmark = pytest.mark.gpu
+
+SYN_TOL = 1e-5
+GRAD_TOL = 1e-4
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from fwihip import lib as L
+    L.load()
+    return L
+
+
+@pytest.fixture(scope="module")
+def o32():
+    return Oracle(np.float32)
+
+
+@pytest.fixture(scope="module")
+def o64():
+    return Oracle(np.float64)
+
+
+def small_geom():
+    return Geometry(nx=150, nz=72, npml=10, dx=10.0, dz=10.0, dt=1e-3)
+
+
+def small_case(nshots=3, nt=300, seed=0):
+    g = small_geom()
+    rng = np.random.default_rng(seed)
+    vel = (2200 + 300 * rng.standard_normal((g.nz, g.nx))).astype(np.float32)
+    vel = np.clip(vel, 1500, 3200).astype(np.float32)
+    shots = []
+    for j in range(nshots):
+        irx = np.arange(j % 2, g.nx, 2, dtype=np.int32)
+        ntj = nt - 37 * j  # uneven nt inside one batch
+        shots.append(Shot(nt=ntj, isx=20 + 45 * j, isz=2 + j, wav=ricker(ntj, g.dt, 20.0), irx=irx,
+                          irz=np.full(len(irx), 2 + (j % 3), np.int32)))
+    return g, vel, shots
+
+
+# ---------------------------------------------------------------- per step
+def test_cpml_coe_matches_oracle(lib, o32):
+    for npml, dx, dt in ((20, 10.0, 1e-3), (12, 12.5, 1e-3), (7, 5.0, 4e-4)):
+        a, b = lib.cpml_coe(npml, dx, dt)
+        ao, bo = o32.cpml_coe(npml, dx, dt)
+        np.testing.assert_array_equal(a, ao)
+        np.testing.assert_array_equal(b, bo)
+
+
+def test_op_pml_steps_match_oracle(lib, o32, torch_cuda):
+    torch = torch_cuda
+    g = Geometry(nx=70, nz=40, npml=9, dx=10.0, dz=8.0, dt=1e-3)
+    pnx, pnz, npml = g.pnx, g.pnz, g.npml
+    coe = o32.fdcoe(g.dx, g.dz)
+    tab = o32.cpml_tables(npml, g.dx, g.dz, g.dt)
+    st = o32.new_pml_state(pnx, pnz, npml)
+    dev = {k: torch.zeros(v.shape, dtype=torch.float32, device="cuda") for k, v in st.items()}
+    rng = np.random.default_rng(11)
+    for step in range(6):
+        p = rng.standard_normal((pnz, pnx)).astype(np.float32)
+        o32.op_pml(p, st, coe, tab, npml)
+        pd = torch.from_numpy(p).cuda()
+        lib.op_pml(g, pd, dev["u"], dev["v"], dev["ux"], dev["vz"], dev["u0"], dev["u1"], dev["v1"],
+                   dev["ux0"], dev["ux1"], dev["vz1"])
+        torch.cuda.synchronize()
+        for k in st:
+            got = dev[k].cpu().numpy()
+            np.testing.assert_allclose(got, st[k], rtol=2e-6, atol=1e-6 * max(1.0, np.abs(st[k]).max()),
+                                       err_msg=f"step {step} array {k}")
+
+
+def test_op_interior_time_integral_bdr(lib, o32, torch_cuda):
+    torch = torch_cuda
+    g = Geometry(nx=60, nz=44, npml=6, dx=10.0, dz=10.0, dt=1e-3)
+    rng = np.random.default_rng(12)
+    coe = o32.fdcoe(g.dx, g.dz)
+    p = rng.standard_normal((g.nz, g.nx)).astype(np.float32)
+    u, v, ux, vz = o32.op_interior(p, coe)
+    du = torch.zeros(u.shape, device="cuda")
+    dv = torch.zeros(v.shape, device="cuda")
+    dux = torch.zeros(ux.shape, device="cuda")
+    dvz = torch.zeros(vz.shape, device="cuda")
+    lib.op_interior(g, torch.from_numpy(p).cuda(), du, dv, dux, dvz)
+    torch.cuda.synchronize()
+    for a, b in ((du, u), (dv, v), (dux, ux), (dvz, vz)):
+        np.testing.assert_allclose(a.cpu().numpy(), b, rtol=2e-6, atol=1e-6)
+    pold = rng.standard_normal(p.shape).astype(np.float32)
+    ekp = rng.uniform(0.01, 0.1, p.shape).astype(np.float32)
+    ref = o32.time_integral(p, pold, ux, vz, ekp)
+    out = torch.zeros(p.shape, device="cuda")
+    lib.time_integral(out, torch.from_numpy(p).cuda(), torch.from_numpy(pold).cuda(), torch.from_numpy(ux).cuda(),
+                      torch.from_numpy(vz).cuda(), torch.from_numpy(ekp).cuda())
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(out.cpu().numpy(), ref)
+    pf = rng.standard_normal((g.pnz, g.pnx)).astype(np.float32)
+    ud, lr = o32.write_bdr(pf, g.nx, g.nz, g.npml)
+    dud = torch.zeros((6, g.nx), device="cuda")
+    dlr = torch.zeros((g.nz, 6), device="cuda")
+    lib.write_bdr(g, dud, dlr, torch.from_numpy(pf).cuda())
+    pb = torch.zeros((g.nz, g.nx), device="cuda")
+    lib.read_bdr(g, dud, dlr, pb)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(dud.cpu().numpy(), ud)
+    np.testing.assert_array_equal(dlr.cpu().numpy(), lr)
+    np.testing.assert_array_equal(pb.cpu().numpy(), o32.read_bdr(ud, lr, np.zeros((g.nz, g.nx), np.float32)))
+
+
+# ---------------------------------------------------------------- forward
+def test_forward_batched_uneven(lib, o32, o64):
+    g, vel, shots = small_case()
+    ctx = lib.Context(g, max_shots=4, max_nt=300, max_nr=100, tile_z=8)
+    ctx.set_model(vel)
+    ctx.set_shots(shots)
+    syn = ctx.split(ctx.forward())
+    for j, s in enumerate(shots):
+        f32 = o32.forward(g, vel, s, want_strips=True)
+        f64 = o64.forward(g, vel, s)
+        print(f"shot {j}: syn rel32 {rel(syn[j], f32['syn']):.2e} rel64 {rel(syn[j], f64['syn']):.2e}")
+        assert rel(syn[j], f32["syn"]) < SYN_TOL
+        assert rel(syn[j], f64["syn"]) < SYN_TOL
+        ps = ctx.read_state(4, j)
+        xs = ctx.read_state(5, j)
+        assert rel(ps, f32["pstrip"]) < SYN_TOL
+        assert rel(xs, f32["xstrip"]) < SYN_TOL
+        # final wavefields on the physical grid
+        pl = f32["plast"]
+        pn = ctx.read_state(s.nt & 1, j)[:g.nz, g.npml:g.npml + g.nx]
+        assert rel(pn, pl[0]) < SYN_TOL
+
+
+def test_forward_tiles_agree(lib):
+    """Different tile heights and shot groups give identical traces."""
+    g, vel, shots = small_case(nshots=3, nt=200, seed=3)
+    outs = []
+    for tz, sg in ((8, 1), (13, 2), (40, 4), (0, 0)):
+        ctx = lib.Context(g, max_shots=3, max_nt=200, max_nr=100, tile_z=tz, shots_per_group=sg)
+        ctx.set_model(vel)
+        ctx.set_shots(shots)
+        outs.append(ctx.forward())
+        ctx.close()
+    for o in outs[1:]:
+        np.testing.assert_array_equal(o, outs[0])
+
+
+# ---------------------------------------------------------------- adjoint
+def test_adjoint_matches_oracle(lib, o32, o64):
+    g, vel, shots = small_case(nshots=2, nt=260, seed=5)
+    ctx = lib.Context(g, max_shots=2, max_nt=260, max_nr=100, tile_z=16)
+    ctx.set_model(vel)
+    ctx.set_shots(shots)
+    ctx.forward(want_traces=False)
+    rng = np.random.default_rng(9)
+    res = [rng.standard_normal((s.nr, s.nt)).astype(np.float32) for s in shots]
+    ctx.set_residual(ctx.pack(res))
+    grad, gk = ctx.adjoint()
+    g32 = np.zeros_like(grad)
+    g64 = np.zeros_like(grad)
+    for s, r in zip(shots, res):
+        f = o32.forward(g, vel, s, want_strips=True)
+        g32 += o32.backward(g, vel, s, r, f)["grad"]
+        f = o64.forward(g, vel, s, want_strips=True)
+        g64 += o64.backward(g, vel, s, r.astype(np.float64), f)["grad"]
+    print(f"adjoint grad rel32 {rel(grad, g32):.2e} rel64 {rel(grad, g64):.2e}")
+    assert rel(grad, g32) < GRAD_TOL
+    assert rel(grad, g64) < GRAD_TOL
+
+
+def test_gradient_config1(lib, o32, o64):
+    """Config 1: forward, L2 misfit on the device, adjoint, gradient."""
+    g, vel, start, shots = config1()
+    s = shots[0]
+    obs = o64.forward(g, vel, s)["syn"].astype(np.float32)
+    ctx = lib.Context(g, max_shots=1, max_nt=s.nt, max_nr=s.nr)
+    ctx.set_model(start)
+    ctx.set_shots([s])
+    syn = ctx.forward().reshape(s.nr, s.nt)
+    ctx.set_observed(obs.ravel())
+    J = ctx.misfit_l2()
+    grad, _ = ctx.adjoint()
+    J32, syn32, gr32, _ = o32.gradient(g, start, s, obs)
+    J64, syn64, gr64, _ = o64.gradient(g, start.astype(np.float64), s, obs.astype(np.float64))
+    print(f"config1 syn rel32 {rel(syn, syn32):.2e} rel64 {rel(syn, syn64):.2e} "
+          f"grad rel32 {rel(grad, gr32):.2e} rel64 {rel(grad, gr64):.2e} J {J} {J32} {J64}")
+    assert rel(syn, syn32) < SYN_TOL and rel(syn, syn64) < SYN_TOL
+    assert rel(grad, gr32) < GRAD_TOL and rel(grad, gr64) < GRAD_TOL
+    assert abs(J - J64) < 1e-4 * J64
+
+
+def test_batch_equals_sum_of_single_shots(lib):
+    g, vel, shots = small_case(nshots=3, nt=220, seed=7)
+    rng = np.random.default_rng(1)
+    res = [rng.standard_normal((s.nr, s.nt)).astype(np.float32) for s in shots]
+    ctx = lib.Context(g, max_shots=3, max_nt=220, max_nr=100)
+    ctx.set_model(vel)
+    ctx.set_shots(shots)
+    ctx.forward(want_traces=False)
+    ctx.set_residual(ctx.pack(res))
+    gb, _ = ctx.adjoint()
+    gs = np.zeros_like(gb)
+    for s, r in zip(shots, res):
+        ctx.set_shots([s])
+        ctx.forward(want_traces=False)
+        ctx.set_residual(r.ravel())
+        gs += ctx.adjoint()[0]
+    assert rel(gb, gs) < 1e-6
+
+
+def test_reconstruction_recovers_initial_state(lib):
+    """After the backward sweep the physical-grid field is p^0 = 0 (exact
+    time reversal of the forward up to FP32 rounding)."""
+    g, vel, shots = small_case(nshots=1, nt=300, seed=8)
+    ctx = lib.Context(g, max_shots=1, max_nt=300, max_nr=100)
+    ctx.set_model(vel)
+    ctx.set_shots(shots)
+    ctx.forward(want_traces=False)
+    peak = np.abs(ctx.read_state(300 & 1, 0)).max()
+    ctx.set_residual(np.zeros(ctx.ntrace(), np.float32))
+    ctx.adjoint()
+    p0 = ctx.read_state(0, 0)[:g.nz, g.npml:g.npml + g.nx]
+    assert np.abs(p0).max() < 1e-4 * peak
