@@ -18,9 +18,11 @@
 #include "../../include/fwi_hip.h"
 
 extern "C" {
-hipError_t fwi_launch_fwd(const FwiFwdArgs *a, int k, int nblocks, hipStream_t st);
-hipError_t fwi_launch_adj(const FwiAdjArgs *a, int k, int nblocks, hipStream_t st);
-hipError_t fwi_launch_rec(const FwiRecArgs *a, int k, int nblocks, hipStream_t st);
+hipError_t fwi_launch_fwd(const FwiFwdArgs *a, const int *kbase, int koff, int nblocks, hipStream_t st);
+hipError_t fwi_launch_adj(const FwiAdjArgs *a, const int *kbase, int koff, int nblocks, hipStream_t st);
+hipError_t fwi_launch_rec(const FwiRecArgs *a, const int *kbase, int koff, int nblocks, hipStream_t st);
+hipError_t fwi_launch_advance(int *kbase, int delta, hipStream_t st);
+hipError_t fwi_launch_set_int(int *p, int v, hipStream_t st);
 hipError_t fwi_launch_fill_f32(float *p, long long n, float v, hipStream_t st);
 hipError_t fwi_launch_fill_f64(double *p, long long n, double v, hipStream_t st);
 hipError_t fwi_launch_transpose(const float *in, float *out, int rows, int cols, hipStream_t st);
@@ -74,6 +76,10 @@ struct fwi_hip_ctx {
     int *rmap = nullptr;
     long long rmap_cap = 0;
     bool have_fwd = false, have_obs = false, have_res = false;
+    // captured blocks of time steps
+    int *kbase = nullptr;
+    int graph_steps = 100;
+    hipGraphExec_t gx_fwd = nullptr, gx_bwd = nullptr;
     // timing
     bool timing = false;
     double t_ms[3] = {0, 0, 0};
@@ -205,6 +211,8 @@ extern "C" void fwi_hip_destroy(fwi_hip_ctx *c)
 {
     if (!c) return;
     if (c->st) (void)hipStreamSynchronize(c->st);
+    if (c->gx_fwd) (void)hipGraphExecDestroy(c->gx_fwd);
+    if (c->gx_bwd) (void)hipGraphExecDestroy(c->gx_bwd);
     for (void *p : c->allocs) (void)hipFree(p);
     for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
     if (c->own_stream && c->st) (void)hipStreamDestroy(c->st);
@@ -299,6 +307,7 @@ extern "C" int fwi_hip_create(const fwi_hip_desc *desc, fwi_hip_ctx **out)
         {(void **)&c->partial, sizeof(double) * c->npartial},
         {(void **)&c->shots_d, sizeof(FwiShotDev) * S},
         {(void **)&c->rrow, sizeof(int) * S * g.pnz},
+        {(void **)&c->kbase, sizeof(int) * 64},
     };
     for (auto &a : plan) {
         rc = dalloc(c, a.p, a.bytes);
@@ -447,6 +456,9 @@ extern "C" int fwi_hip_set_shots(fwi_hip_ctx *c, int nshots, const fwi_hip_shot 
     HIPCHK(c, hipMemcpyAsync(c->rrow, rrow.data(), sizeof(int) * rrow.size(), hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipMemcpyAsync(c->shots_d, sd.data(), sizeof(FwiShotDev) * nshots, hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
+    if (c->gx_fwd) (void)hipGraphExecDestroy(c->gx_fwd);
+    if (c->gx_bwd) (void)hipGraphExecDestroy(c->gx_bwd);
+    c->gx_fwd = c->gx_bwd = nullptr;
     c->shots_h = sd;
     c->nshots = nshots;
     c->kmax = kmax;
@@ -518,6 +530,38 @@ static int collect_timing(fwi_hip_ctx *c, std::vector<std::pair<int, std::pair<h
     return FWI_HIP_OK;
 }
 
+// Captures `build` (the launches of graph_steps time steps plus the counter
+// advance) into an executable graph.
+template <class F>
+static int capture(fwi_hip_ctx *c, hipGraphExec_t *out, F build)
+{
+    hipGraph_t graph = nullptr;
+    HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+    hipError_t e = build();
+    hipError_t e2 = hipStreamEndCapture(c->st, &graph);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return set_err(c, FWI_HIP_EHIP, "graph capture: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    e = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        *out = nullptr;
+        return set_err(c, FWI_HIP_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    return FWI_HIP_OK;
+}
+
+extern "C" int fwi_hip_set_graph(fwi_hip_ctx *c, int steps)
+{
+    if (!c || steps < 0) return set_err(c, FWI_HIP_EINVAL, "set_graph: bad argument");
+    if (c->gx_fwd) (void)hipGraphExecDestroy(c->gx_fwd);
+    if (c->gx_bwd) (void)hipGraphExecDestroy(c->gx_bwd);
+    c->gx_fwd = c->gx_bwd = nullptr;
+    c->graph_steps = steps;
+    return FWI_HIP_OK;
+}
+
 extern "C" int fwi_hip_forward(fwi_hip_ctx *c, float *traces)
 {
     if (!c) return set_err(c, FWI_HIP_EINVAL, "forward: NULL context");
@@ -543,13 +587,28 @@ extern "C" int fwi_hip_forward(fwi_hip_ctx *c, float *traces)
     a.xstrip = c->xstrip;
     const int nblocks = g.ntx * g.ntz * c->ngroups;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
-    for (int k = 0; k < c->kmax; k++) {
-        hipEvent_t e0, e1;
-        if ((rc = tick(c, &e0))) return rc;
-        HIPCHK(c, fwi_launch_fwd(&a, k, nblocks, c->st));
-        if ((rc = tick(c, &e1))) return rc;
-        if (c->timing) marks.push_back({0, {e0, e1}});
-        if (marks.size() >= 512 && (rc = collect_timing(c, marks))) return rc;
+    const int B = c->graph_steps;
+    if (B > 0 && !c->timing) {
+        if (!c->gx_fwd) {
+            rc = capture(c, &c->gx_fwd, [&]() {
+                hipError_t e = hipSuccess;
+                for (int j = 0; j < B && e == hipSuccess; j++) e = fwi_launch_fwd(&a, c->kbase, j, nblocks, c->st);
+                if (e == hipSuccess) e = fwi_launch_advance(c->kbase, B, c->st);
+                return e;
+            });
+            if (rc) return rc;
+        }
+        HIPCHK(c, fwi_launch_set_int(c->kbase, 0, c->st));
+        for (int blk = 0; blk < (c->kmax + B - 1) / B; blk++) HIPCHK(c, hipGraphLaunch(c->gx_fwd, c->st));
+    } else {
+        for (int k = 0; k < c->kmax; k++) {
+            hipEvent_t e0, e1;
+            if ((rc = tick(c, &e0))) return rc;
+            HIPCHK(c, fwi_launch_fwd(&a, nullptr, k, nblocks, c->st));
+            if ((rc = tick(c, &e1))) return rc;
+            if (c->timing) marks.push_back({0, {e0, e1}});
+            if (marks.size() >= 512 && (rc = collect_timing(c, marks))) return rc;
+        }
     }
     if ((rc = collect_timing(c, marks))) return rc;
     for (int s = 0; s < c->nshots; s++) {
@@ -643,18 +702,36 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
     const int nb_adj = g.ntx * g.ntz * c->ngroups;
     const int nb_rec = g.bntx * g.bntz * c->ngroups;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
-    for (int k = c->kmax - 1; k >= 0; k--) {
-        hipEvent_t e0, e1, e2;
-        if ((rc = tick(c, &e0))) return rc;
-        HIPCHK(c, fwi_launch_adj(&a, k, nb_adj, c->st));
-        if ((rc = tick(c, &e1))) return rc;
-        HIPCHK(c, fwi_launch_rec(&r, k, nb_rec, c->st));
-        if ((rc = tick(c, &e2))) return rc;
-        if (c->timing) {
-            marks.push_back({1, {e0, e1}});
-            marks.push_back({2, {e1, e2}});
+    const int B = c->graph_steps;
+    if (B > 0 && !c->timing) {
+        if (!c->gx_bwd) {
+            rc = capture(c, &c->gx_bwd, [&]() {
+                hipError_t e = hipSuccess;
+                for (int j = 0; j < B && e == hipSuccess; j++) {
+                    e = fwi_launch_adj(&a, c->kbase, -j, nb_adj, c->st);
+                    if (e == hipSuccess) e = fwi_launch_rec(&r, c->kbase, -j, nb_rec, c->st);
+                }
+                if (e == hipSuccess) e = fwi_launch_advance(c->kbase, -B, c->st);
+                return e;
+            });
+            if (rc) return rc;
         }
-        if (marks.size() >= 512 && (rc = collect_timing(c, marks))) return rc;
+        HIPCHK(c, fwi_launch_set_int(c->kbase, c->kmax - 1, c->st));
+        for (int blk = 0; blk < (c->kmax + B - 1) / B; blk++) HIPCHK(c, hipGraphLaunch(c->gx_bwd, c->st));
+    } else {
+        for (int k = c->kmax - 1; k >= 0; k--) {
+            hipEvent_t e0, e1, e2;
+            if ((rc = tick(c, &e0))) return rc;
+            HIPCHK(c, fwi_launch_adj(&a, nullptr, k, nb_adj, c->st));
+            if ((rc = tick(c, &e1))) return rc;
+            HIPCHK(c, fwi_launch_rec(&r, nullptr, k, nb_rec, c->st));
+            if ((rc = tick(c, &e2))) return rc;
+            if (c->timing) {
+                marks.push_back({1, {e0, e1}});
+                marks.push_back({2, {e1, e2}});
+            }
+            if (marks.size() >= 512 && (rc = collect_timing(c, marks))) return rc;
+        }
     }
     if ((rc = collect_timing(c, marks))) return rc;
     HIPCHK(c, fwi_launch_grad_finalize(c->gacc, c->ngroups, nb, c->vel, c->gk, c->grad, c->st));
@@ -790,5 +867,46 @@ extern "C" int fwi_hip_read_bdr(const fwi_hip_grid *g, const float *ud, const fl
 {
     if (!g || !ud || !lr || !pb) return set_err(nullptr, FWI_HIP_EINVAL, "read_bdr: NULL");
     HIPCHK0(fwi_launch_ref_read_bdr(ud, lr, pb, g->nx, g->nz, (hipStream_t)stream));
+    return FWI_HIP_OK;
+}
+
+// --------------------------------------------------------------------------
+// device memory helpers
+
+extern "C" int fwi_hip_device_count(int *n)
+{
+    if (!n) return set_err(nullptr, FWI_HIP_EINVAL, "device_count: NULL");
+    *n = 0;
+    hipError_t e = hipGetDeviceCount(n);
+    if (e != hipSuccess) {
+        *n = 0;
+        (void)hipGetLastError();
+    }
+    return FWI_HIP_OK;
+}
+
+extern "C" int fwi_hip_dmalloc(void **ptr, size_t bytes)
+{
+    if (!ptr) return set_err(nullptr, FWI_HIP_EINVAL, "dmalloc: NULL");
+    *ptr = nullptr;
+    hipError_t e = hipMalloc(ptr, bytes ? bytes : 4);
+    if (e != hipSuccess) return set_err(nullptr, FWI_HIP_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    e = hipMemset(*ptr, 0, bytes ? bytes : 4);
+    if (e != hipSuccess) return set_err(nullptr, FWI_HIP_EHIP, "hipMemset: %s", hipGetErrorString(e));
+    return FWI_HIP_OK;
+}
+
+extern "C" int fwi_hip_dfree(void *ptr)
+{
+    if (ptr) HIPCHK0(hipFree(ptr));
+    return FWI_HIP_OK;
+}
+
+extern "C" int fwi_hip_memcpy(void *dst, const void *src, size_t bytes, int kind)
+{
+    if (!dst || !src || kind < 1 || kind > 3) return set_err(nullptr, FWI_HIP_EINVAL, "memcpy: bad argument");
+    const hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice : (kind == 2 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+    HIPCHK0(hipDeviceSynchronize());
+    HIPCHK0(hipMemcpy(dst, src, bytes, k));
     return FWI_HIP_OK;
 }

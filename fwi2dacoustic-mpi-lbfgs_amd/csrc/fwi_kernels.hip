@@ -48,6 +48,21 @@ __device__ __forceinline__ int frame_pos(int bz, int bx, int nx, int nz)
 // forward: x chain of one row (u nodes, CPML on u, ux nodes, CPML on ux)
 // ============================================================================
 
+// Column that lane `lane` (< 6) loads for the x halo of a 64-column tile:
+// lanes 0..2 -> x0-3..x0-1, lanes 3..5 -> x0+64..x0+66 (-1: no halo lane).
+__device__ __forceinline__ int halo_col(int x0, int lane)
+{
+    return (lane < 3) ? x0 - 3 + lane : ((lane < 6) ? x0 + 61 + lane : -1);
+}
+
+// Prefetch depth of the row queue of the marching kernels: rows are loaded
+// this many iterations before they are used, so that several rows of loads
+// per wave are in flight while the current row is computed.
+#ifndef FWI_PREFETCH
+#define FWI_PREFETCH 3
+#endif
+#define PD FWI_PREFETCH
+
 // u at half node j (between p[j-1] and p[j]); pi = LDS index of p[j]
 __device__ __forceinline__ float fwd_u_node(const FwiGeomDev &g, const float *tab, const float *Ps,
                                             int j, int pi, int r, const float *xh_o, float *xh_n,
@@ -100,19 +115,14 @@ __device__ __forceinline__ float fwd_ux_node(const FwiGeomDev &g, const float *t
 }
 
 template <bool FAST>
-__device__ __forceinline__ float fwd_xrow(const FwiGeomDev &g, const float *tab, const float *prow,
-                                          float pself, float *Ps, float *Us, int x0, int lane, int r,
+__device__ __forceinline__ float fwd_xrow(const FwiGeomDev &g, const float *tab, float pself, float phalo,
+                                          float *Ps, float *Us, int x0, int lane, int r,
                                           const float *xh_o, float *xh_n, const float *xd_o, float *xd_n)
 {
     wave_sync();
     Ps[lane + 3] = pself;
-    if (lane < 3) {
-        const int q = x0 - 3 + lane;
-        Ps[lane] = (q >= 0) ? prow[q] : 0.f;
-    } else if (lane < 6) {
-        const int q = x0 + 61 + lane;
-        Ps[lane + 64] = (q < g.pnx) ? prow[q] : 0.f;
-    }
+    if (lane < 3) Ps[lane] = phalo;
+    else if (lane < 6) Ps[lane + 64] = phalo;
     wave_sync();
     if (FAST) {
         Us[lane] = g.c0x * (Ps[lane + 2] - Ps[lane + 1]) + g.c1x * (Ps[lane + 3] - Ps[lane]);
@@ -146,8 +156,11 @@ __device__ __forceinline__ float leapfrog(float pnow, float pold, float k_times_
     return (float)(2.0 * (double)pnow - (double)pold + (double)k_times_x);
 }
 
-__global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_fwd_step(FwiFwdArgs A, int k)
+__global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_fwd_step(FwiFwdArgs A, const int *kbase, int koff)
 {
+    // step index: koff alone, or a device counter plus koff inside a captured graph
+    const int k = (kbase ? *kbase : 0) + koff;
+    if (k < 0) return;
     const FwiGeomDev &g = A.g;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int ntiles = g.ntx * g.ntz;
@@ -186,14 +199,32 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_fwd_step(FwiFwdArgs A, i
     float v0w = 0.f, v1w = 0.f, v2w = 0.f, v3w = 0.f;  // v[r-4..r-1]
     float q0 = 0.f, q1 = 0.f, q2 = 0.f, q3 = 0.f;      // ux[r-3..r]
 
-    for (int r = z0 - 3; r <= z1 + 2; ++r) {
-        float pr = 0.f;
-        if (r >= 0 && r < pnz) pr = pnow[(long long)r * ld + x];
+    // row queue: iteration r needs p[r] (own column and halo) and, for its
+    // output row i = r - 3, p^{k-1}[i] and ekp[i]; loaded PD iterations ahead
+    const int rend = z1 + 2;
+    const int hq = halo_col(x0, lane);
+    float qs[PD], qh[PD], qo[PD], qk[PD];
+    auto fetch = [&](int r, float &fs, float &fh, float &fo, float &fk) {
+        fs = fh = fo = fk = 0.f;
+        if (r > rend) return;
+        if (r >= 0 && r < pnz) fs = pnow[(long long)r * ld + x];
+        if (r >= z0 && r < z1 && hq >= 0 && hq < pnx) fh = pnow[(long long)r * ld + hq];
+        const int io = r - 3;
+        if (io >= z0 && io < z1) {
+            fo = pold[(long long)io * ld + x];
+            fk = ekp[(long long)io * ld + x];
+        }
+    };
+#pragma unroll
+    for (int d = 0; d < PD; ++d) fetch(z0 - 3 + d, qs[d], qh[d], qo[d], qk[d]);
+
+    // unrolled by PD so that every queue slot is a fixed register: a load
+    // stays in flight until its row is used PD iterations later
+    auto body = [&](int r, float pr, float ph, float pold_i, float ekp_i) {
         float uxr = 0.f;
         if (r >= z0 && r < z1) {
-            const float *prow = pnow + (long long)r * ld;
-            uxr = fastx ? fwd_xrow<true>(g, tab, prow, pr, Ps, Us, x0, lane, r, xh_o, xh_n, xd_o, xd_n)
-                        : fwd_xrow<false>(g, tab, prow, pr, Ps, Us, x0, lane, r, xh_o, xh_n, xd_o, xd_n);
+            uxr = fastx ? fwd_xrow<true>(g, tab, pr, ph, Ps, Us, x0, lane, r, xh_o, xh_n, xd_o, xd_n)
+                        : fwd_xrow<false>(g, tab, pr, ph, Ps, Us, x0, lane, r, xh_o, xh_n, xd_o, xd_n);
         }
         p0w = p1w; p1w = p2w; p2w = p3w; p3w = pr;
         q0 = q1; q1 = q2; q2 = q3; q3 = uxr;
@@ -240,7 +271,7 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_fwd_step(FwiFwdArgs A, i
             if (i == sh.isz && x == sh.isx) ux = ux + A.b.srcf[sh.src_off + k];
             const float X = ux + vz;
             const long long o = (long long)i * ld + x;
-            const float pnew = leapfrog(p0w, pold[o], ekp[o] * X);
+            const float pnew = leapfrog(p0w, pold_i, ekp_i * X);
             if (x < pnx) pold[o] = pnew;
             const int slot = A.b.rrow[sh.rrow_off + i];
             if (slot >= 0) {
@@ -259,6 +290,17 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_fwd_step(FwiFwdArgs A, i
                     A.pstrip[so2 + pos] = pnew;
                     A.xstrip[so2 + pos] = X;
                 }
+            }
+        }
+    };
+    for (int rb = z0 - 3; rb <= rend; rb += PD) {
+#pragma unroll
+        for (int d = 0; d < PD; ++d) {
+            const int r = rb + d;
+            if (r <= rend) {
+                const float pr = qs[d], ph = qh[d], po = qo[d], pk = qk[d];
+                fetch(r + PD, qs[d], qh[d], qo[d], qk[d]);
+                body(r, pr, ph, po, pk);
             }
         }
     }
@@ -363,8 +405,11 @@ __device__ __forceinline__ float adj_xrow(const FwiGeomDev &g, const float *tab,
     return gxE(g, i) * Us[ui] + gxF(g, i + 1) * Us[ui + 1] + gxG(g, i - 1) * Us[ui - 1] + gxH(g, i + 2) * Us[ui + 2];
 }
 
-__global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_adj_step(FwiAdjArgs A, int k)
+__global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_adj_step(FwiAdjArgs A, const int *kbase, int koff)
 {
+    // step index: koff alone, or a device counter plus koff inside a captured graph
+    const int k = (kbase ? *kbase : 0) + koff;
+    if (k < 0) return;
     const FwiGeomDev &g = A.g;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int ntiles = g.ntx * g.ntz;
@@ -496,8 +541,11 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_adj_step(FwiAdjArgs A, i
 // ============================================================================
 // reconstruction (physical grid) + zero-lag imaging
 // ============================================================================
-__global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_rec_step(FwiRecArgs A, int k)
+__global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_rec_step(FwiRecArgs A, const int *kbase, int koff)
 {
+    // step index: koff alone, or a device counter plus koff inside a captured graph
+    const int k = (kbase ? *kbase : 0) + koff;
+    if (k < 0) return;
     const FwiGeomDev &g = A.g;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int ntiles = g.bntx * g.bntz;
@@ -599,6 +647,17 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_rec_step(FwiRecArgs A, i
 // ============================================================================
 // small kernels
 // ============================================================================
+// moves the step counter of a captured block of steps
+__global__ void k_advance(int *kbase, int delta)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) kbase[0] += delta;
+}
+
+__global__ void k_set_int(int *p, int v)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) p[0] = v;
+}
+
 __global__ void k_fill_f32(float *p, long long n, float v)
 {
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
@@ -822,21 +881,33 @@ __global__ void k_ref_read_bdr_lr(const float *lr, float *pb, int nx, int nz)
 // ============================================================================
 extern "C" {
 
-hipError_t fwi_launch_fwd(const FwiFwdArgs *a, int k, int nblocks, hipStream_t st)
+hipError_t fwi_launch_fwd(const FwiFwdArgs *a, const int *kbase, int koff, int nblocks, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_fwd_step, dim3(nblocks), dim3(FWI_TX * a->b.sg), 0, st, *a, k);
+    hipLaunchKernelGGL(k_fwd_step, dim3(nblocks), dim3(FWI_TX * a->b.sg), 0, st, *a, kbase, koff);
     return hipGetLastError();
 }
 
-hipError_t fwi_launch_adj(const FwiAdjArgs *a, int k, int nblocks, hipStream_t st)
+hipError_t fwi_launch_adj(const FwiAdjArgs *a, const int *kbase, int koff, int nblocks, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_adj_step, dim3(nblocks), dim3(FWI_TX * a->b.sg), 0, st, *a, k);
+    hipLaunchKernelGGL(k_adj_step, dim3(nblocks), dim3(FWI_TX * a->b.sg), 0, st, *a, kbase, koff);
     return hipGetLastError();
 }
 
-hipError_t fwi_launch_rec(const FwiRecArgs *a, int k, int nblocks, hipStream_t st)
+hipError_t fwi_launch_rec(const FwiRecArgs *a, const int *kbase, int koff, int nblocks, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_rec_step, dim3(nblocks), dim3(FWI_TX * a->b.sg), 0, st, *a, k);
+    hipLaunchKernelGGL(k_rec_step, dim3(nblocks), dim3(FWI_TX * a->b.sg), 0, st, *a, kbase, koff);
+    return hipGetLastError();
+}
+
+hipError_t fwi_launch_advance(int *kbase, int delta, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, kbase, delta);
+    return hipGetLastError();
+}
+
+hipError_t fwi_launch_set_int(int *p, int v, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, st, p, v);
     return hipGetLastError();
 }
 

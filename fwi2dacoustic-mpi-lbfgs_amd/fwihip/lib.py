@@ -22,6 +22,7 @@ EXPORTS = (
     "fwi_hip_set_residual", "fwi_hip_adjoint", "fwi_hip_synchronize", "fwi_hip_set_timing",
     "fwi_hip_kernel_time", "fwi_hip_read_state", "fwi_hip_op_pml", "fwi_hip_op_interior",
     "fwi_hip_time_integral", "fwi_hip_write_bdr", "fwi_hip_read_bdr", "fwi_hip_cpml_coe",
+    "fwi_hip_set_graph", "fwi_hip_device_count", "fwi_hip_dmalloc", "fwi_hip_dfree", "fwi_hip_memcpy",
 )
 
 
@@ -171,6 +172,9 @@ class Context:
     def synchronize(self):
         self._chk(self.lib.fwi_hip_synchronize(self.h))
 
+    def set_graph(self, steps):
+        self._chk(self.lib.fwi_hip_set_graph(self.h, int(steps)))
+
     def set_timing(self, on):
         self._chk(self.lib.fwi_hip_set_timing(self.h, int(on)))
 
@@ -201,9 +205,53 @@ def cpml_coe(npml, dx, dt, max_f=MAX_F, max_v=MAX_V, coef_r=COEF_R):
     return a, b
 
 
-def _dp(t):
-    return ctypes.c_void_p(t.data_ptr())
+def device_count():
+    n = ctypes.c_int(0)
+    load().fwi_hip_device_count(ctypes.byref(n))
+    return n.value
 
+
+class DevArray:
+    """A float32 device buffer owned through the library's own HIP runtime
+    (callers must not mix in a second HIP runtime, e.g. torch's bundled one,
+    in the same process)."""
+
+    def __init__(self, shape, init=None):
+        self.lib = load()
+        self.shape = tuple(shape)
+        self.nbytes = int(np.prod(self.shape)) * 4
+        self.ptr = ctypes.c_void_p()
+        rc = self.lib.fwi_hip_dmalloc(ctypes.byref(self.ptr), ctypes.c_size_t(self.nbytes))
+        if rc:
+            raise FwiError(rc, self.lib.fwi_hip_last_error(None).decode())
+        if init is not None:
+            self.upload(init)
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, np.float32)
+        assert a.size * 4 == self.nbytes
+        rc = self.lib.fwi_hip_memcpy(self.ptr, _p(a), ctypes.c_size_t(self.nbytes), 1)
+        if rc:
+            raise FwiError(rc, self.lib.fwi_hip_last_error(None).decode())
+
+    def numpy(self):
+        out = np.empty(self.shape, np.float32)
+        rc = self.lib.fwi_hip_memcpy(_p(out), self.ptr, ctypes.c_size_t(self.nbytes), 2)
+        if rc:
+            raise FwiError(rc, self.lib.fwi_hip_last_error(None).decode())
+        return out
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                self.lib.fwi_hip_dfree(self.ptr)
+                self.ptr = ctypes.c_void_p()
+        except Exception:
+            pass
+
+
+def _dp(t):
+    return t.ptr if isinstance(t, DevArray) else ctypes.c_void_p(t.data_ptr())
 
 def op_pml(geom, p, u, v, ux, vz, u0, u1, v1, ux0, ux1, vz1):
     """a_lap_opt_2d_pml on torch device tensors (in place)."""

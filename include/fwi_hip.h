@@ -98,6 +98,11 @@ int fwi_hip_set_residual(fwi_hip_ctx *ctx, const float *res);
  * ekp * dJ/dekp; either may be NULL (then it stays on the device).         */
 int fwi_hip_adjoint(fwi_hip_ctx *ctx, double *grad, double *gk);
 
+/* Time steps per captured HIP graph (default 100; 0 = plain launches).
+ * The kernels of a block of steps read the step index from a device
+ * counter, so one executable graph is replayed for every block.           */
+int fwi_hip_set_graph(fwi_hip_ctx *ctx, int steps);
+
 /* Wait for all work of the context. */
 int fwi_hip_synchronize(fwi_hip_ctx *ctx);
 
@@ -112,6 +117,14 @@ int fwi_hip_kernel_time(fwi_hip_ctx *ctx, int which, double *ms, long long *laun
  *   what 0/1: p buffer 0/1 [pnz][pnx]   2/3: adjoint buffer 0/1 [pnz][pnx]
  *   4/5: boundary / operator frames of the shot [nt][6 nx + 6 nz]          */
 int fwi_hip_read_state(fwi_hip_ctx *ctx, int what, int shot, float *out);
+
+/* ---- device memory for callers of the per-step entry points ------------
+ * (no counterpart in the reference, which has no device).  kind: 1 host to
+ * device, 2 device to host, 3 device to device.  Synchronous.             */
+int fwi_hip_device_count(int *n);
+int fwi_hip_dmalloc(void **ptr, size_t bytes);
+int fwi_hip_dfree(void *ptr);
+int fwi_hip_memcpy(void *dst, const void *src, size_t bytes, int kind);
 
 /* ---- per-step entry points on caller-owned device buffers (dense rows) --
  * Shapes follow fwi_init (fwi_2d.c:230-296) for the f-grid p[pnz][pnx],

@@ -67,3 +67,29 @@ def test_cpml_coe_host_matches_oracle():
     ao, bo = o.cpml_coe(20, 10.0, 1e-3)
     np.testing.assert_array_equal(a, ao)
     np.testing.assert_array_equal(b, bo)
+
+
+def test_plan_layout_and_sampling_check():
+    """fwi_plan_alloc builds the reference's shapes (fwi_2d.c:148-165), the
+    ctypes mirror of header.h reads them back, and grad_all_shots rejects
+    (nt0-1)*jump != nt-1 (fwi_2d.c:195) before touching the device."""
+    import plan_ctypes as PC
+    h = PC.lib()
+    p = h.fwi_plan_alloc(30, 20, 5, 41, 2, 7, 10.0, 10.0, 1e-3)
+    P = p.contents
+    f, b = P.grid.contents.f.contents, P.grid.contents.b.contents
+    assert (f.pnx, f.pnz, f.unx, f.unz, f.vnx, f.vnz) == (40, 25, 41, 25, 40, 26)
+    assert (b.pnx, b.pnz, b.unx, b.vnz) == (30, 20, 31, 21)
+    msf = P.msf.contents
+    assert (msf.nt, msf.nr, msf.jump) == (21, 7, 2)
+    assert abs(P.sou.contents.iarea - 0.01) < 1e-9
+    base = P.grid.contents.base.contents
+    assert (base.nx, base.nz, base.npml, base.nt, base.free_surface) == (30, 20, 5, 41, 1)
+    off = PC.FwiMsf.nt.offset
+    ctypes.c_size_t.from_address(ctypes.addressof(msf) + off).value = 20
+    arr = (ctypes.POINTER(PC.FwiPlan) * 1)(p)
+    assert h.grad_all_shots(arr, 1, 0) == L.EINVAL
+    assert b"nt0" in h.grad4_last_error()
+    ctypes.c_size_t.from_address(ctypes.addressof(msf) + off).value = 21
+    h.fwi_plan_free(p)
+    assert not h.fwi_plan_alloc(30, 20, 5, 40, 2, 7, 10.0, 10.0, 1e-3)  # (nt-1) % jump != 0

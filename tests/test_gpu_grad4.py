@@ -1,0 +1,76 @@
+"""The drop-in boundary: grad4_1shot / grad_all_shots on reference-layout
+plans (include/fwi_plan.h), against the oracle."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import Oracle
+from fwihip import lib as L
+from fwihip.workloads import Geometry, Shot, ricker
+import plan_ctypes as PC
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def h():
+    L.load()
+    if L.device_count() < 1:
+        pytest.skip("no GPU")
+    return PC.lib()
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+def case(nt=241, seed=4):
+    g = Geometry(nx=120, nz=60, npml=12, dx=10.0, dz=10.0, dt=1e-3)
+    rng = np.random.default_rng(seed)
+    vel = np.clip(2300 + 250 * rng.standard_normal((g.nz, g.nx)), 1600, 3200).astype(np.float32)
+    irx = np.arange(1, g.nx, 3, dtype=np.int32)
+    shot = Shot(nt=nt, isx=57, isz=2, wav=ricker(nt, g.dt, 18.0), irx=irx, irz=np.full(len(irx), 2, np.int32))
+    return g, vel, shot
+
+
+@pytest.mark.parametrize("jump", [1, 2])
+def test_grad4_1shot_matches_oracle(h, jump):
+    g, vel, shot = case()
+    o = Oracle(np.float32)
+    true = o.forward(g, vel * 1.03, shot)["syn"]
+    obs0 = np.ascontiguousarray(true[:, ::jump])
+    p = PC.make_plan(h, g, shot, obs0, vel, jump=jump)
+    h.grad4_1shot(p, 0)
+    P = p.contents
+    body = PC.view(P.grad.contents.body, (g.nz, g.nx)).copy()
+    syn = PC.view(P.msf.contents.syn[0], obs0.shape).copy()
+    val = PC.view(P.msf.contents.val, (2,)).copy()
+    # oracle: same driver, residual on the sampled steps
+    f = o.forward(g, vel, shot, want_strips=True)
+    res = np.zeros_like(f["syn"])
+    res[:, ::jump] = f["syn"][:, ::jump] - obs0
+    b = o.backward(g, vel, shot, res, f)
+    J = 0.5 * np.sum((f["syn"][:, ::jump].astype(np.float64) - obs0) ** 2)
+    assert rel(syn, f["syn"][:, ::jump]) < 1e-5
+    assert rel(body, b["grad"]) < 1e-4
+    assert abs(val[0] - J) < 1e-5 * J and val[0] == val[1]
+    h.fwi_plan_free(p)
+
+
+def test_grad_all_shots_sums_shots(h):
+    g, vel, shot = case(nt=201, seed=6)
+    o = Oracle(np.float32)
+    shots = [shot, Shot(nt=201, isx=20, isz=3, wav=shot.wav, irx=shot.irx, irz=shot.irz)]
+    plans = []
+    gsum = np.zeros((g.nz, g.nx))
+    for s in shots:
+        obs0 = o.forward(g, vel * 0.98, s)["syn"]
+        plans.append(PC.make_plan(h, g, s, obs0, vel))
+        gsum += o.gradient(g, vel, s, obs0)[2]
+    arr = (ctypes.POINTER(PC.FwiPlan) * 2)(*plans)
+    assert h.grad_all_shots(arr, 2, 0) == 0, h.grad4_last_error()
+    body = PC.view(plans[0].contents.grad.contents.body, (g.nz, g.nx))
+    assert rel(body, gsum) < 1e-4
+    for p in plans:
+        h.fwi_plan_free(p)
