@@ -29,6 +29,7 @@ struct FwiShotDev {
     long long trace_off;  // into syn/res (time-major nt x nr)
     long long strip_off;  // into pstrip/xstrip (nt x sl)
     int rrow_off;         // into rrow (pnz entries: receiver-row slot or -1)
+    int rz_lo, rz_hi;     // rows holding receivers: [rz_lo, rz_hi] (empty if lo > hi)
     int pad_;
 };
 

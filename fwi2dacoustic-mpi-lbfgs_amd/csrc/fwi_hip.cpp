@@ -196,7 +196,7 @@ static void cpml_tables(const fwi_hip_grid &g, std::vector<float> &tab)
 static int check_grid(fwi_hip_ctx *c, const fwi_hip_grid &g)
 {
     if (g.nx < 8 || g.nz < 8) return set_err(c, FWI_HIP_EINVAL, "grid %dx%d too small (need nx, nz >= 8)", g.nx, g.nz);
-    if (g.npml < 2) return set_err(c, FWI_HIP_EINVAL, "npml=%d: need at least 2 CPML cells", g.npml);
+    if (g.npml < 2 || g.npml > 64) return set_err(c, FWI_HIP_EINVAL, "npml=%d: need 2..64 CPML cells", g.npml);
     if (!(g.dx > 0) || !(g.dz > 0) || !(g.dt > 0)) return set_err(c, FWI_HIP_EINVAL, "dx, dz, dt must be > 0");
     if (!(g.max_v > 0) || !(g.coef_r > 0) || !(g.coef_r < 1) || !(g.max_f >= 0))
         return set_err(c, FWI_HIP_EINVAL, "CPML meta: need max_v > 0, 0 < coef_r < 1, max_f >= 0");
@@ -407,6 +407,8 @@ extern "C" int fwi_hip_set_shots(fwi_hip_ctx *c, int nshots, const fwi_hip_shot 
         d.strip_off = soff;
         soff += (long long)sh.nt * g.sl;
         d.rrow_off = s * g.pnz;
+        d.rz_lo = g.pnz;
+        d.rz_hi = -1;
         std::map<int, int> rowslot;
         for (int r = 0; r < sh.nr; r++) {
             const int x = sh.irx[r], z = sh.irz[r];
@@ -426,6 +428,8 @@ extern "C" int fwi_hip_set_shots(fwi_hip_ctx *c, int nshots, const fwi_hip_shot 
             if (cell >= 0)
                 return set_err(c, FWI_HIP_EINVAL, "shot %d: receivers %d and %d share cell (%d,%d)", s, cell, r, x, z);
             cell = r;
+            d.rz_lo = std::min(d.rz_lo, z);
+            d.rz_hi = std::max(d.rz_hi, z);
         }
         c->nt_h[s] = sh.nt;
         c->nr_h[s] = sh.nr;

@@ -35,6 +35,17 @@ __device__ __forceinline__ void wave_sync()
 // CPML table rows: 0 a0x, 1 b0x, 2 a1x, 3 b1x, 4 a1z, 5 b1z; T = 2 npml + 1
 #define TB(row, i) tab[(row) * T + (i)]
 
+// Workgroups are dispatched round-robin over the 8 XCDs (blockIdx % 8), each
+// with its own L2.  The grid is padded to a multiple of 8 and XCD x gets the
+// contiguous logical range [x*per, (x+1)*per): neighbouring tiles of a shot,
+// which read each other's halo rows, then share an L2.  -1 = padding block.
+__device__ __forceinline__ int xcd_remap(int b, int n)
+{
+    const int per = (n + 7) >> 3;
+    const int l = (b & 7) * per + (b >> 3);
+    return l < n ? l : -1;
+}
+
 // frame position of physical cell (bz, bx) inside one step of the strips
 __device__ __forceinline__ int frame_pos(int bz, int bx, int nx, int nz)
 {
@@ -55,13 +66,24 @@ __device__ __forceinline__ int halo_col(int x0, int lane)
     return (lane < 3) ? x0 - 3 + lane : ((lane < 6) ? x0 + 61 + lane : -1);
 }
 
-// Prefetch depth of the row queue of the marching kernels: rows are loaded
-// this many iterations before they are used, so that several rows of loads
-// per wave are in flight while the current row is computed.
-#ifndef FWI_PREFETCH
-#define FWI_PREFETCH 3
+// Rows per load block of the marching kernels: a wave issues the loads of
+// RB rows (own column, halo, in-place field, ekp) before computing any of
+// them, so RB rows of memory traffic are in flight per wave instead of one.
+#ifndef FWI_ROWBLOCK
+#define FWI_ROWBLOCK 4
 #endif
-#define PD FWI_PREFETCH
+#define RB FWI_ROWBLOCK
+
+// CPML tables staged in LDS once per workgroup (6 rows of 2 npml + 1)
+#define FWI_MAXNPML 64
+#define FWI_TABN (6 * (2 * FWI_MAXNPML + 1))
+
+__device__ __forceinline__ void stage_tables(float *stab, const float *tab, int npml)
+{
+    const int n = 6 * (2 * npml + 1);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) stab[i] = tab[i];
+    __syncthreads();
+}
 
 // u at half node j (between p[j-1] and p[j]); pi = LDS index of p[j]
 __device__ __forceinline__ float fwd_u_node(const FwiGeomDev &g, const float *tab, const float *Ps,
@@ -164,9 +186,13 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_fwd_step(FwiFwdArgs A, c
     const FwiGeomDev &g = A.g;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int ntiles = g.ntx * g.ntz;
-    const int grp = blockIdx.x / ntiles, t = blockIdx.x - grp * ntiles;
+    const int lb = xcd_remap(blockIdx.x, ntiles * A.b.ngroups);
+    if (lb < 0) return;
+    const int grp = lb / ntiles, t = lb - grp * ntiles;
     const int tzi = t / g.ntx, txi = t - tzi * g.ntx;
     const int s = grp * A.b.sg + wv;
+    __shared__ float stab[FWI_TABN];
+    stage_tables(stab, A.b.tab, g.npml);
     if (s >= A.b.nshots) return;
     const FwiShotDev sh = A.b.shots[s];
     if (k >= sh.nt) return;
@@ -175,7 +201,8 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_fwd_step(FwiFwdArgs A, c
     float *Ps = lds[wv][0], *Us = lds[wv][1];
 
     const int npml = g.npml, pnx = g.pnx, pnz = g.pnz, ld = g.ld, T = 2 * npml + 1;
-    const float *tab = A.b.tab;
+    const float *tab = stab;
+    const float fsrc = A.b.srcf[sh.src_off + k];
     const int x0 = txi * TX, x = x0 + lane;
     const int z0 = tzi * g.tz, z1 = min(z0 + g.tz, pnz);
     const long long so = (long long)s * A.b.fs;
@@ -199,11 +226,10 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_fwd_step(FwiFwdArgs A, c
     float v0w = 0.f, v1w = 0.f, v2w = 0.f, v3w = 0.f;  // v[r-4..r-1]
     float q0 = 0.f, q1 = 0.f, q2 = 0.f, q3 = 0.f;      // ux[r-3..r]
 
-    // row queue: iteration r needs p[r] (own column and halo) and, for its
-    // output row i = r - 3, p^{k-1}[i] and ekp[i]; loaded PD iterations ahead
+    // iteration r needs p[r] (own column and halo) and, for its output row
+    // i = r - 3, p^{k-1}[i] and ekp[i]; loaded a block of RB rows at a time
     const int rend = z1 + 2;
     const int hq = halo_col(x0, lane);
-    float qs[PD], qh[PD], qo[PD], qk[PD];
     auto fetch = [&](int r, float &fs, float &fh, float &fo, float &fk) {
         fs = fh = fo = fk = 0.f;
         if (r > rend) return;
@@ -215,11 +241,6 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_fwd_step(FwiFwdArgs A, c
             fk = ekp[(long long)io * ld + x];
         }
     };
-#pragma unroll
-    for (int d = 0; d < PD; ++d) fetch(z0 - 3 + d, qs[d], qh[d], qo[d], qk[d]);
-
-    // unrolled by PD so that every queue slot is a fixed register: a load
-    // stays in flight until its row is used PD iterations later
     auto body = [&](int r, float pr, float ph, float pold_i, float ekp_i) {
         float uxr = 0.f;
         if (r >= z0 && r < z1) {
@@ -268,15 +289,17 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_fwd_step(FwiFwdArgs A, c
                 zd_n[q * ld + x] = ps;
             }
             float ux = q0;
-            if (i == sh.isz && x == sh.isx) ux = ux + A.b.srcf[sh.src_off + k];
+            if (i == sh.isz && x == sh.isx) ux = ux + fsrc;
             const float X = ux + vz;
             const long long o = (long long)i * ld + x;
             const float pnew = leapfrog(p0w, pold_i, ekp_i * X);
             if (x < pnx) pold[o] = pnew;
-            const int slot = A.b.rrow[sh.rrow_off + i];
-            if (slot >= 0) {
-                const int ri = A.b.rmap[slot * ld + x];
-                if (ri >= 0) A.syn[sh.trace_off + (long long)k * sh.nr + ri] = pnew;
+            if (i >= sh.rz_lo && i <= sh.rz_hi) {
+                const int slot = A.b.rrow[sh.rrow_off + i];
+                if (slot >= 0) {
+                    const int ri = A.b.rmap[slot * ld + x];
+                    if (ri >= 0) A.syn[sh.trace_off + (long long)k * sh.nr + ri] = pnew;
+                }
             }
             if (in_bx && i < nz && (i < 3 || i >= nz - 3 || bx < 3 || bx >= nx - 3)) {
                 const long long so2 = sh.strip_off + (long long)k * g.sl;
@@ -293,16 +316,13 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_fwd_step(FwiFwdArgs A, c
             }
         }
     };
-    for (int rb = z0 - 3; rb <= rend; rb += PD) {
+    for (int rb = z0 - 3; rb <= rend; rb += RB) {
+        float bs[RB], bh[RB], bo[RB], bk[RB];
 #pragma unroll
-        for (int d = 0; d < PD; ++d) {
-            const int r = rb + d;
-            if (r <= rend) {
-                const float pr = qs[d], ph = qh[d], po = qo[d], pk = qk[d];
-                fetch(r + PD, qs[d], qh[d], qo[d], qk[d]);
-                body(r, pr, ph, po, pk);
-            }
-        }
+        for (int d = 0; d < RB; ++d) fetch(rb + d, bs[d], bh[d], bo[d], bk[d]);
+#pragma unroll
+        for (int d = 0; d < RB; ++d)
+            if (rb + d <= rend) body(rb + d, bs[d], bh[d], bo[d], bk[d]);
     }
 }
 
@@ -374,7 +394,7 @@ __device__ __forceinline__ float adj_u_node(const FwiGeomDev &g, const float *ta
 }
 
 template <bool FAST>
-__device__ __forceinline__ float adj_xrow(const FwiGeomDev &g, const float *tab, const float *wrow, float wself,
+__device__ __forceinline__ float adj_xrow(const FwiGeomDev &g, const float *tab, float wself, float whalo,
                                           float *Ws, float *Us, int x0, int lane, int r,
                                           const float *xh_o, float *xh_n, const float *xd_o, float *xd_n)
 {
@@ -383,9 +403,9 @@ __device__ __forceinline__ float adj_xrow(const FwiGeomDev &g, const float *tab,
     if (!FAST) wm = adj_filt_d(g, tab, x0 + lane, wm, r, xd_o, xd_n, x0 + lane < g.pnx);
     Ws[lane + 3] = wm;
     if (lane < 6) {
-        const int q = (lane < 3) ? x0 - 3 + lane : x0 + 61 + lane;
+        const int q = halo_col(x0, lane);
         const int idx = (lane < 3) ? lane : lane + 64;
-        float wh = (q >= 0 && q < g.pnx) ? wrow[q] : 0.f;
+        float wh = whalo;
         if (!FAST) wh = adj_filt_d(g, tab, q, wh, r, xd_o, xd_n, false);
         Ws[idx] = wh;
     }
@@ -413,9 +433,13 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_adj_step(FwiAdjArgs A, c
     const FwiGeomDev &g = A.g;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int ntiles = g.ntx * g.ntz;
-    const int grp = blockIdx.x / ntiles, t = blockIdx.x - grp * ntiles;
+    const int lb = xcd_remap(blockIdx.x, ntiles * A.b.ngroups);
+    if (lb < 0) return;
+    const int grp = lb / ntiles, t = lb - grp * ntiles;
     const int tzi = t / g.ntx, txi = t - tzi * g.ntx;
     const int s = grp * A.b.sg + wv;
+    __shared__ float stab[FWI_TABN];
+    stage_tables(stab, A.b.tab, g.npml);
     if (s >= A.b.nshots) return;
     const FwiShotDev sh = A.b.shots[s];
     if (k >= sh.nt) return;
@@ -424,7 +448,7 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_adj_step(FwiAdjArgs A, c
     float *Ws = lds[wv][0], *Us = lds[wv][1];
 
     const int npml = g.npml, pnx = g.pnx, pnz = g.pnz, ld = g.ld, T = 2 * npml + 1;
-    const float *tab = A.b.tab;
+    const float *tab = stab;
     const int x0 = txi * TX, x = x0 + lane;
     const int z0 = tzi * g.tz, z1 = min(z0 + g.tz, pnz);
     const long long so = (long long)s * A.b.fs;
@@ -459,14 +483,24 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_adj_step(FwiAdjArgs A, c
     float q0 = 0.f, q1 = 0.f, q2 = 0.f, q3 = 0.f;  // out_x[r-3..r]
     float wz0 = 0.f, tv0 = 0.f, tv1 = 0.f;          // top rows held by the first tile
 
-    for (int r = z0 - 3; r <= z1 + 2; ++r) {
-        float wr = 0.f;
-        if (r >= 0 && r < pnz) wr = mnow[(long long)r * ld + x];
+    const int rend = z1 + 2;
+    const int hq = halo_col(x0, lane);
+    auto fetch = [&](int r, float &fs, float &fh, float &fo, float &fk) {
+        fs = fh = fo = fk = 0.f;
+        if (r > rend) return;
+        if (r >= 0 && r < pnz) fs = mnow[(long long)r * ld + x];
+        if (r >= z0 && r < z1 && hq >= 0 && hq < pnx) fh = mnow[(long long)r * ld + hq];
+        const int io = r - 3;
+        if (io >= z0 && io < z1) {
+            fo = mold[(long long)io * ld + x];
+            fk = ekp[(long long)io * ld + x];
+        }
+    };
+    auto body = [&](int r, float wr, float wh, float mold_i, float ekp_i) {
         float oxr = 0.f;
         if (r >= z0 && r < z1) {
-            const float *wrow = mnow + (long long)r * ld;
-            oxr = fastx ? adj_xrow<true>(g, tab, wrow, wr, Ws, Us, x0, lane, r, xh_o, xh_n, xd_o, xd_n)
-                        : adj_xrow<false>(g, tab, wrow, wr, Ws, Us, x0, lane, r, xh_o, xh_n, xd_o, xd_n);
+            oxr = fastx ? adj_xrow<true>(g, tab, wr, wh, Ws, Us, x0, lane, r, xh_o, xh_n, xd_o, xd_n)
+                        : adj_xrow<false>(g, tab, wr, wh, Ws, Us, x0, lane, r, xh_o, xh_n, xd_o, xd_n);
         }
         // z chain: C^T of the vz-node CPML
         float wz = wr;
@@ -526,15 +560,25 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_adj_step(FwiAdjArgs A, c
                 if (i == pnz - 2) oz -= izf * b2;
             }
             float out = q0 + oz;
-            const int slot = A.b.rrow[sh.rrow_off + i];
-            if (slot >= 0) {
-                const int ri = A.b.rmap[slot * ld + x];
-                if (ri >= 0) out = out + A.res[sh.trace_off + (long long)k * sh.nr + ri];
+            if (i >= sh.rz_lo && i <= sh.rz_hi) {
+                const int slot = A.b.rrow[sh.rrow_off + i];
+                if (slot >= 0) {
+                    const int ri = A.b.rmap[slot * ld + x];
+                    if (ri >= 0) out = out + A.res[sh.trace_off + (long long)k * sh.nr + ri];
+                }
             }
             const long long o = (long long)i * ld + x;
-            const float mnew = leapfrog(m0, mold[o], ekp[o] * out);
+            const float mnew = leapfrog(m0, mold_i, ekp_i * out);
             if (x < pnx) mold[o] = mnew;
         }
+    };
+    for (int rb = z0 - 3; rb <= rend; rb += RB) {
+        float bs[RB], bh[RB], bo[RB], bk[RB];
+#pragma unroll
+        for (int d = 0; d < RB; ++d) fetch(rb + d, bs[d], bh[d], bo[d], bk[d]);
+#pragma unroll
+        for (int d = 0; d < RB; ++d)
+            if (rb + d <= rend) body(rb + d, bs[d], bh[d], bo[d], bk[d]);
     }
 }
 
@@ -549,11 +593,16 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_rec_step(FwiRecArgs A, c
     const FwiGeomDev &g = A.g;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int ntiles = g.bntx * g.bntz;
-    const int grp = blockIdx.x / ntiles, t = blockIdx.x - grp * ntiles;
+    const int lb = xcd_remap(blockIdx.x, ntiles * A.b.ngroups);
+    if (lb < 0) return;
+    const int grp = lb / ntiles, t = lb - grp * ntiles;
     const int tzi = t / g.bntx, txi = t - tzi * g.bntx;
     const int s = grp * A.b.sg + wv;
+    // every wave of the workgroup runs the whole loop (the imaging sum over
+    // the group's shots meets at a barrier per row); inactive waves add 0
     bool active = s < A.b.nshots;
     FwiShotDev sh;
+    sh.nt = 0;
     if (active) {
         sh = A.b.shots[s];
         active = k < sh.nt;
@@ -567,6 +616,7 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_rec_step(FwiRecArgs A, c
     const int bx0 = txi * TX, bx = bx0 + lane, fx = bx + npml;
     const int z0 = tzi * g.tz, z1 = min(z0 + g.tz, nz);
     const bool colok = bx < nx;
+    const bool live = active && colok;
     const long long so = (long long)(active ? s : 0) * A.b.fs;
     const int cur = k & 1;
     const float *pn = (cur ? A.P1 : A.P0) + so;  // p^k
@@ -575,26 +625,41 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_rec_step(FwiRecArgs A, c
     const float *ekp = A.b.ekp;
     const float c0x = g.c0x, c1x = g.c1x, c0z = g.c0z, c1z = g.c1z;
     double *gacc = A.gacc + (long long)grp * nx * nz;
+    const float fsrc = active ? A.b.srcf[sh.src_off + k] : 0.f;
+    const int hq = halo_col(bx0, lane);
 
     float p0w = 0.f, p1w = 0.f, p2w = 0.f, p3w = 0.f;
     float v0w = 0.f, v1w = 0.f, v2w = 0.f, v3w = 0.f;
     float q0 = 0.f, q1 = 0.f, q2 = 0.f, q3 = 0.f;
 
-    for (int r = z0 - 3; r <= z1 + 2; ++r) {
-        float pr = 0.f;
-        if (active && r >= 0 && r < nz && colok) pr = pn[(long long)r * ld + fx];
+    // iteration r needs p^k[r] (own column and halo); its output row i = r - 3
+    // needs p^{k+1}[i], mu^{k+1}[i], ekp[i] and (wave 0) the FP64 accumulator
+    const int rend = z1 + 2;
+    auto fetch = [&](int r, float &fs, float &fh, float &fp, float &fm, float &fk, double &fg) {
+        fs = fh = fp = fm = fk = 0.f;
+        fg = 0.0;
+        if (r > rend) return;
+        if (live && r >= 0 && r < nz) fs = pn[(long long)r * ld + fx];
+        if (active && r >= z0 && r < z1 && r >= 3 && r < nz - 3 && hq >= 0 && hq < nx)
+            fh = pn[(long long)r * ld + npml + hq];
+        const int io = r - 3;
+        if (io >= z0 && io < z1) {
+            const long long o = (long long)io * ld + fx;
+            if (live) {
+                fp = pp[o];
+                fm = mu[o];
+                fk = ekp[o];
+            }
+            if (wv == 0 && colok) fg = gacc[(long long)io * nx + bx];
+        }
+    };
+    auto body = [&](int r, float pr, float ph, float pp_i, float mu_i, float ekp_i, double g_i) {
         float uxr = 0.f;
         if (active && r >= z0 && r < z1 && r >= 3 && r < nz - 3) {
-            const float *prow = pn + (long long)r * ld + npml;  // physical column 0
             wave_sync();
             Ps[lane + 3] = pr;
-            if (lane < 3) {
-                const int q = bx0 - 3 + lane;
-                Ps[lane] = (q >= 0) ? prow[q] : 0.f;
-            } else if (lane < 6) {
-                const int q = bx0 + 61 + lane;
-                Ps[lane + 64] = (q < nx) ? prow[q] : 0.f;
-            }
+            if (lane < 3) Ps[lane] = ph;
+            else if (lane < 6) Ps[lane + 64] = ph;
             wave_sync();
             Us[lane] = c0x * (Ps[lane + 2] - Ps[lane + 1]) + c1x * (Ps[lane + 3] - Ps[lane]);
             if (lane < 3)
@@ -610,37 +675,45 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_rec_step(FwiRecArgs A, c
         const int i = r - 3;
         if (i >= z0 && i < z1) {
             double img = 0.0;
-            if (active && colok) {
+            if (live) {
                 const long long so2 = sh.strip_off + (long long)k * g.sl;
                 const bool inner = (i >= 3 && i < nz - 3 && bx >= 3 && bx < nx - 3);
                 float X;
                 if (inner) {
                     float ux = q0;
-                    if (i == sh.isz && fx == sh.isx) ux = ux + A.b.srcf[sh.src_off + k];
+                    if (i == sh.isz && fx == sh.isx) ux = ux + fsrc;
                     const float vz = c0z * (v2w - v1w) + c1z * (v3w - v0w);
                     X = ux + vz;
                 } else {
                     X = A.xstrip[so2 + frame_pos(i, bx, nx, nz)];
                 }
-                const long long o = (long long)i * ld + fx;
-                img = (double)mu[o] * (double)X;
+                img = (double)mu_i * (double)X;
                 if (k >= 1) {
                     float pm;
-                    if (inner) pm = leapfrog(p0w, pp[o], ekp[o] * X);
+                    if (inner) pm = leapfrog(p0w, pp_i, ekp_i * X);
                     else pm = (k >= 2) ? A.pstrip[sh.strip_off + (long long)(k - 2) * g.sl + frame_pos(i, bx, nx, nz)]
                                        : 0.f;
-                    pp[o] = pm;
+                    pp[(long long)i * ld + fx] = pm;
                 }
             }
             red[wv][lane] = img;
             __syncthreads();
             if (wv == 0 && colok) {
-                double acc = 0.0;
+                double acc = g_i;
                 for (int w = 0; w < sg; ++w) acc += red[w][lane];
-                gacc[(long long)i * nx + bx] += acc;
+                gacc[(long long)i * nx + bx] = acc;
             }
             __syncthreads();
         }
+    };
+    for (int rb = z0 - 3; rb <= rend; rb += RB) {
+        float bs[RB], bh[RB], bp[RB], bm[RB], bk[RB];
+        double bg[RB];
+#pragma unroll
+        for (int d = 0; d < RB; ++d) fetch(rb + d, bs[d], bh[d], bp[d], bm[d], bk[d], bg[d]);
+#pragma unroll
+        for (int d = 0; d < RB; ++d)
+            if (rb + d <= rend) body(rb + d, bs[d], bh[d], bp[d], bm[d], bk[d], bg[d]);
     }
 }
 
@@ -883,19 +956,19 @@ extern "C" {
 
 hipError_t fwi_launch_fwd(const FwiFwdArgs *a, const int *kbase, int koff, int nblocks, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_fwd_step, dim3(nblocks), dim3(FWI_TX * a->b.sg), 0, st, *a, kbase, koff);
+    hipLaunchKernelGGL(k_fwd_step, dim3(((nblocks + 7) / 8) * 8), dim3(FWI_TX * a->b.sg), 0, st, *a, kbase, koff);
     return hipGetLastError();
 }
 
 hipError_t fwi_launch_adj(const FwiAdjArgs *a, const int *kbase, int koff, int nblocks, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_adj_step, dim3(nblocks), dim3(FWI_TX * a->b.sg), 0, st, *a, kbase, koff);
+    hipLaunchKernelGGL(k_adj_step, dim3(((nblocks + 7) / 8) * 8), dim3(FWI_TX * a->b.sg), 0, st, *a, kbase, koff);
     return hipGetLastError();
 }
 
 hipError_t fwi_launch_rec(const FwiRecArgs *a, const int *kbase, int koff, int nblocks, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_rec_step, dim3(nblocks), dim3(FWI_TX * a->b.sg), 0, st, *a, kbase, koff);
+    hipLaunchKernelGGL(k_rec_step, dim3(((nblocks + 7) / 8) * 8), dim3(FWI_TX * a->b.sg), 0, st, *a, kbase, koff);
     return hipGetLastError();
 }
 

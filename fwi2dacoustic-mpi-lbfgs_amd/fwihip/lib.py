@@ -9,7 +9,7 @@ import os
 import numpy as np
 
 PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(PKG, "libfwi_hip.so")
+LIB_PATH = os.environ.get("FWI_HIP_LIBRARY") or os.path.join(PKG, "libfwi_hip.so")
 
 OK, EINVAL, ENOMEM, EHIP, ESTATE = 0, 1, 2, 3, 4
 
