@@ -14,6 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libfwi_hip.so")
+LIB_TORCH = os.path.join(HERE, "libfwi_hip_torch.so")
 INC = os.path.join(os.path.dirname(HERE), "include")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -45,7 +46,29 @@ def build(verbose=True):
     tmp = LIB + ".tmp"
     run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)
+    # Second link of the same objects against the HIP runtime that PyTorch
+    # bundles, so that one process holds a single HIP runtime when the
+    # library works on torch tensors (fwihip.autograd); C callers use LIB.
+    tlib = torch_hip_dir()
+    if tlib:
+        tmp = LIB_TORCH + ".tmp"
+        run(["g++", "-shared", "-fPIC", "-o", tmp, *objs, f"-L{tlib}", "-l:libamdhip64.so",
+             f"-Wl,-rpath,{tlib}", "-Wl,--no-undefined"])
+        os.replace(tmp, LIB_TORCH)
     return LIB
+
+
+def torch_hip_dir():
+    """Directory of torch's bundled libamdhip64.so (None without torch)."""
+    try:
+        import importlib.util
+        spec = importlib.util.find_spec("torch")
+    except Exception:
+        return None
+    if not spec or not spec.origin:
+        return None
+    d = os.path.join(os.path.dirname(spec.origin), "lib")
+    return d if os.path.exists(os.path.join(d, "libamdhip64.so")) else None
 
 
 if __name__ == "__main__":

@@ -40,7 +40,11 @@ struct FwiGeomDev {
     int bntx, bntz;      // physical-grid tiles
     int sl;              // strip length per step = 6 nx + 6 nz
     float c0x, c1x, ix, c0z, c1z, iz, iz840, iz24;
+    int ntz2;            // f-grid tiles of the 2-D tile kernels (FWI_TZ2 rows each)
+    int pad_;
 };
+
+#define FWI_TZ2 32       // rows of a 2-D tile (4 waves x 8 output rows)
 
 struct FwiBatchDev {
     int nshots, sg, ngroups;

@@ -19,6 +19,7 @@
 
 extern "C" {
 hipError_t fwi_launch_fwd(const FwiFwdArgs *a, const int *kbase, int koff, int nblocks, hipStream_t st);
+hipError_t fwi_launch_fwd_tile(const FwiFwdArgs *a, const int *kbase, int koff, hipStream_t st);
 hipError_t fwi_launch_adj(const FwiAdjArgs *a, const int *kbase, int koff, int nblocks, hipStream_t st);
 hipError_t fwi_launch_rec(const FwiRecArgs *a, const int *kbase, int koff, int nblocks, hipStream_t st);
 hipError_t fwi_launch_advance(int *kbase, int delta, hipStream_t st);
@@ -79,6 +80,7 @@ struct fwi_hip_ctx {
     // captured blocks of time steps
     int *kbase = nullptr;
     int graph_steps = 100;
+    int fwd_kind = 0;  // 0: 2-D tile kernel, 1: z-marching kernel
     hipGraphExec_t gx_fwd = nullptr, gx_bwd = nullptr;
     // timing
     bool timing = false;
@@ -256,6 +258,7 @@ extern "C" int fwi_hip_create(const fwi_hip_desc *desc, fwi_hip_ctx **out)
     g.bntx = (g.nx + FWI_TX - 1) / FWI_TX;
     g.bntz = (g.nz + tz - 1) / tz;
     g.sl = 6 * g.nx + 6 * g.nz;
+    g.ntz2 = (g.pnz + FWI_TZ2 - 1) / FWI_TZ2;
     Coe co = fd_coe(gr.dx, gr.dz);
     g.c0x = co.c0x; g.c1x = co.c1x; g.ix = co.ix;
     g.c0z = co.c0z; g.c1z = co.c1z; g.iz = co.iz;
@@ -556,6 +559,15 @@ static int capture(fwi_hip_ctx *c, hipGraphExec_t *out, F build)
     return FWI_HIP_OK;
 }
 
+extern "C" int fwi_hip_set_kernels(fwi_hip_ctx *c, int fwd_kind)
+{
+    if (!c || fwd_kind < 0 || fwd_kind > 1) return set_err(c, FWI_HIP_EINVAL, "set_kernels: bad argument");
+    if (c->gx_fwd) (void)hipGraphExecDestroy(c->gx_fwd);
+    c->gx_fwd = nullptr;
+    c->fwd_kind = fwd_kind;
+    return FWI_HIP_OK;
+}
+
 extern "C" int fwi_hip_set_graph(fwi_hip_ctx *c, int steps)
 {
     if (!c || steps < 0) return set_err(c, FWI_HIP_EINVAL, "set_graph: bad argument");
@@ -596,7 +608,9 @@ extern "C" int fwi_hip_forward(fwi_hip_ctx *c, float *traces)
         if (!c->gx_fwd) {
             rc = capture(c, &c->gx_fwd, [&]() {
                 hipError_t e = hipSuccess;
-                for (int j = 0; j < B && e == hipSuccess; j++) e = fwi_launch_fwd(&a, c->kbase, j, nblocks, c->st);
+                for (int j = 0; j < B && e == hipSuccess; j++)
+                    e = c->fwd_kind ? fwi_launch_fwd(&a, c->kbase, j, nblocks, c->st)
+                                    : fwi_launch_fwd_tile(&a, c->kbase, j, c->st);
                 if (e == hipSuccess) e = fwi_launch_advance(c->kbase, B, c->st);
                 return e;
             });
@@ -608,7 +622,8 @@ extern "C" int fwi_hip_forward(fwi_hip_ctx *c, float *traces)
         for (int k = 0; k < c->kmax; k++) {
             hipEvent_t e0, e1;
             if ((rc = tick(c, &e0))) return rc;
-            HIPCHK(c, fwi_launch_fwd(&a, nullptr, k, nblocks, c->st));
+            HIPCHK(c, c->fwd_kind ? fwi_launch_fwd(&a, nullptr, k, nblocks, c->st)
+                                  : fwi_launch_fwd_tile(&a, nullptr, k, c->st));
             if ((rc = tick(c, &e1))) return rc;
             if (c->timing) marks.push_back({0, {e0, e1}});
             if (marks.size() >= 512 && (rc = collect_timing(c, marks))) return rc;

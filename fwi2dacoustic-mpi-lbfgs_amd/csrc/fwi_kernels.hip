@@ -21,6 +21,7 @@
 // -ffp-contract=off): float stencils, the free-surface rows and the leapfrog
 // sum evaluated in double as C evaluates the reference's double literals.
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include "fwi_dev.h"
 
 #define TX FWI_TX
@@ -73,6 +74,11 @@ __device__ __forceinline__ int halo_col(int x0, int lane)
 #define FWI_ROWBLOCK 4
 #endif
 #define RB FWI_ROWBLOCK
+
+// 1: the main blocks of a march load block b+1 while block b is computed
+#ifndef FWI_DOUBLEBUF
+#define FWI_DOUBLEBUF 1
+#endif
 
 // CPML tables staged in LDS once per workgroup (6 rows of 2 npml + 1)
 #define FWI_MAXNPML 64
@@ -136,27 +142,53 @@ __device__ __forceinline__ float fwd_ux_node(const FwiGeomDev &g, const float *t
     return ux;
 }
 
-template <bool FAST>
-__device__ __forceinline__ float fwd_xrow(const FwiGeomDev &g, const float *tab, float pself, float phalo,
-                                          float *Ps, float *Us, int x0, int lane, int r,
-                                          const float *xh_o, float *xh_n, const float *xd_o, float *xd_n)
+// x chain of a block of RB rows: all rows staged in LDS, then all u nodes,
+// then all ux nodes (3 wave syncs per block).  Rows outside [z0, z1) are
+// skipped (their ux is 0 and never used).
+template <bool FAST, int N>
+__device__ __forceinline__ void fwd_xblock(const FwiGeomDev &g, const float *tab, const float *bs, const float *bh,
+                                           float (*Ps)[72], float (*Us)[72], int x0, int lane, int rb, int z0,
+                                           int z1, const float *xh_o, float *xh_n, const float *xd_o, float *xd_n,
+                                           float *bux)
 {
     wave_sync();
-    Ps[lane + 3] = pself;
-    if (lane < 3) Ps[lane] = phalo;
-    else if (lane < 6) Ps[lane + 64] = phalo;
-    wave_sync();
-    if (FAST) {
-        Us[lane] = g.c0x * (Ps[lane + 2] - Ps[lane + 1]) + g.c1x * (Ps[lane + 3] - Ps[lane]);
-        if (lane < 3)
-            Us[64 + lane] = g.c0x * (Ps[lane + 66] - Ps[lane + 65]) + g.c1x * (Ps[lane + 67] - Ps[lane + 64]);
-    } else {
-        Us[lane] = fwd_u_node(g, tab, Ps, x0 - 1 + lane, lane + 2, r, xh_o, xh_n, x0);
-        if (lane < 3) Us[64 + lane] = fwd_u_node(g, tab, Ps, x0 + 63 + lane, lane + 66, r, xh_o, xh_n, x0);
+#pragma unroll
+    for (int d = 0; d < N; ++d) {
+        const int r = rb + d;
+        if (r >= z0 && r < z1) {
+            Ps[d][lane + 3] = bs[d];
+            if (lane < 3) Ps[d][lane] = bh[d];
+            else if (lane < 6) Ps[d][lane + 64] = bh[d];
+        }
     }
     wave_sync();
-    if (FAST) return g.c0x * (Us[lane + 2] - Us[lane + 1]) + g.c1x * (Us[lane + 3] - Us[lane]);
-    return fwd_ux_node(g, tab, Us, x0 + lane, lane + 1, r, xd_o, xd_n);
+#pragma unroll
+    for (int d = 0; d < N; ++d) {
+        const int r = rb + d;
+        if (r >= z0 && r < z1) {
+            const float *P = Ps[d];
+            float *U = Us[d];
+            if (FAST) {
+                U[lane] = g.c0x * (P[lane + 2] - P[lane + 1]) + g.c1x * (P[lane + 3] - P[lane]);
+                if (lane < 3)
+                    U[64 + lane] = g.c0x * (P[lane + 66] - P[lane + 65]) + g.c1x * (P[lane + 67] - P[lane + 64]);
+            } else {
+                U[lane] = fwd_u_node(g, tab, P, x0 - 1 + lane, lane + 2, r, xh_o, xh_n, x0);
+                if (lane < 3) U[64 + lane] = fwd_u_node(g, tab, P, x0 + 63 + lane, lane + 66, r, xh_o, xh_n, x0);
+            }
+        }
+    }
+    wave_sync();
+#pragma unroll
+    for (int d = 0; d < N; ++d) {
+        const int r = rb + d;
+        bux[d] = 0.f;
+        if (r >= z0 && r < z1) {
+            const float *U = Us[d];
+            bux[d] = FAST ? g.c0x * (U[lane + 2] - U[lane + 1]) + g.c1x * (U[lane + 3] - U[lane])
+                          : fwd_ux_node(g, tab, U, x0 + lane, lane + 1, r, xd_o, xd_n);
+        }
+    }
 }
 
 __device__ __forceinline__ float v_row0(float p0, float p1, float p2, float p3, float iz840)
@@ -197,8 +229,9 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_fwd_step(FwiFwdArgs A, c
     const FwiShotDev sh = A.b.shots[s];
     if (k >= sh.nt) return;
 
-    __shared__ float lds[FWI_MAXSG][2][72];
-    float *Ps = lds[wv][0], *Us = lds[wv][1];
+    __shared__ float lds[FWI_MAXSG][2][RB][72];
+    float(*Ps)[72] = lds[wv][0];
+    float(*Us)[72] = lds[wv][1];
 
     const int npml = g.npml, pnx = g.pnx, pnz = g.pnz, ld = g.ld, T = 2 * npml + 1;
     const float *tab = stab;
@@ -241,12 +274,7 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_fwd_step(FwiFwdArgs A, c
             fk = ekp[(long long)io * ld + x];
         }
     };
-    auto body = [&](int r, float pr, float ph, float pold_i, float ekp_i) {
-        float uxr = 0.f;
-        if (r >= z0 && r < z1) {
-            uxr = fastx ? fwd_xrow<true>(g, tab, pr, ph, Ps, Us, x0, lane, r, xh_o, xh_n, xd_o, xd_n)
-                        : fwd_xrow<false>(g, tab, pr, ph, Ps, Us, x0, lane, r, xh_o, xh_n, xd_o, xd_n);
-        }
+    auto body = [&](int r, float pr, float uxr, float pold_i, float ekp_i) {
         p0w = p1w; p1w = p2w; p2w = p3w; p3w = pr;
         q0 = q1; q1 = q2; q2 = q3; q3 = uxr;
 
@@ -316,13 +344,272 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_fwd_step(FwiFwdArgs A, c
             }
         }
     };
-    for (int rb = z0 - 3; rb <= rend; rb += RB) {
-        float bs[RB], bh[RB], bo[RB], bk[RB];
+    // interior rows: no special row, strip, frame, source or receiver
+    auto plain_body = [&](int r, float pr, float uxr, float pold_i, float ekp_i) {
+        p0w = p1w; p1w = p2w; p2w = p3w; p3w = pr;
+        q0 = q1; q1 = q2; q2 = q3; q3 = uxr;
+        const float vj = g.c0z * (p2w - p1w) + g.c1z * (p3w - p0w);
+        v0w = v1w; v1w = v2w; v2w = v3w; v3w = vj;
+        const float vz = g.c0z * (v2w - v1w) + g.c1z * (v3w - v0w);
+        const float X = q0 + vz;
+        pold[(r - 3) * ld + x] = leapfrog(p0w, pold_i, ekp_i * X);
+    };
+    // any N rows: general x chain and bodies
+    auto gen_block = [&](auto Nc, int rb) {
+        constexpr int N = decltype(Nc)::value;
+        float bs[N], bh[N], bo[N], bk[N], bu[N];
 #pragma unroll
-        for (int d = 0; d < RB; ++d) fetch(rb + d, bs[d], bh[d], bo[d], bk[d]);
+        for (int d = 0; d < N; ++d) fetch(rb + d, bs[d], bh[d], bo[d], bk[d]);
+        if (rb + N - 1 >= z0 && rb < z1) {
+            if (fastx) fwd_xblock<true, N>(g, tab, bs, bh, Ps, Us, x0, lane, rb, z0, z1, xh_o, xh_n, xd_o, xd_n, bu);
+            else fwd_xblock<false, N>(g, tab, bs, bh, Ps, Us, x0, lane, rb, z0, z1, xh_o, xh_n, xd_o, xd_n, bu);
+        } else {
 #pragma unroll
-        for (int d = 0; d < RB; ++d)
-            if (rb + d <= rend) body(rb + d, bs[d], bh[d], bo[d], bk[d]);
+            for (int d = 0; d < N; ++d) bu[d] = 0.f;
+        }
+#pragma unroll
+        for (int d = 0; d < N; ++d)
+            if (rb + d <= rend) body(rb + d, bs[d], bu[d], bo[d], bk[d]);
+    };
+    // a main block: RB rows that all need the x chain and all output a row
+    auto run_block = [&](int rb, const float *bs, const float *bh, const float *bo, const float *bk) {
+        float bu[RB];
+        const int ilo = rb - 3, ihi = rb + RB - 4;
+        const bool plain = fastx && ilo >= 3 && ihi < nz - 3 && (sh.isz < ilo || sh.isz > ihi) &&
+                           (sh.rz_hi < ilo || sh.rz_lo > ihi);
+        if (plain) {
+            fwd_xblock<true, RB>(g, tab, bs, bh, Ps, Us, x0, lane, rb, z0, z1, xh_o, xh_n, xd_o, xd_n, bu);
+#pragma unroll
+            for (int d = 0; d < RB; ++d) plain_body(rb + d, bs[d], bu[d], bo[d], bk[d]);
+        } else {
+            if (fastx) fwd_xblock<true, RB>(g, tab, bs, bh, Ps, Us, x0, lane, rb, z0, z1, xh_o, xh_n, xd_o, xd_n, bu);
+            else fwd_xblock<false, RB>(g, tab, bs, bh, Ps, Us, x0, lane, rb, z0, z1, xh_o, xh_n, xd_o, xd_n, bu);
+#pragma unroll
+            for (int d = 0; d < RB; ++d) body(rb + d, bs[d], bu[d], bo[d], bk[d]);
+        }
+    };
+    using I3 = std::integral_constant<int, 3>;
+    // warm-up rows z0-3 .. z0+2 (no output yet)
+    gen_block(I3{}, z0 - 3);
+    gen_block(I3{}, z0);
+    // main rows z0+3 .. z1-1, double-buffered: the loads of the next block
+    // are in flight while the current block is computed
+    int rb = z0 + 3;
+    const int rmain = z1 - 1;
+    if (rb + RB - 1 <= rmain) {
+        float As[RB], Ah[RB], Ao[RB], Ak[RB], Bs[RB], Bh[RB], Bo[RB], Bk[RB];
+#pragma unroll
+        for (int d = 0; d < RB; ++d) fetch(rb + d, As[d], Ah[d], Ao[d], Ak[d]);
+        for (;;) {
+            const bool nb = rb + 2 * RB - 1 <= rmain;
+            if (nb && FWI_DOUBLEBUF) {
+#pragma unroll
+                for (int d = 0; d < RB; ++d) fetch(rb + RB + d, Bs[d], Bh[d], Bo[d], Bk[d]);
+            }
+            run_block(rb, As, Ah, Ao, Ak);
+            rb += RB;
+            if (!nb) break;
+            if (!FWI_DOUBLEBUF) {
+#pragma unroll
+                for (int d = 0; d < RB; ++d) fetch(rb + d, Bs[d], Bh[d], Bo[d], Bk[d]);
+            }
+            const bool na = rb + 2 * RB - 1 <= rmain;
+            if (na && FWI_DOUBLEBUF) {
+#pragma unroll
+                for (int d = 0; d < RB; ++d) fetch(rb + RB + d, As[d], Ah[d], Ao[d], Ak[d]);
+            }
+            run_block(rb, Bs, Bh, Bo, Bk);
+            rb += RB;
+            if (!na) break;
+            if (!FWI_DOUBLEBUF) {
+#pragma unroll
+                for (int d = 0; d < RB; ++d) fetch(rb + d, As[d], Ah[d], Ao[d], Ak[d]);
+            }
+        }
+    }
+    // tail rows up to z1+2
+    for (; rb <= rend; rb += 3) gen_block(I3{}, rb);
+}
+
+// ============================================================================
+// forward, 2-D tile form: one workgroup (4 waves) per 64 x FWI_TZ2 tile of one
+// shot.  All loads of the tile are issued at once; p (with a 3-cell halo) is
+// staged in LDS, the half-node fields u and v are formed in LDS, then each
+// wave finishes 8 output rows.  Same arithmetic and CPML ownership rules as
+// k_fwd_step.
+// ============================================================================
+#define TR2 (FWI_TZ2 / 4)          // output rows per wave
+#define PR2 (FWI_TZ2 + 6)          // staged p rows
+#define VR2 (FWI_TZ2 + 3)          // v-node rows
+#define PM2 ((PR2 + 3) / 4)        // staged p rows per wave
+#define VM2 ((VR2 + 3) / 4)        // v-node rows per wave
+
+// v at half node j (between p[j-1] and p[j]) from staged rows; P row of p[z] is z - z0 + 3
+__device__ __forceinline__ float fwd_v_node(const FwiGeomDev &g, const float *tab, const float (*P)[72], int j,
+                                            int z0, int c, int x, const float *zh_o, float *zh_n, bool own)
+{
+    const int pnz = g.pnz, npml = g.npml, T = 2 * npml + 1, ld = g.ld;
+    if (j < 0 || j >= pnz) return 0.f;
+    const int b = j - z0 + 3;  // P row of p[j]
+    float v;
+    if (j == 0) v = v_row0(P[b][c], P[b + 1][c], P[b + 2][c], P[b + 3][c], g.iz840);
+    else if (j == 1) v = v_row1(P[b - 1][c], P[b][c], P[b + 1][c], P[b + 2][c], g.iz840);
+    else if (j == pnz - 1) v = (P[b][c] - P[b - 1][c]) * g.iz;
+    else v = g.c0z * (P[b][c] - P[b - 1][c]) + g.c1z * (P[b + 1][c] - P[b - 2][c]);
+    if (j >= pnz - npml) {
+        const int q = j - (pnz - npml);
+        float ps = zh_o[q * ld + x];
+        ps = TB(5, 2 * q) * ps + TB(4, 2 * q) * v;
+        v = v + ps;
+        if (own) zh_n[q * ld + x] = ps;
+    }
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_fwd_tile(FwiFwdArgs A, const int *kbase, int koff)
+{
+    const int k = (kbase ? *kbase : 0) + koff;
+    if (k < 0) return;
+    const FwiGeomDev &g = A.g;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int ntiles = g.ntx * g.ntz2;
+    const int lb = xcd_remap(blockIdx.x, ntiles * A.b.nshots);
+    if (lb < 0) return;
+    const int s = lb / ntiles, t = lb - s * ntiles;
+    const int tzi = t / g.ntx, txi = t - tzi * g.ntx;
+    __shared__ float stab[FWI_TABN];
+    __shared__ float P[PR2][72];
+    __shared__ float U[FWI_TZ2][72];
+    __shared__ float V[VR2][64];
+    stage_tables(stab, A.b.tab, g.npml);
+    const FwiShotDev sh = A.b.shots[s];
+    if (k >= sh.nt) return;  // the whole workgroup belongs to shot s
+
+    const int npml = g.npml, pnx = g.pnx, pnz = g.pnz, ld = g.ld, T = 2 * npml + 1;
+    const float *tab = stab;
+    const int x0 = txi * TX, x = x0 + lane;
+    const int z0 = tzi * FWI_TZ2, z1 = min(z0 + FWI_TZ2, pnz);
+    const long long so = (long long)s * A.b.fs;
+    const int cur = k & 1, nxt = cur ^ 1;
+    const float *pnow = (cur ? A.P1 : A.P0) + so;
+    float *pold = (cur ? A.P0 : A.P1) + so;
+    const float *ekp = A.b.ekp;
+    const float *xh_o = A.xh + s * A.b.xh_s + cur * A.b.xh_b;
+    float *xh_n = A.xh + s * A.b.xh_s + nxt * A.b.xh_b;
+    const float *xd_o = A.xd + s * A.b.xd_s + cur * A.b.xd_b;
+    float *xd_n = A.xd + s * A.b.xd_s + nxt * A.b.xd_b;
+    const float *zh_o = A.zh + s * A.b.zh_s + cur * A.b.zh_b;
+    float *zh_n = A.zh + s * A.b.zh_s + nxt * A.b.zh_b;
+    const float *zd_o = A.zd + s * A.b.zd_s + cur * A.b.zd_b;
+    float *zd_n = A.zd + s * A.b.zd_s + nxt * A.b.zd_b;
+    const bool fastx = (x0 >= npml + 3) && (x0 + 66 <= pnx - npml - 1);
+    const int hq = halo_col(x0, lane);
+    const int ibase = z0 + w * TR2;  // first output row of this wave
+
+    // ---- phase 1: every load of the tile in flight at once
+    float pv[PM2], ph[PM2], po[TR2], pk[TR2];
+#pragma unroll
+    for (int m = 0; m < PM2; ++m) {
+        const int rr = w + 4 * m, zr = z0 - 3 + rr;
+        pv[m] = ph[m] = 0.f;
+        if (rr < PR2 && zr >= 0 && zr < pnz) {
+            pv[m] = pnow[zr * ld + x];
+            if (hq >= 0 && hq < pnx) ph[m] = pnow[zr * ld + hq];
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < TR2; ++m) {
+        const int i = ibase + m;
+        po[m] = pk[m] = 0.f;
+        if (i < z1) {
+            po[m] = pold[i * ld + x];
+            pk[m] = ekp[i * ld + x];
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < PM2; ++m) {
+        const int rr = w + 4 * m;
+        if (rr < PR2) {
+            P[rr][lane + 3] = pv[m];
+            if (lane < 3) P[rr][lane] = ph[m];
+            else if (lane < 6) P[rr][lane + 64] = ph[m];
+        }
+    }
+    __syncthreads();
+
+    // ---- phase 2: half-node fields u (rows z0..z1-1) and v (nodes z0-1..z0+TZ2+1)
+#pragma unroll
+    for (int m = 0; m < TR2; ++m) {
+        const int rr = w + 4 * m, zr = z0 + rr;
+        if (zr < z1) {
+            const float *Pr = P[rr + 3];
+            if (fastx) {
+                U[rr][lane] = g.c0x * (Pr[lane + 2] - Pr[lane + 1]) + g.c1x * (Pr[lane + 3] - Pr[lane]);
+                if (lane < 3)
+                    U[rr][64 + lane] =
+                        g.c0x * (Pr[lane + 66] - Pr[lane + 65]) + g.c1x * (Pr[lane + 67] - Pr[lane + 64]);
+            } else {
+                U[rr][lane] = fwd_u_node(g, tab, Pr, x0 - 1 + lane, lane + 2, zr, xh_o, xh_n, x0);
+                if (lane < 3) U[rr][64 + lane] = fwd_u_node(g, tab, Pr, x0 + 63 + lane, lane + 66, zr, xh_o, xh_n, x0);
+            }
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < VM2; ++m) {
+        const int rr = w + 4 * m;
+        if (rr < VR2) {
+            const int j = z0 - 1 + rr;
+            V[rr][lane] = fwd_v_node(g, tab, P, j, z0, lane + 3, x, zh_o, zh_n, j >= z0 && j < z1);
+        }
+    }
+    __syncthreads();
+
+    // ---- phase 3: output rows
+    const int bx = x - npml, nx = g.nx, nz = g.nz;
+    const bool in_bx = (bx >= 0) && (bx < nx);
+#pragma unroll
+    for (int m = 0; m < TR2; ++m) {
+        const int i = ibase + m;
+        if (i >= z1) break;
+        const int rr = i - z0;
+        float ux;
+        if (fastx) ux = g.c0x * (U[rr][lane + 2] - U[rr][lane + 1]) + g.c1x * (U[rr][lane + 3] - U[rr][lane]);
+        else ux = fwd_ux_node(g, tab, U[rr], x, lane + 1, i, xd_o, xd_n);
+        float vz;
+        if (i == 0) vz = vz_row0(V[1][lane], V[2][lane], V[3][lane], V[4][lane], V[5][lane], g.iz24);
+        else if (i == pnz - 1) vz = (V[rr + 2][lane] - V[rr + 1][lane]) * g.iz;
+        else vz = g.c0z * (V[rr + 2][lane] - V[rr + 1][lane]) + g.c1z * (V[rr + 3][lane] - V[rr][lane]);
+        if (i >= pnz - npml) {
+            const int q = i - (pnz - npml);
+            float ps = zd_o[q * ld + x];
+            ps = TB(5, 2 * q + 1) * ps + TB(4, 2 * q + 1) * vz;
+            vz = vz + ps;
+            zd_n[q * ld + x] = ps;
+        }
+        if (i == sh.isz && x == sh.isx) ux = ux + A.b.srcf[sh.src_off + k];
+        const float X = ux + vz;
+        const float pnew = leapfrog(P[rr + 3][lane + 3], po[m], pk[m] * X);
+        if (x < pnx) pold[i * ld + x] = pnew;
+        if (i >= sh.rz_lo && i <= sh.rz_hi) {
+            const int slot = A.b.rrow[sh.rrow_off + i];
+            if (slot >= 0) {
+                const int ri = A.b.rmap[slot * ld + x];
+                if (ri >= 0) A.syn[sh.trace_off + (long long)k * sh.nr + ri] = pnew;
+            }
+        }
+        if (in_bx && i < nz && (i < 3 || i >= nz - 3 || bx < 3 || bx >= nx - 3)) {
+            const long long so2 = sh.strip_off + (long long)k * g.sl;
+            if (i < 3 || i >= nz - 3) {
+                const int pos = (i < 3) ? i * nx + bx : (3 + nz - 1 - i) * nx + bx;
+                A.pstrip[so2 + pos] = pnew;
+                A.xstrip[so2 + pos] = X;
+            }
+            if (bx < 3 || bx >= nx - 3) {
+                const int pos = 6 * nx + i * 6 + ((bx < 3) ? bx : 3 + (nx - 1 - bx));
+                A.pstrip[so2 + pos] = pnew;
+                A.xstrip[so2 + pos] = X;
+            }
+        }
     }
 }
 
@@ -393,36 +680,63 @@ __device__ __forceinline__ float adj_u_node(const FwiGeomDev &g, const float *ta
     return u;
 }
 
-template <bool FAST>
-__device__ __forceinline__ float adj_xrow(const FwiGeomDev &g, const float *tab, float wself, float whalo,
-                                          float *Ws, float *Us, int x0, int lane, int r,
-                                          const float *xh_o, float *xh_n, const float *xd_o, float *xd_n)
+template <bool FAST, int N>
+__device__ __forceinline__ void adj_xblock(const FwiGeomDev &g, const float *tab, const float *bs, const float *bh,
+                                           float (*Ws)[72], float (*Us)[72], int x0, int lane, int rb, int z0,
+                                           int z1, const float *xh_o, float *xh_n, const float *xd_o, float *xd_n,
+                                           float *box)
 {
+    const int hq = halo_col(x0, lane);
+    const int hidx = (lane < 3) ? lane : lane + 64;
     wave_sync();
-    float wm = wself;
-    if (!FAST) wm = adj_filt_d(g, tab, x0 + lane, wm, r, xd_o, xd_n, x0 + lane < g.pnx);
-    Ws[lane + 3] = wm;
-    if (lane < 6) {
-        const int q = halo_col(x0, lane);
-        const int idx = (lane < 3) ? lane : lane + 64;
-        float wh = whalo;
-        if (!FAST) wh = adj_filt_d(g, tab, q, wh, r, xd_o, xd_n, false);
-        Ws[idx] = wh;
+#pragma unroll
+    for (int d = 0; d < N; ++d) {
+        const int r = rb + d;
+        if (r >= z0 && r < z1) {
+            float wm = bs[d];
+            if (!FAST) wm = adj_filt_d(g, tab, x0 + lane, wm, r, xd_o, xd_n, x0 + lane < g.pnx);
+            Ws[d][lane + 3] = wm;
+            if (lane < 6) {
+                float wh = bh[d];
+                if (!FAST) wh = adj_filt_d(g, tab, hq, wh, r, xd_o, xd_n, false);
+                Ws[d][hidx] = wh;
+            }
+        }
     }
     wave_sync();
-    if (FAST) {
-        Us[lane] = g.c0x * (Ws[lane + 1] - Ws[lane + 2]) + g.c1x * (Ws[lane] - Ws[lane + 3]);
-        if (lane < 3)
-            Us[64 + lane] = g.c0x * (Ws[lane + 65] - Ws[lane + 66]) + g.c1x * (Ws[lane + 64] - Ws[lane + 67]);
-    } else {
-        Us[lane] = adj_u_node(g, tab, Ws, x0 - 1 + lane, lane + 2, r, xh_o, xh_n, x0);
-        if (lane < 3) Us[64 + lane] = adj_u_node(g, tab, Ws, x0 + 63 + lane, lane + 66, r, xh_o, xh_n, x0);
+#pragma unroll
+    for (int d = 0; d < N; ++d) {
+        const int r = rb + d;
+        if (r >= z0 && r < z1) {
+            const float *W = Ws[d];
+            float *U = Us[d];
+            if (FAST) {
+                U[lane] = g.c0x * (W[lane + 1] - W[lane + 2]) + g.c1x * (W[lane] - W[lane + 3]);
+                if (lane < 3)
+                    U[64 + lane] = g.c0x * (W[lane + 65] - W[lane + 66]) + g.c1x * (W[lane + 64] - W[lane + 67]);
+            } else {
+                U[lane] = adj_u_node(g, tab, W, x0 - 1 + lane, lane + 2, r, xh_o, xh_n, x0);
+                if (lane < 3) U[64 + lane] = adj_u_node(g, tab, W, x0 + 63 + lane, lane + 66, r, xh_o, xh_n, x0);
+            }
+        }
     }
     wave_sync();
-    if (FAST) return g.c0x * (Us[lane + 1] - Us[lane + 2]) + g.c1x * (Us[lane] - Us[lane + 3]);
-    const int i = x0 + lane, ui = lane + 1;
-    if (i >= g.pnx) return 0.f;
-    return gxE(g, i) * Us[ui] + gxF(g, i + 1) * Us[ui + 1] + gxG(g, i - 1) * Us[ui - 1] + gxH(g, i + 2) * Us[ui + 2];
+#pragma unroll
+    for (int d = 0; d < N; ++d) {
+        const int r = rb + d;
+        box[d] = 0.f;
+        if (r >= z0 && r < z1) {
+            const float *U = Us[d];
+            if (FAST) {
+                box[d] = g.c0x * (U[lane + 1] - U[lane + 2]) + g.c1x * (U[lane] - U[lane + 3]);
+            } else {
+                const int i = x0 + lane, ui = lane + 1;
+                box[d] = (i >= g.pnx) ? 0.f
+                                      : gxE(g, i) * U[ui] + gxF(g, i + 1) * U[ui + 1] + gxG(g, i - 1) * U[ui - 1] +
+                                            gxH(g, i + 2) * U[ui + 2];
+            }
+        }
+    }
 }
 
 __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_adj_step(FwiAdjArgs A, const int *kbase, int koff)
@@ -444,8 +758,9 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_adj_step(FwiAdjArgs A, c
     const FwiShotDev sh = A.b.shots[s];
     if (k >= sh.nt) return;
 
-    __shared__ float lds[FWI_MAXSG][2][72];
-    float *Ws = lds[wv][0], *Us = lds[wv][1];
+    __shared__ float lds[FWI_MAXSG][2][RB][72];
+    float(*Ws)[72] = lds[wv][0];
+    float(*Us)[72] = lds[wv][1];
 
     const int npml = g.npml, pnx = g.pnx, pnz = g.pnz, ld = g.ld, T = 2 * npml + 1;
     const float *tab = stab;
@@ -496,12 +811,7 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_adj_step(FwiAdjArgs A, c
             fk = ekp[(long long)io * ld + x];
         }
     };
-    auto body = [&](int r, float wr, float wh, float mold_i, float ekp_i) {
-        float oxr = 0.f;
-        if (r >= z0 && r < z1) {
-            oxr = fastx ? adj_xrow<true>(g, tab, wr, wh, Ws, Us, x0, lane, r, xh_o, xh_n, xd_o, xd_n)
-                        : adj_xrow<false>(g, tab, wr, wh, Ws, Us, x0, lane, r, xh_o, xh_n, xd_o, xd_n);
-        }
+    auto body = [&](int r, float wr, float oxr, float mold_i, float ekp_i) {
         // z chain: C^T of the vz-node CPML
         float wz = wr;
         if (r >= pnz - npml && r < pnz) {
@@ -572,14 +882,85 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_adj_step(FwiAdjArgs A, c
             if (x < pnx) mold[o] = mnew;
         }
     };
-    for (int rb = z0 - 3; rb <= rend; rb += RB) {
-        float bs[RB], bh[RB], bo[RB], bk[RB];
+    // interior rows: fast transposed stencils, no strip and no receiver
+    auto plain_body = [&](int r, float wr, float oxr, float mold_i, float ekp_i) {
+        a0 = a1; a1 = a2; a2 = a3; a3 = wr;
+        m0 = m1; m1 = m2; m2 = m3; m3 = wr;
+        q0 = q1; q1 = q2; q2 = q3; q3 = oxr;
+        const float tv = c0z * (a1 - a2) + c1z * (a0 - a3);
+        b0 = b1; b1 = b2; b2 = b3; b3 = tv;
+        const float oz = c0z * (b1 - b2) + c1z * (b0 - b3);
+        const float out = q0 + oz;
+        mold[(r - 3) * ld + x] = leapfrog(m0, mold_i, ekp_i * out);
+    };
+    auto gen_block = [&](auto Nc, int rb) {
+        constexpr int N = decltype(Nc)::value;
+        float bs[N], bh[N], bo[N], bk[N], bu[N];
 #pragma unroll
-        for (int d = 0; d < RB; ++d) fetch(rb + d, bs[d], bh[d], bo[d], bk[d]);
+        for (int d = 0; d < N; ++d) fetch(rb + d, bs[d], bh[d], bo[d], bk[d]);
+        if (rb + N - 1 >= z0 && rb < z1) {
+            if (fastx) adj_xblock<true, N>(g, tab, bs, bh, Ws, Us, x0, lane, rb, z0, z1, xh_o, xh_n, xd_o, xd_n, bu);
+            else adj_xblock<false, N>(g, tab, bs, bh, Ws, Us, x0, lane, rb, z0, z1, xh_o, xh_n, xd_o, xd_n, bu);
+        } else {
 #pragma unroll
-        for (int d = 0; d < RB; ++d)
-            if (rb + d <= rend) body(rb + d, bs[d], bh[d], bo[d], bk[d]);
+            for (int d = 0; d < N; ++d) bu[d] = 0.f;
+        }
+#pragma unroll
+        for (int d = 0; d < N; ++d)
+            if (rb + d <= rend) body(rb + d, bs[d], bu[d], bo[d], bk[d]);
+    };
+    auto run_block = [&](int rb, const float *bs, const float *bh, const float *bo, const float *bk) {
+        float bu[RB];
+        const int ilo = rb - 3, ihi = rb + RB - 4;
+        const bool plain = fastx && ilo >= 5 && ihi < g.nz - 3 && (sh.rz_hi < ilo || sh.rz_lo > ihi);
+        if (plain) {
+            adj_xblock<true, RB>(g, tab, bs, bh, Ws, Us, x0, lane, rb, z0, z1, xh_o, xh_n, xd_o, xd_n, bu);
+#pragma unroll
+            for (int d = 0; d < RB; ++d) plain_body(rb + d, bs[d], bu[d], bo[d], bk[d]);
+        } else {
+            if (fastx) adj_xblock<true, RB>(g, tab, bs, bh, Ws, Us, x0, lane, rb, z0, z1, xh_o, xh_n, xd_o, xd_n, bu);
+            else adj_xblock<false, RB>(g, tab, bs, bh, Ws, Us, x0, lane, rb, z0, z1, xh_o, xh_n, xd_o, xd_n, bu);
+#pragma unroll
+            for (int d = 0; d < RB; ++d) body(rb + d, bs[d], bu[d], bo[d], bk[d]);
+        }
+    };
+    using I3 = std::integral_constant<int, 3>;
+    gen_block(I3{}, z0 - 3);
+    gen_block(I3{}, z0);
+    int rb = z0 + 3;
+    const int rmain = z1 - 1;
+    if (rb + RB - 1 <= rmain) {
+        float As[RB], Ah[RB], Ao[RB], Ak[RB], Bs[RB], Bh[RB], Bo[RB], Bk[RB];
+#pragma unroll
+        for (int d = 0; d < RB; ++d) fetch(rb + d, As[d], Ah[d], Ao[d], Ak[d]);
+        for (;;) {
+            const bool nb = rb + 2 * RB - 1 <= rmain;
+            if (nb && FWI_DOUBLEBUF) {
+#pragma unroll
+                for (int d = 0; d < RB; ++d) fetch(rb + RB + d, Bs[d], Bh[d], Bo[d], Bk[d]);
+            }
+            run_block(rb, As, Ah, Ao, Ak);
+            rb += RB;
+            if (!nb) break;
+            if (!FWI_DOUBLEBUF) {
+#pragma unroll
+                for (int d = 0; d < RB; ++d) fetch(rb + d, Bs[d], Bh[d], Bo[d], Bk[d]);
+            }
+            const bool na = rb + 2 * RB - 1 <= rmain;
+            if (na && FWI_DOUBLEBUF) {
+#pragma unroll
+                for (int d = 0; d < RB; ++d) fetch(rb + RB + d, As[d], Ah[d], Ao[d], Ak[d]);
+            }
+            run_block(rb, Bs, Bh, Bo, Bk);
+            rb += RB;
+            if (!na) break;
+            if (!FWI_DOUBLEBUF) {
+#pragma unroll
+                for (int d = 0; d < RB; ++d) fetch(rb + d, As[d], Ah[d], Ao[d], Ak[d]);
+            }
+        }
     }
+    for (; rb <= rend; rb += 3) gen_block(I3{}, rb);
 }
 
 // ============================================================================
@@ -608,9 +989,11 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_rec_step(FwiRecArgs A, c
         active = k < sh.nt;
     }
 
-    __shared__ float lds[FWI_MAXSG][2][72];
+    __shared__ float lds[FWI_MAXSG][2][RB][72];
     __shared__ double red[FWI_MAXSG][TX];
-    float *Ps = lds[wv][0], *Us = lds[wv][1];
+    __shared__ double redb[2][RB][FWI_MAXSG][TX];  // one barrier per interior block
+    float(*Ps)[72] = lds[wv][0];
+    float(*Us)[72] = lds[wv][1];
 
     const int npml = g.npml, nx = g.nx, nz = g.nz, ld = g.ld, sg = A.b.sg;
     const int bx0 = txi * TX, bx = bx0 + lane, fx = bx + npml;
@@ -653,20 +1036,42 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_rec_step(FwiRecArgs A, c
             if (wv == 0 && colok) fg = gacc[(long long)io * nx + bx];
         }
     };
-    auto body = [&](int r, float pr, float ph, float pp_i, float mu_i, float ekp_i, double g_i) {
-        float uxr = 0.f;
-        if (active && r >= z0 && r < z1 && r >= 3 && r < nz - 3) {
-            wave_sync();
-            Ps[lane + 3] = pr;
-            if (lane < 3) Ps[lane] = ph;
-            else if (lane < 6) Ps[lane + 64] = ph;
-            wave_sync();
-            Us[lane] = c0x * (Ps[lane + 2] - Ps[lane + 1]) + c1x * (Ps[lane + 3] - Ps[lane]);
-            if (lane < 3)
-                Us[64 + lane] = c0x * (Ps[lane + 66] - Ps[lane + 65]) + c1x * (Ps[lane + 67] - Ps[lane + 64]);
-            wave_sync();
-            uxr = c0x * (Us[lane + 2] - Us[lane + 1]) + c1x * (Us[lane + 3] - Us[lane]);
+    // x chain of the interior operator for the block's rows in [z0, z1) n [3, nz-3)
+    auto xblock = [&](auto Nc, int rb, const float *bs, const float *bh, float *bu) {
+        constexpr int N = decltype(Nc)::value;
+        wave_sync();
+#pragma unroll
+        for (int d = 0; d < N; ++d) {
+            const int r = rb + d;
+            if (r >= z0 && r < z1 && r >= 3 && r < nz - 3) {
+                Ps[d][lane + 3] = bs[d];
+                if (lane < 3) Ps[d][lane] = bh[d];
+                else if (lane < 6) Ps[d][lane + 64] = bh[d];
+            }
         }
+        wave_sync();
+#pragma unroll
+        for (int d = 0; d < N; ++d) {
+            const int r = rb + d;
+            if (r >= z0 && r < z1 && r >= 3 && r < nz - 3) {
+                const float *P = Ps[d];
+                Us[d][lane] = c0x * (P[lane + 2] - P[lane + 1]) + c1x * (P[lane + 3] - P[lane]);
+                if (lane < 3)
+                    Us[d][64 + lane] = c0x * (P[lane + 66] - P[lane + 65]) + c1x * (P[lane + 67] - P[lane + 64]);
+            }
+        }
+        wave_sync();
+#pragma unroll
+        for (int d = 0; d < N; ++d) {
+            const int r = rb + d;
+            bu[d] = 0.f;
+            if (r >= z0 && r < z1 && r >= 3 && r < nz - 3) {
+                const float *U = Us[d];
+                bu[d] = c0x * (U[lane + 2] - U[lane + 1]) + c1x * (U[lane + 3] - U[lane]);
+            }
+        }
+    };
+    auto body = [&](int r, float pr, float uxr, float pp_i, float mu_i, float ekp_i, double g_i) {
         p0w = p1w; p1w = p2w; p2w = p3w; p3w = pr;
         q0 = q1; q1 = q2; q2 = q3; q3 = uxr;
         const float vj = c0z * (p2w - p1w) + c1z * (p3w - p0w);  // v[r-1]; used only where all taps are inside
@@ -706,15 +1111,110 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_rec_step(FwiRecArgs A, c
             __syncthreads();
         }
     };
-    for (int rb = z0 - 3; rb <= rend; rb += RB) {
-        float bs[RB], bh[RB], bp[RB], bm[RB], bk[RB];
-        double bg[RB];
+    auto gen_block = [&](auto Nc, int rb) {
+        constexpr int N = decltype(Nc)::value;
+        float bs[N], bh[N], bp[N], bm[N], bk[N], bu[N];
+        double bg[N];
 #pragma unroll
-        for (int d = 0; d < RB; ++d) fetch(rb + d, bs[d], bh[d], bp[d], bm[d], bk[d], bg[d]);
+        for (int d = 0; d < N; ++d) fetch(rb + d, bs[d], bh[d], bp[d], bm[d], bk[d], bg[d]);
+        if (active && rb + N - 1 >= z0 && rb < z1) xblock(Nc, rb, bs, bh, bu);
+        else {
 #pragma unroll
-        for (int d = 0; d < RB; ++d)
-            if (rb + d <= rend) body(rb + d, bs[d], bh[d], bp[d], bm[d], bk[d], bg[d]);
+            for (int d = 0; d < N; ++d) bu[d] = 0.f;
+        }
+#pragma unroll
+        for (int d = 0; d < N; ++d)
+            if (rb + d <= rend) body(rb + d, bs[d], bu[d], bp[d], bm[d], bk[d], bg[d]);
+    };
+    // interior blocks: the decision depends on the tile and rows only (the
+    // same for every wave of the workgroup, which meets at one barrier)
+    const bool fastb = bx0 >= 3 && bx0 + 63 < nx - 3;
+    int blk = 0;
+    auto run_block = [&](int rb, const float *bs, const float *bh, const float *bp, const float *bm,
+                         const float *bk, const double *bg) {
+        const int ilo = rb - 3, ihi = rb + RB - 4;
+        if (fastb && ilo >= 3 && ihi < nz - 3) {
+            float bu[RB];
+            if (active) xblock(std::integral_constant<int, RB>{}, rb, bs, bh, bu);
+            double(*rd)[FWI_MAXSG][TX] = redb[blk & 1];
+#pragma unroll
+            for (int d = 0; d < RB; ++d) {
+                p0w = p1w; p1w = p2w; p2w = p3w; p3w = bs[d];
+                q0 = q1; q1 = q2; q2 = q3; q3 = active ? bu[d] : 0.f;
+                const float vj = c0z * (p2w - p1w) + c1z * (p3w - p0w);
+                v0w = v1w; v1w = v2w; v2w = v3w; v3w = vj;
+                const int i = rb + d - 3;
+                double img = 0.0;
+                if (active) {
+                    float ux = q0;
+                    if (i == sh.isz && fx == sh.isx) ux = ux + fsrc;
+                    const float vz = c0z * (v2w - v1w) + c1z * (v3w - v0w);
+                    const float X = ux + vz;
+                    img = (double)bm[d] * (double)X;
+                    if (k >= 1) pp[i * ld + fx] = leapfrog(p0w, bp[d], bk[d] * X);
+                }
+                rd[d][wv][lane] = img;
+            }
+            __syncthreads();
+            if (wv == 0) {
+#pragma unroll
+                for (int d = 0; d < RB; ++d) {
+                    double acc = bg[d];
+                    for (int w = 0; w < sg; ++w) acc += rd[d][w][lane];
+                    gacc[(rb + d - 3) * nx + bx] = acc;
+                }
+            }
+            ++blk;
+        } else {
+            float bu[RB];
+            if (active) xblock(std::integral_constant<int, RB>{}, rb, bs, bh, bu);
+            else {
+#pragma unroll
+                for (int d = 0; d < RB; ++d) bu[d] = 0.f;
+            }
+#pragma unroll
+            for (int d = 0; d < RB; ++d) body(rb + d, bs[d], bu[d], bp[d], bm[d], bk[d], bg[d]);
+        }
+    };
+    using I3 = std::integral_constant<int, 3>;
+    gen_block(I3{}, z0 - 3);
+    gen_block(I3{}, z0);
+    int rb = z0 + 3;
+    const int rmain = z1 - 1;
+    if (rb + RB - 1 <= rmain) {
+        float As[RB], Ah[RB], Ap[RB], Am[RB], Ak[RB], Bs[RB], Bh[RB], Bp[RB], Bm[RB], Bk[RB];
+        double Ag[RB], Bg[RB];
+#pragma unroll
+        for (int d = 0; d < RB; ++d) fetch(rb + d, As[d], Ah[d], Ap[d], Am[d], Ak[d], Ag[d]);
+        for (;;) {
+            const bool nb = rb + 2 * RB - 1 <= rmain;
+            if (nb && FWI_DOUBLEBUF) {
+#pragma unroll
+                for (int d = 0; d < RB; ++d) fetch(rb + RB + d, Bs[d], Bh[d], Bp[d], Bm[d], Bk[d], Bg[d]);
+            }
+            run_block(rb, As, Ah, Ap, Am, Ak, Ag);
+            rb += RB;
+            if (!nb) break;
+            if (!FWI_DOUBLEBUF) {
+#pragma unroll
+                for (int d = 0; d < RB; ++d) fetch(rb + d, Bs[d], Bh[d], Bp[d], Bm[d], Bk[d], Bg[d]);
+            }
+            const bool na = rb + 2 * RB - 1 <= rmain;
+            if (na && FWI_DOUBLEBUF) {
+#pragma unroll
+                for (int d = 0; d < RB; ++d) fetch(rb + RB + d, As[d], Ah[d], Ap[d], Am[d], Ak[d], Ag[d]);
+            }
+            run_block(rb, Bs, Bh, Bp, Bm, Bk, Bg);
+            rb += RB;
+            if (!na) break;
+            if (!FWI_DOUBLEBUF) {
+#pragma unroll
+                for (int d = 0; d < RB; ++d) fetch(rb + d, As[d], Ah[d], Ap[d], Am[d], Ak[d], Ag[d]);
+            }
+        }
     }
+    for (; rb <= rend; rb += 3) gen_block(I3{}, rb);
+    __syncthreads();  // the last interior block's partial sums are read before the waves leave
 }
 
 // ============================================================================
@@ -957,6 +1457,13 @@ extern "C" {
 hipError_t fwi_launch_fwd(const FwiFwdArgs *a, const int *kbase, int koff, int nblocks, hipStream_t st)
 {
     hipLaunchKernelGGL(k_fwd_step, dim3(((nblocks + 7) / 8) * 8), dim3(FWI_TX * a->b.sg), 0, st, *a, kbase, koff);
+    return hipGetLastError();
+}
+
+hipError_t fwi_launch_fwd_tile(const FwiFwdArgs *a, const int *kbase, int koff, hipStream_t st)
+{
+    const int n = a->g.ntx * a->g.ntz2 * a->b.nshots;
+    hipLaunchKernelGGL(k_fwd_tile, dim3(((n + 7) / 8) * 8), dim3(256), 0, st, *a, kbase, koff);
     return hipGetLastError();
 }
 

@@ -10,6 +10,9 @@ import numpy as np
 
 PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.environ.get("FWI_HIP_LIBRARY") or os.path.join(PKG, "libfwi_hip.so")
+# the same library linked against PyTorch's bundled HIP runtime: chosen when
+# torch is already imported, so that the process holds one HIP runtime
+LIB_TORCH_PATH = os.path.join(PKG, "libfwi_hip_torch.so")
 
 OK, EINVAL, ENOMEM, EHIP, ESTATE = 0, 1, 2, 3, 4
 
@@ -22,7 +25,7 @@ EXPORTS = (
     "fwi_hip_set_residual", "fwi_hip_adjoint", "fwi_hip_synchronize", "fwi_hip_set_timing",
     "fwi_hip_kernel_time", "fwi_hip_read_state", "fwi_hip_op_pml", "fwi_hip_op_interior",
     "fwi_hip_time_integral", "fwi_hip_write_bdr", "fwi_hip_read_bdr", "fwi_hip_cpml_coe",
-    "fwi_hip_set_graph", "fwi_hip_device_count", "fwi_hip_dmalloc", "fwi_hip_dfree", "fwi_hip_memcpy",
+    "fwi_hip_set_graph", "fwi_hip_set_kernels", "fwi_hip_device_count", "fwi_hip_dmalloc", "fwi_hip_dfree", "fwi_hip_memcpy",
 )
 
 
@@ -56,10 +59,14 @@ def load():
     """Load the library (raises if it is missing: the HIP path is the product)."""
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise FileNotFoundError(f"{LIB_PATH} is not built: run __graft_entry__.build() "
+        import sys
+        path = LIB_PATH
+        if "torch" in sys.modules and not os.environ.get("FWI_HIP_LIBRARY"):
+            path = LIB_TORCH_PATH
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} is not built: run __graft_entry__.build() "
                                     "or python fwi2dacoustic-mpi-lbfgs_amd/build.py")
-        lib = ctypes.CDLL(LIB_PATH)
+        lib = ctypes.CDLL(path)
         for name in EXPORTS:
             fn = getattr(lib, name)
             fn.restype = ctypes.c_int
@@ -171,6 +178,9 @@ class Context:
 
     def synchronize(self):
         self._chk(self.lib.fwi_hip_synchronize(self.h))
+
+    def set_kernels(self, fwd_kind):
+        self._chk(self.lib.fwi_hip_set_kernels(self.h, int(fwd_kind)))
 
     def set_graph(self, steps):
         self._chk(self.lib.fwi_hip_set_graph(self.h, int(steps)))

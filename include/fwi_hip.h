@@ -103,6 +103,10 @@ int fwi_hip_adjoint(fwi_hip_ctx *ctx, double *grad, double *gk);
  * counter, so one executable graph is replayed for every block.           */
 int fwi_hip_set_graph(fwi_hip_ctx *ctx, int steps);
 
+/* Forward kernel form: 0 = 2-D tile kernel (default), 1 = z-marching
+ * kernel.  Both compute the same values bit for bit.                      */
+int fwi_hip_set_kernels(fwi_hip_ctx *ctx, int fwd_kind);
+
 /* Wait for all work of the context. */
 int fwi_hip_synchronize(fwi_hip_ctx *ctx);
 

@@ -142,11 +142,13 @@ def test_forward_batched_uneven(lib, o32, o64):
 
 
 def test_forward_tiles_agree(lib):
-    """Different tile heights and shot groups give identical traces."""
+    """Different tile heights, shot groups and the two forward kernel forms
+    (z-marching, 2-D tile) give identical traces."""
     g, vel, shots = small_case(nshots=3, nt=200, seed=3)
     outs = []
-    for tz, sg in ((8, 1), (13, 2), (40, 4), (0, 0)):
+    for tz, sg, kind in ((8, 1, 1), (13, 2, 1), (40, 4, 1), (0, 0, 1), (0, 0, 0)):
         ctx = lib.Context(g, max_shots=3, max_nt=200, max_nr=100, tile_z=tz, shots_per_group=sg)
+        ctx.set_kernels(kind)
         ctx.set_model(vel)
         ctx.set_shots(shots)
         outs.append(ctx.forward())
