@@ -9,7 +9,9 @@ gradient copied back.  Synthetic seeded layered/faulted model.
 
 Prints ONE JSON line (rank 0):
   value       shot-gradients/s (whole job)
-  roofline    dominant kernel's algorithmic bytes / HIP-event duration vs 8 TB/s
+  roofline    dominant kernel's algorithmic bytes (the minimum HBM traffic of
+              one launch, see algorithmic_bytes) / its HIP-event duration,
+              against the 8 TB/s HBM3E peak
   cpu_baseline  the FP32 OpenMP oracle (a restatement of the reference CPU
               path) on a bounded sample of the same workload, on this host
 """
@@ -24,7 +26,6 @@ PKG = os.path.join(ROOT, "fwi2dacoustic-mpi-lbfgs_amd")
 sys.path.insert(0, PKG)
 
 HBM_PEAK = 8.0e12          # MI355X datasheet HBM3E bandwidth, B/s
-BYTES_PER_CELL_UPDATE = 16  # read p^n, p^{n-1}, ekp; write p^{n+1} (FP32)
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "traffic_1024x320_64shots.json")
 
 
@@ -36,13 +37,36 @@ def parse():
     ap.add_argument("--shots", type=int, default=64)
     ap.add_argument("--nt", type=int, default=4000)
     ap.add_argument("--tile-z", type=int, default=0)
-    ap.add_argument("--group", type=int, default=0)
+    ap.add_argument("--group", type=int, default=0, help="shots per workgroup of the z-marching kernels")
+    ap.add_argument("--kernels", type=int, default=0, help="0: 2-D tile kernels, 1: z-marching kernels")
+    ap.add_argument("--rec-group", type=int, default=0, help="shots per workgroup of the tile imaging kernel")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-single", action="store_true", help="skip the one-shot-per-launch comparison")
     ap.add_argument("--cpu-nt", type=int, default=1500, help="time steps of the CPU sample shot")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (HBM counter calibration for rocprofv3 --pmc)")
     return ap.parse_args()
+
+
+def algorithmic_bytes(g, nshots, nr):
+    """Minimum HBM bytes of one launch of each step kernel (FP32 fields):
+    forward   per propagation-grid cell and shot: read p^n, p^{n-1}, write
+              p^{n+1} (12 B); ekp once per cell (shared by the shots); CPML
+              memory read + write (8 B per live node); frame strips p and X
+              written (8 B per frame cell); one trace sample per receiver;
+    adjoint   the same for mu, without frames (the residual is read instead);
+    imaging   per physical cell and shot: read p^k, p^{k+1}, mu^{k+1}, write
+              p^{k-1} (16 B); ekp once; frames read (8 B per frame cell); the
+              FP64 image accumulator read + written once (16 B per cell)."""
+    f = g.pnx * g.pnz
+    b = g.nx * g.nz
+    n1 = g.npml + 1
+    cpml = (2 * n1 + 2 * g.npml) * g.pnz + (n1 + g.npml) * g.pnx
+    frame = 6 * g.nx + 6 * g.nz
+    fwd = nshots * (12 * f + 8 * cpml + 8 * frame + 4 * nr) + 4 * f
+    adj = nshots * (12 * f + 8 * cpml + 4 * nr) + 4 * f
+    rec = nshots * (16 * b + 8 * frame) + 4 * b + 16 * b
+    return fwd, adj, rec
 
 
 def cpu_baseline(g, vel, shot, nt_sample):
@@ -91,6 +115,9 @@ def main():
     nr = max(s.nr for s in shots)
     ctx = lib.Context(g, max_shots=len(shots), max_nt=args.nt, max_nr=nr, tile_z=args.tile_z,
                       shots_per_group=args.group)
+    ctx.set_kernels(args.kernels)
+    if args.rec_group:
+        ctx.set_rec_group(args.rec_group)
     ctx.set_model(vel)
     ctx.set_shots(shots)
     obs = ctx.forward()
@@ -121,14 +148,14 @@ def main():
     ctx.set_timing(True)
     step()
     kern = {}
-    names = ("k_fwd_step", "k_adj_step", "k_rec_step")
-    cells_launch = (g.pnx * g.pnz * nshots, g.pnx * g.pnz * nshots, g.nx * g.nz * nshots)
+    names = (("k_fwd_tile", "k_adj_tile", "k_rec_tile") if args.kernels == 0
+             else ("k_fwd_step", "k_adj_step", "k_rec_step"))
+    alg = algorithmic_bytes(g, nshots, nr)
     for w, name in enumerate(names):
         ms, n = ctx.kernel_time(w)
         avg = ms / max(n, 1) * 1e-3
-        byts = BYTES_PER_CELL_UPDATE * cells_launch[w]
-        kern[name] = dict(launches=n, avg_us=avg * 1e6, algorithmic_bytes=byts,
-                          achieved_GBps=byts / avg / 1e9 if avg > 0 else None)
+        kern[name] = dict(launches=n, avg_us=avg * 1e6, algorithmic_bytes=alg[w],
+                          achieved_GBps=alg[w] / avg / 1e9 if avg > 0 else None)
     ctx.set_timing(False)
     dom = max(names, key=lambda k: kern[k]["avg_us"] * kern[k]["launches"])
     traffic = None
@@ -145,6 +172,7 @@ def main():
         # the plain one-shot-per-launch GPU form, on the first 4 shots
         sub = shots[:4]
         c1 = lib.Context(g, max_shots=1, max_nt=args.nt, max_nr=nr, tile_z=args.tile_z)
+        c1.set_kernels(args.kernels)
         c1.set_model(vel)
         t1 = 0.0
         for s in sub:
@@ -188,7 +216,7 @@ def main():
                                f"nt={args.nt}, {nshots} shots x {nr} receivers, npml={g.npml}, free surface",
                    "nx": g.nx, "nz": g.nz, "nt": args.nt, "nshots": nshots, "nr": nr, "npml": g.npml},
         "gcell_updates_per_s": gcell,
-        "hbm_frac_whole_job": gcell * 1e9 * BYTES_PER_CELL_UPDATE / HBM_PEAK,
+        "hbm_frac_whole_job": sum(alg) * args.nt * args.steps / wall / HBM_PEAK,
         "misfit": J,
         "roofline": roof,
         "kernels": kern,

@@ -19,8 +19,13 @@
 
 extern "C" {
 hipError_t fwi_launch_fwd(const FwiFwdArgs *a, const int *kbase, int koff, int nblocks, hipStream_t st);
-hipError_t fwi_launch_fwd_tile(const FwiFwdArgs *a, const int *kbase, int koff, hipStream_t st);
+hipError_t fwi_launch_fwd_tile(const FwiFwdArgs *a, const FwiFwdArgs *a_dev, const int *kbase, int koff,
+                               hipStream_t st);
 hipError_t fwi_launch_adj(const FwiAdjArgs *a, const int *kbase, int koff, int nblocks, hipStream_t st);
+hipError_t fwi_launch_adj_tile(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, const int *kbase, int koff,
+                               hipStream_t st);
+hipError_t fwi_launch_rec_tile(const FwiRecArgs *a, const FwiRecArgs *a_dev, const int *kbase, int koff,
+                               int gsize, int ngroups, hipStream_t st);
 hipError_t fwi_launch_rec(const FwiRecArgs *a, const int *kbase, int koff, int nblocks, hipStream_t st);
 hipError_t fwi_launch_advance(int *kbase, int delta, hipStream_t st);
 hipError_t fwi_launch_set_int(int *p, int v, hipStream_t st);
@@ -79,8 +84,16 @@ struct fwi_hip_ctx {
     bool have_fwd = false, have_obs = false, have_res = false;
     // captured blocks of time steps
     int *kbase = nullptr;
+    // parameter blocks of the step kernels that read them from device memory
+    FwiFwdArgs fwd_args_h;
+    FwiFwdArgs *fwd_args_d = nullptr;
+    FwiAdjArgs adj_args_h;
+    FwiAdjArgs *adj_args_d = nullptr;
+    FwiRecArgs rec_args_h;
+    FwiRecArgs *rec_args_d = nullptr;
     int graph_steps = 100;
-    int fwd_kind = 0;  // 0: 2-D tile kernel, 1: z-marching kernel
+    int fwd_kind = 0;  // 0: 2-D tile kernels, 1: z-marching kernels
+    int rec_group = 8; // shots per workgroup of the 2-D tile reconstruction kernel
     hipGraphExec_t gx_fwd = nullptr, gx_bwd = nullptr;
     // timing
     bool timing = false;
@@ -284,7 +297,7 @@ extern "C" int fwi_hip_create(const fwi_hip_desc *desc, fwi_hip_ctx **out)
     c->xd_b = (long long)g.pnz * 2 * g.npml;
     c->zh_b = (long long)n1 * g.ld;
     c->zd_b = (long long)g.npml * g.ld;
-    const int maxgroups = (int)((S + c->sg - 1) / c->sg);
+    const int maxgroups = (int)S;  // image accumulators: at most one group per shot, whatever the grouping
     struct A { void **p; size_t bytes; } plan[] = {
         {(void **)&c->ekp, sizeof(float) * c->fs},
         {(void **)&c->vel, sizeof(float) * gr.nx * gr.nz},
@@ -311,6 +324,9 @@ extern "C" int fwi_hip_create(const fwi_hip_desc *desc, fwi_hip_ctx **out)
         {(void **)&c->shots_d, sizeof(FwiShotDev) * S},
         {(void **)&c->rrow, sizeof(int) * S * g.pnz},
         {(void **)&c->kbase, sizeof(int) * 64},
+        {(void **)&c->fwd_args_d, sizeof(FwiFwdArgs)},
+        {(void **)&c->adj_args_d, sizeof(FwiAdjArgs)},
+        {(void **)&c->rec_args_d, sizeof(FwiRecArgs)},
     };
     for (auto &a : plan) {
         rc = dalloc(c, a.p, a.bytes);
@@ -563,8 +579,18 @@ extern "C" int fwi_hip_set_kernels(fwi_hip_ctx *c, int fwd_kind)
 {
     if (!c || fwd_kind < 0 || fwd_kind > 1) return set_err(c, FWI_HIP_EINVAL, "set_kernels: bad argument");
     if (c->gx_fwd) (void)hipGraphExecDestroy(c->gx_fwd);
-    c->gx_fwd = nullptr;
+    if (c->gx_bwd) (void)hipGraphExecDestroy(c->gx_bwd);
+    c->gx_fwd = c->gx_bwd = nullptr;
     c->fwd_kind = fwd_kind;
+    return FWI_HIP_OK;
+}
+
+extern "C" int fwi_hip_set_rec_group(fwi_hip_ctx *c, int shots)
+{
+    if (!c || shots < 1) return set_err(c, FWI_HIP_EINVAL, "set_rec_group: need shots >= 1");
+    if (c->gx_bwd) (void)hipGraphExecDestroy(c->gx_bwd);
+    c->gx_bwd = nullptr;
+    c->rec_group = shots;
     return FWI_HIP_OK;
 }
 
@@ -601,6 +627,9 @@ extern "C" int fwi_hip_forward(fwi_hip_ctx *c, float *traces)
     a.syn = c->syn_tm;
     a.pstrip = c->pstrip;
     a.xstrip = c->xstrip;
+    c->fwd_args_h = a;  // host copy stays alive while the stream reads it
+    HIPCHK(c, hipMemcpyAsync(c->fwd_args_d, &c->fwd_args_h, sizeof(FwiFwdArgs), hipMemcpyHostToDevice, c->st));
+    const FwiFwdArgs *ad = c->fwd_args_d;
     const int nblocks = g.ntx * g.ntz * c->ngroups;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
     const int B = c->graph_steps;
@@ -610,7 +639,7 @@ extern "C" int fwi_hip_forward(fwi_hip_ctx *c, float *traces)
                 hipError_t e = hipSuccess;
                 for (int j = 0; j < B && e == hipSuccess; j++)
                     e = c->fwd_kind ? fwi_launch_fwd(&a, c->kbase, j, nblocks, c->st)
-                                    : fwi_launch_fwd_tile(&a, c->kbase, j, c->st);
+                                    : fwi_launch_fwd_tile(&a, ad, c->kbase, j, c->st);
                 if (e == hipSuccess) e = fwi_launch_advance(c->kbase, B, c->st);
                 return e;
             });
@@ -623,7 +652,7 @@ extern "C" int fwi_hip_forward(fwi_hip_ctx *c, float *traces)
             hipEvent_t e0, e1;
             if ((rc = tick(c, &e0))) return rc;
             HIPCHK(c, c->fwd_kind ? fwi_launch_fwd(&a, nullptr, k, nblocks, c->st)
-                                  : fwi_launch_fwd_tile(&a, nullptr, k, c->st));
+                                  : fwi_launch_fwd_tile(&a, ad, nullptr, k, c->st));
             if ((rc = tick(c, &e1))) return rc;
             if (c->timing) marks.push_back({0, {e0, e1}});
             if (marks.size() >= 512 && (rc = collect_timing(c, marks))) return rc;
@@ -686,9 +715,12 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
         return set_err(c, FWI_HIP_ESTATE, "adjoint: forward and a residual (misfit_l2 or set_residual) first");
     const FwiGeomDev &g = c->g;
     const long long S = c->nshots, nb = (long long)g.nx * g.nz;
+    const bool tile = c->fwd_kind == 0;
+    const int rgs = std::min(c->rec_group, c->nshots);        // tile path: shots per workgroup
+    const int ngr = tile ? (c->nshots + rgs - 1) / rgs : c->ngroups;  // image accumulators in use
     HIPCHK(c, fwi_launch_fill_f32(c->M0, S * c->fs, 0.f, c->st));
     HIPCHK(c, fwi_launch_fill_f32(c->M1, S * c->fs, 0.f, c->st));
-    HIPCHK(c, fwi_launch_fill_f64(c->gacc, (long long)c->ngroups * nb, 0.0, c->st));
+    HIPCHK(c, fwi_launch_fill_f64(c->gacc, (long long)ngr * nb, 0.0, c->st));
     int rc = zero_cpml(c);
     if (rc) return rc;
     for (int s = 0; s < c->nshots; s++) {
@@ -718,8 +750,21 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
     r.pstrip = c->pstrip;
     r.xstrip = c->xstrip;
     r.gacc = c->gacc;
+    c->adj_args_h = a;
+    c->rec_args_h = r;
+    HIPCHK(c, hipMemcpyAsync(c->adj_args_d, &c->adj_args_h, sizeof(FwiAdjArgs), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->rec_args_d, &c->rec_args_h, sizeof(FwiRecArgs), hipMemcpyHostToDevice, c->st));
+    const FwiAdjArgs *ad = c->adj_args_d;
+    const FwiRecArgs *rd = c->rec_args_d;
     const int nb_adj = g.ntx * g.ntz * c->ngroups;
     const int nb_rec = g.bntx * g.bntz * c->ngroups;
+    auto launch_adj = [&](const int *kb, int koff) {
+        return tile ? fwi_launch_adj_tile(&a, ad, kb, koff, c->st) : fwi_launch_adj(&a, kb, koff, nb_adj, c->st);
+    };
+    auto launch_rec = [&](const int *kb, int koff) {
+        return tile ? fwi_launch_rec_tile(&r, rd, kb, koff, rgs, ngr, c->st)
+                    : fwi_launch_rec(&r, kb, koff, nb_rec, c->st);
+    };
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
     const int B = c->graph_steps;
     if (B > 0 && !c->timing) {
@@ -727,8 +772,8 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
             rc = capture(c, &c->gx_bwd, [&]() {
                 hipError_t e = hipSuccess;
                 for (int j = 0; j < B && e == hipSuccess; j++) {
-                    e = fwi_launch_adj(&a, c->kbase, -j, nb_adj, c->st);
-                    if (e == hipSuccess) e = fwi_launch_rec(&r, c->kbase, -j, nb_rec, c->st);
+                    e = launch_adj(c->kbase, -j);
+                    if (e == hipSuccess) e = launch_rec(c->kbase, -j);
                 }
                 if (e == hipSuccess) e = fwi_launch_advance(c->kbase, -B, c->st);
                 return e;
@@ -741,9 +786,9 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
         for (int k = c->kmax - 1; k >= 0; k--) {
             hipEvent_t e0, e1, e2;
             if ((rc = tick(c, &e0))) return rc;
-            HIPCHK(c, fwi_launch_adj(&a, nullptr, k, nb_adj, c->st));
+            HIPCHK(c, launch_adj(nullptr, k));
             if ((rc = tick(c, &e1))) return rc;
-            HIPCHK(c, fwi_launch_rec(&r, nullptr, k, nb_rec, c->st));
+            HIPCHK(c, launch_rec(nullptr, k));
             if ((rc = tick(c, &e2))) return rc;
             if (c->timing) {
                 marks.push_back({1, {e0, e1}});
@@ -753,7 +798,7 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
         }
     }
     if ((rc = collect_timing(c, marks))) return rc;
-    HIPCHK(c, fwi_launch_grad_finalize(c->gacc, c->ngroups, nb, c->vel, c->gk, c->grad, c->st));
+    HIPCHK(c, fwi_launch_grad_finalize(c->gacc, ngr, nb, c->vel, c->gk, c->grad, c->st));
     if (grad) HIPCHK(c, hipMemcpyAsync(grad, c->grad, sizeof(double) * nb, hipMemcpyDeviceToHost, c->st));
     if (gk) HIPCHK(c, hipMemcpyAsync(gk, c->gk, sizeof(double) * nb, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));

@@ -47,6 +47,25 @@ __device__ __forceinline__ int xcd_remap(int b, int n)
     return l < n ? l : -1;
 }
 
+// Raw buffer access: a 128-bit resource (base, size in bytes) in SGPRs, a
+// per-lane byte offset (the column, fixed for a wave) and a uniform byte
+// offset (the row), so a row costs one scalar add instead of a 64-bit vector
+// address.  Every access stays inside [0, bytes) by construction.
+typedef __amdgpu_buffer_rsrc_t brsrc;
+#define FWI_BUF_WORD3 0x00020000  // raw 32-bit dwords, gfx9-family resource format
+__device__ __forceinline__ brsrc buf_rsrc(const void *base, unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, FWI_BUF_WORD3);
+}
+__device__ __forceinline__ float buf_ld(brsrc r, int voff, int soff)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ void buf_st(float v, brsrc r, int voff, int soff)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
+}
+
 // frame position of physical cell (bz, bx) inside one step of the strips
 __device__ __forceinline__ int frame_pos(int bz, int bx, int nx, int nz)
 {
@@ -433,67 +452,83 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_fwd_step(FwiFwdArgs A, c
 
 // ============================================================================
 // forward, 2-D tile form: one workgroup (4 waves) per 64 x FWI_TZ2 tile of one
-// shot.  All loads of the tile are issued at once; p (with a 3-cell halo) is
-// staged in LDS, the half-node fields u and v are formed in LDS, then each
-// wave finishes 8 output rows.  Same arithmetic and CPML ownership rules as
-// k_fwd_step.
+// shot; wave w owns the FR contiguous output rows r0 = z0 + w FR ...  A wave
+// loads the FR + 6 rows of p its rows depend on (3-row halo above and below)
+// into registers, so v (z-derivative of p) and vz (z-derivative of v) never
+// leave registers; only the x chain (u, then ux) goes through LDS, on rows
+// private to the wave (wave-level sync only).  Every load of a wave is issued
+// before any arithmetic.  Tiles that touch no CPML strip, free-surface row,
+// frame strip, source or receiver row run the interior body, which has no
+// lane-dependent branch; the others run the general body.  Same arithmetic
+// and CPML ownership rules as k_fwd_step.
 // ============================================================================
-#define TR2 (FWI_TZ2 / 4)          // output rows per wave
-#define PR2 (FWI_TZ2 + 6)          // staged p rows
-#define VR2 (FWI_TZ2 + 3)          // v-node rows
-#define PM2 ((PR2 + 3) / 4)        // staged p rows per wave
-#define VM2 ((VR2 + 3) / 4)        // v-node rows per wave
+#define FR 8              // output rows per wave
+#define FPR (FR + 6)      // p rows per wave: r0-3 .. r0+FR+2
+#define FVN (FR + 3)      // v nodes per wave: r0-1 .. r0+FR+1
+#define HC 72             // LDS row: 3 halo | 64 | 3 halo | 2 spare
+#define HJ 71             // spare column that takes the stores of idle lanes
+static_assert(FWI_TZ2 == 4 * FR, "a tile is 4 waves of FR rows");
+static_assert(6 * FR <= 64, "one halo load per wave");
 
-// v at half node j (between p[j-1] and p[j]) from staged rows; P row of p[z] is z - z0 + 3
-__device__ __forceinline__ float fwd_v_node(const FwiGeomDev &g, const float *tab, const float (*P)[72], int j,
-                                            int z0, int c, int x, const float *zh_o, float *zh_n, bool own)
+// true when no cell of the tile (64 x FWI_TZ2 at x0, z0), nor any u/v node it
+// evaluates, needs a CPML, edge, free-surface, frame, source or receiver rule
+__device__ __forceinline__ bool fwd_tile_interior(const FwiGeomDev &g, const FwiShotDev &sh, int x0, int z0)
 {
-    const int pnz = g.pnz, npml = g.npml, T = 2 * npml + 1, ld = g.ld;
-    if (j < 0 || j >= pnz) return 0.f;
-    const int b = j - z0 + 3;  // P row of p[j]
-    float v;
-    if (j == 0) v = v_row0(P[b][c], P[b + 1][c], P[b + 2][c], P[b + 3][c], g.iz840);
-    else if (j == 1) v = v_row1(P[b - 1][c], P[b][c], P[b + 1][c], P[b + 2][c], g.iz840);
-    else if (j == pnz - 1) v = (P[b][c] - P[b - 1][c]) * g.iz;
-    else v = g.c0z * (P[b][c] - P[b - 1][c]) + g.c1z * (P[b + 1][c] - P[b - 2][c]);
-    if (j >= pnz - npml) {
-        const int q = j - (pnz - npml);
-        float ps = zh_o[q * ld + x];
-        ps = TB(5, 2 * q) * ps + TB(4, 2 * q) * v;
-        v = v + ps;
-        if (own) zh_n[q * ld + x] = ps;
-    }
-    return v;
+    const int z1 = z0 + FWI_TZ2;
+    if (x0 < g.npml + 3 || x0 + 66 > g.pnx - g.npml - 1) return false;  // u nodes x0-1 .. x0+65
+    if (z0 < 3 || z1 > g.nz - 3) return false;                           // v nodes z0-1 .. z1+1
+    if (sh.isz >= z0 && sh.isz < z1 && sh.isx >= x0 && sh.isx < x0 + TX) return false;
+    if (sh.rz_lo <= sh.rz_hi && sh.rz_lo < z1 && sh.rz_hi >= z0) return false;
+    return true;
 }
 
-__global__ __launch_bounds__(256) void k_fwd_tile(FwiFwdArgs A, const int *kbase, int koff)
+template <bool EDGE>
+__device__ __forceinline__ void fwd_tile_rows(const FwiFwdArgs &A, const FwiShotDev &sh, const float *tab,
+                                              float (*Ps)[HC], float (*Us)[HC], int s, int k, int x0, int r0,
+                                              int lane)
 {
-    const int k = (kbase ? *kbase : 0) + koff;
-    if (k < 0) return;
     const FwiGeomDev &g = A.g;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int ntiles = g.ntx * g.ntz2;
-    const int lb = xcd_remap(blockIdx.x, ntiles * A.b.nshots);
-    if (lb < 0) return;
-    const int s = lb / ntiles, t = lb - s * ntiles;
-    const int tzi = t / g.ntx, txi = t - tzi * g.ntx;
-    __shared__ float stab[FWI_TABN];
-    __shared__ float P[PR2][72];
-    __shared__ float U[FWI_TZ2][72];
-    __shared__ float V[VR2][64];
-    stage_tables(stab, A.b.tab, g.npml);
-    const FwiShotDev sh = A.b.shots[s];
-    if (k >= sh.nt) return;  // the whole workgroup belongs to shot s
-
     const int npml = g.npml, pnx = g.pnx, pnz = g.pnz, ld = g.ld, T = 2 * npml + 1;
-    const float *tab = stab;
-    const int x0 = txi * TX, x = x0 + lane;
-    const int z0 = tzi * FWI_TZ2, z1 = min(z0 + FWI_TZ2, pnz);
+    const int x = x0 + lane;
     const long long so = (long long)s * A.b.fs;
     const int cur = k & 1, nxt = cur ^ 1;
     const float *pnow = (cur ? A.P1 : A.P0) + so;
     float *pold = (cur ? A.P0 : A.P1) + so;
-    const float *ekp = A.b.ekp;
+    const unsigned fbytes = (unsigned)(A.b.fs * sizeof(float));
+    const brsrc rnow = buf_rsrc(pnow, fbytes), rold = buf_rsrc(pold, fbytes), rk = buf_rsrc(A.b.ekp, fbytes);
+    const int vo = x * 4, rowb = ld * 4;  // byte offsets: column (per lane), row pitch
+
+    // ---- every load of the wave
+    float p[FPR], po[FR], pk[FR];
+#pragma unroll
+    for (int m = 0; m < FPR; ++m) {
+        const int zr = r0 - 3 + m;
+        p[m] = (!EDGE || (zr >= 0 && zr < pnz)) ? buf_ld(rnow, vo, zr * rowb) : 0.f;
+    }
+    // x halo of the FR own rows in one load: lane = 6 hd + hc, hc < 3 left, else right
+    const int hd = lane / 6, hc = lane - 6 * hd;
+    const int hrow = r0 + hd, hcol = (hc < 3) ? x0 - 3 + hc : x0 + 61 + hc;
+    bool hok = lane < 6 * FR;
+    if (EDGE) hok = hok && hrow < pnz && hcol >= 0 && hcol < pnx;
+    float hv = buf_ld(rnow, hok ? (hd * ld + hcol) * 4 : vo, r0 * rowb);
+    hv = hok ? hv : 0.f;
+#pragma unroll
+    for (int d = 0; d < FR; ++d) {
+        const int i = r0 + d;
+        const bool ok = !EDGE || i < pnz;
+        po[d] = ok ? buf_ld(rold, vo, i * rowb) : 0.f;
+        pk[d] = ok ? buf_ld(rk, vo, i * rowb) : 0.f;
+    }
+
+    // ---- stage the own rows (with halo) for the x chain
+#pragma unroll
+    for (int d = 0; d < FR; ++d) Ps[d][lane + 3] = p[d + 3];
+    {
+        const bool hl = lane < 6 * FR;
+        Ps[hl ? hd : 0][hl ? ((hc < 3) ? hc : hc + 64) : HJ] = hv;
+    }
+    wave_sync();
+
     const float *xh_o = A.xh + s * A.b.xh_s + cur * A.b.xh_b;
     float *xh_n = A.xh + s * A.b.xh_s + nxt * A.b.xh_b;
     const float *xd_o = A.xd + s * A.b.xd_s + cur * A.b.xd_b;
@@ -502,94 +537,92 @@ __global__ __launch_bounds__(256) void k_fwd_tile(FwiFwdArgs A, const int *kbase
     float *zh_n = A.zh + s * A.b.zh_s + nxt * A.b.zh_b;
     const float *zd_o = A.zd + s * A.b.zd_s + cur * A.b.zd_b;
     float *zd_n = A.zd + s * A.b.zd_s + nxt * A.b.zd_b;
-    const bool fastx = (x0 >= npml + 3) && (x0 + 66 <= pnx - npml - 1);
-    const int hq = halo_col(x0, lane);
-    const int ibase = z0 + w * TR2;  // first output row of this wave
+    // u nodes x0-1 .. x0+65 and p nodes x0 .. x0+63 clear of the x CPML and edges
+    const bool fastx = !EDGE || ((x0 >= npml + 3) && (x0 + 66 <= pnx - npml - 1));
 
-    // ---- phase 1: every load of the tile in flight at once
-    float pv[PM2], ph[PM2], po[TR2], pk[TR2];
+    // ---- u nodes x0-1+c, c = 0..66 (LDS column c); c = lane, then c = 64..66 for all rows
 #pragma unroll
-    for (int m = 0; m < PM2; ++m) {
-        const int rr = w + 4 * m, zr = z0 - 3 + rr;
-        pv[m] = ph[m] = 0.f;
-        if (rr < PR2 && zr >= 0 && zr < pnz) {
-            pv[m] = pnow[zr * ld + x];
-            if (hq >= 0 && hq < pnx) ph[m] = pnow[zr * ld + hq];
-        }
+    for (int d = 0; d < FR; ++d) {
+        if (EDGE && r0 + d >= pnz) break;
+        const float *P = Ps[d];
+        if (fastx) Us[d][lane] = g.c0x * (P[lane + 2] - P[lane + 1]) + g.c1x * (P[lane + 3] - P[lane]);
+        else Us[d][lane] = fwd_u_node(g, tab, P, x0 - 1 + lane, lane + 2, r0 + d, xh_o, xh_n, x0);
     }
-#pragma unroll
-    for (int m = 0; m < TR2; ++m) {
-        const int i = ibase + m;
-        po[m] = pk[m] = 0.f;
-        if (i < z1) {
-            po[m] = pold[i * ld + x];
-            pk[m] = ekp[i * ld + x];
-        }
+    {
+        // lane = 3 ud + (uc - 64); idle lanes evaluate node x0+64 of row 0 (owned by
+        // the next tile, so no CPML store) into the spare column
+        const int ud0 = lane / 3;
+        bool uok = lane < 3 * FR;
+        if (EDGE) uok = uok && r0 + ud0 < pnz;
+        const int ud = uok ? ud0 : 0, uc = uok ? 64 + lane - 3 * ud0 : 65;
+        const float *P = Ps[ud];
+        float u;
+        if (fastx) u = g.c0x * (P[uc + 2] - P[uc + 1]) + g.c1x * (P[uc + 3] - P[uc]);
+        else u = fwd_u_node(g, tab, P, x0 - 1 + uc, uc + 2, r0 + ud, xh_o, xh_n, x0);
+        Us[ud][uok ? uc : HJ] = u;
     }
-#pragma unroll
-    for (int m = 0; m < PM2; ++m) {
-        const int rr = w + 4 * m;
-        if (rr < PR2) {
-            P[rr][lane + 3] = pv[m];
-            if (lane < 3) P[rr][lane] = ph[m];
-            else if (lane < 6) P[rr][lane + 64] = ph[m];
-        }
-    }
-    __syncthreads();
+    wave_sync();
 
-    // ---- phase 2: half-node fields u (rows z0..z1-1) and v (nodes z0-1..z0+TZ2+1)
+    // ---- v nodes r0-1+n from the register rows (p[j] is p[n+2])
+    float v[FVN];
 #pragma unroll
-    for (int m = 0; m < TR2; ++m) {
-        const int rr = w + 4 * m, zr = z0 + rr;
-        if (zr < z1) {
-            const float *Pr = P[rr + 3];
-            if (fastx) {
-                U[rr][lane] = g.c0x * (Pr[lane + 2] - Pr[lane + 1]) + g.c1x * (Pr[lane + 3] - Pr[lane]);
-                if (lane < 3)
-                    U[rr][64 + lane] =
-                        g.c0x * (Pr[lane + 66] - Pr[lane + 65]) + g.c1x * (Pr[lane + 67] - Pr[lane + 64]);
-            } else {
-                U[rr][lane] = fwd_u_node(g, tab, Pr, x0 - 1 + lane, lane + 2, zr, xh_o, xh_n, x0);
-                if (lane < 3) U[rr][64 + lane] = fwd_u_node(g, tab, Pr, x0 + 63 + lane, lane + 66, zr, xh_o, xh_n, x0);
+    for (int n = 0; n < FVN; ++n) {
+        const int j = r0 - 1 + n;
+        float vv;
+        if (!EDGE) {
+            vv = g.c0z * (p[n + 2] - p[n + 1]) + g.c1z * (p[n + 3] - p[n]);
+        } else {
+            if (j < 0 || j >= pnz) vv = 0.f;
+            else if (n == 1 && j == 0) vv = v_row0(p[n + 2], p[n + 3], p[n + 4], p[n + 5], g.iz840);
+            else if (n == 2 && j == 1) vv = v_row1(p[n + 1], p[n + 2], p[n + 3], p[n + 4], g.iz840);
+            else if (j == pnz - 1) vv = (p[n + 2] - p[n + 1]) * g.iz;
+            else vv = g.c0z * (p[n + 2] - p[n + 1]) + g.c1z * (p[n + 3] - p[n]);
+            if (j >= pnz - npml && j < pnz) {
+                const int q = j - (pnz - npml);
+                float ps = zh_o[q * ld + x];
+                ps = TB(5, 2 * q) * ps + TB(4, 2 * q) * vv;
+                vv = vv + ps;
+                if (n >= 1 && n <= FR) zh_n[q * ld + x] = ps;  // own nodes r0 .. r0+FR-1
             }
         }
+        v[n] = vv;
     }
-#pragma unroll
-    for (int m = 0; m < VM2; ++m) {
-        const int rr = w + 4 * m;
-        if (rr < VR2) {
-            const int j = z0 - 1 + rr;
-            V[rr][lane] = fwd_v_node(g, tab, P, j, z0, lane + 3, x, zh_o, zh_n, j >= z0 && j < z1);
-        }
-    }
-    __syncthreads();
 
-    // ---- phase 3: output rows
+    // ---- output rows
     const int bx = x - npml, nx = g.nx, nz = g.nz;
-    const bool in_bx = (bx >= 0) && (bx < nx);
 #pragma unroll
-    for (int m = 0; m < TR2; ++m) {
-        const int i = ibase + m;
-        if (i >= z1) break;
-        const int rr = i - z0;
-        float ux;
-        if (fastx) ux = g.c0x * (U[rr][lane + 2] - U[rr][lane + 1]) + g.c1x * (U[rr][lane + 3] - U[rr][lane]);
-        else ux = fwd_ux_node(g, tab, U[rr], x, lane + 1, i, xd_o, xd_n);
-        float vz;
-        if (i == 0) vz = vz_row0(V[1][lane], V[2][lane], V[3][lane], V[4][lane], V[5][lane], g.iz24);
-        else if (i == pnz - 1) vz = (V[rr + 2][lane] - V[rr + 1][lane]) * g.iz;
-        else vz = g.c0z * (V[rr + 2][lane] - V[rr + 1][lane]) + g.c1z * (V[rr + 3][lane] - V[rr][lane]);
-        if (i >= pnz - npml) {
-            const int q = i - (pnz - npml);
-            float ps = zd_o[q * ld + x];
-            ps = TB(5, 2 * q + 1) * ps + TB(4, 2 * q + 1) * vz;
-            vz = vz + ps;
-            zd_n[q * ld + x] = ps;
+    for (int d = 0; d < FR; ++d) {
+        const int i = r0 + d;
+        if (EDGE && i >= pnz) break;
+        const float *U = Us[d];
+        float ux, vz;
+        ux = fastx ? g.c0x * (U[lane + 2] - U[lane + 1]) + g.c1x * (U[lane + 3] - U[lane])
+                   : fwd_ux_node(g, tab, U, x, lane + 1, i, xd_o, xd_n);
+        if (!EDGE) {
+            vz = g.c0z * (v[d + 2] - v[d + 1]) + g.c1z * (v[d + 3] - v[d]);
+        } else {
+            if (d == 0 && i == 0) vz = vz_row0(v[d + 1], v[d + 2], v[d + 3], v[d + 4], v[d + 5], g.iz24);  // nodes 0..4
+            else if (i == pnz - 1) vz = (v[d + 2] - v[d + 1]) * g.iz;
+            else vz = g.c0z * (v[d + 2] - v[d + 1]) + g.c1z * (v[d + 3] - v[d]);
+            if (i >= pnz - npml) {
+                const int q = i - (pnz - npml);
+                float ps = zd_o[q * ld + x];
+                ps = TB(5, 2 * q + 1) * ps + TB(4, 2 * q + 1) * vz;
+                vz = vz + ps;
+                zd_n[q * ld + x] = ps;
+            }
+            if (i == sh.isz) {
+                const float sv = A.b.srcf[sh.src_off + k];
+                ux = (x == sh.isx) ? ux + sv : ux;
+            }
         }
-        if (i == sh.isz && x == sh.isx) ux = ux + A.b.srcf[sh.src_off + k];
         const float X = ux + vz;
-        const float pnew = leapfrog(P[rr + 3][lane + 3], po[m], pk[m] * X);
-        if (x < pnx) pold[i * ld + x] = pnew;
+        const float pnew = leapfrog(p[d + 3], po[d], pk[d] * X);
+        if (!EDGE) {
+            buf_st(pnew, rold, vo, i * rowb);
+            continue;
+        }
+        if (x < pnx) buf_st(pnew, rold, vo, i * rowb);
         if (i >= sh.rz_lo && i <= sh.rz_hi) {
             const int slot = A.b.rrow[sh.rrow_off + i];
             if (slot >= 0) {
@@ -597,7 +630,7 @@ __global__ __launch_bounds__(256) void k_fwd_tile(FwiFwdArgs A, const int *kbase
                 if (ri >= 0) A.syn[sh.trace_off + (long long)k * sh.nr + ri] = pnew;
             }
         }
-        if (in_bx && i < nz && (i < 3 || i >= nz - 3 || bx < 3 || bx >= nx - 3)) {
+        if (bx >= 0 && bx < nx && i < nz && (i < 3 || i >= nz - 3 || bx < 3 || bx >= nx - 3)) {
             const long long so2 = sh.strip_off + (long long)k * g.sl;
             if (i < 3 || i >= nz - 3) {
                 const int pos = (i < 3) ? i * nx + bx : (3 + nz - 1 - i) * nx + bx;
@@ -610,6 +643,36 @@ __global__ __launch_bounds__(256) void k_fwd_tile(FwiFwdArgs A, const int *kbase
                 A.xstrip[so2 + pos] = X;
             }
         }
+    }
+}
+
+// The parameter block lives in device memory (not in kernel arguments, which
+// are all loaded at entry): each body then loads only the fields it uses, and
+// the interior body keeps its scalar registers free of the general body's.
+__global__ __launch_bounds__(256) void k_fwd_tile(const FwiFwdArgs *__restrict__ Ap, const int *kbase, int koff)
+{
+    const int k = (kbase ? *kbase : 0) + koff;
+    if (k < 0) return;
+    const FwiFwdArgs &A = *Ap;
+    const FwiGeomDev &g = A.g;
+    // the wave index is uniform within a wave: keep it (and the rows) in SGPRs
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int ntiles = g.ntx * g.ntz2;
+    const int lb = xcd_remap(blockIdx.x, ntiles * A.b.nshots);
+    if (lb < 0) return;
+    const int s = lb / ntiles, t = lb - s * ntiles;
+    const int tzi = t / g.ntx, txi = t - tzi * g.ntx;
+    const FwiShotDev sh = A.b.shots[s];
+    if (k >= sh.nt) return;  // the whole workgroup belongs to shot s
+    const int x0 = txi * TX, z0 = tzi * FWI_TZ2, r0 = z0 + w * FR;
+    __shared__ float stab[FWI_TABN];
+    __shared__ float Psh[4][FR][HC];
+    __shared__ float Ush[4][FR][HC];
+    if (fwd_tile_interior(g, sh, x0, z0)) {
+        fwd_tile_rows<false>(A, sh, stab, Psh[w], Ush[w], s, k, x0, r0, lane);
+    } else {
+        stage_tables(stab, A.b.tab, g.npml);
+        if (r0 < g.pnz) fwd_tile_rows<true>(A, sh, stab, Psh[w], Ush[w], s, k, x0, r0, lane);
     }
 }
 
@@ -964,6 +1027,236 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_adj_step(FwiAdjArgs A, c
 }
 
 // ============================================================================
+// adjoint, 2-D tile form: the exact transpose of k_fwd_tile's step with the
+// same tiling (4 waves x FR contiguous rows of one shot per workgroup).
+// mu^{k+2} rows r0-3 .. r0+FR+2 sit in registers; the z chain (C_d^T, H_z^T,
+// C_h^T, G_z^T with the transposed free-surface rows) runs in registers, the
+// x chain (C_d^T on the staged row, H_x^T + C_h^T at u nodes, G_x^T) through
+// wave-private LDS rows.  Same arithmetic and CPML ownership as k_adj_step.
+// ============================================================================
+template <bool EDGE>
+__device__ __forceinline__ void adj_tile_rows(const FwiAdjArgs &A, const FwiShotDev &sh, const float *tab,
+                                              float (*Ws)[HC], float (*Us)[HC], int s, int k, int x0, int r0,
+                                              int lane)
+{
+    const FwiGeomDev &g = A.g;
+    const int npml = g.npml, pnx = g.pnx, pnz = g.pnz, ld = g.ld, T = 2 * npml + 1;
+    const int x = x0 + lane;
+    const long long so = (long long)s * A.b.fs;
+    // mu^{k+2} in M[k&1] (read), mu^{k+3} in M[(k+1)&1] overwritten by mu^{k+1}
+    const int cur = k & 1, prv = cur ^ 1;
+    const float *mnow = (cur ? A.M1 : A.M0) + so;
+    float *mold = (cur ? A.M0 : A.M1) + so;
+    const unsigned fbytes = (unsigned)(A.b.fs * sizeof(float));
+    const brsrc rnow = buf_rsrc(mnow, fbytes), rold = buf_rsrc(mold, fbytes), rk = buf_rsrc(A.b.ekp, fbytes);
+    const int vo = x * 4, rowb = ld * 4;
+
+    // ---- every load of the wave
+    float m[FPR], mo[FR], mk[FR];
+#pragma unroll
+    for (int j = 0; j < FPR; ++j) {
+        const int zr = r0 - 3 + j;
+        m[j] = (!EDGE || (zr >= 0 && zr < pnz)) ? buf_ld(rnow, vo, zr * rowb) : 0.f;
+    }
+    const int hd = lane / 6, hc = lane - 6 * hd;
+    const int hrow = r0 + hd, hcol = (hc < 3) ? x0 - 3 + hc : x0 + 61 + hc;
+    bool hok = lane < 6 * FR;
+    if (EDGE) hok = hok && hrow < pnz && hcol >= 0 && hcol < pnx;
+    float hv = buf_ld(rnow, hok ? (hd * ld + hcol) * 4 : vo, r0 * rowb);
+    hv = hok ? hv : 0.f;
+#pragma unroll
+    for (int d = 0; d < FR; ++d) {
+        const int i = r0 + d;
+        const bool ok = !EDGE || i < pnz;
+        mo[d] = ok ? buf_ld(rold, vo, i * rowb) : 0.f;
+        mk[d] = ok ? buf_ld(rk, vo, i * rowb) : 0.f;
+    }
+
+    // chi after step k goes to buffer (k & 1); it was read from buffer ((k+1) & 1)
+    const float *xh_o = A.xh + s * A.b.xh_s + prv * A.b.xh_b;
+    float *xh_n = A.xh + s * A.b.xh_s + cur * A.b.xh_b;
+    const float *xd_o = A.xd + s * A.b.xd_s + prv * A.b.xd_b;
+    float *xd_n = A.xd + s * A.b.xd_s + cur * A.b.xd_b;
+    const float *zh_o = A.zh + s * A.b.zh_s + prv * A.b.zh_b;
+    float *zh_n = A.zh + s * A.b.zh_s + cur * A.b.zh_b;
+    const float *zd_o = A.zd + s * A.b.zd_s + prv * A.b.zd_b;
+    float *zd_n = A.zd + s * A.b.zd_s + cur * A.b.zd_b;
+
+    // columns x0-3 .. x0+66 clear of the x CPML and of the grid edges: compact
+    // x stencils (the form k_adj_step uses on such tiles)
+    const bool fastx = !EDGE || ((x0 >= npml + 3) && (x0 + 66 <= pnx - npml - 1));
+    // ---- x chain, step 1: w' = C_d^T mu on the own rows and the halo nodes
+#pragma unroll
+    for (int d = 0; d < FR; ++d) {
+        float wm = m[d + 3];
+        if (!fastx && r0 + d < pnz) wm = adj_filt_d(g, tab, x, wm, r0 + d, xd_o, xd_n, x < pnx);
+        Ws[d][lane + 3] = wm;
+    }
+    {
+        const bool hl = lane < 6 * FR;
+        if (!fastx && hok) hv = adj_filt_d(g, tab, hcol, hv, hrow, xd_o, xd_n, false);
+        Ws[hl ? hd : 0][hl ? ((hc < 3) ? hc : hc + 64) : HJ] = hv;
+    }
+    wave_sync();
+    // ---- step 2: H_x^T then C_h^T at u nodes x0-1+c (LDS column c)
+#pragma unroll
+    for (int d = 0; d < FR; ++d) {
+        if (EDGE && r0 + d >= pnz) break;
+        const float *W = Ws[d];
+        if (fastx) Us[d][lane] = g.c0x * (W[lane + 1] - W[lane + 2]) + g.c1x * (W[lane] - W[lane + 3]);
+        else Us[d][lane] = adj_u_node(g, tab, W, x0 - 1 + lane, lane + 2, r0 + d, xh_o, xh_n, x0);
+    }
+    {
+        const int ud0 = lane / 3;
+        bool uok = lane < 3 * FR;
+        if (EDGE) uok = uok && r0 + ud0 < pnz;
+        const int ud = uok ? ud0 : 0, uc = uok ? 64 + lane - 3 * ud0 : 65;
+        const float *W = Ws[ud];
+        float u;
+        if (fastx) u = g.c0x * (W[uc + 1] - W[uc + 2]) + g.c1x * (W[uc] - W[uc + 3]);
+        else u = adj_u_node(g, tab, W, x0 - 1 + uc, uc + 2, r0 + ud, xh_o, xh_n, x0);
+        Us[ud][uok ? uc : HJ] = u;
+    }
+    wave_sync();
+
+    // ---- z chain in registers: a = C_d^T mu (rows r0-3+j), tv = C_h^T H_z^T a (nodes r0-1+n)
+    const float c0z = g.c0z, c1z = g.c1z, izf = g.iz;
+    float a[FPR];
+#pragma unroll
+    for (int j = 0; j < FPR; ++j) {
+        const int r = r0 - 3 + j;
+        float wz = m[j];
+        if (EDGE && r >= pnz - npml && r < pnz) {
+            const int q = r - (pnz - npml);
+            float c = zd_o[q * ld + x];
+            c = TB(5, 2 * q + 1) * c + TB(4, 2 * q + 1) * wz;
+            wz = wz + c;
+            if (j >= 3 && j < FR + 3) zd_n[q * ld + x] = c;  // own rows
+        }
+        a[j] = wz;
+    }
+    float tv[FVN];
+#pragma unroll
+    for (int n = 0; n < FVN; ++n) {
+        const int jn = r0 - 1 + n;
+        float t = 0.f;
+        if (!EDGE) {
+            t = c0z * (a[n + 1] - a[n + 2]) + c1z * (a[n] - a[n + 3]);
+        } else if (jn >= 0 && jn <= pnz) {
+            if (jn > 4 && jn + 1 <= pnz - 2) {
+                t = c0z * (a[n + 1] - a[n + 2]) + c1z * (a[n] - a[n + 3]);
+            } else {
+                // rows i in [1, pnz-1) of vz: taps v[i+1]=c0, v[i]=-c0, v[i+2]=c1, v[i-1]=-c1
+                if (jn - 1 >= 1 && jn - 1 < pnz - 1) t += c0z * a[n + 1];
+                if (jn >= 1 && jn < pnz - 1) t -= c0z * a[n + 2];
+                if (jn - 2 >= 1 && jn - 2 < pnz - 1) t += c1z * a[n];
+                if (jn + 1 >= 1 && jn + 1 < pnz - 1) t -= c1z * a[n + 3];
+                if (r0 == 0 && n >= 1 && n <= 5) {  // vz row 0 (free surface) reads v nodes 0..4
+                    const double i24 = (double)g.iz24;
+                    const double w0 = (n == 1) ? -22. : (n == 2) ? 17. : (n == 3) ? 9. : (n == 4) ? -5. : 1.;
+                    t += (float)(w0 * i24) * a[3];
+                }
+                if (jn == pnz) t += izf * a[n + 1];
+                if (jn == pnz - 1) t -= izf * a[n + 2];
+            }
+            if (jn >= pnz - npml && jn < pnz) {
+                const int q = jn - (pnz - npml);
+                float c = zh_o[q * ld + x];
+                c = TB(5, 2 * q) * c + TB(4, 2 * q) * t;
+                t = t + c;
+                if (n >= 1 && n <= FR) zh_n[q * ld + x] = c;  // own nodes r0 .. r0+FR-1
+            }
+        }
+        tv[n] = t;
+    }
+
+    // ---- output rows: out = G_x^T u~ + G_z^T tv (+ residual), then the leapfrog
+#pragma unroll
+    for (int d = 0; d < FR; ++d) {
+        const int i = r0 + d;
+        if (EDGE && i >= pnz) break;
+        const float *U = Us[d];
+        float ox, oz;
+        if (fastx) {
+            ox = g.c0x * (U[lane + 1] - U[lane + 2]) + g.c1x * (U[lane] - U[lane + 3]);
+        } else {
+            const int ui = lane + 1;
+            ox = (x >= pnx) ? 0.f
+                            : gxE(g, x) * U[ui] + gxF(g, x + 1) * U[ui + 1] + gxG(g, x - 1) * U[ui - 1] +
+                                  gxH(g, x + 2) * U[ui + 2];
+        }
+        if (!EDGE) {
+            oz = c0z * (tv[d + 1] - tv[d + 2]) + c1z * (tv[d] - tv[d + 3]);
+        } else {
+            if (i > 3 && i - 1 >= 2 && i + 2 < pnz - 1) {
+                oz = c0z * (tv[d + 1] - tv[d + 2]) + c1z * (tv[d] - tv[d + 3]);
+            } else {
+                // rows j in [2, pnz-1) of v: taps p[j]=c0, p[j-1]=-c0, p[j+1]=c1, p[j-2]=-c1
+                oz = 0.f;
+                if (i >= 2 && i < pnz - 1) oz += c0z * tv[d + 1];
+                if (i + 1 >= 2 && i + 1 < pnz - 1) oz -= c0z * tv[d + 2];
+                if (i - 1 >= 2 && i - 1 < pnz - 1) oz += c1z * tv[d];
+                if (i + 2 >= 2 && i + 2 < pnz - 1) oz -= c1z * tv[d + 3];
+                if (r0 == 0 && d <= 3) {  // v rows 0 and 1 (free surface) read p rows 0..3
+                    const double i840 = (double)g.iz840;
+                    const double w0 = (d == 0) ? 3675.0 : (d == 1) ? -1225.0 : (d == 2) ? 441.0 : -75.0;
+                    const double w1 = (d == 0) ? -1085.0 : (d == 1) ? 1015.0 : (d == 2) ? -63.0 : 5.0;
+                    oz += (float)(w0 * i840) * tv[1] + (float)(w1 * i840) * tv[2];
+                }
+                if (i == pnz - 1) oz += izf * tv[d + 1];
+                if (i == pnz - 2) oz -= izf * tv[d + 2];
+            }
+        }
+        float out = ox + oz;
+        if (EDGE && i >= sh.rz_lo && i <= sh.rz_hi) {
+            const int slot = A.b.rrow[sh.rrow_off + i];
+            if (slot >= 0) {
+                const int ri = A.b.rmap[slot * ld + x];
+                if (ri >= 0) out = out + A.res[sh.trace_off + (long long)k * sh.nr + ri];
+            }
+        }
+        const float mnew = leapfrog(m[d + 3], mo[d], mk[d] * out);
+        if (!EDGE || x < pnx) buf_st(mnew, rold, vo, i * rowb);
+    }
+}
+
+// no source term in the adjoint; otherwise the forward's interior rule
+__device__ __forceinline__ bool adj_tile_interior(const FwiGeomDev &g, const FwiShotDev &sh, int x0, int z0)
+{
+    const int z1 = z0 + FWI_TZ2;
+    if (x0 < g.npml + 3 || x0 + 66 > g.pnx - g.npml - 1) return false;
+    if (z0 < 6 || z1 > g.nz - 3) return false;
+    if (sh.rz_lo <= sh.rz_hi && sh.rz_lo < z1 && sh.rz_hi >= z0) return false;
+    return true;
+}
+
+__global__ __launch_bounds__(256) void k_adj_tile(const FwiAdjArgs *__restrict__ Ap, const int *kbase, int koff)
+{
+    const int k = (kbase ? *kbase : 0) + koff;
+    if (k < 0) return;
+    const FwiAdjArgs &A = *Ap;
+    const FwiGeomDev &g = A.g;
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int ntiles = g.ntx * g.ntz2;
+    const int lb = xcd_remap(blockIdx.x, ntiles * A.b.nshots);
+    if (lb < 0) return;
+    const int s = lb / ntiles, t = lb - s * ntiles;
+    const int tzi = t / g.ntx, txi = t - tzi * g.ntx;
+    const FwiShotDev sh = A.b.shots[s];
+    if (k >= sh.nt) return;
+    const int x0 = txi * TX, z0 = tzi * FWI_TZ2, r0 = z0 + w * FR;
+    __shared__ float stab[FWI_TABN];
+    __shared__ float Wsh[4][FR][HC];
+    __shared__ float Ush[4][FR][HC];
+    if (adj_tile_interior(g, sh, x0, z0)) {
+        adj_tile_rows<false>(A, sh, stab, Wsh[w], Ush[w], s, k, x0, r0, lane);
+    } else {
+        stage_tables(stab, A.b.tab, g.npml);
+        if (r0 < g.pnz) adj_tile_rows<true>(A, sh, stab, Wsh[w], Ush[w], s, k, x0, r0, lane);
+    }
+}
+
+// ============================================================================
 // reconstruction (physical grid) + zero-lag imaging
 // ============================================================================
 __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_rec_step(FwiRecArgs A, const int *kbase, int koff)
@@ -1218,6 +1511,171 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_rec_step(FwiRecArgs A, c
 }
 
 // ============================================================================
+// reconstruction + zero-lag imaging, 2-D tile form on the physical grid: one
+// workgroup per 64 x FWI_TZ2 physical tile and group of shots, wave w owning
+// the FR rows r0 = z0 + w FR.  The workgroup walks the shots of its group;
+// per shot a wave reconstructs p^{k-1} on its rows (interior operator with
+// the z chain in registers and the x chain through wave-private LDS; frame
+// cells from the saved strips) and adds mu^{k+1} X^k to FP64 accumulators
+// held in registers.  The group's image sum meets the accumulator in HBM once
+// per step.  No workgroup barrier.  Same arithmetic as k_rec_step.
+// ============================================================================
+template <bool EDGE>
+__device__ __forceinline__ void rec_tile_shot(const FwiRecArgs &A, const FwiShotDev &sh, int s, int k, int bx0,
+                                              int r0, int lane, float (*Ps)[HC], float (*Us)[HC], const float *pk,
+                                              double *acc)
+{
+    const FwiGeomDev &g = A.g;
+    const int npml = g.npml, nx = g.nx, nz = g.nz, ld = g.ld;
+    const int bx = bx0 + lane, fx = bx + npml;
+    const bool colok = !EDGE || bx < nx;
+    const long long so = (long long)s * A.b.fs;
+    const int cur = k & 1;
+    const float *pn = (cur ? A.P1 : A.P0) + so;  // p^k
+    float *pp = (cur ? A.P0 : A.P1) + so;        // p^{k+1} -> p^{k-1}
+    const float *mu = (cur ? A.M0 : A.M1) + so;  // mu^{k+1}
+    const unsigned fbytes = (unsigned)(A.b.fs * sizeof(float));
+    const brsrc rn = buf_rsrc(pn, fbytes), rq = buf_rsrc(pp, fbytes), rm = buf_rsrc(mu, fbytes);
+    const int vo = (colok ? fx : npml) * 4, rowb = ld * 4;
+
+    float p[FPR], pq[FR], pm[FR];
+#pragma unroll
+    for (int j = 0; j < FPR; ++j) {
+        const int zr = r0 - 3 + j;
+        p[j] = (!EDGE || (zr >= 0 && zr < nz)) ? buf_ld(rn, vo, zr * rowb) : 0.f;
+        if (EDGE && !colok) p[j] = 0.f;
+    }
+    const int hd = lane / 6, hc = lane - 6 * hd;
+    const int hrow = r0 + hd, hcol = (hc < 3) ? bx0 - 3 + hc : bx0 + 61 + hc;
+    bool hok = lane < 6 * FR;
+    if (EDGE) hok = hok && hrow < nz && hcol >= 0 && hcol < nx;
+    float hv = buf_ld(rn, hok ? (hd * ld + hcol + npml) * 4 : vo, r0 * rowb);
+    hv = hok ? hv : 0.f;
+#pragma unroll
+    for (int d = 0; d < FR; ++d) {
+        const int i = r0 + d;
+        const bool ok = !EDGE || i < nz;
+        pq[d] = ok ? buf_ld(rq, vo, i * rowb) : 0.f;
+        pm[d] = ok ? buf_ld(rm, vo, i * rowb) : 0.f;
+    }
+
+    wave_sync();  // the previous shot's reads of Ps / Us are done
+#pragma unroll
+    for (int d = 0; d < FR; ++d) Ps[d][lane + 3] = p[d + 3];
+    {
+        const bool hl = lane < 6 * FR;
+        Ps[hl ? hd : 0][hl ? ((hc < 3) ? hc : hc + 64) : HJ] = hv;
+    }
+    wave_sync();
+    const float c0x = g.c0x, c1x = g.c1x, c0z = g.c0z, c1z = g.c1z;
+#pragma unroll
+    for (int d = 0; d < FR; ++d) {
+        const float *P = Ps[d];
+        Us[d][lane] = c0x * (P[lane + 2] - P[lane + 1]) + c1x * (P[lane + 3] - P[lane]);
+    }
+    {
+        const int ud0 = lane / 3;
+        const bool uok = lane < 3 * FR;
+        const int ud = uok ? ud0 : 0, uc = uok ? 64 + lane - 3 * ud0 : 64;
+        const float *P = Ps[ud];
+        Us[ud][uok ? uc : HJ] = c0x * (P[uc + 2] - P[uc + 1]) + c1x * (P[uc + 3] - P[uc]);
+    }
+    wave_sync();
+    float v[FVN];
+#pragma unroll
+    for (int n = 0; n < FVN; ++n) v[n] = c0z * (p[n + 2] - p[n + 1]) + c1z * (p[n + 3] - p[n]);
+
+    const long long so2 = sh.strip_off + (long long)k * g.sl;
+#pragma unroll
+    for (int d = 0; d < FR; ++d) {
+        const int i = r0 + d;
+        if (EDGE && i >= nz) break;
+        const float *U = Us[d];
+        float ux = c0x * (U[lane + 2] - U[lane + 1]) + c1x * (U[lane + 3] - U[lane]);
+        const float vz = c0z * (v[d + 2] - v[d + 1]) + c1z * (v[d + 3] - v[d]);
+        if (EDGE && i == sh.isz) {
+            const float fsrc = A.b.srcf[sh.src_off + k];
+            ux = (fx == sh.isx) ? ux + fsrc : ux;
+        }
+        float X = ux + vz;
+        bool inner = true;
+        if (EDGE) {
+            inner = i >= 3 && i < nz - 3 && bx >= 3 && bx < nx - 3;
+            if (!inner && colok) X = A.xstrip[so2 + frame_pos(i, bx, nx, nz)];
+        }
+        if (colok) acc[d] += (double)pm[d] * (double)X;
+        if (k >= 1) {
+            float pnew;
+            if (!EDGE || inner) pnew = leapfrog(p[d + 3], pq[d], pk[d] * X);
+            else pnew = (k >= 2 && colok) ? A.pstrip[sh.strip_off + (long long)(k - 2) * g.sl + frame_pos(i, bx, nx, nz)]
+                                          : 0.f;
+            if (colok) buf_st(pnew, rq, vo, i * rowb);
+        }
+    }
+}
+
+// every cell of the physical tile (bx0, z0) is inner and the source is elsewhere
+__device__ __forceinline__ bool rec_tile_interior(const FwiGeomDev &g, const FwiShotDev &sh, int bx0, int z0)
+{
+    if (bx0 < 3 || bx0 + 66 >= g.nx) return false;
+    if (z0 < 3 || z0 + FWI_TZ2 > g.nz - 3) return false;
+    const int sx = sh.isx - g.npml;
+    if (sh.isz >= z0 && sh.isz < z0 + FWI_TZ2 && sx >= bx0 && sx < bx0 + TX) return false;
+    return true;
+}
+
+__global__ __launch_bounds__(256) void k_rec_tile(const FwiRecArgs *__restrict__ Ap, const int *kbase, int koff,
+                                                  int gsize, int ngroups)
+{
+    const int k = (kbase ? *kbase : 0) + koff;
+    if (k < 0) return;
+    const FwiRecArgs &A = *Ap;
+    const FwiGeomDev &g = A.g;
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int bntz2 = (g.nz + FWI_TZ2 - 1) / FWI_TZ2;
+    const int ntiles = g.bntx * bntz2;
+    const int lb = xcd_remap(blockIdx.x, ntiles * ngroups);
+    if (lb < 0) return;
+    const int grp = lb / ntiles, t = lb - grp * ntiles;
+    const int tzi = t / g.bntx, txi = t - tzi * g.bntx;
+    const int bx0 = txi * TX, z0 = tzi * FWI_TZ2, r0 = z0 + w * FR;
+    if (r0 >= g.nz) return;  // no barrier follows
+    __shared__ float Psh[4][FR][HC];
+    __shared__ float Ush[4][FR][HC];
+    const int nx = g.nx, nz = g.nz, ld = g.ld, npml = g.npml;
+    const int bx = bx0 + lane;
+    const bool colok = bx < nx;
+    // ekp of the own rows, shared by every shot
+    float pk[FR];
+    {
+        const brsrc rk = buf_rsrc(A.b.ekp, (unsigned)(A.b.fs * sizeof(float)));
+        const int vo = (colok ? bx + npml : npml) * 4;
+#pragma unroll
+        for (int d = 0; d < FR; ++d) pk[d] = (r0 + d < nz) ? buf_ld(rk, vo, (r0 + d) * ld * 4) : 0.f;
+    }
+    double acc[FR];
+#pragma unroll
+    for (int d = 0; d < FR; ++d) acc[d] = 0.0;
+    const int s_lo = grp * gsize, s_hi = min(s_lo + gsize, A.b.nshots);
+    for (int s = s_lo; s < s_hi; ++s) {
+        const FwiShotDev sh = A.b.shots[s];
+        if (k >= sh.nt) continue;
+        if (rec_tile_interior(g, sh, bx0, z0))
+            rec_tile_shot<false>(A, sh, s, k, bx0, r0, lane, Psh[w], Ush[w], pk, acc);
+        else
+            rec_tile_shot<true>(A, sh, s, k, bx0, r0, lane, Psh[w], Ush[w], pk, acc);
+    }
+    double *gacc = A.gacc + (long long)grp * nx * nz;
+    if (colok) {
+#pragma unroll
+        for (int d = 0; d < FR; ++d) {
+            const int i = r0 + d;
+            if (i < nz) gacc[(long long)i * nx + bx] += acc[d];
+        }
+    }
+}
+
+// ============================================================================
 // small kernels
 // ============================================================================
 // moves the step counter of a captured block of steps
@@ -1460,10 +1918,27 @@ hipError_t fwi_launch_fwd(const FwiFwdArgs *a, const int *kbase, int koff, int n
     return hipGetLastError();
 }
 
-hipError_t fwi_launch_fwd_tile(const FwiFwdArgs *a, const int *kbase, int koff, hipStream_t st)
+hipError_t fwi_launch_fwd_tile(const FwiFwdArgs *a, const FwiFwdArgs *a_dev, const int *kbase, int koff,
+                               hipStream_t st)
 {
     const int n = a->g.ntx * a->g.ntz2 * a->b.nshots;
-    hipLaunchKernelGGL(k_fwd_tile, dim3(((n + 7) / 8) * 8), dim3(256), 0, st, *a, kbase, koff);
+    hipLaunchKernelGGL(k_fwd_tile, dim3(((n + 7) / 8) * 8), dim3(256), 0, st, a_dev, kbase, koff);
+    return hipGetLastError();
+}
+
+hipError_t fwi_launch_adj_tile(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, const int *kbase, int koff,
+                               hipStream_t st)
+{
+    const int n = a->g.ntx * a->g.ntz2 * a->b.nshots;
+    hipLaunchKernelGGL(k_adj_tile, dim3(((n + 7) / 8) * 8), dim3(256), 0, st, a_dev, kbase, koff);
+    return hipGetLastError();
+}
+
+hipError_t fwi_launch_rec_tile(const FwiRecArgs *a, const FwiRecArgs *a_dev, const int *kbase, int koff,
+                               int gsize, int ngroups, hipStream_t st)
+{
+    const int n = a->g.bntx * ((a->g.nz + FWI_TZ2 - 1) / FWI_TZ2) * ngroups;
+    hipLaunchKernelGGL(k_rec_tile, dim3(((n + 7) / 8) * 8), dim3(256), 0, st, a_dev, kbase, koff, gsize, ngroups);
     return hipGetLastError();
 }
 

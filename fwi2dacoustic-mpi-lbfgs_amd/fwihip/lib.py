@@ -25,7 +25,7 @@ EXPORTS = (
     "fwi_hip_set_residual", "fwi_hip_adjoint", "fwi_hip_synchronize", "fwi_hip_set_timing",
     "fwi_hip_kernel_time", "fwi_hip_read_state", "fwi_hip_op_pml", "fwi_hip_op_interior",
     "fwi_hip_time_integral", "fwi_hip_write_bdr", "fwi_hip_read_bdr", "fwi_hip_cpml_coe",
-    "fwi_hip_set_graph", "fwi_hip_set_kernels", "fwi_hip_device_count", "fwi_hip_dmalloc", "fwi_hip_dfree", "fwi_hip_memcpy",
+    "fwi_hip_set_graph", "fwi_hip_set_kernels", "fwi_hip_set_rec_group", "fwi_hip_device_count", "fwi_hip_dmalloc", "fwi_hip_dfree", "fwi_hip_memcpy",
 )
 
 
@@ -179,8 +179,13 @@ class Context:
     def synchronize(self):
         self._chk(self.lib.fwi_hip_synchronize(self.h))
 
-    def set_kernels(self, fwd_kind):
-        self._chk(self.lib.fwi_hip_set_kernels(self.h, int(fwd_kind)))
+    def set_kernels(self, kind):
+        """0: 2-D tile kernels (default), 1: z-marching kernels."""
+        self._chk(self.lib.fwi_hip_set_kernels(self.h, int(kind)))
+
+    def set_rec_group(self, shots):
+        """Shots per workgroup of the tile reconstruction + imaging kernel."""
+        self._chk(self.lib.fwi_hip_set_rec_group(self.h, int(shots)))
 
     def set_graph(self, steps):
         self._chk(self.lib.fwi_hip_set_graph(self.h, int(steps)))

@@ -103,9 +103,14 @@ int fwi_hip_adjoint(fwi_hip_ctx *ctx, double *grad, double *gk);
  * counter, so one executable graph is replayed for every block.           */
 int fwi_hip_set_graph(fwi_hip_ctx *ctx, int steps);
 
-/* Forward kernel form: 0 = 2-D tile kernel (default), 1 = z-marching
- * kernel.  Both compute the same values bit for bit.                      */
-int fwi_hip_set_kernels(fwi_hip_ctx *ctx, int fwd_kind);
+/* Kernel form of the forward, adjoint and reconstruction steps: 0 = 2-D
+ * tile kernels (default), 1 = z-marching kernels.  Traces agree bit for
+ * bit; gradients differ only in the FP64 summation order over shots.
+ * (No counterpart in the reference, which has one CPU loop nest.)        */
+int fwi_hip_set_kernels(fwi_hip_ctx *ctx, int kind);
+/* Shots per workgroup of the tile reconstruction + imaging kernel (default
+ * 8): the group's image sum meets the FP64 accumulator once per step.     */
+int fwi_hip_set_rec_group(fwi_hip_ctx *ctx, int shots);
 
 /* Wait for all work of the context. */
 int fwi_hip_synchronize(fwi_hip_ctx *ctx);
