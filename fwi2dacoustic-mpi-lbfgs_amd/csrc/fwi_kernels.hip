@@ -1512,13 +1512,14 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_rec_step(FwiRecArgs A, c
 
 // ============================================================================
 // reconstruction + zero-lag imaging, 2-D tile form on the physical grid: one
-// workgroup per 64 x FWI_TZ2 physical tile and group of shots, wave w owning
-// the FR rows r0 = z0 + w FR.  The workgroup walks the shots of its group;
-// per shot a wave reconstructs p^{k-1} on its rows (interior operator with
-// the z chain in registers and the x chain through wave-private LDS; frame
-// cells from the saved strips) and adds mu^{k+1} X^k to FP64 accumulators
-// held in registers.  The group's image sum meets the accumulator in HBM once
-// per step.  No workgroup barrier.  Same arithmetic as k_rec_step.
+// workgroup (4 waves) per 64 x FR physical tile and group of shots; wave w
+// takes the shots s_lo + w, s_lo + w + 4, ...  Per shot a wave reconstructs
+// p^{k-1} on the tile's rows (interior operator with the z chain in registers
+// and the x chain through wave-private LDS; frame cells from the saved
+// strips) and adds mu^{k+1} X^k to FP64 accumulators held in registers.  At
+// the end the four partial images meet in LDS (fixed order, so the sum is
+// deterministic) and the group's image is added to its FP64 accumulator in
+// HBM once per step.  Same arithmetic as k_rec_step.
 // ============================================================================
 template <bool EDGE>
 __device__ __forceinline__ void rec_tile_shot(const FwiRecArgs &A, const FwiShotDev &sh, int s, int k, int bx0,
@@ -1614,13 +1615,13 @@ __device__ __forceinline__ void rec_tile_shot(const FwiRecArgs &A, const FwiShot
     }
 }
 
-// every cell of the physical tile (bx0, z0) is inner and the source is elsewhere
+// every cell of the physical tile (bx0, z0: 64 x FR) is inner and the source is elsewhere
 __device__ __forceinline__ bool rec_tile_interior(const FwiGeomDev &g, const FwiShotDev &sh, int bx0, int z0)
 {
     if (bx0 < 3 || bx0 + 66 >= g.nx) return false;
-    if (z0 < 3 || z0 + FWI_TZ2 > g.nz - 3) return false;
+    if (z0 < 3 || z0 + FR > g.nz - 3) return false;
     const int sx = sh.isx - g.npml;
-    if (sh.isz >= z0 && sh.isz < z0 + FWI_TZ2 && sx >= bx0 && sx < bx0 + TX) return false;
+    if (sh.isz >= z0 && sh.isz < z0 + FR && sx >= bx0 && sx < bx0 + TX) return false;
     return true;
 }
 
@@ -1632,20 +1633,20 @@ __global__ __launch_bounds__(256) void k_rec_tile(const FwiRecArgs *__restrict__
     const FwiRecArgs &A = *Ap;
     const FwiGeomDev &g = A.g;
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int bntz2 = (g.nz + FWI_TZ2 - 1) / FWI_TZ2;
-    const int ntiles = g.bntx * bntz2;
+    const int ntz = (g.nz + FR - 1) / FR;
+    const int ntiles = g.bntx * ntz;
     const int lb = xcd_remap(blockIdx.x, ntiles * ngroups);
     if (lb < 0) return;
     const int grp = lb / ntiles, t = lb - grp * ntiles;
     const int tzi = t / g.bntx, txi = t - tzi * g.bntx;
-    const int bx0 = txi * TX, z0 = tzi * FWI_TZ2, r0 = z0 + w * FR;
-    if (r0 >= g.nz) return;  // no barrier follows
-    __shared__ float Psh[4][FR][HC];
-    __shared__ float Ush[4][FR][HC];
+    const int bx0 = txi * TX, r0 = tzi * FR;
+    // per wave: staged p rows and u rows while propagating, then its partial image (FP64)
+    __shared__ __attribute__((aligned(16))) float lds[4][2][FR][HC];
+    static_assert(sizeof(lds[0]) >= FR * TX * sizeof(double), "partial image fits the wave's rows");
     const int nx = g.nx, nz = g.nz, ld = g.ld, npml = g.npml;
     const int bx = bx0 + lane;
     const bool colok = bx < nx;
-    // ekp of the own rows, shared by every shot
+    // ekp of the tile's rows, shared by every shot
     float pk[FR];
     {
         const brsrc rk = buf_rsrc(A.b.ekp, (unsigned)(A.b.fs * sizeof(float)));
@@ -1657,21 +1658,28 @@ __global__ __launch_bounds__(256) void k_rec_tile(const FwiRecArgs *__restrict__
 #pragma unroll
     for (int d = 0; d < FR; ++d) acc[d] = 0.0;
     const int s_lo = grp * gsize, s_hi = min(s_lo + gsize, A.b.nshots);
-    for (int s = s_lo; s < s_hi; ++s) {
+    for (int s = s_lo + w; s < s_hi; s += 4) {
         const FwiShotDev sh = A.b.shots[s];
         if (k >= sh.nt) continue;
-        if (rec_tile_interior(g, sh, bx0, z0))
-            rec_tile_shot<false>(A, sh, s, k, bx0, r0, lane, Psh[w], Ush[w], pk, acc);
+        if (rec_tile_interior(g, sh, bx0, r0))
+            rec_tile_shot<false>(A, sh, s, k, bx0, r0, lane, lds[w][0], lds[w][1], pk, acc);
         else
-            rec_tile_shot<true>(A, sh, s, k, bx0, r0, lane, Psh[w], Ush[w], pk, acc);
+            rec_tile_shot<true>(A, sh, s, k, bx0, r0, lane, lds[w][0], lds[w][1], pk, acc);
     }
-    double *gacc = A.gacc + (long long)grp * nx * nz;
-    if (colok) {
+    // partial images of the four waves -> LDS; wave w then finishes rows 2w, 2w+1
+    wave_sync();
+    double(*red)[TX] = reinterpret_cast<double(*)[TX]>(&lds[w][0][0][0]);
 #pragma unroll
-        for (int d = 0; d < FR; ++d) {
-            const int i = r0 + d;
-            if (i < nz) gacc[(long long)i * nx + bx] += acc[d];
-        }
+    for (int d = 0; d < FR; ++d) red[d][lane] = acc[d];
+    __syncthreads();
+    double *gacc = A.gacc + (long long)grp * nx * nz;
+#pragma unroll
+    for (int e = 0; e < FR / 4; ++e) {
+        const int d = w * (FR / 4) + e, i = r0 + d;
+        double sum = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) sum += reinterpret_cast<double(*)[TX]>(&lds[q][0][0][0])[d][lane];
+        if (colok && i < nz) gacc[(long long)i * nx + bx] += sum;
     }
 }
 
@@ -1937,7 +1945,7 @@ hipError_t fwi_launch_adj_tile(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, con
 hipError_t fwi_launch_rec_tile(const FwiRecArgs *a, const FwiRecArgs *a_dev, const int *kbase, int koff,
                                int gsize, int ngroups, hipStream_t st)
 {
-    const int n = a->g.bntx * ((a->g.nz + FWI_TZ2 - 1) / FWI_TZ2) * ngroups;
+    const int n = a->g.bntx * ((a->g.nz + FR - 1) / FR) * ngroups;
     hipLaunchKernelGGL(k_rec_tile, dim3(((n + 7) / 8) * 8), dim3(256), 0, st, a_dev, kbase, koff, gsize, ngroups);
     return hipGetLastError();
 }
