@@ -9,9 +9,13 @@ gradient copied back.  Synthetic seeded layered/faulted model.
 
 Prints ONE JSON line (rank 0):
   value       shot-gradients/s (whole job)
-  roofline    dominant kernel's algorithmic bytes (the minimum HBM traffic of
-              one launch, see algorithmic_bytes) / its HIP-event duration,
-              against the 8 TB/s HBM3E peak
+  roofline    dominant kernel's algorithmic bytes / its HIP-event duration,
+              against the 8 TB/s HBM3E peak.  Algorithmic bytes: 16 B per
+              wavefield cell-update (read p^n, p^{n-1}, ekp; write p^{n+1}),
+              over the cells the kernel updates (propagation grid for the
+              forward and adjoint, physical grid for the reconstruction), times
+              the shots; CPML memory, frames, traces and the image are not
+              credited (they are listed as min_traffic_bytes)
   cpu_baseline  the FP32 OpenMP oracle (a restatement of the reference CPU
               path) on a bounded sample of the same workload, on this host
 """
@@ -26,6 +30,7 @@ PKG = os.path.join(ROOT, "fwi2dacoustic-mpi-lbfgs_amd")
 sys.path.insert(0, PKG)
 
 HBM_PEAK = 8.0e12          # MI355X datasheet HBM3E bandwidth, B/s
+BYTES_PER_CELL_UPDATE = 16  # read p^n, p^{n-1}, ekp; write p^{n+1} (FP32)
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "traffic_1024x320_64shots.json")
 
 
@@ -48,7 +53,15 @@ def parse():
     return ap.parse_args()
 
 
-def algorithmic_bytes(g, nshots, nr):
+def algorithmic_bytes(g, nshots):
+    """Credited bytes of one launch of each step kernel: 16 B per cell-update."""
+    f = g.pnx * g.pnz
+    b = g.nx * g.nz
+    return (BYTES_PER_CELL_UPDATE * f * nshots, BYTES_PER_CELL_UPDATE * f * nshots,
+            BYTES_PER_CELL_UPDATE * b * nshots)
+
+
+def min_traffic_bytes(g, nshots, nr):
     """Minimum HBM bytes of one launch of each step kernel (FP32 fields):
     forward   per propagation-grid cell and shot: read p^n, p^{n-1}, write
               p^{n+1} (12 B); ekp once per cell (shared by the shots); CPML
@@ -150,11 +163,12 @@ def main():
     kern = {}
     names = (("k_fwd_tile", "k_adj_tile", "k_rec_tile") if args.kernels == 0
              else ("k_fwd_step", "k_adj_step", "k_rec_step"))
-    alg = algorithmic_bytes(g, nshots, nr)
+    alg = algorithmic_bytes(g, nshots)
+    mint = min_traffic_bytes(g, nshots, nr)
     for w, name in enumerate(names):
         ms, n = ctx.kernel_time(w)
         avg = ms / max(n, 1) * 1e-3
-        kern[name] = dict(launches=n, avg_us=avg * 1e6, algorithmic_bytes=alg[w],
+        kern[name] = dict(launches=n, avg_us=avg * 1e6, algorithmic_bytes=alg[w], min_traffic_bytes=mint[w],
                           achieved_GBps=alg[w] / avg / 1e9 if avg > 0 else None)
     ctx.set_timing(False)
     dom = max(names, key=lambda k: kern[k]["avg_us"] * kern[k]["launches"])
@@ -216,7 +230,7 @@ def main():
                                f"nt={args.nt}, {nshots} shots x {nr} receivers, npml={g.npml}, free surface",
                    "nx": g.nx, "nz": g.nz, "nt": args.nt, "nshots": nshots, "nr": nr, "npml": g.npml},
         "gcell_updates_per_s": gcell,
-        "hbm_frac_whole_job": sum(alg) * args.nt * args.steps / wall / HBM_PEAK,
+        "hbm_frac_whole_job": gcell * 1e9 * BYTES_PER_CELL_UPDATE / HBM_PEAK,
         "misfit": J,
         "roofline": roof,
         "kernels": kern,

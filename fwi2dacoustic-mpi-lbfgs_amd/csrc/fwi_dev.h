@@ -44,7 +44,9 @@ struct FwiGeomDev {
     int pad_;
 };
 
+#ifndef FWI_TZ2
 #define FWI_TZ2 32       // rows of a 2-D tile (4 waves x 8 output rows)
+#endif
 
 struct FwiBatchDev {
     int nshots, sg, ngroups;

@@ -462,13 +462,40 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_fwd_step(FwiFwdArgs A, c
 // lane-dependent branch; the others run the general body.  Same arithmetic
 // and CPML ownership rules as k_fwd_step.
 // ============================================================================
-#define FR 8              // output rows per wave
+#define FR (FWI_TZ2 / 4) // output rows per wave
 #define FPR (FR + 6)      // p rows per wave: r0-3 .. r0+FR+2
 #define FVN (FR + 3)      // v nodes per wave: r0-1 .. r0+FR+1
 #define HC 72             // LDS row: 3 halo | 64 | 3 halo | 2 spare
 #define HJ 71             // spare column that takes the stores of idle lanes
-static_assert(FWI_TZ2 == 4 * FR, "a tile is 4 waves of FR rows");
-static_assert(6 * FR <= 64, "one halo load per wave");
+#define HP ((6 * FR + 63) / 64)  // x-halo loads per wave: 6 values per own row
+static_assert(FWI_TZ2 % 4 == 0 && FR >= 4, "a tile is 4 waves of FR rows");
+static_assert(3 * FR <= 64, "one pass for the three extra u nodes of every row");
+
+// The x halo of a wave's nrow rows: value h of lane l is number l + 64 h =
+// 6 d + c of the list (row d, column c: c < 3 left of the tile, else right).
+template <int NP>
+struct XHalo {
+    int d[NP], col[NP];  // row within the wave, column (grid)
+    bool in[NP];         // a real entry (not past the 6 nrow values)
+};
+template <int NP>
+__device__ __forceinline__ void xhalo_plan(XHalo<NP> &h, int lane, int x0, int nrow)
+{
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+        const int l = lane + 64 * q, d = l / 6, c = l - 6 * d;
+        h.in[q] = d < nrow;
+        h.d[q] = h.in[q] ? d : 0;
+        h.col[q] = (c < 3) ? x0 - 3 + c : x0 + 61 + c;
+    }
+}
+// LDS column of entry q (the spare column for idle entries)
+template <int NP>
+__device__ __forceinline__ int xhalo_lds_col(const XHalo<NP> &h, int q, int x0)
+{
+    const int c = h.col[q] - x0 + 3;
+    return h.in[q] ? c : HJ;
+}
 
 // true when no cell of the tile (64 x FWI_TZ2 at x0, z0), nor any u/v node it
 // evaluates, needs a CPML, edge, free-surface, frame, source or receiver rule
@@ -505,13 +532,17 @@ __device__ __forceinline__ void fwd_tile_rows(const FwiFwdArgs &A, const FwiShot
         const int zr = r0 - 3 + m;
         p[m] = (!EDGE || (zr >= 0 && zr < pnz)) ? buf_ld(rnow, vo, zr * rowb) : 0.f;
     }
-    // x halo of the FR own rows in one load: lane = 6 hd + hc, hc < 3 left, else right
-    const int hd = lane / 6, hc = lane - 6 * hd;
-    const int hrow = r0 + hd, hcol = (hc < 3) ? x0 - 3 + hc : x0 + 61 + hc;
-    bool hok = lane < 6 * FR;
-    if (EDGE) hok = hok && hrow < pnz && hcol >= 0 && hcol < pnx;
-    float hv = buf_ld(rnow, hok ? (hd * ld + hcol) * 4 : vo, r0 * rowb);
-    hv = hok ? hv : 0.f;
+    // x halo of the FR own rows, HP loads
+    XHalo<HP> xh;
+    xhalo_plan(xh, lane, x0, FR);
+    float hv[HP];
+#pragma unroll
+    for (int q = 0; q < HP; ++q) {
+        bool ok = xh.in[q];
+        if (EDGE) ok = ok && r0 + xh.d[q] < pnz && xh.col[q] >= 0 && xh.col[q] < pnx;
+        hv[q] = buf_ld(rnow, ok ? (xh.d[q] * ld + xh.col[q]) * 4 : vo, r0 * rowb);
+        hv[q] = ok ? hv[q] : 0.f;
+    }
 #pragma unroll
     for (int d = 0; d < FR; ++d) {
         const int i = r0 + d;
@@ -523,10 +554,8 @@ __device__ __forceinline__ void fwd_tile_rows(const FwiFwdArgs &A, const FwiShot
     // ---- stage the own rows (with halo) for the x chain
 #pragma unroll
     for (int d = 0; d < FR; ++d) Ps[d][lane + 3] = p[d + 3];
-    {
-        const bool hl = lane < 6 * FR;
-        Ps[hl ? hd : 0][hl ? ((hc < 3) ? hc : hc + 64) : HJ] = hv;
-    }
+#pragma unroll
+    for (int q = 0; q < HP; ++q) Ps[xh.d[q]][xhalo_lds_col(xh, q, x0)] = hv[q];
     wave_sync();
 
     const float *xh_o = A.xh + s * A.b.xh_s + cur * A.b.xh_b;
@@ -649,7 +678,12 @@ __device__ __forceinline__ void fwd_tile_rows(const FwiFwdArgs &A, const FwiShot
 // The parameter block lives in device memory (not in kernel arguments, which
 // are all loaded at entry): each body then loads only the fields it uses, and
 // the interior body keeps its scalar registers free of the general body's.
-__global__ __launch_bounds__(256) void k_fwd_tile(const FwiFwdArgs *__restrict__ Ap, const int *kbase, int koff)
+// The shot table comes as its own read-only kernel argument so that its
+// descriptors are read through the scalar cache (a pointer fetched from the
+// parameter block would be read with vector loads, one memory latency per
+// wave before its field loads can issue).
+__global__ __launch_bounds__(256) void k_fwd_tile(const FwiFwdArgs *__restrict__ Ap,
+                                                  const FwiShotDev *__restrict__ shots, const int *kbase, int koff)
 {
     const int k = (kbase ? *kbase : 0) + koff;
     if (k < 0) return;
@@ -662,7 +696,7 @@ __global__ __launch_bounds__(256) void k_fwd_tile(const FwiFwdArgs *__restrict__
     if (lb < 0) return;
     const int s = lb / ntiles, t = lb - s * ntiles;
     const int tzi = t / g.ntx, txi = t - tzi * g.ntx;
-    const FwiShotDev sh = A.b.shots[s];
+    const FwiShotDev sh = shots[s];
     if (k >= sh.nt) return;  // the whole workgroup belongs to shot s
     const int x0 = txi * TX, z0 = tzi * FWI_TZ2, r0 = z0 + w * FR;
     __shared__ float stab[FWI_TABN];
@@ -1058,12 +1092,17 @@ __device__ __forceinline__ void adj_tile_rows(const FwiAdjArgs &A, const FwiShot
         const int zr = r0 - 3 + j;
         m[j] = (!EDGE || (zr >= 0 && zr < pnz)) ? buf_ld(rnow, vo, zr * rowb) : 0.f;
     }
-    const int hd = lane / 6, hc = lane - 6 * hd;
-    const int hrow = r0 + hd, hcol = (hc < 3) ? x0 - 3 + hc : x0 + 61 + hc;
-    bool hok = lane < 6 * FR;
-    if (EDGE) hok = hok && hrow < pnz && hcol >= 0 && hcol < pnx;
-    float hv = buf_ld(rnow, hok ? (hd * ld + hcol) * 4 : vo, r0 * rowb);
-    hv = hok ? hv : 0.f;
+    XHalo<HP> xh;
+    xhalo_plan(xh, lane, x0, FR);
+    float hv[HP];
+    bool hok[HP];
+#pragma unroll
+    for (int q = 0; q < HP; ++q) {
+        hok[q] = xh.in[q];
+        if (EDGE) hok[q] = hok[q] && r0 + xh.d[q] < pnz && xh.col[q] >= 0 && xh.col[q] < pnx;
+        hv[q] = buf_ld(rnow, hok[q] ? (xh.d[q] * ld + xh.col[q]) * 4 : vo, r0 * rowb);
+        hv[q] = hok[q] ? hv[q] : 0.f;
+    }
 #pragma unroll
     for (int d = 0; d < FR; ++d) {
         const int i = r0 + d;
@@ -1092,10 +1131,11 @@ __device__ __forceinline__ void adj_tile_rows(const FwiAdjArgs &A, const FwiShot
         if (!fastx && r0 + d < pnz) wm = adj_filt_d(g, tab, x, wm, r0 + d, xd_o, xd_n, x < pnx);
         Ws[d][lane + 3] = wm;
     }
-    {
-        const bool hl = lane < 6 * FR;
-        if (!fastx && hok) hv = adj_filt_d(g, tab, hcol, hv, hrow, xd_o, xd_n, false);
-        Ws[hl ? hd : 0][hl ? ((hc < 3) ? hc : hc + 64) : HJ] = hv;
+#pragma unroll
+    for (int q = 0; q < HP; ++q) {
+        float w = hv[q];
+        if (!fastx && hok[q]) w = adj_filt_d(g, tab, xh.col[q], w, r0 + xh.d[q], xd_o, xd_n, false);
+        Ws[xh.d[q]][xhalo_lds_col(xh, q, x0)] = w;
     }
     wave_sync();
     // ---- step 2: H_x^T then C_h^T at u nodes x0-1+c (LDS column c)
@@ -1230,7 +1270,8 @@ __device__ __forceinline__ bool adj_tile_interior(const FwiGeomDev &g, const Fwi
     return true;
 }
 
-__global__ __launch_bounds__(256) void k_adj_tile(const FwiAdjArgs *__restrict__ Ap, const int *kbase, int koff)
+__global__ __launch_bounds__(256) void k_adj_tile(const FwiAdjArgs *__restrict__ Ap,
+                                                  const FwiShotDev *__restrict__ shots, const int *kbase, int koff)
 {
     const int k = (kbase ? *kbase : 0) + koff;
     if (k < 0) return;
@@ -1242,7 +1283,7 @@ __global__ __launch_bounds__(256) void k_adj_tile(const FwiAdjArgs *__restrict__
     if (lb < 0) return;
     const int s = lb / ntiles, t = lb - s * ntiles;
     const int tzi = t / g.ntx, txi = t - tzi * g.ntx;
-    const FwiShotDev sh = A.b.shots[s];
+    const FwiShotDev sh = shots[s];
     if (k >= sh.nt) return;
     const int x0 = txi * TX, z0 = tzi * FWI_TZ2, r0 = z0 + w * FR;
     __shared__ float stab[FWI_TABN];
@@ -1521,6 +1562,17 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_rec_step(FwiRecArgs A, c
 // deterministic) and the group's image is added to its FP64 accumulator in
 // HBM once per step.  Same arithmetic as k_rec_step.
 // ============================================================================
+// rows per wave of the reconstruction kernel (its FP64 image accumulators
+// grow with it, so it is set apart from the forward/adjoint row count)
+#ifndef FWI_REC_ROWS
+#define FWI_REC_ROWS 8
+#endif
+#define RR FWI_REC_ROWS
+#define RPR (RR + 6)
+#define RVN (RR + 3)
+#define RHP ((6 * RR + 63) / 64)
+static_assert(RR % 4 == 0 && 3 * RR <= 64, "reconstruction rows per wave");
+
 template <bool EDGE>
 __device__ __forceinline__ void rec_tile_shot(const FwiRecArgs &A, const FwiShotDev &sh, int s, int k, int bx0,
                                               int r0, int lane, float (*Ps)[HC], float (*Us)[HC], const float *pk,
@@ -1539,21 +1591,25 @@ __device__ __forceinline__ void rec_tile_shot(const FwiRecArgs &A, const FwiShot
     const brsrc rn = buf_rsrc(pn, fbytes), rq = buf_rsrc(pp, fbytes), rm = buf_rsrc(mu, fbytes);
     const int vo = (colok ? fx : npml) * 4, rowb = ld * 4;
 
-    float p[FPR], pq[FR], pm[FR];
+    float p[RPR], pq[RR], pm[RR];
 #pragma unroll
-    for (int j = 0; j < FPR; ++j) {
+    for (int j = 0; j < RPR; ++j) {
         const int zr = r0 - 3 + j;
         p[j] = (!EDGE || (zr >= 0 && zr < nz)) ? buf_ld(rn, vo, zr * rowb) : 0.f;
         if (EDGE && !colok) p[j] = 0.f;
     }
-    const int hd = lane / 6, hc = lane - 6 * hd;
-    const int hrow = r0 + hd, hcol = (hc < 3) ? bx0 - 3 + hc : bx0 + 61 + hc;
-    bool hok = lane < 6 * FR;
-    if (EDGE) hok = hok && hrow < nz && hcol >= 0 && hcol < nx;
-    float hv = buf_ld(rn, hok ? (hd * ld + hcol + npml) * 4 : vo, r0 * rowb);
-    hv = hok ? hv : 0.f;
+    XHalo<RHP> xh;
+    xhalo_plan(xh, lane, bx0, RR);
+    float hv[RHP];
 #pragma unroll
-    for (int d = 0; d < FR; ++d) {
+    for (int q = 0; q < RHP; ++q) {
+        bool ok = xh.in[q];
+        if (EDGE) ok = ok && r0 + xh.d[q] < nz && xh.col[q] >= 0 && xh.col[q] < nx;
+        hv[q] = buf_ld(rn, ok ? (xh.d[q] * ld + xh.col[q] + npml) * 4 : vo, r0 * rowb);
+        hv[q] = ok ? hv[q] : 0.f;
+    }
+#pragma unroll
+    for (int d = 0; d < RR; ++d) {
         const int i = r0 + d;
         const bool ok = !EDGE || i < nz;
         pq[d] = ok ? buf_ld(rq, vo, i * rowb) : 0.f;
@@ -1562,33 +1618,31 @@ __device__ __forceinline__ void rec_tile_shot(const FwiRecArgs &A, const FwiShot
 
     wave_sync();  // the previous shot's reads of Ps / Us are done
 #pragma unroll
-    for (int d = 0; d < FR; ++d) Ps[d][lane + 3] = p[d + 3];
-    {
-        const bool hl = lane < 6 * FR;
-        Ps[hl ? hd : 0][hl ? ((hc < 3) ? hc : hc + 64) : HJ] = hv;
-    }
+    for (int d = 0; d < RR; ++d) Ps[d][lane + 3] = p[d + 3];
+#pragma unroll
+    for (int q = 0; q < RHP; ++q) Ps[xh.d[q]][xhalo_lds_col(xh, q, bx0)] = hv[q];
     wave_sync();
     const float c0x = g.c0x, c1x = g.c1x, c0z = g.c0z, c1z = g.c1z;
 #pragma unroll
-    for (int d = 0; d < FR; ++d) {
+    for (int d = 0; d < RR; ++d) {
         const float *P = Ps[d];
         Us[d][lane] = c0x * (P[lane + 2] - P[lane + 1]) + c1x * (P[lane + 3] - P[lane]);
     }
     {
         const int ud0 = lane / 3;
-        const bool uok = lane < 3 * FR;
+        const bool uok = lane < 3 * RR;
         const int ud = uok ? ud0 : 0, uc = uok ? 64 + lane - 3 * ud0 : 64;
         const float *P = Ps[ud];
         Us[ud][uok ? uc : HJ] = c0x * (P[uc + 2] - P[uc + 1]) + c1x * (P[uc + 3] - P[uc]);
     }
     wave_sync();
-    float v[FVN];
+    float v[RVN];
 #pragma unroll
-    for (int n = 0; n < FVN; ++n) v[n] = c0z * (p[n + 2] - p[n + 1]) + c1z * (p[n + 3] - p[n]);
+    for (int n = 0; n < RVN; ++n) v[n] = c0z * (p[n + 2] - p[n + 1]) + c1z * (p[n + 3] - p[n]);
 
     const long long so2 = sh.strip_off + (long long)k * g.sl;
 #pragma unroll
-    for (int d = 0; d < FR; ++d) {
+    for (int d = 0; d < RR; ++d) {
         const int i = r0 + d;
         if (EDGE && i >= nz) break;
         const float *U = Us[d];
@@ -1615,17 +1669,18 @@ __device__ __forceinline__ void rec_tile_shot(const FwiRecArgs &A, const FwiShot
     }
 }
 
-// every cell of the physical tile (bx0, z0: 64 x FR) is inner and the source is elsewhere
+// every cell of the physical tile (bx0, z0: 64 x RR) is inner and the source is elsewhere
 __device__ __forceinline__ bool rec_tile_interior(const FwiGeomDev &g, const FwiShotDev &sh, int bx0, int z0)
 {
     if (bx0 < 3 || bx0 + 66 >= g.nx) return false;
-    if (z0 < 3 || z0 + FR > g.nz - 3) return false;
+    if (z0 < 3 || z0 + RR > g.nz - 3) return false;
     const int sx = sh.isx - g.npml;
-    if (sh.isz >= z0 && sh.isz < z0 + FR && sx >= bx0 && sx < bx0 + TX) return false;
+    if (sh.isz >= z0 && sh.isz < z0 + RR && sx >= bx0 && sx < bx0 + TX) return false;
     return true;
 }
 
-__global__ __launch_bounds__(256) void k_rec_tile(const FwiRecArgs *__restrict__ Ap, const int *kbase, int koff,
+__global__ __launch_bounds__(256) void k_rec_tile(const FwiRecArgs *__restrict__ Ap,
+                                                  const FwiShotDev *__restrict__ shots, const int *kbase, int koff,
                                                   int gsize, int ngroups)
 {
     const int k = (kbase ? *kbase : 0) + koff;
@@ -1633,33 +1688,33 @@ __global__ __launch_bounds__(256) void k_rec_tile(const FwiRecArgs *__restrict__
     const FwiRecArgs &A = *Ap;
     const FwiGeomDev &g = A.g;
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int ntz = (g.nz + FR - 1) / FR;
+    const int ntz = (g.nz + RR - 1) / RR;
     const int ntiles = g.bntx * ntz;
     const int lb = xcd_remap(blockIdx.x, ntiles * ngroups);
     if (lb < 0) return;
     const int grp = lb / ntiles, t = lb - grp * ntiles;
     const int tzi = t / g.bntx, txi = t - tzi * g.bntx;
-    const int bx0 = txi * TX, r0 = tzi * FR;
+    const int bx0 = txi * TX, r0 = tzi * RR;
     // per wave: staged p rows and u rows while propagating, then its partial image (FP64)
-    __shared__ __attribute__((aligned(16))) float lds[4][2][FR][HC];
-    static_assert(sizeof(lds[0]) >= FR * TX * sizeof(double), "partial image fits the wave's rows");
+    __shared__ __attribute__((aligned(16))) float lds[4][2][RR][HC];
+    static_assert(sizeof(lds[0]) >= RR * TX * sizeof(double), "partial image fits the wave's rows");
     const int nx = g.nx, nz = g.nz, ld = g.ld, npml = g.npml;
     const int bx = bx0 + lane;
     const bool colok = bx < nx;
     // ekp of the tile's rows, shared by every shot
-    float pk[FR];
+    float pk[RR];
     {
         const brsrc rk = buf_rsrc(A.b.ekp, (unsigned)(A.b.fs * sizeof(float)));
         const int vo = (colok ? bx + npml : npml) * 4;
 #pragma unroll
-        for (int d = 0; d < FR; ++d) pk[d] = (r0 + d < nz) ? buf_ld(rk, vo, (r0 + d) * ld * 4) : 0.f;
+        for (int d = 0; d < RR; ++d) pk[d] = (r0 + d < nz) ? buf_ld(rk, vo, (r0 + d) * ld * 4) : 0.f;
     }
-    double acc[FR];
+    double acc[RR];
 #pragma unroll
-    for (int d = 0; d < FR; ++d) acc[d] = 0.0;
+    for (int d = 0; d < RR; ++d) acc[d] = 0.0;
     const int s_lo = grp * gsize, s_hi = min(s_lo + gsize, A.b.nshots);
     for (int s = s_lo + w; s < s_hi; s += 4) {
-        const FwiShotDev sh = A.b.shots[s];
+        const FwiShotDev sh = shots[s];
         if (k >= sh.nt) continue;
         if (rec_tile_interior(g, sh, bx0, r0))
             rec_tile_shot<false>(A, sh, s, k, bx0, r0, lane, lds[w][0], lds[w][1], pk, acc);
@@ -1670,12 +1725,12 @@ __global__ __launch_bounds__(256) void k_rec_tile(const FwiRecArgs *__restrict__
     wave_sync();
     double(*red)[TX] = reinterpret_cast<double(*)[TX]>(&lds[w][0][0][0]);
 #pragma unroll
-    for (int d = 0; d < FR; ++d) red[d][lane] = acc[d];
+    for (int d = 0; d < RR; ++d) red[d][lane] = acc[d];
     __syncthreads();
     double *gacc = A.gacc + (long long)grp * nx * nz;
 #pragma unroll
-    for (int e = 0; e < FR / 4; ++e) {
-        const int d = w * (FR / 4) + e, i = r0 + d;
+    for (int e = 0; e < RR / 4; ++e) {
+        const int d = w * (RR / 4) + e, i = r0 + d;
         double sum = 0.0;
 #pragma unroll
         for (int q = 0; q < 4; ++q) sum += reinterpret_cast<double(*)[TX]>(&lds[q][0][0][0])[d][lane];
@@ -1930,7 +1985,7 @@ hipError_t fwi_launch_fwd_tile(const FwiFwdArgs *a, const FwiFwdArgs *a_dev, con
                                hipStream_t st)
 {
     const int n = a->g.ntx * a->g.ntz2 * a->b.nshots;
-    hipLaunchKernelGGL(k_fwd_tile, dim3(((n + 7) / 8) * 8), dim3(256), 0, st, a_dev, kbase, koff);
+    hipLaunchKernelGGL(k_fwd_tile, dim3(((n + 7) / 8) * 8), dim3(256), 0, st, a_dev, a->b.shots, kbase, koff);
     return hipGetLastError();
 }
 
@@ -1938,15 +1993,16 @@ hipError_t fwi_launch_adj_tile(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, con
                                hipStream_t st)
 {
     const int n = a->g.ntx * a->g.ntz2 * a->b.nshots;
-    hipLaunchKernelGGL(k_adj_tile, dim3(((n + 7) / 8) * 8), dim3(256), 0, st, a_dev, kbase, koff);
+    hipLaunchKernelGGL(k_adj_tile, dim3(((n + 7) / 8) * 8), dim3(256), 0, st, a_dev, a->b.shots, kbase, koff);
     return hipGetLastError();
 }
 
 hipError_t fwi_launch_rec_tile(const FwiRecArgs *a, const FwiRecArgs *a_dev, const int *kbase, int koff,
                                int gsize, int ngroups, hipStream_t st)
 {
-    const int n = a->g.bntx * ((a->g.nz + FR - 1) / FR) * ngroups;
-    hipLaunchKernelGGL(k_rec_tile, dim3(((n + 7) / 8) * 8), dim3(256), 0, st, a_dev, kbase, koff, gsize, ngroups);
+    const int n = a->g.bntx * ((a->g.nz + RR - 1) / RR) * ngroups;
+    hipLaunchKernelGGL(k_rec_tile, dim3(((n + 7) / 8) * 8), dim3(256), 0, st, a_dev, a->b.shots, kbase, koff, gsize,
+                       ngroups);
     return hipGetLastError();
 }
 

@@ -8,8 +8,13 @@
  * on the GPU -> L2 misfit and adjoint source on the host in FP64 ->
  * exact-adjoint back-propagation, reconstruction and zero-lag imaging on the
  * GPU -> gradient copied back once.  Data sampled every msf->jump steps
- * (fwi_2d.c:195): the adjoint source is the residual placed on those steps
- * (the exact transpose of the subsampling).
+ * (fwi_2d.c:195).  The adjoint source at the propagation rate is, by
+ * grad4_set_residual_mode:
+ *   0 (default) the residual placed on the sampled steps, zero between them
+ *     (the exact transpose of the subsampling: the gradient is exact);
+ *   1 the residual upsampled by the natural cubic spline of trcinp
+ *     (spline_interp.c:19-38), the reference's resampling of traces between
+ *     the data rate and the propagation rate.
  */
 #include <math.h>
 #include <stdarg.h>
@@ -18,11 +23,20 @@
 #include <string.h>
 
 #include "fwi_hip.h"
+#include "fwi_io.h"
 #include "fwi_plan.h"
 
 static fwi_hip_ctx *g_ctx = NULL;
 static long g_key[8];
 static char g_msg[512] = "";
+static int g_res_mode = 0;
+
+int grad4_set_residual_mode(int mode)
+{
+    if (mode < 0 || mode > 1) return FWI_HIP_EINVAL;
+    g_res_mode = mode;
+    return FWI_HIP_OK;
+}
 
 static int fail(int code, const char *fmt, ...)
 {
@@ -116,9 +130,9 @@ int grad_all_shots(fwi_plan **plans, int nshots, int k)
         if (rc) rc = fail(rc, "forward: %s", fwi_hip_last_error(g_ctx));
     }
     if (!rc) {
-        /* host misfit step: sample every jump, L2, adjoint source on the samples */
+        /* host misfit step: sample every jump, L2, adjoint source at the fine rate */
         long long o = 0;
-        for (int s = 0; s < nshots; s++) {
+        for (int s = 0; s < nshots && !rc; s++) {
             fwi_msf *m = plans[s]->msf;
             const int nr = (int)m->nr, nt0 = (int)m->nt, jump = (int)m->jump;
             double J = 0.0;
@@ -130,15 +144,27 @@ int grad_all_shots(fwi_plan **plans, int nshots, int k)
                     const float res = syn - m->obs0[r][i0];
                     J += 0.5 * (double)res * (double)res;
                 }
-                for (int it = nt - 1; it >= 0; it--) /* residual at fine dt, zero between samples */
-                    t[it] = (it % jump == 0) ? t[it] - m->obs0[r][it / jump] : 0.0f;
+                if (g_res_mode == 0 || jump == 1) {
+                    for (int it = nt - 1; it >= 0; it--) /* residual on the samples, zero between them */
+                        t[it] = (it % jump == 0) ? t[it] - m->obs0[r][it / jump] : 0.0f;
+                } else {
+                    float *r0 = (float *)malloc(sizeof(float) * nt0);
+                    if (!r0) { rc = fail(FWI_HIP_ENOMEM, "host memory"); break; }
+                    for (int i0 = 0; i0 < nt0; i0++) r0[i0] = m->syn[r][i0] - m->obs0[r][i0];
+                    float *rows0[1] = {r0}, *rows1[1] = {t};
+                    int e = fwi_trcinp(rows0, nt0, m->dt, rows1, nt, b->dt, 1);
+                    free(r0);
+                    if (e) { rc = fail(e, "trcinp: %s", fwi_io_last_error()); break; }
+                }
             }
             m->val[0] = (float)J;
             m->val[1] = (float)J;
             o += (long long)nr * nt;
         }
-        rc = fwi_hip_set_residual(g_ctx, tr);
-        if (rc) rc = fail(rc, "set_residual: %s", fwi_hip_last_error(g_ctx));
+        if (!rc) {
+            rc = fwi_hip_set_residual(g_ctx, tr);
+            if (rc) rc = fail(rc, "set_residual: %s", fwi_hip_last_error(g_ctx));
+        }
     }
     if (!rc) {
         rc = fwi_hip_adjoint(g_ctx, grad, NULL);
@@ -209,6 +235,8 @@ fwi_plan *fwi_plan_alloc(int nx, int nz, int npml, int nt, int jump, int nr, flo
     p->model->md = alloc2d(nx, nz);
     p->model->kp = alloc2d(nx, nz);
     p->model->ekp = alloc2d(gf->mnx, gf->mnz);
+    p->model->btry = (float *)calloc(nx, sizeof(float));
+    p->model->ibtry = (int *)calloc(nx, sizeof(int));
     const int nt0 = (nt - 1) / jump + 1;
     const fwi_msf msf = {.nt = (size_t)nt0, .nr = (size_t)nr, .jump = (size_t)jump, .dt = dt * jump};
     p->msf = (fwi_msf *)malloc(sizeof(fwi_msf));
@@ -217,6 +245,8 @@ fwi_plan *fwi_plan_alloc(int nx, int nz, int npml, int nt, int jump, int nr, flo
     p->msf->obs = alloc2d(nt0, nr);
     p->msf->syn = alloc2d(nt0, nr);
     p->msf->val = (float *)calloc(2, sizeof(float));
+    p->msf->fbt = (float *)calloc(nr ? nr : 1, sizeof(float));
+    p->msf->ifbt = (int *)calloc(nr ? nr : 1, sizeof(int));
     p->sou = (fwi_sou *)calloc(1, sizeof(fwi_sou));
     p->sou->iarea = 1.0 / (dx * dz);
     p->sou->ixyz_sr = (int *)calloc((size_t)(1 + nr) * 2, sizeof(int));
@@ -236,11 +266,15 @@ void fwi_plan_free(fwi_plan *p)
     free2d(p->model->md);
     free2d(p->model->kp);
     free2d(p->model->ekp);
+    free(p->model->btry);
+    free(p->model->ibtry);
     free(p->model);
     free2d(p->msf->obs0);
     free2d(p->msf->obs);
     free2d(p->msf->syn);
     free(p->msf->val);
+    free(p->msf->fbt);
+    free(p->msf->ifbt);
     free(p->msf);
     free(p->sou->ixyz_sr);
     free(p->sou->xyz_sr);

@@ -70,7 +70,26 @@ def lib():
     h.grad_all_shots.argtypes = [ctypes.POINTER(ctypes.POINTER(FwiPlan)), c_int, c_int]
     h.grad_all_shots.restype = c_int
     h.grad4_last_error.restype = ctypes.c_char_p
+    h.fwi_read_f32.argtypes = [ctypes.c_char_p, PF, c_size_t, c_int]
+    h.fwi_read_f32.restype = c_int
+    h.fwi_write_f32.argtypes = [ctypes.c_char_p, PF, c_size_t]
+    h.fwi_write_f32.restype = c_int
+    h.fwi_plan_read_inputs.argtypes = [ctypes.POINTER(FwiPlan), ctypes.c_char_p, c_int, ctypes.c_char_p, c_int,
+                                       c_int, c_float]
+    h.fwi_plan_read_inputs.restype = c_int
+    h.fwi_trcinp.argtypes = [PPF, c_int, c_float, PPF, c_int, c_float, c_int]
+    h.fwi_trcinp.restype = c_int
+    h.fwi_io_last_error.restype = ctypes.c_char_p
+    h.grad4_set_residual_mode.argtypes = [c_int]
+    h.grad4_set_residual_mode.restype = c_int
     return h
+
+
+def rows(a):
+    """row-pointer array (float **) over a C-contiguous float32 2-D numpy array"""
+    assert a.dtype == np.float32 and a.flags.c_contiguous and a.ndim == 2
+    base = a.ctypes.data
+    return (PF * a.shape[0])(*[ctypes.cast(base + i * a.strides[0], PF) for i in range(a.shape[0])])
 
 
 def view(ptr, shape):

@@ -30,8 +30,11 @@ def test_library_exports_every_declared_symbol():
 
 def test_plan_header_symbols_exported():
     lib = L.load()
-    for n in declared("fwi_plan.h"):
-        assert hasattr(lib, n), n
+    for hdr in ("fwi_plan.h", "fwi_io.h"):
+        names = declared(hdr)
+        assert names, hdr
+        for n in names:
+            assert hasattr(lib, n), n
 
 
 def _desc(**kw):

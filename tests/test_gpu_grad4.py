@@ -74,3 +74,32 @@ def test_grad_all_shots_sums_shots(h):
     assert rel(body, gsum) < 1e-4
     for p in plans:
         h.fwi_plan_free(p)
+
+
+def test_grad4_spline_residual_mode(h):
+    """Residual mode 1: the data-rate residual is upsampled by the natural
+    cubic spline of trcinp (spline_interp.c:19-38) before back-propagation;
+    the oracle gets the same residual from scipy's natural spline."""
+    from scipy.interpolate import CubicSpline
+    jump = 4
+    g, vel, shot = case(nt=241, seed=9)
+    o = Oracle(np.float32)
+    obs0 = np.ascontiguousarray(o.forward(g, vel * 1.02, shot)["syn"][:, ::jump])
+    p = PC.make_plan(h, g, shot, obs0, vel, jump=jump)
+    assert h.grad4_set_residual_mode(1) == 0
+    try:
+        h.grad4_1shot(p, 0)
+    finally:
+        h.grad4_set_residual_mode(0)
+    body = PC.view(p.contents.grad.contents.body, (g.nz, g.nx)).copy()
+    f = o.forward(g, vel, shot, want_strips=True)
+    r0 = (f["syn"][:, ::jump] - obs0).astype(np.float64)
+    dt = np.float32(g.dt)
+    x0 = (np.arange(r0.shape[1], dtype=np.float32) * np.float32(dt * jump)).astype(np.float64)
+    x1 = (np.arange(shot.nt, dtype=np.float32) * dt).astype(np.float64)
+    res = CubicSpline(x0, r0, axis=1, bc_type="natural")(x1)
+    res[:, 0], res[:, -1] = r0[:, 0], r0[:, -1]
+    b = o.backward(g, vel, shot, res.astype(np.float32), f)
+    print(f"spline-residual grad rel {rel(body, b['grad']):.2e}")
+    assert rel(body, b["grad"]) < 1e-4
+    h.fwi_plan_free(p)
