@@ -16,6 +16,7 @@
 //   write_bdr, fwi_2d.c:893-910), one block per step.
 //   traces: time-major [shot][nt][nr] while propagating.
 #pragma once
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #define FWI_TX 64      // x cells per wave (one lane per column)
@@ -47,6 +48,49 @@ struct FwiGeomDev {
 #ifndef FWI_TZ2
 #define FWI_TZ2 32       // rows of a 2-D tile (4 waves x 8 output rows)
 #endif
+
+// rows per wave of the tile reconstruction kernel (its FP64 image
+// accumulators grow with it, so it is set apart from the forward/adjoint)
+#ifndef FWI_REC_ROWS
+#define FWI_REC_ROWS 8
+#endif
+
+// Strip classes of the 2-D tile kernels.  A wave owns a strip of 64 columns x
+// FWI_STRIP rows; a strip is "interior" when none of its cells, nor any u/v
+// node it evaluates, needs a CPML, edge, free-surface, frame, source or
+// receiver rule.  Interior and general strips run in separate launches (each
+// with its own register budget); the host builds the lists with these same
+// functions.
+#define FWI_STRIP (FWI_TZ2 / 4)
+// forward strip at (x0, r0)
+__host__ __device__ inline bool fwi_fwd_strip_interior(const FwiGeomDev &g, const FwiShotDev &sh, int x0, int r0)
+{
+    const int r1 = r0 + FWI_STRIP;
+    if (x0 < g.npml + 3 || x0 + 66 > g.pnx - g.npml - 1) return false;  // u nodes x0-1 .. x0+65
+    if (r0 < 3 || r1 > g.nz - 3) return false;                           // v nodes r0-1 .. r1+1
+    if (sh.isz >= r0 && sh.isz < r1 && sh.isx >= x0 && sh.isx < x0 + FWI_TX) return false;
+    if (sh.rz_lo <= sh.rz_hi && sh.rz_lo < r1 && sh.rz_hi >= r0) return false;
+    return true;
+}
+// adjoint strip: no source term; the transposed free-surface rows reach node 4
+__host__ __device__ inline bool fwi_adj_strip_interior(const FwiGeomDev &g, const FwiShotDev &sh, int x0, int r0)
+{
+    const int r1 = r0 + FWI_STRIP;
+    if (x0 < g.npml + 3 || x0 + 66 > g.pnx - g.npml - 1) return false;
+    if (r0 < 6 || r1 > g.nz - 3) return false;
+    if (sh.rz_lo <= sh.rz_hi && sh.rz_lo < r1 && sh.rz_hi >= r0) return false;
+    return true;
+}
+// reconstruction tile: 64 x FWI_REC_ROWS physical cells at (bx0, z0), every
+// cell inner (3 or more from the edge) and the source elsewhere
+__host__ __device__ inline bool fwi_rec_tile_interior(const FwiGeomDev &g, const FwiShotDev &sh, int bx0, int z0)
+{
+    if (bx0 < 3 || bx0 + 66 >= g.nx) return false;
+    if (z0 < 3 || z0 + FWI_REC_ROWS > g.nz - 3) return false;
+    const int sx = sh.isx - g.npml;
+    if (sh.isz >= z0 && sh.isz < z0 + FWI_REC_ROWS && sx >= bx0 && sx < bx0 + FWI_TX) return false;
+    return true;
+}
 
 struct FwiBatchDev {
     int nshots, sg, ngroups;

@@ -19,13 +19,13 @@
 
 extern "C" {
 hipError_t fwi_launch_fwd(const FwiFwdArgs *a, const int *kbase, int koff, int nblocks, hipStream_t st);
-hipError_t fwi_launch_fwd_tile(const FwiFwdArgs *a, const FwiFwdArgs *a_dev, const int *kbase, int koff,
-                               hipStream_t st);
+hipError_t fwi_launch_fwd_tile(const FwiFwdArgs *a, const FwiFwdArgs *a_dev, const int *const *tl, const int *ntl,
+                               const int *kbase, int koff, hipStream_t st);
 hipError_t fwi_launch_adj(const FwiAdjArgs *a, const int *kbase, int koff, int nblocks, hipStream_t st);
-hipError_t fwi_launch_adj_tile(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, const int *kbase, int koff,
-                               hipStream_t st);
-hipError_t fwi_launch_rec_tile(const FwiRecArgs *a, const FwiRecArgs *a_dev, const int *kbase, int koff,
-                               int gsize, int ngroups, hipStream_t st);
+hipError_t fwi_launch_adj_tile(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, const int *const *tl, const int *ntl,
+                               const int *kbase, int koff, hipStream_t st);
+hipError_t fwi_launch_rec_tile(const FwiRecArgs *a, const FwiRecArgs *a_dev, const int *const *tl, const int *ntl,
+                               const int *kbase, int koff, int gsize, hipStream_t st);
 hipError_t fwi_launch_rec(const FwiRecArgs *a, const int *kbase, int koff, int nblocks, hipStream_t st);
 hipError_t fwi_launch_advance(int *kbase, int delta, hipStream_t st);
 hipError_t fwi_launch_set_int(int *p, int v, hipStream_t st);
@@ -94,6 +94,10 @@ struct fwi_hip_ctx {
     int graph_steps = 100;
     int fwd_kind = 0;  // 0: 2-D tile kernels, 1: z-marching kernels
     int rec_group = 8; // shots per workgroup of the 2-D tile reconstruction kernel
+    // work lists of the 2-D tile kernels, [0] interior, [1] general (strips; imaging: group x tile)
+    int *tl_fwd[2] = {nullptr, nullptr}, *tl_adj[2] = {nullptr, nullptr}, *tl_rec[2] = {nullptr, nullptr};
+    int n_fwd[2] = {0, 0}, n_adj[2] = {0, 0}, n_rec[2] = {0, 0};
+    int rec_lists_for = -1;  // rec_group the reconstruction lists were built for (-1: none)
     hipGraphExec_t gx_fwd = nullptr, gx_bwd = nullptr;
     // timing
     bool timing = false;
@@ -325,6 +329,12 @@ extern "C" int fwi_hip_create(const fwi_hip_desc *desc, fwi_hip_ctx **out)
         {(void **)&c->rrow, sizeof(int) * S * g.pnz},
         {(void **)&c->kbase, sizeof(int) * 64},
         {(void **)&c->fwd_args_d, sizeof(FwiFwdArgs)},
+        {(void **)&c->tl_fwd[0], sizeof(int) * S * g.ntx * ((g.pnz + FWI_STRIP - 1) / FWI_STRIP)},
+        {(void **)&c->tl_fwd[1], sizeof(int) * S * g.ntx * ((g.pnz + FWI_STRIP - 1) / FWI_STRIP)},
+        {(void **)&c->tl_adj[0], sizeof(int) * S * g.ntx * ((g.pnz + FWI_STRIP - 1) / FWI_STRIP)},
+        {(void **)&c->tl_adj[1], sizeof(int) * S * g.ntx * ((g.pnz + FWI_STRIP - 1) / FWI_STRIP)},
+        {(void **)&c->tl_rec[0], sizeof(int) * S * g.bntx * ((g.nz + FWI_REC_ROWS - 1) / FWI_REC_ROWS)},
+        {(void **)&c->tl_rec[1], sizeof(int) * S * g.bntx * ((g.nz + FWI_REC_ROWS - 1) / FWI_REC_ROWS)},
         {(void **)&c->adj_args_d, sizeof(FwiAdjArgs)},
         {(void **)&c->rec_args_d, sizeof(FwiRecArgs)},
     };
@@ -478,6 +488,29 @@ extern "C" int fwi_hip_set_shots(fwi_hip_ctx *c, int nshots, const fwi_hip_shot 
     HIPCHK(c, hipMemcpyAsync(c->rmap, rmap.data(), sizeof(int) * rmap.size(), hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipMemcpyAsync(c->rrow, rrow.data(), sizeof(int) * rrow.size(), hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipMemcpyAsync(c->shots_d, sd.data(), sizeof(FwiShotDev) * nshots, hipMemcpyHostToDevice, c->st));
+    // strip lists of the forward and adjoint tile kernels (logical strip (s nsz + sz) ntx + tx)
+    {
+        const int nsz = (g.pnz + FWI_STRIP - 1) / FWI_STRIP;
+        std::vector<int> lf[2], la[2];
+        for (int s = 0; s < nshots; s++)
+            for (int sz = 0; sz < nsz; sz++)
+                for (int tx = 0; tx < g.ntx; tx++) {
+                    const int x0 = tx * FWI_TX, r0 = sz * FWI_STRIP, l = (s * nsz + sz) * g.ntx + tx;
+                    lf[fwi_fwd_strip_interior(g, sd[s], x0, r0) ? 0 : 1].push_back(l);
+                    la[fwi_adj_strip_interior(g, sd[s], x0, r0) ? 0 : 1].push_back(l);
+                }
+        for (int q = 0; q < 2; q++) {
+            c->n_fwd[q] = (int)lf[q].size();
+            c->n_adj[q] = (int)la[q].size();
+            if (c->n_fwd[q])
+                HIPCHK(c, hipMemcpyAsync(c->tl_fwd[q], lf[q].data(), sizeof(int) * lf[q].size(), hipMemcpyHostToDevice,
+                                         c->st));
+            if (c->n_adj[q])
+                HIPCHK(c, hipMemcpyAsync(c->tl_adj[q], la[q].data(), sizeof(int) * la[q].size(), hipMemcpyHostToDevice,
+                                         c->st));
+        }
+        c->rec_lists_for = -1;
+    }
     HIPCHK(c, hipStreamSynchronize(c->st));
     if (c->gx_fwd) (void)hipGraphExecDestroy(c->gx_fwd);
     if (c->gx_bwd) (void)hipGraphExecDestroy(c->gx_bwd);
@@ -591,6 +624,7 @@ extern "C" int fwi_hip_set_rec_group(fwi_hip_ctx *c, int shots)
     if (c->gx_bwd) (void)hipGraphExecDestroy(c->gx_bwd);
     c->gx_bwd = nullptr;
     c->rec_group = shots;
+    c->rec_lists_for = -1;
     return FWI_HIP_OK;
 }
 
@@ -639,7 +673,7 @@ extern "C" int fwi_hip_forward(fwi_hip_ctx *c, float *traces)
                 hipError_t e = hipSuccess;
                 for (int j = 0; j < B && e == hipSuccess; j++)
                     e = c->fwd_kind ? fwi_launch_fwd(&a, c->kbase, j, nblocks, c->st)
-                                    : fwi_launch_fwd_tile(&a, ad, c->kbase, j, c->st);
+                                    : fwi_launch_fwd_tile(&a, ad, c->tl_fwd, c->n_fwd, c->kbase, j, c->st);
                 if (e == hipSuccess) e = fwi_launch_advance(c->kbase, B, c->st);
                 return e;
             });
@@ -652,7 +686,7 @@ extern "C" int fwi_hip_forward(fwi_hip_ctx *c, float *traces)
             hipEvent_t e0, e1;
             if ((rc = tick(c, &e0))) return rc;
             HIPCHK(c, c->fwd_kind ? fwi_launch_fwd(&a, nullptr, k, nblocks, c->st)
-                                  : fwi_launch_fwd_tile(&a, ad, nullptr, k, c->st));
+                                  : fwi_launch_fwd_tile(&a, ad, c->tl_fwd, c->n_fwd, nullptr, k, c->st));
             if ((rc = tick(c, &e1))) return rc;
             if (c->timing) marks.push_back({0, {e0, e1}});
             if (marks.size() >= 512 && (rc = collect_timing(c, marks))) return rc;
@@ -756,13 +790,36 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
     HIPCHK(c, hipMemcpyAsync(c->rec_args_d, &c->rec_args_h, sizeof(FwiRecArgs), hipMemcpyHostToDevice, c->st));
     const FwiAdjArgs *ad = c->adj_args_d;
     const FwiRecArgs *rd = c->rec_args_d;
+    if (tile && c->rec_lists_for != rgs) {
+        // items (group, tile) of the reconstruction kernel; interior when the tile is
+        // interior for every shot of the group
+        const int ntz = (g.nz + FWI_REC_ROWS - 1) / FWI_REC_ROWS, ntiles = g.bntx * ntz;
+        std::vector<int> lr[2];
+        for (int grp = 0; grp < ngr; grp++)
+            for (int t = 0; t < ntiles; t++) {
+                const int bx0 = (t % g.bntx) * FWI_TX, z0 = (t / g.bntx) * FWI_REC_ROWS;
+                bool in = true;
+                for (int s = grp * rgs; s < std::min((grp + 1) * rgs, c->nshots) && in; s++)
+                    in = fwi_rec_tile_interior(g, c->shots_h[s], bx0, z0);
+                lr[in ? 0 : 1].push_back(grp * ntiles + t);
+            }
+        for (int q = 0; q < 2; q++) {
+            c->n_rec[q] = (int)lr[q].size();
+            if (c->n_rec[q])
+                HIPCHK(c, hipMemcpyAsync(c->tl_rec[q], lr[q].data(), sizeof(int) * lr[q].size(), hipMemcpyHostToDevice,
+                                         c->st));
+        }
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        c->rec_lists_for = rgs;
+    }
     const int nb_adj = g.ntx * g.ntz * c->ngroups;
     const int nb_rec = g.bntx * g.bntz * c->ngroups;
     auto launch_adj = [&](const int *kb, int koff) {
-        return tile ? fwi_launch_adj_tile(&a, ad, kb, koff, c->st) : fwi_launch_adj(&a, kb, koff, nb_adj, c->st);
+        return tile ? fwi_launch_adj_tile(&a, ad, c->tl_adj, c->n_adj, kb, koff, c->st)
+                    : fwi_launch_adj(&a, kb, koff, nb_adj, c->st);
     };
     auto launch_rec = [&](const int *kb, int koff) {
-        return tile ? fwi_launch_rec_tile(&r, rd, kb, koff, rgs, ngr, c->st)
+        return tile ? fwi_launch_rec_tile(&r, rd, c->tl_rec, c->n_rec, kb, koff, rgs, c->st)
                     : fwi_launch_rec(&r, kb, koff, nb_rec, c->st);
     };
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;

@@ -103,10 +103,30 @@ __device__ __forceinline__ int halo_col(int x0, int lane)
 #define FWI_MAXNPML 64
 #define FWI_TABN (6 * (2 * FWI_MAXNPML + 1))
 
+// any workgroup size (the z-marching kernels)
 __device__ __forceinline__ void stage_tables(float *stab, const float *tab, int npml)
 {
     const int n = 6 * (2 * npml + 1);
     for (int i = threadIdx.x; i < n; i += blockDim.x) stab[i] = tab[i];
+    __syncthreads();
+}
+// 256-thread workgroups (the tile kernels): every load issued before the first
+// LDS store, so the staging costs one memory latency instead of one per pass
+__device__ __forceinline__ void stage_tables_256(float *stab, const float *tab, int npml)
+{
+    constexpr int NP = (FWI_TABN + 255) / 256;
+    const int n = 6 * (2 * npml + 1);
+    float t[NP];
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+        const int i = threadIdx.x + 256 * q;
+        t[q] = tab[i < n ? i : 0];
+    }
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+        const int i = threadIdx.x + 256 * q;
+        if (i < n) stab[i] = t[q];
+    }
     __syncthreads();
 }
 
@@ -497,16 +517,115 @@ __device__ __forceinline__ int xhalo_lds_col(const XHalo<NP> &h, int q, int x0)
     return h.in[q] ? c : HJ;
 }
 
-// true when no cell of the tile (64 x FWI_TZ2 at x0, z0), nor any u/v node it
-// evaluates, needs a CPML, edge, free-surface, frame, source or receiver rule
-__device__ __forceinline__ bool fwd_tile_interior(const FwiGeomDev &g, const FwiShotDev &sh, int x0, int z0)
+// CPML memory index of u node j (x strips, row r), or -1 outside the strips
+__device__ __forceinline__ int xh_idx(const FwiGeomDev &g, int j, int r)
 {
-    const int z1 = z0 + FWI_TZ2;
-    if (x0 < g.npml + 3 || x0 + 66 > g.pnx - g.npml - 1) return false;  // u nodes x0-1 .. x0+65
-    if (z0 < 3 || z1 > g.nz - 3) return false;                           // v nodes z0-1 .. z1+1
-    if (sh.isz >= z0 && sh.isz < z1 && sh.isx >= x0 && sh.isx < x0 + TX) return false;
-    if (sh.rz_lo <= sh.rz_hi && sh.rz_lo < z1 && sh.rz_hi >= z0) return false;
-    return true;
+    const int npml = g.npml, n1 = npml + 1;
+    if (j >= 0 && j <= npml) return r * 2 * n1 + j;
+    if (j >= g.pnx - npml && j <= g.pnx) return r * 2 * n1 + n1 + (j - (g.pnx - npml));
+    return -1;
+}
+// CPML memory index of p node i (x strips, row r), or -1 outside the strips
+__device__ __forceinline__ int xd_idx(const FwiGeomDev &g, int i, int r)
+{
+    const int npml = g.npml;
+    if (i >= 0 && i < npml) return r * 2 * npml + i;
+    if (i >= g.pnx - npml && i < g.pnx) return r * 2 * npml + npml + (i - (g.pnx - npml));
+    return -1;
+}
+// u at half node j with the CPML update from the old memory value po (new in pn):
+// the arithmetic of fwd_u_node, memory traffic left to the caller
+__device__ __forceinline__ float fwd_u_edge(const FwiGeomDev &g, const float *tab, const float *Ps, int j, int pi,
+                                            float po, float &pn)
+{
+    const int pnx = g.pnx, npml = g.npml, T = 2 * npml + 1;
+    float u;
+    if (j <= 0 || j >= pnx) u = 0.f;
+    else if (j == 1 || j == pnx - 1) u = g.ix * (Ps[pi] - Ps[pi - 1]);
+    else u = g.c0x * (Ps[pi] - Ps[pi - 1]) + g.c1x * (Ps[pi + 1] - Ps[pi - 2]);
+    pn = 0.f;
+    if (j >= 0 && j <= npml) {
+        pn = TB(1, 2 * j) * po + TB(0, 2 * j) * u;
+        u = u + pn;
+    } else if (j >= pnx - npml && j <= pnx) {
+        const int q = j - (pnx - npml);
+        pn = TB(3, 2 * q) * po + TB(2, 2 * q) * u;
+        u = u + pn;
+    }
+    return u;
+}
+// ux at p node i (ui = LDS index of u[i]) with the CPML update: fwd_ux_node's arithmetic
+__device__ __forceinline__ float fwd_ux_edge(const FwiGeomDev &g, const float *tab, const float *Us, int i, int ui,
+                                             float po, float &pn)
+{
+    const int pnx = g.pnx, npml = g.npml, T = 2 * npml + 1;
+    pn = 0.f;
+    if (i >= pnx) return 0.f;
+    float ux;
+    if (i == 0 || i == pnx - 1) ux = (Us[ui + 1] - Us[ui]) * g.ix;
+    else ux = g.c0x * (Us[ui + 1] - Us[ui]) + g.c1x * (Us[ui + 2] - Us[ui - 1]);
+    if (i < npml) {
+        pn = TB(1, 2 * i + 1) * po + TB(0, 2 * i + 1) * ux;
+        ux = ux + pn;
+    } else if (i >= pnx - npml) {
+        const int q = i - (pnx - npml);
+        pn = TB(3, 2 * q + 1) * po + TB(2, 2 * q + 1) * ux;
+        ux = ux + pn;
+    }
+    return ux;
+}
+
+
+// Branch-free x chain of the general tiles: per-node coefficients, computed
+// once per wave, that reproduce fwd_u_edge / fwd_ux_edge exactly.  The
+// stencil is cA (f[+0] - f[-1]) + cB (f[+1] - f[-2]) (cB = 0 on the 2nd-order
+// edge nodes, both 0 outside the grid, so the added terms are exact zeros);
+// the CPML update is pn = cb po + ca x, x + pn (ca = cb = 0 outside the strips).
+struct XCoef {
+    float A, B, ca, cb;
+    int col;  // column of the node's CPML memory within its row (-1: none)
+};
+// u node j (between p[j-1] and p[j])
+__device__ __forceinline__ XCoef fwd_u_coef(const FwiGeomDev &g, const float *tab, int j)
+{
+    const int pnx = g.pnx, npml = g.npml, T = 2 * npml + 1, n1 = npml + 1;
+    XCoef c;
+    c.A = (j <= 0 || j >= pnx) ? 0.f : ((j == 1 || j == pnx - 1) ? g.ix : g.c0x);
+    c.B = (j <= 1 || j >= pnx - 1) ? 0.f : g.c1x;
+    c.ca = c.cb = 0.f;
+    c.col = -1;
+    if (j >= 0 && j <= npml) {
+        c.cb = TB(1, 2 * j);
+        c.ca = TB(0, 2 * j);
+        c.col = j;
+    } else if (j >= pnx - npml && j <= pnx) {
+        const int q = j - (pnx - npml);
+        c.cb = TB(3, 2 * q);
+        c.ca = TB(2, 2 * q);
+        c.col = n1 + q;
+    }
+    return c;
+}
+// ux node i (p node)
+__device__ __forceinline__ XCoef fwd_ux_coef(const FwiGeomDev &g, const float *tab, int i)
+{
+    const int pnx = g.pnx, npml = g.npml, T = 2 * npml + 1;
+    XCoef c;
+    c.A = (i < 0 || i >= pnx) ? 0.f : ((i == 0 || i == pnx - 1) ? g.ix : g.c0x);
+    c.B = (i <= 0 || i >= pnx - 1) ? 0.f : g.c1x;
+    c.ca = c.cb = 0.f;
+    c.col = -1;
+    if (i >= 0 && i < npml) {
+        c.cb = TB(1, 2 * i + 1);
+        c.ca = TB(0, 2 * i + 1);
+        c.col = i;
+    } else if (i >= pnx - npml && i < pnx) {
+        const int q = i - (pnx - npml);
+        c.cb = TB(3, 2 * q + 1);
+        c.ca = TB(2, 2 * q + 1);
+        c.col = npml + q;
+    }
+    return c;
 }
 
 template <bool EDGE>
@@ -551,13 +670,6 @@ __device__ __forceinline__ void fwd_tile_rows(const FwiFwdArgs &A, const FwiShot
         pk[d] = ok ? buf_ld(rk, vo, i * rowb) : 0.f;
     }
 
-    // ---- stage the own rows (with halo) for the x chain
-#pragma unroll
-    for (int d = 0; d < FR; ++d) Ps[d][lane + 3] = p[d + 3];
-#pragma unroll
-    for (int q = 0; q < HP; ++q) Ps[xh.d[q]][xhalo_lds_col(xh, q, x0)] = hv[q];
-    wave_sync();
-
     const float *xh_o = A.xh + s * A.b.xh_s + cur * A.b.xh_b;
     float *xh_n = A.xh + s * A.b.xh_s + nxt * A.b.xh_b;
     const float *xd_o = A.xd + s * A.b.xd_s + cur * A.b.xd_b;
@@ -568,26 +680,106 @@ __device__ __forceinline__ void fwd_tile_rows(const FwiFwdArgs &A, const FwiShot
     float *zd_n = A.zd + s * A.b.zd_s + nxt * A.b.zd_b;
     // u nodes x0-1 .. x0+65 and p nodes x0 .. x0+63 clear of the x CPML and edges
     const bool fastx = !EDGE || ((x0 >= npml + 3) && (x0 + 66 <= pnx - npml - 1));
+    const int ud0 = lane / 3;  // the extra-u-node pass: lane = 3 ud + (uc - 64)
+    bool uok = lane < 3 * FR;
+    if (EDGE) uok = uok && r0 + ud0 < pnz;
+    // idle lanes evaluate node x0+64 of row 0 (owned by the next tile, so no CPML store)
+    const int ud = uok ? ud0 : 0, uc = uok ? 64 + lane - 3 * ud0 : 65;
+
+    // ---- general tiles: every other load before the first store.  Vector
+    // memory operations retire in order, so a load issued after a store waits
+    // for that store; the CPML memory, receiver map and source sample are
+    // therefore read here, not where they are used.
+    float zho[FVN], zdo[FR], xuo[FR], xuho = 0.f, xdo[FR], sv = 0.f;
+    int rix[FR];
+    XCoef cu, cuh, cx;  // x chain coefficients of this lane's u node, extra u node and ux node
+    const int n1 = npml + 1;
+    // which special rules can occur in this strip at all (uniform): each is
+    // tested once here instead of per row and node
+    const bool zplain = !EDGE || (r0 >= 3 && r0 + FR + 1 < pnz - npml);  // v nodes and vz rows: plain
+    const bool has_recv = EDGE && sh.rz_lo <= sh.rz_hi && sh.rz_lo < r0 + FR && sh.rz_hi >= r0;
+    const bool has_src = EDGE && sh.isz >= r0 && sh.isz < r0 + FR && sh.isx >= x0 && sh.isx < x0 + TX;
+    const bool frame_rows = EDGE && (r0 < 3 || (r0 + FR > g.nz - 3 && r0 < g.nz));
+    const bool frame_cols = EDGE && ((x0 < npml + 3 && x0 + TX > npml) ||
+                                     (x0 < npml + g.nx && x0 + TX > npml + g.nx - 3));
+    if (EDGE && !fastx) {
+        cu = fwd_u_coef(g, tab, x0 - 1 + lane);
+        cuh = fwd_u_coef(g, tab, x0 - 1 + uc);
+        cx = fwd_ux_coef(g, tab, x);
+    }
+    if (EDGE) {
+        if (!zplain) {
+#pragma unroll
+            for (int n = 0; n < FVN; ++n) {
+                const int j = r0 - 1 + n;
+                zho[n] = (j >= pnz - npml && j < pnz) ? zh_o[(j - (pnz - npml)) * ld + x] : 0.f;
+            }
+#pragma unroll
+            for (int d = 0; d < FR; ++d) {
+                const int i = r0 + d;
+                zdo[d] = (i >= pnz - npml && i < pnz) ? zd_o[(i - (pnz - npml)) * ld + x] : 0.f;
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < FR; ++d) {
+            const int i = r0 + d;
+            xuo[d] = xdo[d] = 0.f;
+            if (!fastx && i < pnz) {  // x strips: per lane, clamped index, no divergence
+                xuo[d] = xh_o[i * 2 * n1 + (cu.col >= 0 ? cu.col : 0)];
+                xdo[d] = xd_o[i * 2 * npml + (cx.col >= 0 ? cx.col : 0)];
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < FR; ++d) rix[d] = -1;
+        if (has_recv) {  // every row slot first, then the lanes' receiver indices
+            int slot[FR];
+#pragma unroll
+            for (int d = 0; d < FR; ++d) {
+                const int i = r0 + d;
+                const bool in = i >= sh.rz_lo && i <= sh.rz_hi;
+                slot[d] = A.b.rrow[sh.rrow_off + (in ? i : 0)];
+                slot[d] = in ? slot[d] : -1;
+            }
+#pragma unroll
+            for (int d = 0; d < FR; ++d) rix[d] = (slot[d] >= 0) ? A.b.rmap[slot[d] * ld + x] : -1;
+        }
+        if (!fastx) xuho = xh_o[(r0 + ud) * 2 * n1 + (cuh.col >= 0 ? cuh.col : 0)];
+        if (has_src) sv = A.b.srcf[sh.src_off + k];
+    }
+
+    // ---- stage the own rows (with halo) for the x chain
+#pragma unroll
+    for (int d = 0; d < FR; ++d) Ps[d][lane + 3] = p[d + 3];
+#pragma unroll
+    for (int q = 0; q < HP; ++q) Ps[xh.d[q]][xhalo_lds_col(xh, q, x0)] = hv[q];
+    wave_sync();
 
     // ---- u nodes x0-1+c, c = 0..66 (LDS column c); c = lane, then c = 64..66 for all rows
 #pragma unroll
     for (int d = 0; d < FR; ++d) {
         if (EDGE && r0 + d >= pnz) break;
         const float *P = Ps[d];
-        if (fastx) Us[d][lane] = g.c0x * (P[lane + 2] - P[lane + 1]) + g.c1x * (P[lane + 3] - P[lane]);
-        else Us[d][lane] = fwd_u_node(g, tab, P, x0 - 1 + lane, lane + 2, r0 + d, xh_o, xh_n, x0);
+        if (fastx) {
+            Us[d][lane] = g.c0x * (P[lane + 2] - P[lane + 1]) + g.c1x * (P[lane + 3] - P[lane]);
+        } else {
+            float u = cu.A * (P[lane + 2] - P[lane + 1]) + cu.B * (P[lane + 3] - P[lane]);
+            const float pn = cu.cb * xuo[d] + cu.ca * u;
+            u = u + pn;
+            Us[d][lane] = u;
+            if (cu.col >= 0 && lane >= 1) xh_n[(r0 + d) * 2 * n1 + cu.col] = pn;  // owned: x0 <= j < x0+64
+        }
     }
     {
-        // lane = 3 ud + (uc - 64); idle lanes evaluate node x0+64 of row 0 (owned by
-        // the next tile, so no CPML store) into the spare column
-        const int ud0 = lane / 3;
-        bool uok = lane < 3 * FR;
-        if (EDGE) uok = uok && r0 + ud0 < pnz;
-        const int ud = uok ? ud0 : 0, uc = uok ? 64 + lane - 3 * ud0 : 65;
         const float *P = Ps[ud];
         float u;
-        if (fastx) u = g.c0x * (P[uc + 2] - P[uc + 1]) + g.c1x * (P[uc + 3] - P[uc]);
-        else u = fwd_u_node(g, tab, P, x0 - 1 + uc, uc + 2, r0 + ud, xh_o, xh_n, x0);
+        if (fastx) {
+            u = g.c0x * (P[uc + 2] - P[uc + 1]) + g.c1x * (P[uc + 3] - P[uc]);
+        } else {
+            u = cuh.A * (P[uc + 2] - P[uc + 1]) + cuh.B * (P[uc + 3] - P[uc]);
+            const float pn = cuh.cb * xuho + cuh.ca * u;
+            u = u + pn;
+            if (cuh.col >= 0 && uc == 64) xh_n[(r0 + ud) * 2 * n1 + cuh.col] = pn;  // owned: node x0+63
+        }
         Us[ud][uok ? uc : HJ] = u;
     }
     wave_sync();
@@ -598,7 +790,7 @@ __device__ __forceinline__ void fwd_tile_rows(const FwiFwdArgs &A, const FwiShot
     for (int n = 0; n < FVN; ++n) {
         const int j = r0 - 1 + n;
         float vv;
-        if (!EDGE) {
+        if (zplain) {
             vv = g.c0z * (p[n + 2] - p[n + 1]) + g.c1z * (p[n + 3] - p[n]);
         } else {
             if (j < 0 || j >= pnz) vv = 0.f;
@@ -608,8 +800,7 @@ __device__ __forceinline__ void fwd_tile_rows(const FwiFwdArgs &A, const FwiShot
             else vv = g.c0z * (p[n + 2] - p[n + 1]) + g.c1z * (p[n + 3] - p[n]);
             if (j >= pnz - npml && j < pnz) {
                 const int q = j - (pnz - npml);
-                float ps = zh_o[q * ld + x];
-                ps = TB(5, 2 * q) * ps + TB(4, 2 * q) * vv;
+                const float ps = TB(5, 2 * q) * zho[n] + TB(4, 2 * q) * vv;
                 vv = vv + ps;
                 if (n >= 1 && n <= FR) zh_n[q * ld + x] = ps;  // own nodes r0 .. r0+FR-1
             }
@@ -625,9 +816,15 @@ __device__ __forceinline__ void fwd_tile_rows(const FwiFwdArgs &A, const FwiShot
         if (EDGE && i >= pnz) break;
         const float *U = Us[d];
         float ux, vz;
-        ux = fastx ? g.c0x * (U[lane + 2] - U[lane + 1]) + g.c1x * (U[lane + 3] - U[lane])
-                   : fwd_ux_node(g, tab, U, x, lane + 1, i, xd_o, xd_n);
-        if (!EDGE) {
+        if (fastx) {
+            ux = g.c0x * (U[lane + 2] - U[lane + 1]) + g.c1x * (U[lane + 3] - U[lane]);
+        } else {
+            ux = cx.A * (U[lane + 2] - U[lane + 1]) + cx.B * (U[lane + 3] - U[lane]);
+            const float pn = cx.cb * xdo[d] + cx.ca * ux;
+            ux = ux + pn;
+            if (cx.col >= 0) xd_n[i * 2 * npml + cx.col] = pn;
+        }
+        if (zplain) {
             vz = g.c0z * (v[d + 2] - v[d + 1]) + g.c1z * (v[d + 3] - v[d]);
         } else {
             if (d == 0 && i == 0) vz = vz_row0(v[d + 1], v[d + 2], v[d + 3], v[d + 4], v[d + 5], g.iz24);  // nodes 0..4
@@ -635,16 +832,12 @@ __device__ __forceinline__ void fwd_tile_rows(const FwiFwdArgs &A, const FwiShot
             else vz = g.c0z * (v[d + 2] - v[d + 1]) + g.c1z * (v[d + 3] - v[d]);
             if (i >= pnz - npml) {
                 const int q = i - (pnz - npml);
-                float ps = zd_o[q * ld + x];
-                ps = TB(5, 2 * q + 1) * ps + TB(4, 2 * q + 1) * vz;
+                const float ps = TB(5, 2 * q + 1) * zdo[d] + TB(4, 2 * q + 1) * vz;
                 vz = vz + ps;
                 zd_n[q * ld + x] = ps;
             }
-            if (i == sh.isz) {
-                const float sv = A.b.srcf[sh.src_off + k];
-                ux = (x == sh.isx) ? ux + sv : ux;
-            }
         }
+        if (has_src && i == sh.isz) ux = (x == sh.isx) ? ux + sv : ux;
         const float X = ux + vz;
         const float pnew = leapfrog(p[d + 3], po[d], pk[d] * X);
         if (!EDGE) {
@@ -652,14 +845,9 @@ __device__ __forceinline__ void fwd_tile_rows(const FwiFwdArgs &A, const FwiShot
             continue;
         }
         if (x < pnx) buf_st(pnew, rold, vo, i * rowb);
-        if (i >= sh.rz_lo && i <= sh.rz_hi) {
-            const int slot = A.b.rrow[sh.rrow_off + i];
-            if (slot >= 0) {
-                const int ri = A.b.rmap[slot * ld + x];
-                if (ri >= 0) A.syn[sh.trace_off + (long long)k * sh.nr + ri] = pnew;
-            }
-        }
-        if (bx >= 0 && bx < nx && i < nz && (i < 3 || i >= nz - 3 || bx < 3 || bx >= nx - 3)) {
+        if (has_recv && rix[d] >= 0) A.syn[sh.trace_off + (long long)k * sh.nr + rix[d]] = pnew;
+        if ((frame_rows || frame_cols) && bx >= 0 && bx < nx && i < nz &&
+            (i < 3 || i >= nz - 3 || bx < 3 || bx >= nx - 3)) {
             const long long so2 = sh.strip_off + (long long)k * g.sl;
             if (i < 3 || i >= nz - 3) {
                 const int pos = (i < 3) ? i * nx + bx : (3 + nz - 1 - i) * nx + bx;
@@ -678,36 +866,38 @@ __device__ __forceinline__ void fwd_tile_rows(const FwiFwdArgs &A, const FwiShot
 // The parameter block lives in device memory (not in kernel arguments, which
 // are all loaded at entry): each body then loads only the fields it uses, and
 // the interior body keeps its scalar registers free of the general body's.
-// The shot table comes as its own read-only kernel argument so that its
-// descriptors are read through the scalar cache (a pointer fetched from the
-// parameter block would be read with vector loads, one memory latency per
-// wave before its field loads can issue).
+// The shot table and the tile list come as their own read-only kernel
+// arguments so that they are read through the scalar cache (a pointer fetched
+// from the parameter block would be read with vector loads, one memory
+// latency per wave before its field loads can issue).  One instantiation
+// runs the interior strips, the other the general strips: each list holds the
+// logical strips (s nsz + sz) ntx + tx of its class (fwi_fwd_strip_interior),
+// four consecutive entries per workgroup, one per wave.
+template <bool EDGE>
 __global__ __launch_bounds__(256) void k_fwd_tile(const FwiFwdArgs *__restrict__ Ap,
-                                                  const FwiShotDev *__restrict__ shots, const int *kbase, int koff)
+                                                  const FwiShotDev *__restrict__ shots,
+                                                  const int *__restrict__ strips, int nst, const int *kbase, int koff)
 {
+    static_assert(FR == FWI_STRIP, "one strip per wave");
     const int k = (kbase ? *kbase : 0) + koff;
     if (k < 0) return;
     const FwiFwdArgs &A = *Ap;
     const FwiGeomDev &g = A.g;
     // the wave index is uniform within a wave: keep it (and the rows) in SGPRs
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int ntiles = g.ntx * g.ntz2;
-    const int lb = xcd_remap(blockIdx.x, ntiles * A.b.nshots);
+    const int lb = xcd_remap(blockIdx.x, (nst + 3) >> 2);
     if (lb < 0) return;
-    const int s = lb / ntiles, t = lb - s * ntiles;
-    const int tzi = t / g.ntx, txi = t - tzi * g.ntx;
-    const FwiShotDev sh = shots[s];
-    if (k >= sh.nt) return;  // the whole workgroup belongs to shot s
-    const int x0 = txi * TX, z0 = tzi * FWI_TZ2, r0 = z0 + w * FR;
-    __shared__ float stab[FWI_TABN];
     __shared__ float Psh[4][FR][HC];
     __shared__ float Ush[4][FR][HC];
-    if (fwd_tile_interior(g, sh, x0, z0)) {
-        fwd_tile_rows<false>(A, sh, stab, Psh[w], Ush[w], s, k, x0, r0, lane);
-    } else {
-        stage_tables(stab, A.b.tab, g.npml);
-        if (r0 < g.pnz) fwd_tile_rows<true>(A, sh, stab, Psh[w], Ush[w], s, k, x0, r0, lane);
-    }
+    __shared__ float stab[EDGE ? FWI_TABN : 1];
+    if (EDGE) stage_tables_256(stab, A.b.tab, g.npml);  // the only workgroup barrier
+    const int e = 4 * lb + w;
+    if (e >= nst) return;
+    const int nsz = (g.pnz + FR - 1) / FR, l = strips[e];
+    const int tx = l % g.ntx, rest = l / g.ntx, sz = rest % nsz, s = rest / nsz;
+    const FwiShotDev sh = shots[s];
+    if (k >= sh.nt) return;
+    fwd_tile_rows<EDGE>(A, sh, stab, Psh[w], Ush[w], s, k, tx * TX, sz * FR, lane);
 }
 
 // ============================================================================
@@ -773,6 +963,42 @@ __device__ __forceinline__ float adj_u_node(const FwiGeomDev &g, const float *ta
         c = TB(3, 2 * q) * c + TB(2, 2 * q) * u;
         u = u + c;
         if (own) xh_n[r * 2 * n1 + n1 + q] = c;
+    }
+    return u;
+}
+
+// C^T of the ux-node CPML at p node q from the old memory co (new value in
+// cn): the arithmetic of adj_filt_d, memory traffic left to the caller
+__device__ __forceinline__ float adj_filt_edge(const FwiGeomDev &g, const float *tab, int q, float w, float co,
+                                               float &cn)
+{
+    const int pnx = g.pnx, npml = g.npml, T = 2 * npml + 1;
+    cn = 0.f;
+    if (q >= 0 && q < npml) {
+        cn = TB(1, 2 * q + 1) * co + TB(0, 2 * q + 1) * w;
+        w = w + cn;
+    } else if (q >= pnx - npml && q < pnx) {
+        const int m = q - (pnx - npml);
+        cn = TB(3, 2 * m + 1) * co + TB(2, 2 * m + 1) * w;
+        w = w + cn;
+    }
+    return w;
+}
+// H^T gather at u node j then C^T of the u-node CPML: the arithmetic of adj_u_node
+__device__ __forceinline__ float adj_u_edge(const FwiGeomDev &g, const float *tab, const float *Ws, int j, int wi,
+                                            float co, float &cn)
+{
+    const int pnx = g.pnx, npml = g.npml, T = 2 * npml + 1;
+    cn = 0.f;
+    if (j < 0 || j > pnx) return 0.f;
+    float u = hxA(g, j - 1) * Ws[wi - 1] + hxB(g, j) * Ws[wi] + hxC(g, j - 2) * Ws[wi - 2] + hxD(g, j + 1) * Ws[wi + 1];
+    if (j <= npml) {
+        cn = TB(1, 2 * j) * co + TB(0, 2 * j) * u;
+        u = u + cn;
+    } else if (j >= pnx - npml) {
+        const int q = j - (pnx - npml);
+        cn = TB(3, 2 * q) * co + TB(2, 2 * q) * u;
+        u = u + cn;
     }
     return u;
 }
@@ -1124,17 +1350,104 @@ __device__ __forceinline__ void adj_tile_rows(const FwiAdjArgs &A, const FwiShot
     // columns x0-3 .. x0+66 clear of the x CPML and of the grid edges: compact
     // x stencils (the form k_adj_step uses on such tiles)
     const bool fastx = !EDGE || ((x0 >= npml + 3) && (x0 + 66 <= pnx - npml - 1));
+    const int ud0 = lane / 3;  // the extra-u-node pass: lane = 3 ud + (uc - 64)
+    bool uok = lane < 3 * FR;
+    if (EDGE) uok = uok && r0 + ud0 < pnz;
+    const int ud = uok ? ud0 : 0, uc = uok ? 64 + lane - 3 * ud0 : 65;
+
+    // ---- general tiles: every other load before the first store (vector memory
+    // operations retire in order: a load issued after a store waits for it)
+    float xdo[FR], xdho[HP], xuo[FR], xuho = 0.f, zdo[FPR], zho[FVN], rsv[FR];
+    bool rhas[FR];
+    // x strips, branch free: per-node coefficients computed once per wave (the
+    // arithmetic of adj_filt_edge, adj_u_edge and the G_x^T gather; the CPML
+    // coefficients are those of the forward's nodes, fwd_u_coef / fwd_ux_coef)
+    XCoef cf, cu, cuh, cfh[HP];
+    float hA = 0.f, hB = 0.f, hC = 0.f, hD = 0.f, hhA = 0.f, hhB = 0.f, hhC = 0.f, hhD = 0.f;
+    float gE = 0.f, gF = 0.f, gG = 0.f, gH = 0.f;
+    const int n1 = npml + 1;
+    // which special rules can occur in this strip at all (uniform)
+    const bool zplain = !EDGE || (r0 >= 6 && r0 + FR + 2 < pnz - npml);  // z chain: plain transposed stencils
+    const bool has_recv = EDGE && sh.rz_lo <= sh.rz_hi && sh.rz_lo < r0 + FR && sh.rz_hi >= r0;
+    if (EDGE && !fastx) {
+        cf = fwd_ux_coef(g, tab, x);
+#pragma unroll
+        for (int q = 0; q < HP; ++q) cfh[q] = fwd_ux_coef(g, tab, xh.col[q]);
+        const int j = x0 - 1 + lane, jh = x0 - 1 + uc;
+        cu = fwd_u_coef(g, tab, j);
+        cuh = fwd_u_coef(g, tab, jh);
+        hA = hxA(g, j - 1); hB = hxB(g, j); hC = hxC(g, j - 2); hD = hxD(g, j + 1);
+        hhA = hxA(g, jh - 1); hhB = hxB(g, jh); hhC = hxC(g, jh - 2); hhD = hxD(g, jh + 1);
+        gE = gxE(g, x); gF = gxF(g, x + 1); gG = gxG(g, x - 1); gH = gxH(g, x + 2);
+    }
+    if (EDGE) {
+#pragma unroll
+        for (int d = 0; d < FR; ++d) {
+            const int i = r0 + d;
+            xdo[d] = xuo[d] = 0.f;
+            if (!fastx && i < pnz) {  // x strips: per lane, clamped index, no divergence
+                xdo[d] = xd_o[i * 2 * npml + (cf.col >= 0 ? cf.col : 0)];
+                xuo[d] = xh_o[i * 2 * n1 + (cu.col >= 0 ? cu.col : 0)];
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < FR; ++d) {
+            rhas[d] = false;
+            rsv[d] = 0.f;
+        }
+        if (has_recv) {  // every row slot first, then the receiver indices, then the samples
+            int slot[FR], rix[FR];
+#pragma unroll
+            for (int d = 0; d < FR; ++d) {
+                const int i = r0 + d;
+                const bool in = i >= sh.rz_lo && i <= sh.rz_hi;
+                slot[d] = A.b.rrow[sh.rrow_off + (in ? i : 0)];
+                slot[d] = in ? slot[d] : -1;
+            }
+#pragma unroll
+            for (int d = 0; d < FR; ++d) rix[d] = (slot[d] >= 0) ? A.b.rmap[slot[d] * ld + x] : -1;
+#pragma unroll
+            for (int d = 0; d < FR; ++d) {
+                rhas[d] = rix[d] >= 0;
+                rsv[d] = (slot[d] >= 0) ? A.res[sh.trace_off + (long long)k * sh.nr + (rix[d] >= 0 ? rix[d] : 0)]
+                                        : 0.f;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < HP; ++q) {
+            xdho[q] = 0.f;
+            if (!fastx) xdho[q] = xd_o[(r0 + xh.d[q]) * 2 * npml + ((hok[q] && cfh[q].col >= 0) ? cfh[q].col : 0)];
+        }
+        if (!fastx) xuho = xh_o[(r0 + ud) * 2 * n1 + (cuh.col >= 0 ? cuh.col : 0)];
+        if (!zplain) {
+#pragma unroll
+            for (int j = 0; j < FPR; ++j) {
+                const int r = r0 - 3 + j;
+                zdo[j] = (r >= pnz - npml && r < pnz) ? zd_o[(r - (pnz - npml)) * ld + x] : 0.f;
+            }
+#pragma unroll
+            for (int n = 0; n < FVN; ++n) {
+                const int jn = r0 - 1 + n;
+                zho[n] = (jn >= pnz - npml && jn < pnz) ? zh_o[(jn - (pnz - npml)) * ld + x] : 0.f;
+            }
+        }
+    }
+
     // ---- x chain, step 1: w' = C_d^T mu on the own rows and the halo nodes
 #pragma unroll
     for (int d = 0; d < FR; ++d) {
         float wm = m[d + 3];
-        if (!fastx && r0 + d < pnz) wm = adj_filt_d(g, tab, x, wm, r0 + d, xd_o, xd_n, x < pnx);
+        if (!fastx && r0 + d < pnz) {
+            const float cn = cf.cb * xdo[d] + cf.ca * wm;
+            wm = wm + cn;
+            if (cf.col >= 0) xd_n[(r0 + d) * 2 * npml + cf.col] = cn;  // x < pnx whenever col >= 0
+        }
         Ws[d][lane + 3] = wm;
     }
 #pragma unroll
     for (int q = 0; q < HP; ++q) {
         float w = hv[q];
-        if (!fastx && hok[q]) w = adj_filt_d(g, tab, xh.col[q], w, r0 + xh.d[q], xd_o, xd_n, false);
+        if (!fastx && hok[q]) w = w + (cfh[q].cb * xdho[q] + cfh[q].ca * w);  // halo node: owned by the neighbour
         Ws[xh.d[q]][xhalo_lds_col(xh, q, x0)] = w;
     }
     wave_sync();
@@ -1143,18 +1456,27 @@ __device__ __forceinline__ void adj_tile_rows(const FwiAdjArgs &A, const FwiShot
     for (int d = 0; d < FR; ++d) {
         if (EDGE && r0 + d >= pnz) break;
         const float *W = Ws[d];
-        if (fastx) Us[d][lane] = g.c0x * (W[lane + 1] - W[lane + 2]) + g.c1x * (W[lane] - W[lane + 3]);
-        else Us[d][lane] = adj_u_node(g, tab, W, x0 - 1 + lane, lane + 2, r0 + d, xh_o, xh_n, x0);
+        if (fastx) {
+            Us[d][lane] = g.c0x * (W[lane + 1] - W[lane + 2]) + g.c1x * (W[lane] - W[lane + 3]);
+        } else {
+            float u = hA * W[lane + 1] + hB * W[lane + 2] + hC * W[lane] + hD * W[lane + 3];
+            const float cn = cu.cb * xuo[d] + cu.ca * u;
+            u = u + cn;
+            Us[d][lane] = u;
+            if (cu.col >= 0 && lane >= 1) xh_n[(r0 + d) * 2 * n1 + cu.col] = cn;  // owned: x0 <= j < x0+64
+        }
     }
     {
-        const int ud0 = lane / 3;
-        bool uok = lane < 3 * FR;
-        if (EDGE) uok = uok && r0 + ud0 < pnz;
-        const int ud = uok ? ud0 : 0, uc = uok ? 64 + lane - 3 * ud0 : 65;
         const float *W = Ws[ud];
         float u;
-        if (fastx) u = g.c0x * (W[uc + 1] - W[uc + 2]) + g.c1x * (W[uc] - W[uc + 3]);
-        else u = adj_u_node(g, tab, W, x0 - 1 + uc, uc + 2, r0 + ud, xh_o, xh_n, x0);
+        if (fastx) {
+            u = g.c0x * (W[uc + 1] - W[uc + 2]) + g.c1x * (W[uc] - W[uc + 3]);
+        } else {
+            u = hhA * W[uc + 1] + hhB * W[uc + 2] + hhC * W[uc] + hhD * W[uc + 3];
+            const float cn = cuh.cb * xuho + cuh.ca * u;
+            u = u + cn;
+            if (cuh.col >= 0 && uc == 64) xh_n[(r0 + ud) * 2 * n1 + cuh.col] = cn;  // owned: node x0+63
+        }
         Us[ud][uok ? uc : HJ] = u;
     }
     wave_sync();
@@ -1166,10 +1488,9 @@ __device__ __forceinline__ void adj_tile_rows(const FwiAdjArgs &A, const FwiShot
     for (int j = 0; j < FPR; ++j) {
         const int r = r0 - 3 + j;
         float wz = m[j];
-        if (EDGE && r >= pnz - npml && r < pnz) {
+        if (!zplain && r >= pnz - npml && r < pnz) {
             const int q = r - (pnz - npml);
-            float c = zd_o[q * ld + x];
-            c = TB(5, 2 * q + 1) * c + TB(4, 2 * q + 1) * wz;
+            const float c = TB(5, 2 * q + 1) * zdo[j] + TB(4, 2 * q + 1) * wz;
             wz = wz + c;
             if (j >= 3 && j < FR + 3) zd_n[q * ld + x] = c;  // own rows
         }
@@ -1180,7 +1501,7 @@ __device__ __forceinline__ void adj_tile_rows(const FwiAdjArgs &A, const FwiShot
     for (int n = 0; n < FVN; ++n) {
         const int jn = r0 - 1 + n;
         float t = 0.f;
-        if (!EDGE) {
+        if (zplain) {
             t = c0z * (a[n + 1] - a[n + 2]) + c1z * (a[n] - a[n + 3]);
         } else if (jn >= 0 && jn <= pnz) {
             if (jn > 4 && jn + 1 <= pnz - 2) {
@@ -1201,8 +1522,7 @@ __device__ __forceinline__ void adj_tile_rows(const FwiAdjArgs &A, const FwiShot
             }
             if (jn >= pnz - npml && jn < pnz) {
                 const int q = jn - (pnz - npml);
-                float c = zh_o[q * ld + x];
-                c = TB(5, 2 * q) * c + TB(4, 2 * q) * t;
+                const float c = TB(5, 2 * q) * zho[n] + TB(4, 2 * q) * t;
                 t = t + c;
                 if (n >= 1 && n <= FR) zh_n[q * ld + x] = c;  // own nodes r0 .. r0+FR-1
             }
@@ -1220,12 +1540,9 @@ __device__ __forceinline__ void adj_tile_rows(const FwiAdjArgs &A, const FwiShot
         if (fastx) {
             ox = g.c0x * (U[lane + 1] - U[lane + 2]) + g.c1x * (U[lane] - U[lane + 3]);
         } else {
-            const int ui = lane + 1;
-            ox = (x >= pnx) ? 0.f
-                            : gxE(g, x) * U[ui] + gxF(g, x + 1) * U[ui + 1] + gxG(g, x - 1) * U[ui - 1] +
-                                  gxH(g, x + 2) * U[ui + 2];
+            ox = gE * U[lane + 1] + gF * U[lane + 2] + gG * U[lane] + gH * U[lane + 3];
         }
-        if (!EDGE) {
+        if (zplain) {
             oz = c0z * (tv[d + 1] - tv[d + 2]) + c1z * (tv[d] - tv[d + 3]);
         } else {
             if (i > 3 && i - 1 >= 2 && i + 2 < pnz - 1) {
@@ -1248,53 +1565,37 @@ __device__ __forceinline__ void adj_tile_rows(const FwiAdjArgs &A, const FwiShot
             }
         }
         float out = ox + oz;
-        if (EDGE && i >= sh.rz_lo && i <= sh.rz_hi) {
-            const int slot = A.b.rrow[sh.rrow_off + i];
-            if (slot >= 0) {
-                const int ri = A.b.rmap[slot * ld + x];
-                if (ri >= 0) out = out + A.res[sh.trace_off + (long long)k * sh.nr + ri];
-            }
-        }
+        if (has_recv && rhas[d]) out = out + rsv[d];
         const float mnew = leapfrog(m[d + 3], mo[d], mk[d] * out);
         if (!EDGE || x < pnx) buf_st(mnew, rold, vo, i * rowb);
     }
 }
 
-// no source term in the adjoint; otherwise the forward's interior rule
-__device__ __forceinline__ bool adj_tile_interior(const FwiGeomDev &g, const FwiShotDev &sh, int x0, int z0)
-{
-    const int z1 = z0 + FWI_TZ2;
-    if (x0 < g.npml + 3 || x0 + 66 > g.pnx - g.npml - 1) return false;
-    if (z0 < 6 || z1 > g.nz - 3) return false;
-    if (sh.rz_lo <= sh.rz_hi && sh.rz_lo < z1 && sh.rz_hi >= z0) return false;
-    return true;
-}
 
+// interior / general strips of the lists built with fwi_adj_strip_interior
+template <bool EDGE>
 __global__ __launch_bounds__(256) void k_adj_tile(const FwiAdjArgs *__restrict__ Ap,
-                                                  const FwiShotDev *__restrict__ shots, const int *kbase, int koff)
+                                                  const FwiShotDev *__restrict__ shots,
+                                                  const int *__restrict__ strips, int nst, const int *kbase, int koff)
 {
     const int k = (kbase ? *kbase : 0) + koff;
     if (k < 0) return;
     const FwiAdjArgs &A = *Ap;
     const FwiGeomDev &g = A.g;
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int ntiles = g.ntx * g.ntz2;
-    const int lb = xcd_remap(blockIdx.x, ntiles * A.b.nshots);
+    const int lb = xcd_remap(blockIdx.x, (nst + 3) >> 2);
     if (lb < 0) return;
-    const int s = lb / ntiles, t = lb - s * ntiles;
-    const int tzi = t / g.ntx, txi = t - tzi * g.ntx;
-    const FwiShotDev sh = shots[s];
-    if (k >= sh.nt) return;
-    const int x0 = txi * TX, z0 = tzi * FWI_TZ2, r0 = z0 + w * FR;
-    __shared__ float stab[FWI_TABN];
     __shared__ float Wsh[4][FR][HC];
     __shared__ float Ush[4][FR][HC];
-    if (adj_tile_interior(g, sh, x0, z0)) {
-        adj_tile_rows<false>(A, sh, stab, Wsh[w], Ush[w], s, k, x0, r0, lane);
-    } else {
-        stage_tables(stab, A.b.tab, g.npml);
-        if (r0 < g.pnz) adj_tile_rows<true>(A, sh, stab, Wsh[w], Ush[w], s, k, x0, r0, lane);
-    }
+    __shared__ float stab[EDGE ? FWI_TABN : 1];
+    if (EDGE) stage_tables_256(stab, A.b.tab, g.npml);  // the only workgroup barrier
+    const int e = 4 * lb + w;
+    if (e >= nst) return;
+    const int nsz = (g.pnz + FR - 1) / FR, l = strips[e];
+    const int tx = l % g.ntx, rest = l / g.ntx, sz = rest % nsz, s = rest / nsz;
+    const FwiShotDev sh = shots[s];
+    if (k >= sh.nt) return;
+    adj_tile_rows<EDGE>(A, sh, stab, Wsh[w], Ush[w], s, k, tx * TX, sz * FR, lane);
 }
 
 // ============================================================================
@@ -1562,11 +1863,6 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_rec_step(FwiRecArgs A, c
 // deterministic) and the group's image is added to its FP64 accumulator in
 // HBM once per step.  Same arithmetic as k_rec_step.
 // ============================================================================
-// rows per wave of the reconstruction kernel (its FP64 image accumulators
-// grow with it, so it is set apart from the forward/adjoint row count)
-#ifndef FWI_REC_ROWS
-#define FWI_REC_ROWS 8
-#endif
 #define RR FWI_REC_ROWS
 #define RPR (RR + 6)
 #define RVN (RR + 3)
@@ -1641,6 +1937,21 @@ __device__ __forceinline__ void rec_tile_shot(const FwiRecArgs &A, const FwiShot
     for (int n = 0; n < RVN; ++n) v[n] = c0z * (p[n + 2] - p[n + 1]) + c1z * (p[n + 3] - p[n]);
 
     const long long so2 = sh.strip_off + (long long)k * g.sl;
+    // general tiles: the frame values and the source sample are read before the
+    // first store (vector memory operations retire in order)
+    float xsv[RR], psv[RR], fsrc = 0.f;
+    if (EDGE) {
+        const long long so0 = sh.strip_off + (long long)(k - 2) * g.sl;  // p^{k-1} was saved at step k-2
+#pragma unroll
+        for (int d = 0; d < RR; ++d) {
+            const int i = r0 + d;
+            const bool fr = colok && i < nz && !(i >= 3 && i < nz - 3 && bx >= 3 && bx < nx - 3);
+            const int pos = fr ? frame_pos(i, bx, nx, nz) : 0;
+            xsv[d] = A.xstrip[so2 + pos];
+            psv[d] = (k >= 2) ? A.pstrip[so0 + pos] : 0.f;
+        }
+        if (sh.isz >= r0 && sh.isz < r0 + RR) fsrc = A.b.srcf[sh.src_off + k];
+    }
 #pragma unroll
     for (int d = 0; d < RR; ++d) {
         const int i = r0 + d;
@@ -1648,40 +1959,30 @@ __device__ __forceinline__ void rec_tile_shot(const FwiRecArgs &A, const FwiShot
         const float *U = Us[d];
         float ux = c0x * (U[lane + 2] - U[lane + 1]) + c1x * (U[lane + 3] - U[lane]);
         const float vz = c0z * (v[d + 2] - v[d + 1]) + c1z * (v[d + 3] - v[d]);
-        if (EDGE && i == sh.isz) {
-            const float fsrc = A.b.srcf[sh.src_off + k];
-            ux = (fx == sh.isx) ? ux + fsrc : ux;
-        }
+        if (EDGE && i == sh.isz) ux = (fx == sh.isx) ? ux + fsrc : ux;
         float X = ux + vz;
         bool inner = true;
         if (EDGE) {
             inner = i >= 3 && i < nz - 3 && bx >= 3 && bx < nx - 3;
-            if (!inner && colok) X = A.xstrip[so2 + frame_pos(i, bx, nx, nz)];
+            if (!inner && colok) X = xsv[d];
         }
         if (colok) acc[d] += (double)pm[d] * (double)X;
         if (k >= 1) {
             float pnew;
             if (!EDGE || inner) pnew = leapfrog(p[d + 3], pq[d], pk[d] * X);
-            else pnew = (k >= 2 && colok) ? A.pstrip[sh.strip_off + (long long)(k - 2) * g.sl + frame_pos(i, bx, nx, nz)]
-                                          : 0.f;
+            else pnew = colok ? psv[d] : 0.f;
             if (colok) buf_st(pnew, rq, vo, i * rowb);
         }
     }
 }
 
-// every cell of the physical tile (bx0, z0: 64 x RR) is inner and the source is elsewhere
-__device__ __forceinline__ bool rec_tile_interior(const FwiGeomDev &g, const FwiShotDev &sh, int bx0, int z0)
-{
-    if (bx0 < 3 || bx0 + 66 >= g.nx) return false;
-    if (z0 < 3 || z0 + RR > g.nz - 3) return false;
-    const int sx = sh.isx - g.npml;
-    if (sh.isz >= z0 && sh.isz < z0 + RR && sx >= bx0 && sx < bx0 + TX) return false;
-    return true;
-}
 
-__global__ __launch_bounds__(256) void k_rec_tile(const FwiRecArgs *__restrict__ Ap,
-                                                  const FwiShotDev *__restrict__ shots, const int *kbase, int koff,
-                                                  int gsize, int ngroups)
+// items (group, tile) of the lists built with fwi_rec_tile_interior (an item
+// is interior when the tile is interior for every shot of the group)
+template <bool EDGE>
+__device__ __forceinline__ void rec_tile_kernel(const FwiRecArgs *__restrict__ Ap, const FwiShotDev *__restrict__ shots,
+                                                const int *__restrict__ items, int nit, const int *kbase, int koff,
+                                                int gsize)
 {
     const int k = (kbase ? *kbase : 0) + koff;
     if (k < 0) return;
@@ -1690,9 +1991,10 @@ __global__ __launch_bounds__(256) void k_rec_tile(const FwiRecArgs *__restrict__
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int ntz = (g.nz + RR - 1) / RR;
     const int ntiles = g.bntx * ntz;
-    const int lb = xcd_remap(blockIdx.x, ntiles * ngroups);
+    const int lb = xcd_remap(blockIdx.x, nit);
     if (lb < 0) return;
-    const int grp = lb / ntiles, t = lb - grp * ntiles;
+    const int l = items[lb];
+    const int grp = l / ntiles, t = l - grp * ntiles;
     const int tzi = t / g.bntx, txi = t - tzi * g.bntx;
     const int bx0 = txi * TX, r0 = tzi * RR;
     // per wave: staged p rows and u rows while propagating, then its partial image (FP64)
@@ -1716,10 +2018,7 @@ __global__ __launch_bounds__(256) void k_rec_tile(const FwiRecArgs *__restrict__
     for (int s = s_lo + w; s < s_hi; s += 4) {
         const FwiShotDev sh = shots[s];
         if (k >= sh.nt) continue;
-        if (rec_tile_interior(g, sh, bx0, r0))
-            rec_tile_shot<false>(A, sh, s, k, bx0, r0, lane, lds[w][0], lds[w][1], pk, acc);
-        else
-            rec_tile_shot<true>(A, sh, s, k, bx0, r0, lane, lds[w][0], lds[w][1], pk, acc);
+        rec_tile_shot<EDGE>(A, sh, s, k, bx0, r0, lane, lds[w][0], lds[w][1], pk, acc);
     }
     // partial images of the four waves -> LDS; wave w then finishes rows 2w, 2w+1
     wave_sync();
@@ -1736,6 +2035,23 @@ __global__ __launch_bounds__(256) void k_rec_tile(const FwiRecArgs *__restrict__
         for (int q = 0; q < 4; ++q) sum += reinterpret_cast<double(*)[TX]>(&lds[q][0][0][0])[d][lane];
         if (colok && i < nz) gacc[(long long)i * nx + bx] += sum;
     }
+}
+
+// Occupancy targets: the interior items fit 80 VGPRs without spilling when the
+// compiler aims at 6 waves per SIMD (its default schedule takes 98, 4 waves);
+// the general items fit 4 waves (123 VGPRs) without spilling.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_rec_tile_int(
+    const FwiRecArgs *__restrict__ Ap, const FwiShotDev *__restrict__ shots, const int *__restrict__ items, int nit,
+    const int *kbase, int koff, int gsize)
+{
+    rec_tile_kernel<false>(Ap, shots, items, nit, kbase, koff, gsize);
+}
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_rec_tile_gen(const FwiRecArgs *__restrict__ Ap,
+                                                      const FwiShotDev *__restrict__ shots,
+                                                      const int *__restrict__ items, int nit, const int *kbase,
+                                                      int koff, int gsize)
+{
+    rec_tile_kernel<true>(Ap, shots, items, nit, kbase, koff, gsize);
 }
 
 // ============================================================================
@@ -1981,28 +2297,40 @@ hipError_t fwi_launch_fwd(const FwiFwdArgs *a, const int *kbase, int koff, int n
     return hipGetLastError();
 }
 
-hipError_t fwi_launch_fwd_tile(const FwiFwdArgs *a, const FwiFwdArgs *a_dev, const int *kbase, int koff,
-                               hipStream_t st)
+// tile lists: [0] interior tiles, [1] general tiles (device arrays and sizes)
+hipError_t fwi_launch_fwd_tile(const FwiFwdArgs *a, const FwiFwdArgs *a_dev, const int *const *tl, const int *ntl,
+                               const int *kbase, int koff, hipStream_t st)
 {
-    const int n = a->g.ntx * a->g.ntz2 * a->b.nshots;
-    hipLaunchKernelGGL(k_fwd_tile, dim3(((n + 7) / 8) * 8), dim3(256), 0, st, a_dev, a->b.shots, kbase, koff);
+    if (ntl[0] > 0)
+        hipLaunchKernelGGL(k_fwd_tile<false>, dim3(((ntl[0] + 31) / 32) * 8), dim3(256), 0, st, a_dev, a->b.shots,
+                           tl[0], ntl[0], kbase, koff);
+    if (ntl[1] > 0)
+        hipLaunchKernelGGL(k_fwd_tile<true>, dim3(((ntl[1] + 31) / 32) * 8), dim3(256), 0, st, a_dev, a->b.shots,
+                           tl[1], ntl[1], kbase, koff);
     return hipGetLastError();
 }
 
-hipError_t fwi_launch_adj_tile(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, const int *kbase, int koff,
-                               hipStream_t st)
+hipError_t fwi_launch_adj_tile(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, const int *const *tl, const int *ntl,
+                               const int *kbase, int koff, hipStream_t st)
 {
-    const int n = a->g.ntx * a->g.ntz2 * a->b.nshots;
-    hipLaunchKernelGGL(k_adj_tile, dim3(((n + 7) / 8) * 8), dim3(256), 0, st, a_dev, a->b.shots, kbase, koff);
+    if (ntl[0] > 0)
+        hipLaunchKernelGGL(k_adj_tile<false>, dim3(((ntl[0] + 31) / 32) * 8), dim3(256), 0, st, a_dev, a->b.shots,
+                           tl[0], ntl[0], kbase, koff);
+    if (ntl[1] > 0)
+        hipLaunchKernelGGL(k_adj_tile<true>, dim3(((ntl[1] + 31) / 32) * 8), dim3(256), 0, st, a_dev, a->b.shots,
+                           tl[1], ntl[1], kbase, koff);
     return hipGetLastError();
 }
 
-hipError_t fwi_launch_rec_tile(const FwiRecArgs *a, const FwiRecArgs *a_dev, const int *kbase, int koff,
-                               int gsize, int ngroups, hipStream_t st)
+hipError_t fwi_launch_rec_tile(const FwiRecArgs *a, const FwiRecArgs *a_dev, const int *const *tl, const int *ntl,
+                               const int *kbase, int koff, int gsize, hipStream_t st)
 {
-    const int n = a->g.bntx * ((a->g.nz + RR - 1) / RR) * ngroups;
-    hipLaunchKernelGGL(k_rec_tile, dim3(((n + 7) / 8) * 8), dim3(256), 0, st, a_dev, a->b.shots, kbase, koff, gsize,
-                       ngroups);
+    if (ntl[0] > 0)
+        hipLaunchKernelGGL(k_rec_tile_int, dim3(((ntl[0] + 7) / 8) * 8), dim3(256), 0, st, a_dev, a->b.shots, tl[0],
+                           ntl[0], kbase, koff, gsize);
+    if (ntl[1] > 0)
+        hipLaunchKernelGGL(k_rec_tile_gen, dim3(((ntl[1] + 7) / 8) * 8), dim3(256), 0, st, a_dev, a->b.shots, tl[1],
+                           ntl[1], kbase, koff, gsize);
     return hipGetLastError();
 }
 
