@@ -132,6 +132,12 @@ struct FwiRecArgs {
     double *gacc;        // [ngroups][nz][nx]
 };
 
+// streams of a step's interior and general launches (st2 = nullptr: both on st)
+struct FwiPair {
+    hipStream_t st, st2;
+    hipEvent_t fork, join;
+};
+
 // per-step kernels with the reference's loop structure (unit parity)
 struct FwiRefArgs {
     int pnx, pnz, npml;

@@ -20,12 +20,12 @@
 extern "C" {
 hipError_t fwi_launch_fwd(const FwiFwdArgs *a, const int *kbase, int koff, int nblocks, hipStream_t st);
 hipError_t fwi_launch_fwd_tile(const FwiFwdArgs *a, const FwiFwdArgs *a_dev, const int *const *tl, const int *ntl,
-                               const int *kbase, int koff, hipStream_t st);
+                               const int *kbase, int koff, const FwiPair &p);
 hipError_t fwi_launch_adj(const FwiAdjArgs *a, const int *kbase, int koff, int nblocks, hipStream_t st);
 hipError_t fwi_launch_adj_tile(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, const int *const *tl, const int *ntl,
-                               const int *kbase, int koff, hipStream_t st);
+                               const int *kbase, int koff, const FwiPair &p);
 hipError_t fwi_launch_rec_tile(const FwiRecArgs *a, const FwiRecArgs *a_dev, const int *const *tl, const int *ntl,
-                               const int *kbase, int koff, int gsize, hipStream_t st);
+                               const int *kbase, int koff, int gsize, const FwiPair &p);
 hipError_t fwi_launch_rec(const FwiRecArgs *a, const int *kbase, int koff, int nblocks, hipStream_t st);
 hipError_t fwi_launch_advance(int *kbase, int delta, hipStream_t st);
 hipError_t fwi_launch_set_int(int *p, int v, hipStream_t st);
@@ -53,6 +53,13 @@ struct fwi_hip_ctx {
     FwiGeomDev g;
     hipStream_t st = nullptr;
     bool own_stream = false;
+    // second stream for the general strips of a step, which can run beside the
+    // interior ones (default priority, like st); fork/join events order them.
+    // Off by default: on MI355X the overlap gains nothing at 1024x320 x 64 shots
+    // (both launches then share the memory system) and costs host calls.
+    hipStream_t st2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    bool concurrent = false;
     char err[512] = "";
     size_t bytes = 0;
     std::vector<void *> allocs;
@@ -235,6 +242,9 @@ extern "C" void fwi_hip_destroy(fwi_hip_ctx *c)
     for (void *p : c->allocs) (void)hipFree(p);
     for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
     if (c->own_stream && c->st) (void)hipStreamDestroy(c->st);
+    if (c->st2) (void)hipStreamDestroy(c->st2);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     delete c;
 }
 
@@ -292,6 +302,16 @@ extern "C" int fwi_hip_create(const fwi_hip_desc *desc, fwi_hip_ctx **out)
             return rc;
         }
         c->own_stream = true;
+    }
+    {
+        hipError_t e = hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
+        if (e != hipSuccess) {
+            rc = set_err(c, FWI_HIP_EHIP, "second stream: %s", hipGetErrorString(e));
+            fwi_hip_destroy(c);
+            return rc;
+        }
     }
 
     const long long S = desc->max_shots, NT = desc->max_nt, NR = desc->max_nr;
@@ -524,6 +544,16 @@ extern "C" int fwi_hip_set_shots(fwi_hip_ctx *c, int nshots, const fwi_hip_shot 
     return FWI_HIP_OK;
 }
 
+static FwiPair pair_of(const fwi_hip_ctx *c)
+{
+    FwiPair p;
+    p.st = c->st;
+    p.st2 = c->concurrent ? c->st2 : nullptr;
+    p.fork = c->ev_fork;
+    p.join = c->ev_join;
+    return p;
+}
+
 static FwiBatchDev batch_dev(fwi_hip_ctx *c)
 {
     FwiBatchDev b;
@@ -673,7 +703,7 @@ extern "C" int fwi_hip_forward(fwi_hip_ctx *c, float *traces)
                 hipError_t e = hipSuccess;
                 for (int j = 0; j < B && e == hipSuccess; j++)
                     e = c->fwd_kind ? fwi_launch_fwd(&a, c->kbase, j, nblocks, c->st)
-                                    : fwi_launch_fwd_tile(&a, ad, c->tl_fwd, c->n_fwd, c->kbase, j, c->st);
+                                    : fwi_launch_fwd_tile(&a, ad, c->tl_fwd, c->n_fwd, c->kbase, j, pair_of(c));
                 if (e == hipSuccess) e = fwi_launch_advance(c->kbase, B, c->st);
                 return e;
             });
@@ -686,7 +716,7 @@ extern "C" int fwi_hip_forward(fwi_hip_ctx *c, float *traces)
             hipEvent_t e0, e1;
             if ((rc = tick(c, &e0))) return rc;
             HIPCHK(c, c->fwd_kind ? fwi_launch_fwd(&a, nullptr, k, nblocks, c->st)
-                                  : fwi_launch_fwd_tile(&a, ad, c->tl_fwd, c->n_fwd, nullptr, k, c->st));
+                                  : fwi_launch_fwd_tile(&a, ad, c->tl_fwd, c->n_fwd, nullptr, k, pair_of(c)));
             if ((rc = tick(c, &e1))) return rc;
             if (c->timing) marks.push_back({0, {e0, e1}});
             if (marks.size() >= 512 && (rc = collect_timing(c, marks))) return rc;
@@ -815,11 +845,11 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
     const int nb_adj = g.ntx * g.ntz * c->ngroups;
     const int nb_rec = g.bntx * g.bntz * c->ngroups;
     auto launch_adj = [&](const int *kb, int koff) {
-        return tile ? fwi_launch_adj_tile(&a, ad, c->tl_adj, c->n_adj, kb, koff, c->st)
+        return tile ? fwi_launch_adj_tile(&a, ad, c->tl_adj, c->n_adj, kb, koff, pair_of(c))
                     : fwi_launch_adj(&a, kb, koff, nb_adj, c->st);
     };
     auto launch_rec = [&](const int *kb, int koff) {
-        return tile ? fwi_launch_rec_tile(&r, rd, c->tl_rec, c->n_rec, kb, koff, rgs, c->st)
+        return tile ? fwi_launch_rec_tile(&r, rd, c->tl_rec, c->n_rec, kb, koff, rgs, pair_of(c))
                     : fwi_launch_rec(&r, kb, koff, nb_rec, c->st);
     };
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;

@@ -2289,6 +2289,29 @@ __global__ void k_ref_read_bdr_lr(const float *lr, float *pb, int nx, int nz)
 // ============================================================================
 // host-side launchers (C linkage inside the library)
 // ============================================================================
+// The interior and general launches of a step are independent (disjoint
+// cells, the same inputs).  With a second stream they run side by side: the
+// general strips (latency bound) then overlap the interior ones (memory
+// bound).  fork/join events order them against the neighbouring steps.
+template <class LI, class LG>
+static hipError_t launch_pair(const FwiPair &p, bool has_int, bool has_gen, LI launch_int, LG launch_gen)
+{
+    if (p.st2 && has_int && has_gen) {
+        hipError_t e = hipEventRecord(p.fork, p.st);
+        if (e == hipSuccess) e = hipStreamWaitEvent(p.st2, p.fork, 0);
+        if (e != hipSuccess) return e;
+        launch_gen(p.st2);
+        launch_int(p.st);
+        e = hipEventRecord(p.join, p.st2);
+        if (e == hipSuccess) e = hipStreamWaitEvent(p.st, p.join, 0);
+        if (e != hipSuccess) return e;
+    } else {
+        if (has_int) launch_int(p.st);
+        if (has_gen) launch_gen(p.st);
+    }
+    return hipGetLastError();
+}
+
 extern "C" {
 
 hipError_t fwi_launch_fwd(const FwiFwdArgs *a, const int *kbase, int koff, int nblocks, hipStream_t st)
@@ -2299,39 +2322,48 @@ hipError_t fwi_launch_fwd(const FwiFwdArgs *a, const int *kbase, int koff, int n
 
 // tile lists: [0] interior tiles, [1] general tiles (device arrays and sizes)
 hipError_t fwi_launch_fwd_tile(const FwiFwdArgs *a, const FwiFwdArgs *a_dev, const int *const *tl, const int *ntl,
-                               const int *kbase, int koff, hipStream_t st)
+                               const int *kbase, int koff, const FwiPair &p)
 {
-    if (ntl[0] > 0)
-        hipLaunchKernelGGL(k_fwd_tile<false>, dim3(((ntl[0] + 31) / 32) * 8), dim3(256), 0, st, a_dev, a->b.shots,
-                           tl[0], ntl[0], kbase, koff);
-    if (ntl[1] > 0)
-        hipLaunchKernelGGL(k_fwd_tile<true>, dim3(((ntl[1] + 31) / 32) * 8), dim3(256), 0, st, a_dev, a->b.shots,
-                           tl[1], ntl[1], kbase, koff);
-    return hipGetLastError();
+    return launch_pair(
+        p, ntl[0] > 0, ntl[1] > 0,
+        [&](hipStream_t st) {
+            hipLaunchKernelGGL(k_fwd_tile<false>, dim3(((ntl[0] + 31) / 32) * 8), dim3(256), 0, st, a_dev,
+                               a->b.shots, tl[0], ntl[0], kbase, koff);
+        },
+        [&](hipStream_t st) {
+            hipLaunchKernelGGL(k_fwd_tile<true>, dim3(((ntl[1] + 31) / 32) * 8), dim3(256), 0, st, a_dev,
+                               a->b.shots, tl[1], ntl[1], kbase, koff);
+        });
 }
 
 hipError_t fwi_launch_adj_tile(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, const int *const *tl, const int *ntl,
-                               const int *kbase, int koff, hipStream_t st)
+                               const int *kbase, int koff, const FwiPair &p)
 {
-    if (ntl[0] > 0)
-        hipLaunchKernelGGL(k_adj_tile<false>, dim3(((ntl[0] + 31) / 32) * 8), dim3(256), 0, st, a_dev, a->b.shots,
-                           tl[0], ntl[0], kbase, koff);
-    if (ntl[1] > 0)
-        hipLaunchKernelGGL(k_adj_tile<true>, dim3(((ntl[1] + 31) / 32) * 8), dim3(256), 0, st, a_dev, a->b.shots,
-                           tl[1], ntl[1], kbase, koff);
-    return hipGetLastError();
+    return launch_pair(
+        p, ntl[0] > 0, ntl[1] > 0,
+        [&](hipStream_t st) {
+            hipLaunchKernelGGL(k_adj_tile<false>, dim3(((ntl[0] + 31) / 32) * 8), dim3(256), 0, st, a_dev,
+                               a->b.shots, tl[0], ntl[0], kbase, koff);
+        },
+        [&](hipStream_t st) {
+            hipLaunchKernelGGL(k_adj_tile<true>, dim3(((ntl[1] + 31) / 32) * 8), dim3(256), 0, st, a_dev,
+                               a->b.shots, tl[1], ntl[1], kbase, koff);
+        });
 }
 
 hipError_t fwi_launch_rec_tile(const FwiRecArgs *a, const FwiRecArgs *a_dev, const int *const *tl, const int *ntl,
-                               const int *kbase, int koff, int gsize, hipStream_t st)
+                               const int *kbase, int koff, int gsize, const FwiPair &p)
 {
-    if (ntl[0] > 0)
-        hipLaunchKernelGGL(k_rec_tile_int, dim3(((ntl[0] + 7) / 8) * 8), dim3(256), 0, st, a_dev, a->b.shots, tl[0],
-                           ntl[0], kbase, koff, gsize);
-    if (ntl[1] > 0)
-        hipLaunchKernelGGL(k_rec_tile_gen, dim3(((ntl[1] + 7) / 8) * 8), dim3(256), 0, st, a_dev, a->b.shots, tl[1],
-                           ntl[1], kbase, koff, gsize);
-    return hipGetLastError();
+    return launch_pair(
+        p, ntl[0] > 0, ntl[1] > 0,
+        [&](hipStream_t st) {
+            hipLaunchKernelGGL(k_rec_tile_int, dim3(((ntl[0] + 7) / 8) * 8), dim3(256), 0, st, a_dev, a->b.shots,
+                               tl[0], ntl[0], kbase, koff, gsize);
+        },
+        [&](hipStream_t st) {
+            hipLaunchKernelGGL(k_rec_tile_gen, dim3(((ntl[1] + 7) / 8) * 8), dim3(256), 0, st, a_dev, a->b.shots,
+                               tl[1], ntl[1], kbase, koff, gsize);
+        });
 }
 
 hipError_t fwi_launch_adj(const FwiAdjArgs *a, const int *kbase, int koff, int nblocks, hipStream_t st)
