@@ -4,8 +4,9 @@
 
 Device code: hipcc --offload-arch=gfx950, -ffp-contract=off (every
 expression rounds as written, like the C reference) and packed FP32 ops
-disabled.  The C host driver (grad4_1shot.c) and the input/resampling
-layer (fwi_io.c) are compiled with gcc.
+disabled.  The C host driver (grad4_1shot.c), the input/resampling layer
+(fwi_io.c), the trace conditioning (fwi_sigproc.c) and the misfit pool
+(fwi_msf.c) are compiled with gcc.
 """
 import os
 import subprocess
@@ -38,7 +39,7 @@ def build(verbose=True):
         obj = os.path.join(BUILD, src + ".o")
         run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
-    for src in ("grad4_1shot.c", "fwi_io.c"):
+    for src in ("grad4_1shot.c", "fwi_io.c", "fwi_sigproc.c", "fwi_msf.c"):
         path = os.path.join(CSRC, src)
         if os.path.exists(path):
             obj = os.path.join(BUILD, src + ".o")

@@ -33,6 +33,7 @@ extern "C" {
 #define FWI_HIP_ENOMEM 2  /* allocation failed or the HBM budget is exceeded */
 #define FWI_HIP_EHIP 3    /* a HIP runtime call failed */
 #define FWI_HIP_ESTATE 4  /* call out of order (e.g. adjoint before forward) */
+#define FWI_HIP_ERANGE 5  /* a host iteration left the floating-point range (e.g. Sinkhorn scaling underflow) */
 
 typedef struct fwi_hip_ctx fwi_hip_ctx;
 

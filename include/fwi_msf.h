@@ -1,0 +1,116 @@
+/*
+ * fwi_msf.h -- the host misfit pool: misfit values and adjoint sources of
+ * synthetic traces syn against observed traces obs, in FP64 on FP32 traces,
+ * under the reference's names and argument meanings.
+ *
+ * Traces are row pointers obs[ir][it] (alloc.c layout), nr traces of nt
+ * samples.  Every function returns 0 or a FWI_HIP_E* code of fwi_hip.h (the
+ * reference functions return void and exit() on a bad mode string);
+ * fwi_msf_last_error() has the message.  misfit[0] is a diagnostic,
+ * misfit[1] the objective whose gradient with respect to syn is res (up to
+ * the reference's own scalings, stated per function).  sr_xyz is the
+ * sr_location layout (source x,z then receiver x,z pairs, fwi_2d.c:366-371);
+ * a trace whose source-receiver distance lies outside [min, max] contributes
+ * nothing and gets res = 0 (xcorr_msf.c:69-78); sr_xyz = NULL keeps every
+ * trace.
+ */
+#ifndef FWI_MSF_H
+#define FWI_MSF_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* header.h:309-315 (same layout) */
+typedef struct _par_xcorr1d {
+    int nr, nt, npad4Ab;
+    char *conv_how_to_slice; /* "middle" or "leading" */
+} par_xcorr1d;
+
+/* header.h:287-305 (same layout) */
+typedef struct _epsdf1 {
+    int nr, nt, nmodel, npad4Ax, npad4Atr, niter;
+    char *conv_how_to_slice, *reg_type; /* "middle"/"leading"; "diff"/"ide" */
+    float tor0, eps0, Tdepressor;
+} epsdf1;
+
+/* entropy_ot.h:3-16 (same layouts) */
+typedef struct _eot_msf_par {
+    int nt, nr, max_iter;
+    float **M, *u, *v;
+    float eps, tor, dt;
+} eot_msf_par;
+
+typedef struct _eot_log_msf_par {
+    float bar;
+    float **M, *u, *v;
+    float *logp, *logq, *delta_u, *delta_v, vmax0;
+} eot_log_msf_par;
+
+/* L2norm, msf_pool.h:1 (declared only in the reference):
+ * res = syn - obs, misfit[0] = misfit[1] = 1/2 sum res^2. */
+int L2norm(float **obs, float **syn, int nt, int nr, float *sr_xyz, float min, float max, float val[], float **res);
+
+/* Normalised zero-lag cross-correlation (no body in the reference; used by
+ * the inversion with this misfit): per trace c = <s,o> / (|s| |o|),
+ * misfit[1] = sum (1 - c), res = -(o / (|s||o|) - c s / |s|^2);
+ * misfit[0] = sum (s - o)^2.  Traces with |s| = 0 or |o| = 0 are skipped.   */
+int NCCnorm(float **obs, float **syn, int nt, int nr, float *sr_xyz, float min, float max, float val[],
+            float **res);
+
+/* XCORR1D, xcorr_msf.c:4-181.  Per trace, x = the cross-correlation of syn
+ * with obs on nt lags (conv_how_to_slice "middle": lags 1-nt+nt/2 ..
+ * nt/2+... centred at index nt/2; "leading": the first nt lags of the full
+ * correlation), T[i] = |i - nt/2| (middle) or i (leading),
+ * misfit[1] += sum (T x)^2 / sum x^2, res = obs (*) 2 (T^2 x - alpha x) / sum x^2
+ * sliced the same way; misfit[0] += sum |obs[nt-1-i] - syn[i]|.  Traces whose
+ * L1 norms differ by more than a factor 10 are skipped (xcorr_msf.c:77), and
+ * so (unlike the reference, which divides by zero there) are traces with
+ * sum x^2 = 0.  The FFT length (npad4Ab) does not change the result; it
+ * must be >= 0.                                                               */
+int XCORR1D(float **obs, float **syn, float *sr_xyz, float min, float max, void *mpars, float *misfit,
+            float **res);
+
+/* MF1D, lsq_decon_cg.c:82-392 (adaptive matching filter).  Per trace, with
+ * a = 100 obs, b = 100 syn, A w = a (*) w sliced to nt samples and D the
+ * eps0-scaled first difference ("diff") or identity ("ide"): w from niter
+ * CGLS steps on min |A w - b|^2 + |D w|^2 (stop when the residual norm moves
+ * by < tor0), T[i] = Tdepressor |i - nmodel/2| (middle) or Tdepressor i,
+ * misfit[1] += |T w|^2 / |w|^2, and res = y0 from CGNE steps on
+ * [A^T D^T] [y0; y1] = 2 (T^2 w - alpha w) / |w|^2 (the derivative of the
+ * penalty with respect to b = 100 syn, as the reference returns it);
+ * misfit[0] = sum (obs - syn)^2.  "middle" needs an odd nmodel (the
+ * reference writes one element past its buffer otherwise).                   */
+int MF1D(float **obs, float **syn, void *mpars, float *misfit, float **res);
+
+/* Entropic optimal transport between two non-negative traces on the time
+ * axis (entropy_ot.h:18-30; declared only in the reference).  Cost
+ * C_ij = ((i-j) dt)^2, regulariser eps: W = min_P <P,C> + eps sum P (log P - 1)
+ * over couplings of p/|p|_1 and q/|q|_1, by Sinkhorn iterations (at most
+ * max_iter, until the row-marginal L1 error < tor) in FP64.
+ *   entropy_ot: misfit[0] = W, res = dW/dp (through the normalisation).
+ *   entropy_ot_2d_trace_by_trace: per trace p = syn^2 + eta, q = obs^2 + eta
+ *     with eta = 1e-3 max(obs^2) of that trace; misfit[1] = sum W,
+ *     misfit[0] = sum (obs - syn)^2, res = dW/dsyn.
+ *   The _log forms run the same iterations on the log potentials (stable for
+ *     small eps) and take eta = bar max(obs^2) when bar > 0.  The plain forms
+ *     iterate on the scalings exp(f/eps), exp(g/eps) with the banded kernel
+ *     exp(-C/eps) and fall back to the log form where that underflows.
+ * eot_msf_init allocates u, v (nt floats: the final potentials f, g after a
+ * call) and leaves M NULL: the cost is evaluated on the fly as a banded
+ * Toeplitz kernel.                                                           */
+int eot_msf_init(eot_msf_par *par);
+void eot_msf_destroy(eot_msf_par *par);
+int eot_log_msf_init(eot_msf_par *par, eot_log_msf_par *par_exd);
+int entropy_ot(float *p, float *q, eot_msf_par *par, float *msf, float *res);
+int entropy_ot_2d_trace_by_trace(float **obs, float **syn, eot_msf_par *par, float *msf, float **res);
+int entropy_ot_log(float *p, float *q, eot_msf_par *par, eot_log_msf_par *par_exd, float *msf, float *res);
+int entropy_ot_log_2d_trace_by_trace(float **obs, float **syn, eot_msf_par *par, eot_log_msf_par *par_exd,
+                                     float *msf, float **res);
+
+const char *fwi_msf_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
