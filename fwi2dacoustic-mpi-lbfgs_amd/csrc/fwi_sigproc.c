@@ -133,8 +133,28 @@ static void butter_pass(const Butter *b, double *x, int n, int zero_phase)
     }
 }
 
+static int bandpass(float *indat, int nt, int nr, float dt, bool phase, int nphi, int nplo, float fhi, float flo,
+                    float *outdat, int adjoint);
+
 int sfbandpass(float *indat, int nt, int nr, float dt, bool phase, int nphi, int nplo, float fhi, float flo,
                float *outdat)
+{
+    return bandpass(indat, nt, nr, dt, phase, nphi, nplo, fhi, flo, outdat, 0);
+}
+
+int sfbandpass_adjoint(float *indat, int nt, int nr, float dt, bool phase, int nphi, int nplo, float fhi, float flo,
+                       float *outdat)
+{
+    return bandpass(indat, nt, nr, dt, phase, nphi, nplo, fhi, flo, outdat, 1);
+}
+
+/* The cascade from rest is a lower-triangular Toeplitz matrix L per filter,
+ * reversal R.  Causal: F = L_hi L_lo, and since L^T = R L R and the L
+ * commute, F^T = R F R.  Zero phase: each filter is the symmetric
+ * R L R L, but the low- and high-cut passes do not commute, so
+ * F^T = F_lo F_hi: the same passes in the opposite order.                  */
+static int bandpass(float *indat, int nt, int nr, float dt, bool phase, int nphi, int nplo, float fhi, float flo,
+                    float *outdat, int adjoint)
 {
     if (!indat || !outdat || nt < 0 || nr < 0) return sp_fail(FWI_HIP_EINVAL, "sfbandpass: bad arguments");
     if (flo < 0.0f) return sp_fail(FWI_HIP_EINVAL, "sfbandpass: negative flo=%g", flo);
@@ -159,8 +179,18 @@ int sfbandpass(float *indat, int nt, int nr, float dt, bool phase, int nphi, int
     for (int r = 0; r < nr && !rc; r++) {
         const float *src = indat + (size_t)r * nt;
         for (int i = 0; i < nt; i++) tr[i] = src[i];
-        if (use_lo) butter_pass(&blo, tr, nt, !phase);
-        if (use_hi) butter_pass(&bhi, tr, nt, !phase);
+        if (!adjoint) {
+            if (use_lo) butter_pass(&blo, tr, nt, !phase);
+            if (use_hi) butter_pass(&bhi, tr, nt, !phase);
+        } else if (phase) {
+            reverse(tr, nt);
+            if (use_lo) butter_run(&blo, tr, nt);
+            if (use_hi) butter_run(&bhi, tr, nt);
+            reverse(tr, nt);
+        } else {
+            if (use_hi) butter_pass(&bhi, tr, nt, 1);
+            if (use_lo) butter_pass(&blo, tr, nt, 1);
+        }
         float *dst = outdat + (size_t)r * nt;
         for (int i = 0; i < nt; i++) dst[i] = (float)tr[i];
     }

@@ -5,10 +5,17 @@
  *
  * Sequence (the driver this project defines; the oracle implements the
  * same one): forward propagation with receiver recording and frame saving
- * on the GPU -> L2 misfit and adjoint source on the host in FP64 ->
- * exact-adjoint back-propagation, reconstruction and zero-lag imaging on the
- * GPU -> gradient copied back once.  Data sampled every msf->jump steps
- * (fwi_2d.c:195).  The adjoint source at the propagation rate is, by
+ * on the GPU -> misfit and adjoint source on the host in FP64 -> exact-
+ * adjoint back-propagation, reconstruction and zero-lag imaging on the GPU
+ * -> gradient copied back once.  Data sampled every msf->jump steps
+ * (fwi_2d.c:195).
+ *
+ * Host misfit step, per shot, on the data-rate traces: observed and
+ * synthetic traces band-passed (grad4_set_bandpass, sfbandpass) and
+ * first-break muted (grad4_set_fbt_mute, fbt_muter), then the misfit of
+ * grad4_set_misfit (L2 by default; NCC, XCORR1D, MF1D, entropic OT) gives
+ * msf->val and the residual, to which the transposed mute and band-pass
+ * are applied.  The adjoint source at the propagation rate is, by
  * grad4_set_residual_mode:
  *   0 (default) the residual placed on the sampled steps, zero between them
  *     (the exact transpose of the subsampling: the gradient is exact);
@@ -24,12 +31,100 @@
 
 #include "fwi_hip.h"
 #include "fwi_io.h"
+#include "fwi_msf.h"
 #include "fwi_plan.h"
+#include "fwi_sigproc.h"
 
 static fwi_hip_ctx *g_ctx = NULL;
 static long g_key[8];
 static char g_msg[512] = "";
 static int g_res_mode = 0;
+
+static int fail(int code, const char *fmt, ...);
+
+/* host misfit step configuration */
+static struct {
+    int kind;
+    int offsets;          /* select traces by source-receiver distance */
+    float omin, omax;
+    par_xcorr1d xc;
+    epsdf1 mf;
+    char mode[16], reg[16];
+    float eot_eps, eot_tor;
+    int eot_iter;
+    int bp, bp_phase, bp_nphi, bp_nplo;
+    float bp_fhi, bp_flo;
+    int mute, mute_early;
+    float mute_alpha, mute_tune;
+} g_cfg = {.kind = GRAD4_MSF_L2};
+
+static void copy_mode(char *dst, const char *src)
+{
+    snprintf(dst, 16, "%s", src ? src : "");
+}
+
+int grad4_set_misfit(int kind, const void *params, float offset_min, float offset_max)
+{
+    switch (kind) {
+    case GRAD4_MSF_L2:
+    case GRAD4_MSF_NCC:
+        break;
+    case GRAD4_MSF_XCORR1D:
+        if (!params) return FWI_HIP_EINVAL;
+        g_cfg.xc = *(const par_xcorr1d *)params;
+        copy_mode(g_cfg.mode, g_cfg.xc.conv_how_to_slice);
+        /* "leading" slicing returns samples 0..nt-1 of obs (*) y, while the
+         * derivative of its misfit is samples nt-1..2nt-2 (xcorr_msf.c:166-167) */
+        if (strncmp(g_cfg.mode, "middle", 6) != 0) {
+            fail(FWI_HIP_EINVAL, "grad4_set_misfit: XCORR1D needs \"middle\" slicing in the gradient driver");
+            return FWI_HIP_EINVAL;
+        }
+        break;
+    case GRAD4_MSF_MF1D:
+        if (!params) return FWI_HIP_EINVAL;
+        g_cfg.mf = *(const epsdf1 *)params;
+        copy_mode(g_cfg.mode, g_cfg.mf.conv_how_to_slice);
+        copy_mode(g_cfg.reg, g_cfg.mf.reg_type);
+        break;
+    case GRAD4_MSF_EOT: {
+        if (!params) return FWI_HIP_EINVAL;
+        const eot_msf_par *e = (const eot_msf_par *)params;
+        if (!(e->eps > 0.0f) || e->max_iter < 1) return FWI_HIP_EINVAL;
+        g_cfg.eot_eps = e->eps;
+        g_cfg.eot_tor = e->tor;
+        g_cfg.eot_iter = e->max_iter;
+        break;
+    }
+    default:
+        return FWI_HIP_EINVAL;
+    }
+    g_cfg.kind = kind;
+    g_cfg.offsets = offset_min <= offset_max;
+    g_cfg.omin = offset_min;
+    g_cfg.omax = offset_max;
+    return FWI_HIP_OK;
+}
+
+int grad4_set_bandpass(int on, int phase, int nphi, int nplo, float fhi, float flo)
+{
+    if (on && flo < 0.0f) return FWI_HIP_EINVAL;
+    g_cfg.bp = on != 0;
+    g_cfg.bp_phase = phase != 0;
+    g_cfg.bp_nphi = nphi;
+    g_cfg.bp_nplo = nplo;
+    g_cfg.bp_fhi = fhi;
+    g_cfg.bp_flo = flo;
+    return FWI_HIP_OK;
+}
+
+int grad4_set_fbt_mute(int on, float alpha, float tune, int early_gone)
+{
+    g_cfg.mute = on != 0;
+    g_cfg.mute_alpha = alpha;
+    g_cfg.mute_tune = tune;
+    g_cfg.mute_early = early_gone != 0;
+    return FWI_HIP_OK;
+}
 
 int grad4_set_residual_mode(int mode)
 {
@@ -77,6 +172,137 @@ static int ensure_ctx(const fwi_base *b, int nshots, int nt, int nr)
     if (rc) return fail(rc, "fwi_hip_create: %s", fwi_hip_last_error(NULL));
     memcpy(g_key, key, sizeof(key));
     return FWI_HIP_OK;
+}
+
+/* rows of a slab a[n2][n1] */
+static float **slab_rows(float *a, int n1, int n2)
+{
+    float **r = (float **)malloc(sizeof(float *) * (n2 ? n2 : 1));
+    if (r)
+        for (int i = 0; i < n2; i++) r[i] = a + (size_t)i * n1;
+    return r;
+}
+
+/* The host misfit step of one shot.  t holds the shot's nr traces at the
+ * propagation rate (nt samples each) and receives the adjoint source. */
+static int host_misfit(fwi_plan *pl, float *t, int nt, float dt_fine)
+{
+    fwi_msf *m = pl->msf;
+    const int nr = (int)m->nr, nt0 = (int)m->nt, jump = (int)m->jump;
+    const size_t n = (size_t)nr * nt0;
+    float *ob = (float *)malloc(sizeof(float) * (n ? n : 1)), *sy = (float *)malloc(sizeof(float) * (n ? n : 1));
+    float *rs = (float *)calloc(n ? n : 1, sizeof(float));
+    float **obr = slab_rows(ob, nt0, nr), **syr = slab_rows(sy, nt0, nr), **rsr = slab_rows(rs, nt0, nr);
+    int rc = FWI_HIP_OK;
+    if (!ob || !sy || !rs || !obr || !syr || !rsr) {
+        rc = fail(FWI_HIP_ENOMEM, "host memory");
+        goto done;
+    }
+    /* synthetic at the data rate (fwi_2d.c:195) */
+    for (int r = 0; r < nr; r++)
+        for (int i0 = 0; i0 < nt0; i0++) {
+            const float v = t[(size_t)r * nt + (size_t)i0 * jump];
+            m->syn[r][i0] = v;
+            sy[(size_t)r * nt0 + i0] = v;
+            ob[(size_t)r * nt0 + i0] = m->obs0[r][i0];
+        }
+    if (g_cfg.bp) {
+        if (sfbandpass(ob, nt0, nr, m->dt, g_cfg.bp_phase, g_cfg.bp_nphi, g_cfg.bp_nplo, g_cfg.bp_fhi, g_cfg.bp_flo,
+                       ob) ||
+            sfbandpass(sy, nt0, nr, m->dt, g_cfg.bp_phase, g_cfg.bp_nphi, g_cfg.bp_nplo, g_cfg.bp_fhi, g_cfg.bp_flo,
+                       sy)) {
+            rc = fail(FWI_HIP_EINVAL, "sfbandpass: %s", fwi_sigproc_last_error());
+            goto done;
+        }
+    }
+    if (g_cfg.mute) {
+        fbt_muter(m->fbt, ob, nt0, nr, g_cfg.mute_alpha, m->dt, g_cfg.mute_tune, g_cfg.mute_early);
+        fbt_muter(m->fbt, sy, nt0, nr, g_cfg.mute_alpha, m->dt, g_cfg.mute_tune, g_cfg.mute_early);
+    }
+    float *sr = g_cfg.offsets ? pl->sou->xyz_sr : NULL;
+    float val[2] = {0.0f, 0.0f};
+    switch (g_cfg.kind) {
+    case GRAD4_MSF_NCC:
+        rc = NCCnorm(obr, syr, nt0, nr, sr, g_cfg.omin, g_cfg.omax, val, rsr);
+        break;
+    case GRAD4_MSF_XCORR1D: {
+        par_xcorr1d p = g_cfg.xc;
+        p.nr = nr;
+        p.nt = nt0;
+        p.conv_how_to_slice = g_cfg.mode;
+        rc = XCORR1D(obr, syr, sr, g_cfg.omin, g_cfg.omax, &p, val, rsr);
+        break;
+    }
+    case GRAD4_MSF_MF1D: {
+        epsdf1 p = g_cfg.mf;
+        p.nr = nr;
+        p.nt = nt0;
+        p.conv_how_to_slice = g_cfg.mode;
+        p.reg_type = g_cfg.reg;
+        rc = MF1D(obr, syr, &p, val, rsr);
+        break;
+    }
+    case GRAD4_MSF_EOT: {
+        eot_msf_par p;
+        memset(&p, 0, sizeof(p));
+        p.nt = nt0;
+        p.nr = nr;
+        p.max_iter = g_cfg.eot_iter;
+        p.eps = g_cfg.eot_eps;
+        p.tor = g_cfg.eot_tor;
+        p.dt = m->dt;
+        rc = entropy_ot_2d_trace_by_trace(obr, syr, &p, val, rsr);
+        break;
+    }
+    default:
+        rc = L2norm(obr, syr, nt0, nr, sr, g_cfg.omin, g_cfg.omax, val, rsr);
+    }
+    if (rc) {
+        rc = fail(rc, "misfit: %s", fwi_msf_last_error());
+        goto done;
+    }
+    m->val[0] = val[0];
+    m->val[1] = val[1];
+    /* make the residual the derivative of val[1] with respect to syn:
+     * MF1D returns it with respect to 100 syn (lsq_decon_cg.c:183-184,377);
+     * XCORR1D's middle window sits one lag off for even nt
+     * (xcorr_msf.c:43-46,105-106), which shifts its residual one sample */
+    if (g_cfg.kind == GRAD4_MSF_MF1D)
+        for (size_t i = 0; i < n; i++) rs[i] *= 100.0f;
+    if (g_cfg.kind == GRAD4_MSF_XCORR1D && nt0 % 2 == 0)
+        for (int r = 0; r < nr; r++) {
+            float *x = rs + (size_t)r * nt0;
+            memmove(x + 1, x, sizeof(float) * (nt0 - 1));
+            x[0] = 0.0f;
+        }
+    /* transposed conditioning of the residual: mute (diagonal), then F^T */
+    if (g_cfg.mute) fbt_muter(m->fbt, rs, nt0, nr, g_cfg.mute_alpha, m->dt, g_cfg.mute_tune, g_cfg.mute_early);
+    if (g_cfg.bp)
+        sfbandpass_adjoint(rs, nt0, nr, m->dt, g_cfg.bp_phase, g_cfg.bp_nphi, g_cfg.bp_nplo, g_cfg.bp_fhi,
+                           g_cfg.bp_flo, rs);
+    /* adjoint source at the propagation rate */
+    if (g_res_mode == 0 || jump == 1) {
+        for (int r = 0; r < nr; r++) /* residual on the samples, zero between them */
+            for (int it = 0; it < nt; it++)
+                t[(size_t)r * nt + it] = (it % jump == 0) ? rs[(size_t)r * nt0 + it / jump] : 0.0f;
+    } else {
+        float **tr = slab_rows(t, nt, nr);
+        if (!tr) {
+            rc = fail(FWI_HIP_ENOMEM, "host memory");
+            goto done;
+        }
+        int e = fwi_trcinp(rsr, nt0, m->dt, tr, nt, dt_fine, nr);
+        free(tr);
+        if (e) rc = fail(e, "trcinp: %s", fwi_io_last_error());
+    }
+done:
+    free(ob);
+    free(sy);
+    free(rs);
+    free(obr);
+    free(syr);
+    free(rsr);
+    return rc;
 }
 
 int grad_all_shots(fwi_plan **plans, int nshots, int k)
@@ -130,36 +356,10 @@ int grad_all_shots(fwi_plan **plans, int nshots, int k)
         if (rc) rc = fail(rc, "forward: %s", fwi_hip_last_error(g_ctx));
     }
     if (!rc) {
-        /* host misfit step: sample every jump, L2, adjoint source at the fine rate */
         long long o = 0;
         for (int s = 0; s < nshots && !rc; s++) {
-            fwi_msf *m = plans[s]->msf;
-            const int nr = (int)m->nr, nt0 = (int)m->nt, jump = (int)m->jump;
-            double J = 0.0;
-            for (int r = 0; r < nr; r++) {
-                float *t = tr + o + (long long)r * nt;
-                for (int i0 = 0; i0 < nt0; i0++) {
-                    const float syn = t[(long long)i0 * jump];
-                    m->syn[r][i0] = syn;
-                    const float res = syn - m->obs0[r][i0];
-                    J += 0.5 * (double)res * (double)res;
-                }
-                if (g_res_mode == 0 || jump == 1) {
-                    for (int it = nt - 1; it >= 0; it--) /* residual on the samples, zero between them */
-                        t[it] = (it % jump == 0) ? t[it] - m->obs0[r][it / jump] : 0.0f;
-                } else {
-                    float *r0 = (float *)malloc(sizeof(float) * nt0);
-                    if (!r0) { rc = fail(FWI_HIP_ENOMEM, "host memory"); break; }
-                    for (int i0 = 0; i0 < nt0; i0++) r0[i0] = m->syn[r][i0] - m->obs0[r][i0];
-                    float *rows0[1] = {r0}, *rows1[1] = {t};
-                    int e = fwi_trcinp(rows0, nt0, m->dt, rows1, nt, b->dt, 1);
-                    free(r0);
-                    if (e) { rc = fail(e, "trcinp: %s", fwi_io_last_error()); break; }
-                }
-            }
-            m->val[0] = (float)J;
-            m->val[1] = (float)J;
-            o += (long long)nr * nt;
+            rc = host_misfit(plans[s], tr + o, nt, b->dt);
+            o += (long long)plans[s]->msf->nr * nt;
         }
         if (!rc) {
             rc = fwi_hip_set_residual(g_ctx, tr);

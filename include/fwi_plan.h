@@ -140,6 +140,37 @@ int grad_all_shots(fwi_plan **plans, int nshots, int k);
 /* Message of the last failure of grad4_1shot / grad_all_shots. */
 const char *grad4_last_error(void);
 
+/* The host misfit step of grad4_1shot / grad_all_shots (fwi_msf.h,
+ * fwi_sigproc.h), process-wide settings.  The reference leaves this step
+ * inside its missing grad4_1shot; the misfit functions are its pool
+ * (msf_pool.h, xcorr_msf.c, lsq_decon_cg.c, entropy_ot.h).
+ *   kind: GRAD4_MSF_* below; params: par_xcorr1d * (XCORR1D), epsdf1 *
+ *   (MF1D), eot_msf_par * (EOT: eps, tor, max_iter used), else ignored.
+ *   The nr/nt fields of params are set per shot from msf->nr and msf->nt.
+ *   Traces whose source-receiver distance (sou->xyz_sr) lies outside
+ *   [offset_min, offset_max] are left out; offset_min > offset_max keeps all.
+ * msf->val gets the misfit's (diagnostic, objective) pair, and the gradient
+ * is that of val[1]: MF1D's residual (a derivative with respect to 100 syn)
+ * is scaled by 100, XCORR1D's (one sample early for even nt) is shifted
+ * back, and XCORR1D's "leading" slicing, whose residual is not a derivative
+ * of its misfit, is refused.  The default is L2: val[0] = val[1] =
+ * 1/2 sum (syn - obs)^2, residual syn - obs.                                 */
+#define GRAD4_MSF_L2 0
+#define GRAD4_MSF_NCC 1
+#define GRAD4_MSF_XCORR1D 2
+#define GRAD4_MSF_MF1D 3
+#define GRAD4_MSF_EOT 4
+int grad4_set_misfit(int kind, const void *params, float offset_min, float offset_max);
+
+/* Band-pass (sfbandpass, butterworth_filter.c:127-200) of the observed and
+ * synthetic data-rate traces before the misfit, and its transpose on the
+ * residual.  on = 0 switches it off (the default). */
+int grad4_set_bandpass(int on, int phase, int nphi, int nplo, float fhi, float flo);
+
+/* First-break mute (fbt_muter, mute_fun.c:5-23) with msf->fbt, on the
+ * observed and synthetic traces and on the residual.  on = 0: off. */
+int grad4_set_fbt_mute(int on, float alpha, float tune, int early_gone);
+
 /* A plan holding exactly what grad4_1shot reads and writes, with the
  * reference's row-pointer arrays (alloc.c:3-48) and shapes (fwi_2d.c:148-
  * 165, 179-222, 380): for callers without fwi_init's input files.

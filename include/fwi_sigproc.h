@@ -29,6 +29,11 @@ extern "C" {
 int sfbandpass(float *indat, int nt, int nr, float dt, bool phase, int nphi, int nplo, float fhi, float flo,
                float *outdat);
 
+/* The transpose of sfbandpass with the same arguments (no counterpart in
+ * the reference; for adjoint sources of filtered misfits): <F x, y> = <x, F^T y>. */
+int sfbandpass_adjoint(float *indat, int nt, int nr, float dt, bool phase, int nphi, int nplo, float fhi, float flo,
+                       float *outdat);
+
 /* fbt_muter, mute_fun.c:5-23: per trace ix, tt = it*dt - (fbt[ix] + tune),
  * negated unless early_gone; where tt <= 0 the sample is scaled by
  * exp(alpha*tt), or zeroed once alpha*tt <= -10.  obs[ix*nt + it] in place. */

@@ -47,6 +47,7 @@ TAPER_ARGS = [PPF, PF, c_int, c_int, c_float, c_float, c_float, c_int]
 TANH_ARGS = [PPF, PF, c_int, c_int, c_float, c_float, c_float, c_float, PPF]
 SIGPROC = {
     "sfbandpass": [PF, c_int, c_int, c_float, c_bool, c_int, c_int, c_float, c_float, PF],
+    "sfbandpass_adjoint": [PF, c_int, c_int, c_float, c_bool, c_int, c_int, c_float, c_float, PF],
     "fbt_muter": [PF, PF, c_int, c_int, c_float, c_float, c_float, c_bool],
     "win_apply": [PF, PF, c_int, c_int, c_float, c_float, PF],
     "upper_off_srcood": TANH_ARGS,
@@ -84,7 +85,8 @@ def ref_lib():
         return None
     h = ctypes.CDLL(REF_SIGPROC)
     for name, args in SIGPROC.items():
-        _sig(getattr(h, name), args, None)
+        if name != "sfbandpass_adjoint":
+            _sig(getattr(h, name), args, None)
     return h
 
 

@@ -82,6 +82,12 @@ def lib():
     h.fwi_io_last_error.restype = ctypes.c_char_p
     h.grad4_set_residual_mode.argtypes = [c_int]
     h.grad4_set_residual_mode.restype = c_int
+    h.grad4_set_misfit.argtypes = [c_int, ctypes.c_void_p, c_float, c_float]
+    h.grad4_set_misfit.restype = c_int
+    h.grad4_set_bandpass.argtypes = [c_int, c_int, c_int, c_int, c_float, c_float]
+    h.grad4_set_bandpass.restype = c_int
+    h.grad4_set_fbt_mute.argtypes = [c_int, c_float, c_float, c_int]
+    h.grad4_set_fbt_mute.restype = c_int
     return h
 
 

@@ -227,3 +227,24 @@ def test_win_apply_vs_reference(h, ref):
     for r in range(nr):
         inside = (t > win[2 * r] + 1e-4) & (t < win[2 * r + 1] - 1e-4)
         assert np.array_equal(a[r, inside], x[r, inside])
+
+
+@pytest.mark.parametrize("phase", [True, False])
+@pytest.mark.parametrize("flo,fhi", [(3.0, 30.0), (0.0, 30.0), (3.0, 400.0)])
+def test_bandpass_adjoint_dot_product(h, phase, flo, fhi):
+    """<F x, y> = <x, F^T y> for the band-pass and sfbandpass_adjoint."""
+    rng = np.random.default_rng(31)
+    nt, nr, dt = 700, 3, 0.002
+    x = rng.standard_normal((nr, nt)).astype(np.float32)
+    y = rng.standard_normal((nr, nt)).astype(np.float32)
+    fx, fty = np.zeros_like(x), np.zeros_like(y)
+    assert h.sfbandpass(H.fp(x), nt, nr, dt, phase, 5, 4, fhi, flo, H.fp(fx)) == 0
+    assert h.sfbandpass_adjoint(H.fp(y), nt, nr, dt, phase, 5, 4, fhi, flo, H.fp(fty)) == 0
+    a = float((fx.astype(np.float64) * y).sum())
+    b = float((x.astype(np.float64) * fty).sum())
+    assert a == pytest.approx(b, rel=1e-5, abs=1e-5 * np.sqrt((fx.astype(np.float64) ** 2).sum() * nt * nr))
+    if flo > 0 and fhi < 250:
+        # not self-adjoint in general: the transpose differs from the filter itself
+        fy = np.zeros_like(y)
+        assert h.sfbandpass(H.fp(y), nt, nr, dt, phase, 5, 4, fhi, flo, H.fp(fy)) == 0
+        assert not np.allclose(fy, fty, atol=1e-4)
