@@ -5,8 +5,9 @@
 Device code: hipcc --offload-arch=gfx950, -ffp-contract=off (every
 expression rounds as written, like the C reference) and packed FP32 ops
 disabled.  The C host driver (grad4_1shot.c), the input/resampling layer
-(fwi_io.c), the trace conditioning (fwi_sigproc.c) and the misfit pool
-(fwi_msf.c) are compiled with gcc.
+(fwi_io.c), the trace conditioning (fwi_sigproc.c), the misfit pool
+(fwi_msf.c), the gradient post-processing (fwi_gradprec.c) and the L-BFGS
+loop (fwi_lbfgs.c) are compiled with gcc.
 """
 import os
 import subprocess
@@ -39,7 +40,7 @@ def build(verbose=True):
         obj = os.path.join(BUILD, src + ".o")
         run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
-    for src in ("grad4_1shot.c", "fwi_io.c", "fwi_sigproc.c", "fwi_msf.c"):
+    for src in ("grad4_1shot.c", "fwi_io.c", "fwi_sigproc.c", "fwi_msf.c", "fwi_gradprec.c", "fwi_lbfgs.c"):
         path = os.path.join(CSRC, src)
         if os.path.exists(path):
             obj = os.path.join(BUILD, src + ".o")

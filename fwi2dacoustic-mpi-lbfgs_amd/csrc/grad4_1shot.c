@@ -460,6 +460,20 @@ fwi_plan *fwi_plan_alloc(int nx, int nz, int npml, int nt, int jump, int nr, flo
     return p;
 }
 
+int fwi_plan_set_grad(fwi_plan *p, float perc, float alpha_grad, int rectx1, int rectz1, int repeat1, int rectx2,
+                      int rectz2, int repeat2)
+{
+    if (!p || !p->grad || rectx1 < 0 || rectz1 < 0 || repeat1 < 0 || rectx2 < 0 || rectz2 < 0 || repeat2 < 0)
+        return FWI_HIP_EINVAL;
+    const fwi_grad *o = p->grad;
+    const fwi_grad gr = {.id = o->id, .end_t = o->end_t, .start_t = o->start_t, .alpha_grad = alpha_grad,
+                         .perc = perc, .jump = o->jump, .rectx1 = (size_t)rectx1, .rectz1 = (size_t)rectz1,
+                         .repeat1 = (size_t)repeat1, .rectx2 = (size_t)rectx2, .rectz2 = (size_t)rectz2,
+                         .repeat2 = (size_t)repeat2, .body = o->body};
+    memcpy(p->grad, &gr, sizeof(gr));
+    return FWI_HIP_OK;
+}
+
 void fwi_plan_free(fwi_plan *p)
 {
     if (!p) return;

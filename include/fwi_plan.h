@@ -179,6 +179,13 @@ int grad4_set_fbt_mute(int on, float alpha, float tune, int early_gone);
 fwi_plan *fwi_plan_alloc(int nx, int nz, int npml, int nt, int jump, int nr, float dx, float dz, float dt);
 void fwi_plan_free(fwi_plan *plan);
 
+/* The gradient post-processing parameters of plan->grad (header.h:119-127,
+ * const there; fwi_init sets them from its inputs) read by
+ * grad_precond_new5: clip percentage, final scale, and the two smoothing
+ * passes (rect, repeat).  Returns 0 or FWI_HIP_EINVAL. */
+int fwi_plan_set_grad(fwi_plan *plan, float perc, float alpha_grad, int rectx1, int rectz1, int repeat1, int rectx2,
+                      int rectz2, int repeat2);
+
 #ifdef __cplusplus
 }
 #endif

@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_plan_header_symbols_exported():
     lib = L.load()
-    for hdr in ("fwi_plan.h", "fwi_io.h", "fwi_sigproc.h", "fwi_msf.h"):
+    for hdr in ("fwi_plan.h", "fwi_io.h", "fwi_sigproc.h", "fwi_msf.h", "fwi_gradprec.h", "fwi_lbfgs.h"):
         names = declared(hdr)
         assert names, hdr
         for n in names:
