@@ -130,7 +130,14 @@ struct FwiRecArgs {
     const float *M0, *M1;
     const float *pstrip, *xstrip;
     double *gacc;        // [ngroups][nz][nx]
+    int img_jump, img_k0, img_k1;  // steps imaged: k0 <= k <= k1, (k - k0) % jump == 0
 };
+
+// whether reconstruction step k adds to the image
+__host__ __device__ inline bool fwi_image_step(int jump, int k0, int k1, int k)
+{
+    return k >= k0 && k <= k1 && (k - k0) % jump == 0;
+}
 
 // streams of a step's interior and general launches (st2 = nullptr: both on st)
 struct FwiPair {

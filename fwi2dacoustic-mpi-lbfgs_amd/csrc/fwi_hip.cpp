@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <map>
 #include <string>
+#include <climits>
 #include <vector>
 
 #include "fwi_dev.h"
@@ -101,6 +102,7 @@ struct fwi_hip_ctx {
     int graph_steps = 100;
     int fwd_kind = 0;  // 0: 2-D tile kernels, 1: z-marching kernels
     int rec_group = 8; // shots per workgroup of the 2-D tile reconstruction kernel
+    int img_jump = 1, img_k0 = 0, img_k1 = INT_MAX; // imaging window of the adjoint pass
     // work lists of the 2-D tile kernels, [0] interior, [1] general (strips; imaging: group x tile)
     int *tl_fwd[2] = {nullptr, nullptr}, *tl_adj[2] = {nullptr, nullptr}, *tl_rec[2] = {nullptr, nullptr};
     int n_fwd[2] = {0, 0}, n_adj[2] = {0, 0}, n_rec[2] = {0, 0};
@@ -658,6 +660,15 @@ extern "C" int fwi_hip_set_rec_group(fwi_hip_ctx *c, int shots)
     return FWI_HIP_OK;
 }
 
+extern "C" int fwi_hip_set_imaging(fwi_hip_ctx *c, int jump, int k0, int k1)
+{
+    if (!c || jump < 1 || k0 < 0 || k1 < k0) return set_err(c, FWI_HIP_EINVAL, "set_imaging: need jump >= 1, 0 <= k0 <= k1");
+    c->img_jump = jump;
+    c->img_k0 = k0;
+    c->img_k1 = k1;
+    return FWI_HIP_OK;
+}
+
 extern "C" int fwi_hip_set_graph(fwi_hip_ctx *c, int steps)
 {
     if (!c || steps < 0) return set_err(c, FWI_HIP_EINVAL, "set_graph: bad argument");
@@ -814,6 +825,9 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
     r.pstrip = c->pstrip;
     r.xstrip = c->xstrip;
     r.gacc = c->gacc;
+    r.img_jump = c->img_jump;
+    r.img_k0 = c->img_k0;
+    r.img_k1 = c->img_k1;
     c->adj_args_h = a;
     c->rec_args_h = r;
     HIPCHK(c, hipMemcpyAsync(c->adj_args_d, &c->adj_args_h, sizeof(FwiAdjArgs), hipMemcpyHostToDevice, c->st));
@@ -889,6 +903,9 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
     if (grad) HIPCHK(c, hipMemcpyAsync(grad, c->grad, sizeof(double) * nb, hipMemcpyDeviceToHost, c->st));
     if (gk) HIPCHK(c, hipMemcpyAsync(gk, c->gk, sizeof(double) * nb, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
+    // the reconstruction ran the forward's final state back to the start:
+    // another adjoint needs another forward
+    c->have_fwd = false;
     return FWI_HIP_OK;
 }
 

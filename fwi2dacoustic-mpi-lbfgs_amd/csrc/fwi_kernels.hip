@@ -1643,6 +1643,7 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_rec_step(FwiRecArgs A, c
     const float *ekp = A.b.ekp;
     const float c0x = g.c0x, c1x = g.c1x, c0z = g.c0z, c1z = g.c1z;
     double *gacc = A.gacc + (long long)grp * nx * nz;
+    const bool img_on = fwi_image_step(A.img_jump, A.img_k0, A.img_k1, k);  // uniform per launch
     const float fsrc = active ? A.b.srcf[sh.src_off + k] : 0.f;
     const int hq = halo_col(bx0, lane);
 
@@ -1668,7 +1669,7 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_rec_step(FwiRecArgs A, c
                 fm = mu[o];
                 fk = ekp[o];
             }
-            if (wv == 0 && colok) fg = gacc[(long long)io * nx + bx];
+            if (wv == 0 && colok && img_on) fg = gacc[(long long)io * nx + bx];
         }
     };
     // x chain of the interior operator for the block's rows in [z0, z1) n [3, nz-3)
@@ -1738,7 +1739,7 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_rec_step(FwiRecArgs A, c
             }
             red[wv][lane] = img;
             __syncthreads();
-            if (wv == 0 && colok) {
+            if (wv == 0 && colok && img_on) {
                 double acc = g_i;
                 for (int w = 0; w < sg; ++w) acc += red[w][lane];
                 gacc[(long long)i * nx + bx] = acc;
@@ -1791,7 +1792,7 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_rec_step(FwiRecArgs A, c
                 rd[d][wv][lane] = img;
             }
             __syncthreads();
-            if (wv == 0) {
+            if (wv == 0 && img_on) {
 #pragma unroll
                 for (int d = 0; d < RB; ++d) {
                     double acc = bg[d];
@@ -2020,6 +2021,8 @@ __device__ __forceinline__ void rec_tile_kernel(const FwiRecArgs *__restrict__ A
         if (k >= sh.nt) continue;
         rec_tile_shot<EDGE>(A, sh, s, k, bx0, r0, lane, lds[w][0], lds[w][1], pk, acc);
     }
+    // steps outside the imaging window only reconstruct (uniform per launch)
+    if (!fwi_image_step(A.img_jump, A.img_k0, A.img_k1, k)) return;
     // partial images of the four waves -> LDS; wave w then finishes rows 2w, 2w+1
     wave_sync();
     double(*red)[TX] = reinterpret_cast<double(*)[TX]>(&lds[w][0][0][0]);

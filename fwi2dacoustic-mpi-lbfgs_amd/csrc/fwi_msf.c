@@ -230,6 +230,20 @@ int NCCnorm(float **obs, float **syn, int nt, int nr, float *sr_xyz, float min, 
 {
     if (nt < 0 || nr < 0 || !rows_ok(obs, nr) || !rows_ok(syn, nr) || !rows_ok(res, nr) || !val)
         return msf_fail(FWI_HIP_EINVAL, "NCCnorm: bad arguments");
+    /* traces with less than NCC_FLOOR of the gather's largest trace energy
+     * (observed or synthetic) hold no usable waveform, only rounding noise
+     * that the normalisation would amplify by 1/|s| */
+    const double NCC_FLOOR = 1e-8;
+    double smax = 0.0, omax = 0.0;
+    for (int r = 0; r < nr; r++) {
+        double ss = 0.0, oo = 0.0;
+        for (int i = 0; i < nt; i++) {
+            ss += (double)syn[r][i] * syn[r][i];
+            oo += (double)obs[r][i] * obs[r][i];
+        }
+        if (ss > smax) smax = ss;
+        if (oo > omax) omax = oo;
+    }
     double J = 0.0, L2 = 0.0;
     for (int r = 0; r < nr; r++) {
         double ss = 0.0, oo = 0.0, so = 0.0;
@@ -240,7 +254,8 @@ int NCCnorm(float **obs, float **syn, int nt, int nr, float *sr_xyz, float min, 
             so += s * o;
             L2 += (s - o) * (s - o);
         }
-        if (!keep_trace(sr_xyz, min, max, r) || ss == 0.0 || oo == 0.0) {
+        if (!keep_trace(sr_xyz, min, max, r) || ss == 0.0 || oo == 0.0 || ss < NCC_FLOOR * smax ||
+            oo < NCC_FLOOR * omax) {
             for (int i = 0; i < nt; i++) res[r][i] = 0.0f;
             continue;
         }
@@ -440,7 +455,7 @@ int MF1D(float **obs, float **syn, void *mpars, float *misfit, float **res)
             mf_A(&m, pp, w0);
             mf_D(&m, pp, w1);
             const double num = dot(z, z, nm), den = dot(w0, w0, nt) + dot(w1, w1, nm);
-            if (den == 0.0) break;
+            if (num == 0.0 || den == 0.0) break; /* converged exactly: the reference would divide 0/0 */
             const double al = num / den;
             for (int i = 0; i < nm; i++) w[i] += al * pp[i];
             for (int i = 0; i < nt; i++) r0[i] -= al * w0[i];
@@ -474,7 +489,7 @@ int MF1D(float **obs, float **syn, void *mpars, float *misfit, float **res)
         for (int it = 0; it < niter; it++) {
             const double rn0 = sqrt(dot(rr, rr, nm));
             const double nn = rn0 * rn0, dd = dot(w0, w0, nt) + dot(w1, w1, nm);
-            if (dd == 0.0) break;
+            if (nn == 0.0 || dd == 0.0) break;
             const double al = nn / dd;
             for (int i = 0; i < nt; i++) y0[i] += al * w0[i];
             for (int i = 0; i < nm; i++) y1[i] += al * w1[i];

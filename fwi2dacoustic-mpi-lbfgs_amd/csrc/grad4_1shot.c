@@ -23,6 +23,7 @@
  *     (spline_interp.c:19-38), the reference's resampling of traces between
  *     the data rate and the propagation rate.
  */
+#include <limits.h>
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
@@ -193,6 +194,7 @@ static int host_misfit(fwi_plan *pl, float *t, int nt, float dt_fine)
     float *ob = (float *)malloc(sizeof(float) * (n ? n : 1)), *sy = (float *)malloc(sizeof(float) * (n ? n : 1));
     float *rs = (float *)calloc(n ? n : 1, sizeof(float));
     float **obr = slab_rows(ob, nt0, nr), **syr = slab_rows(sy, nt0, nr), **rsr = slab_rows(rs, nt0, nr);
+    unsigned char *drop = NULL; /* traces outside the offset window */
     int rc = FWI_HIP_OK;
     if (!ob || !sy || !rs || !obr || !syr || !rsr) {
         rc = fail(FWI_HIP_ENOMEM, "host memory");
@@ -220,6 +222,24 @@ static int host_misfit(fwi_plan *pl, float *t, int nt, float dt_fine)
         fbt_muter(m->fbt, sy, nt0, nr, g_cfg.mute_alpha, m->dt, g_cfg.mute_tune, g_cfg.mute_early);
     }
     float *sr = g_cfg.offsets ? pl->sou->xyz_sr : NULL;
+    /* the offset window applies to every misfit: traces outside it are left
+     * out (zeroed here, since MF1D and EOT take no offsets) */
+    if (sr) {
+        drop = (unsigned char *)calloc(nr ? nr : 1, 1);
+        if (!drop) {
+            rc = fail(FWI_HIP_ENOMEM, "host memory");
+            goto done;
+        }
+        for (int r = 0; r < nr; r++) {
+            const double dx = (double)sr[2 + 2 * r] - sr[0], dz = (double)sr[3 + 2 * r] - sr[1];
+            const double d = sqrt(dx * dx + dz * dz);
+            if (d < g_cfg.omin || d > g_cfg.omax) {
+                drop[r] = 1;
+                memset(ob + (size_t)r * nt0, 0, sizeof(float) * nt0);
+                memset(sy + (size_t)r * nt0, 0, sizeof(float) * nt0);
+            }
+        }
+    }
     float val[2] = {0.0f, 0.0f};
     switch (g_cfg.kind) {
     case GRAD4_MSF_NCC:
@@ -263,6 +283,9 @@ static int host_misfit(fwi_plan *pl, float *t, int nt, float dt_fine)
     }
     m->val[0] = val[0];
     m->val[1] = val[1];
+    if (drop)
+        for (int r = 0; r < nr; r++)
+            if (drop[r]) memset(rs + (size_t)r * nt0, 0, sizeof(float) * nt0);
     /* make the residual the derivative of val[1] with respect to syn:
      * MF1D returns it with respect to 100 syn (lsq_decon_cg.c:183-184,377);
      * XCORR1D's middle window sits one lag off for even nt
@@ -296,6 +319,7 @@ static int host_misfit(fwi_plan *pl, float *t, int nt, float dt_fine)
         if (e) rc = fail(e, "trcinp: %s", fwi_io_last_error());
     }
 done:
+    free(drop);
     free(ob);
     free(sy);
     free(rs);
@@ -325,6 +349,16 @@ int grad_all_shots(fwi_plan **plans, int nshots, int k)
     if (rc) return rc;
     rc = fwi_hip_set_model(g_ctx, &plans[0]->model->md[0][0]);
     if (rc) return fail(rc, "set_model: %s", fwi_hip_last_error(g_ctx));
+    {
+        /* imaging window from plans[0]->grad: every jump-th step with
+         * start_t <= k dt <= end_t (end_t <= 0: to the last step) */
+        const fwi_grad *gr = plans[0]->grad;
+        const int jump = gr->jump > 0 ? (int)gr->jump : 1;
+        const int k0 = gr->start_t > 0.0f ? (int)ceil((double)gr->start_t / b->dt - 1e-9) : 0;
+        const int k1 = gr->end_t > 0.0f ? (int)floor((double)gr->end_t / b->dt + 1e-9) : INT_MAX;
+        rc = k1 >= k0 ? fwi_hip_set_imaging(g_ctx, jump, k0, k1) : FWI_HIP_EINVAL;
+        if (rc) return fail(rc, "imaging window: start_t=%g end_t=%g jump=%zu", gr->start_t, gr->end_t, gr->jump);
+    }
 
     fwi_hip_shot *sh = (fwi_hip_shot *)calloc(nshots, sizeof(fwi_hip_shot));
     int **cells = (int **)calloc(2 * nshots, sizeof(int *));

@@ -55,6 +55,7 @@ class Engine:
             self._fwd_id = fwd_id
         self.ctx.set_residual(res_np)
         grad, _ = self.ctx.adjoint()
+        self._fwd_id = None  # the adjoint consumed this forward's final wavefield
         return grad
 
 

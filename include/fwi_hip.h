@@ -96,7 +96,9 @@ int fwi_hip_set_residual(fwi_hip_ctx *ctx, const float *res);
 /* Back-propagation of the residual (exact transpose of the forward),
  * source-field reconstruction from the saved frames, zero-lag imaging
  * summed over all shots in FP64.  grad[nz][nx] = dJ/dvel, gk[nz][nx] =
- * ekp * dJ/dekp; either may be NULL (then it stays on the device).         */
+ * ekp * dJ/dekp; either may be NULL (then it stays on the device).  The
+ * reconstruction consumes the forward's final wavefield: a second adjoint
+ * needs a new fwi_hip_forward (FWI_HIP_ESTATE otherwise).                    */
 int fwi_hip_adjoint(fwi_hip_ctx *ctx, double *grad, double *gk);
 
 /* Time steps per captured HIP graph (default 100; 0 = plain launches).
@@ -109,6 +111,12 @@ int fwi_hip_set_graph(fwi_hip_ctx *ctx, int steps);
  * bit; gradients differ only in the FP64 summation order over shots.
  * (No counterpart in the reference, which has one CPU loop nest.)        */
 int fwi_hip_set_kernels(fwi_hip_ctx *ctx, int kind);
+/* Imaging window of the adjoint pass (fwi_grad's jump / start_t / end_t,
+ * header.h:119-127, read by grad4_1shot at header.h:283): the zero-lag
+ * image sums only the reconstruction steps k with k0 <= k <= k1 and
+ * (k - k0) % jump == 0.  Default: every step (jump 1, k0 0, k1 INT_MAX). */
+int fwi_hip_set_imaging(fwi_hip_ctx *ctx, int jump, int k0, int k1);
+
 /* Shots per workgroup of the tile reconstruction + imaging kernel (default
  * 8): the group's image sum meets the FP64 accumulator once per step.     */
 int fwi_hip_set_rec_group(fwi_hip_ctx *ctx, int shots);

@@ -54,7 +54,9 @@ int L2norm(float **obs, float **syn, int nt, int nr, float *sr_xyz, float min, f
 /* Normalised zero-lag cross-correlation (no body in the reference; used by
  * the inversion with this misfit): per trace c = <s,o> / (|s| |o|),
  * misfit[1] = sum (1 - c), res = -(o / (|s||o|) - c s / |s|^2);
- * misfit[0] = sum (s - o)^2.  Traces with |s| = 0 or |o| = 0 are skipped.   */
+ * misfit[0] = sum (s - o)^2.  Traces whose energy |s|^2 or |o|^2 is zero or
+ * below 1e-8 of the gather's largest are skipped (no usable waveform: the
+ * normalisation would amplify their rounding noise by 1/|s|).               */
 int NCCnorm(float **obs, float **syn, int nt, int nr, float *sr_xyz, float min, float max, float val[],
             float **res);
 

@@ -124,7 +124,7 @@ def mf1d(obs, syn, nmodel, niter, mode="middle", reg="diff", tor0=0.0, eps0=1.0,
             w0, w1 = A(p), D(p)
             num = z @ z
             den = w0 @ w0 + w1 @ w1
-            if den == 0.0:
+            if num == 0.0 or den == 0.0:
                 break
             al = num / den
             x = x + al * p
@@ -152,7 +152,7 @@ def mf1d(obs, syn, nmodel, niter, mode="middle", reg="diff", tor0=0.0, eps0=1.0,
             rn0 = np.sqrt(r @ r)
             nn = rn0 * rn0
             dd = p0 @ p0 + p1 @ p1
-            if dd == 0.0:
+            if nn == 0.0 or dd == 0.0:
                 break
             al = nn / dd
             y0 = y0 + al * p0

@@ -24,7 +24,7 @@ def h():
         pytest.skip("no GPU")
     lib = PC.lib()
     yield lib
-    lib.grad4_set_misfit(0, None, 1.0, 0.0)
+    lib.grad4_set_misfit(0, None, 1.0, 0.0)             # L2 over every trace
     lib.grad4_set_bandpass(0, 0, 0, 0, 0.0, 0.0)
     lib.grad4_set_fbt_mute(0, 0.0, 0.0, 0)
 
@@ -33,7 +33,7 @@ def params(kind):
     if kind == 2:
         return HC.ParXcorr1d(0, 0, 3, b"middle")
     if kind == 3:
-        return HC.Epsdf1(0, 0, 41, 0, 0, 41, b"middle", b"diff", 0.0, 0.3, 1.0)
+        return HC.Epsdf1(0, 0, 41, 0, 0, 300, b"middle", b"diff", 0.0, 0.3, 1.0)
     if kind == 4:
         return HC.EotPar(0, 0, 3000, None, None, None, 2e-4, 1e-9, 0.0)
     return None
@@ -48,13 +48,19 @@ def test_driver_gradient_is_the_derivative_of_the_misfit(h, kind, name, nt):
     o = Oracle(np.float32)
     obs0 = o.forward(g, vel * 1.02, shot)["syn"]
     par = params(kind)
-    assert h.grad4_set_misfit(kind, ctypes.byref(par) if par is not None else None, 1.0, 0.0) == 0
+    # receivers beyond 400 m see no wave within nt steps: left out by offset, as
+    # the reference's misfits do (xcorr_msf.c:69-78)
+    assert h.grad4_set_misfit(kind, ctypes.byref(par) if par is not None else None, 0.0, 400.0) == 0
     assert h.grad4_set_bandpass(1, 0, 4, 4, 35.0, 3.0) == 0
     assert h.grad4_set_fbt_mute(1, 30.0, 0.02, 1) == 0
     p = PC.make_plan(h, g, shot, obs0, vel)
     P = p.contents
     fbt = PC.view(P.msf.contents.fbt, (shot.nr,))
     fbt[:] = np.abs(shot.irx - shot.isx) * g.dx / 2300.0
+    xyz = PC.view(P.sou.contents.xyz_sr, (1 + shot.nr, 2))
+    xyz[0] = (shot.isx * g.dx, (shot.isz + 0.5) * g.dz)
+    xyz[1:, 0] = shot.irx * g.dx
+    xyz[1:, 1] = (shot.irz + 0.5) * g.dz
     md = PC.view(P.model.contents.md[0], (g.nz, g.nx))
     val = PC.view(P.msf.contents.val, (2,))
     body = PC.view(P.grad.contents.body, (g.nz, g.nx))
@@ -74,7 +80,10 @@ def test_driver_gradient_is_the_derivative_of_the_misfit(h, kind, name, nt):
         cz, cx = rng.uniform(10, g.nz - 5), rng.uniform(10, g.nx - 10)
         d += rng.standard_normal() * np.exp(-((zz - cz) ** 2 + (xx - cx) ** 2) / (2 * 8.0 ** 2))
     d *= 30.0 / np.abs(d).max()                       # up to 30 m/s
-    e = 0.5
+    # central differences: O(e^2) error; MF1D's penalty bends faster (1.5 % at
+    # e = 0.5, 0.08 % at 0.1), while EOT's small slope needs the larger step
+    # against the FP32 rounding of val[1]
+    e = 0.1 if kind == 3 else 0.5
     Jp, _ = run((vel + e * d).astype(np.float32))
     Jm, _ = run((vel - e * d).astype(np.float32))
     fd = (Jp - Jm) / (2 * e)

@@ -184,6 +184,41 @@ def test_tile_and_march_gradients_agree(lib):
         assert rel(gk, outs[0][2]) < 1e-12
 
 
+@pytest.mark.parametrize("kind", [0, 1])
+def test_imaging_window_is_a_partition_of_the_image(lib, kind):
+    """The image over every step equals the sum of the images over the even
+    and the odd steps (jump 2 from k0 = 0 and k0 = 1), and the sum of the
+    images over [0, 99] and [100, end]; a window past the last step gives
+    zero.  Reconstruction is unchanged, so traces and images add exactly up
+    to the FP64 order of the sums."""
+    g, vel, shots = small_case(nshots=2, nt=210, seed=11)
+    rng = np.random.default_rng(12)
+    res = [rng.standard_normal((s.nr, s.nt)).astype(np.float32) for s in shots]
+    ctx = lib.Context(g, max_shots=2, max_nt=210, max_nr=100)
+    ctx.set_kernels(kind)
+    ctx.set_model(vel)
+    ctx.set_shots(shots)
+
+    def image(jump=1, k0=0, k1=2**31 - 1):
+        ctx.forward(want_traces=False)          # each adjoint consumes a forward
+        ctx.set_residual(ctx.pack(res))
+        ctx.set_imaging(jump, k0, k1)
+        return ctx.adjoint()[1]
+
+    full = image()
+    even, odd = image(2, 0), image(2, 1)
+    early, late = image(1, 0, 99), image(1, 100)
+    none = image(1, 5000)
+    with pytest.raises(lib.FwiError):
+        ctx.adjoint()                           # no forward since the last adjoint
+    ctx.close()
+    assert np.abs(full).max() > 0 and not none.any()
+    for a, b in ((even, odd), (early, late)):
+        assert np.abs(a).max() > 0 and np.abs(b).max() > 0
+        print(f"kind {kind}: split image rel {rel(a + b, full):.2e}")
+        assert rel(a + b, full) < 1e-12
+
+
 # ---------------------------------------------------------------- adjoint
 def test_adjoint_matches_oracle(lib, o32, o64):
     g, vel, shots = small_case(nshots=2, nt=260, seed=5)

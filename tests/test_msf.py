@@ -106,6 +106,13 @@ def test_ncc_closed_forms_and_gradient(h):
         syn2 = np.ascontiguousarray(syn2, np.float32)
         assert h.NCCnorm(H.rows(obs), H.rows(syn2), nt, nr, None, 0.0, 0.0, H.fp(val), H.rows(res)) == 0
         assert val[1] == pytest.approx(want, abs=1e-6)
+    # a trace with no usable waveform (energy below 1e-8 of the largest):
+    # skipped, instead of an adjoint source of order 1/|s|
+    weak = syn.copy()
+    weak[1] *= 1e-5
+    assert h.NCCnorm(H.rows(obs), H.rows(weak), nt, nr, None, 0.0, 0.0, H.fp(val), H.rows(res)) == 0
+    assert not res[1].any() and res[0].any()
+    assert val[1] == pytest.approx(ncc_value(obs[[0, 2]], weak[[0, 2]]), rel=1e-6)
     # zero synthetic: skipped, no NaN
     z = np.zeros_like(obs)
     assert h.NCCnorm(H.rows(obs), H.rows(z), nt, nr, None, 0.0, 0.0, H.fp(val), H.rows(res)) == 0
