@@ -72,7 +72,7 @@ def test_driver_gradient_is_the_derivative_of_the_misfit(h, kind, name, nt):
         return float(val[1]), body.copy()
 
     J0, g0 = run(vel)
-    assert J0 > 0
+    assert J0 != 0.0   # entropic OT includes eps * sum P (log P - 1): its value may be negative
     rng = np.random.default_rng(3)
     zz, xx = np.meshgrid(np.arange(g.nz), np.arange(g.nx), indexing="ij")
     d = np.zeros((g.nz, g.nx))
