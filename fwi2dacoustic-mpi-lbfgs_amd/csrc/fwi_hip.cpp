@@ -663,6 +663,12 @@ extern "C" int fwi_hip_set_rec_group(fwi_hip_ctx *c, int shots)
 extern "C" int fwi_hip_set_imaging(fwi_hip_ctx *c, int jump, int k0, int k1)
 {
     if (!c || jump < 1 || k0 < 0 || k1 < k0) return set_err(c, FWI_HIP_EINVAL, "set_imaging: need jump >= 1, 0 <= k0 <= k1");
+    if ((jump != c->img_jump || k0 != c->img_k0 || k1 != c->img_k1) && c->gx_bwd) {
+        // the z-marching kernels take their arguments by value: a captured
+        // backward graph holds the old window
+        (void)hipGraphExecDestroy(c->gx_bwd);
+        c->gx_bwd = nullptr;
+    }
     c->img_jump = jump;
     c->img_k0 = k0;
     c->img_k1 = k1;
