@@ -45,6 +45,12 @@ def parse():
     ap.add_argument("--group", type=int, default=0, help="shots per workgroup of the z-marching kernels")
     ap.add_argument("--kernels", type=int, default=0, help="0: 2-D tile kernels, 1: z-marching kernels")
     ap.add_argument("--rec-group", type=int, default=0, help="shots per workgroup of the tile imaging kernel")
+    ap.add_argument("--bwd-chunk", type=int, default=-1, help="shots per backward pass (0: all; -1: the default)")
+    ap.add_argument("--workload", default="batched", choices=("batched", "uneven"),
+                    help="batched: the 1024x320 64-shot gradient (the headline line); uneven: 1536x400, "
+                         "96 shots with nt 2000-6000, apertures 256-1536, host Butterworth band-pass")
+    ap.add_argument("--autotune", action="store_true",
+                    help="time short passes of the candidate launch settings first and use the fastest")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-single", action="store_true", help="skip the one-shot-per-launch comparison")
     ap.add_argument("--cpu-nt", type=int, default=1500, help="time steps of the CPU sample shot")
@@ -102,6 +108,87 @@ def cpu_baseline(g, vel, shot, nt_sample):
     return rate, dt, threads
 
 
+def run_uneven(args):
+    """The uneven batch: per-shot nt and receiver apertures in one batch; the
+    misfit step on the host band-passes the synthetic traces (4-pole zero-phase
+    Butterworth, 3-25 Hz, the observed data filtered once), takes the L2
+    residual and applies the transposed filter, over a thread pool (the C
+    calls release the interpreter lock)."""
+    import ctypes
+    from concurrent.futures import ThreadPoolExecutor
+    import numpy as np
+    from fwihip import lib
+    from fwihip.workloads import config3, triangle_smooth
+
+    g, vel, shots = config3(nshots=args.shots if args.shots != 64 else 96)
+    start = triangle_smooth(vel, 25)
+    nr = max(s.nr for s in shots)
+    ntmax = max(s.nt for s in shots)
+    h = lib.load()
+    PF = ctypes.POINTER(ctypes.c_float)
+    for name in ("sfbandpass", "sfbandpass_adjoint"):
+        fn = getattr(h, name)
+        fn.argtypes = [PF, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_bool, ctypes.c_int, ctypes.c_int,
+                       ctypes.c_float, ctypes.c_float, PF]
+        fn.restype = ctypes.c_int
+    flo, fhi = 3.0, 25.0
+
+    def band(x, adjoint=False):
+        y = np.ascontiguousarray(x, np.float32).copy()
+        fn = h.sfbandpass_adjoint if adjoint else h.sfbandpass
+        rc = fn(y.ctypes.data_as(PF), y.shape[1], y.shape[0], g.dt, False, 4, 4, fhi, flo, y.ctypes.data_as(PF))
+        assert rc == 0
+        return y
+
+    pool = ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1))
+    ctx = lib.Context(g, max_shots=len(shots), max_nt=ntmax, max_nr=nr)
+    ctx.set_model(vel)
+    ctx.set_shots(shots)
+    obs_f = list(pool.map(band, ctx.split(ctx.forward())))
+    timing = {"host_s": 0.0}
+
+    def one(a):
+        syn, ob = a
+        r = band(syn) - ob
+        return 0.5 * float((r.astype(np.float64) ** 2).sum()), band(r, adjoint=True)
+
+    def step():
+        ctx.set_model(start)
+        syn = ctx.split(ctx.forward())
+        t0 = time.perf_counter()
+        out = list(pool.map(one, zip(syn, obs_f)))
+        timing["host_s"] += time.perf_counter() - t0
+        ctx.set_residual(ctx.pack([r for _, r in out]))
+        ctx.adjoint(want=False)
+        return sum(j for j, _ in out)
+
+    for _ in range(args.warmup):
+        step()
+    timing["host_s"] = 0.0
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        J = step()
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    cells = sum((g.pnx * g.pnz * 2 + g.nx * g.nz) * s.nt for s in shots)
+    out = {
+        "metric": "shot-gradients/s & Gcell-updates/s, uneven batch 1536x400, 96 shots, nt 2000-6000",
+        "value": len(shots) * args.steps / wall, "unit": "shot-gradients/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded layered/faulted model, Ricker shots of 5-15 Hz, observed data from the true model)",
+        "config": {"workload": f"uneven FWI gradient {g.nx}x{g.nz} dt={g.dt * 1e3:g} ms, {len(shots)} shots, "
+                               f"nt {min(s.nt for s in shots)}-{ntmax}, apertures "
+                               f"{min(s.nr for s in shots)}-{nr} receivers, host 4-pole Butterworth {flo:g}-{fhi:g} Hz",
+                   "nx": g.nx, "nz": g.nz, "nshots": len(shots)},
+        "gcell_updates_per_s": cells * args.steps / wall / 1e9,
+        "host_misfit_ms_per_step": timing["host_s"] / args.steps * 1e3,
+        "misfit": J,
+    }
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -119,6 +206,9 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4, "launches": 5}))
         return
+    if args.workload == "uneven":
+        run_uneven(args)
+        return
     import numpy as np
     from fwihip import lib
     from fwihip.workloads import config2, triangle_smooth
@@ -126,11 +216,22 @@ def main():
     g, vel, shots = config2(nshots=args.shots, nt=args.nt)
     start = triangle_smooth(vel, 25)
     nr = max(s.nr for s in shots)
-    ctx = lib.Context(g, max_shots=len(shots), max_nt=args.nt, max_nr=nr, tile_z=args.tile_z,
-                      shots_per_group=args.group)
-    ctx.set_kernels(args.kernels)
-    if args.rec_group:
-        ctx.set_rec_group(args.rec_group)
+    launch = None
+    if args.autotune:
+        from fwihip import autotune
+        t0 = time.perf_counter()
+        launch, s_step, _ = autotune.tune(g, start, shots, verbose=True)
+        launch["tuning_s"] = round(time.perf_counter() - t0, 1)
+        launch["probe_us_per_step"] = round(s_step * 1e6, 1) if s_step else None
+        ctx = autotune.make_context(g, shots, launch, args.nt)
+    else:
+        ctx = lib.Context(g, max_shots=len(shots), max_nt=args.nt, max_nr=nr, tile_z=args.tile_z,
+                          shots_per_group=args.group)
+        ctx.set_kernels(args.kernels)
+        if args.rec_group:
+            ctx.set_rec_group(args.rec_group)
+        if args.bwd_chunk >= 0:
+            ctx.set_bwd_chunk(args.bwd_chunk)
     ctx.set_model(vel)
     ctx.set_shots(shots)
     obs = ctx.forward()
@@ -161,7 +262,8 @@ def main():
     ctx.set_timing(True)
     step()
     kern = {}
-    names = (("k_fwd_tile", "k_adj_tile", "k_rec_tile") if args.kernels == 0
+    kind = launch["kernels"] if launch else args.kernels
+    names = (("k_fwd_tile", "k_adj_tile", "k_rec_tile") if kind == 0
              else ("k_fwd_step", "k_adj_step", "k_rec_step"))
     alg = algorithmic_bytes(g, nshots)
     mint = min_traffic_bytes(g, nshots, nr)
@@ -228,7 +330,9 @@ def main():
         "data": "synthetic (seeded layered/faulted model, Ricker shots, observed data from the true model)",
         "config": {"workload": f"batched FWI gradient {g.nx}x{g.nz} dx=dz={g.dx:g} m dt={g.dt * 1e3:g} ms "
                                f"nt={args.nt}, {nshots} shots x {nr} receivers, npml={g.npml}, free surface",
-                   "nx": g.nx, "nz": g.nz, "nt": args.nt, "nshots": nshots, "nr": nr, "npml": g.npml},
+                   "nx": g.nx, "nz": g.nz, "nt": args.nt, "nshots": nshots, "nr": nr, "npml": g.npml,
+                   "launch": launch or {"kernels": args.kernels, "rec_group": args.rec_group or 8,
+                                        "bwd_chunk": max(args.bwd_chunk, 0), "graph": 100}},
         "gcell_updates_per_s": gcell,
         "hbm_frac_whole_job": gcell * 1e9 * BYTES_PER_CELL_UPDATE / HBM_PEAK,
         "misfit": J,

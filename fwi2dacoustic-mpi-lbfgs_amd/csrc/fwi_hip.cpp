@@ -107,7 +107,10 @@ struct fwi_hip_ctx {
     int *tl_fwd[2] = {nullptr, nullptr}, *tl_adj[2] = {nullptr, nullptr}, *tl_rec[2] = {nullptr, nullptr};
     int n_fwd[2] = {0, 0}, n_adj[2] = {0, 0}, n_rec[2] = {0, 0};
     int rec_lists_for = -1;  // rec_group the reconstruction lists were built for (-1: none)
-    hipGraphExec_t gx_fwd = nullptr, gx_bwd = nullptr;
+    hipGraphExec_t gx_fwd = nullptr;
+    std::vector<hipGraphExec_t> gx_bwd; // one captured block of backward steps per shot chunk
+    int bwd_chunk = 0;                  // shots per backward pass (0: all in one)
+    std::vector<int> adj_pre[2], rec_pre[2]; // list entries before shot s / before group g
     // timing
     bool timing = false;
     double t_ms[3] = {0, 0, 0};
@@ -233,6 +236,13 @@ static int check_grid(fwi_hip_ctx *c, const fwi_hip_grid &g)
 
 // --------------------------------------------------------------------------
 extern "C" const char *fwi_hip_last_error(const fwi_hip_ctx *c) { return c ? c->err : g_err; }
+static void drop_bwd(fwi_hip_ctx *c)
+{
+    for (hipGraphExec_t &x : c->gx_bwd)
+        if (x) (void)hipGraphExecDestroy(x);
+    c->gx_bwd.clear();
+}
+
 extern "C" size_t fwi_hip_bytes(const fwi_hip_ctx *c) { return c ? c->bytes : 0; }
 
 extern "C" void fwi_hip_destroy(fwi_hip_ctx *c)
@@ -240,7 +250,7 @@ extern "C" void fwi_hip_destroy(fwi_hip_ctx *c)
     if (!c) return;
     if (c->st) (void)hipStreamSynchronize(c->st);
     if (c->gx_fwd) (void)hipGraphExecDestroy(c->gx_fwd);
-    if (c->gx_bwd) (void)hipGraphExecDestroy(c->gx_bwd);
+    drop_bwd(c);
     for (void *p : c->allocs) (void)hipFree(p);
     for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
     if (c->own_stream && c->st) (void)hipStreamDestroy(c->st);
@@ -514,13 +524,17 @@ extern "C" int fwi_hip_set_shots(fwi_hip_ctx *c, int nshots, const fwi_hip_shot 
     {
         const int nsz = (g.pnz + FWI_STRIP - 1) / FWI_STRIP;
         std::vector<int> lf[2], la[2];
-        for (int s = 0; s < nshots; s++)
+        for (int q = 0; q < 2; q++) c->adj_pre[q].assign(nshots + 1, 0);
+        for (int s = 0; s < nshots; s++) {
+            for (int q = 0; q < 2; q++) c->adj_pre[q][s] = (int)la[q].size();
             for (int sz = 0; sz < nsz; sz++)
                 for (int tx = 0; tx < g.ntx; tx++) {
                     const int x0 = tx * FWI_TX, r0 = sz * FWI_STRIP, l = (s * nsz + sz) * g.ntx + tx;
                     lf[fwi_fwd_strip_interior(g, sd[s], x0, r0) ? 0 : 1].push_back(l);
                     la[fwi_adj_strip_interior(g, sd[s], x0, r0) ? 0 : 1].push_back(l);
                 }
+        }
+        for (int q = 0; q < 2; q++) c->adj_pre[q][nshots] = (int)la[q].size();
         for (int q = 0; q < 2; q++) {
             c->n_fwd[q] = (int)lf[q].size();
             c->n_adj[q] = (int)la[q].size();
@@ -535,8 +549,8 @@ extern "C" int fwi_hip_set_shots(fwi_hip_ctx *c, int nshots, const fwi_hip_shot 
     }
     HIPCHK(c, hipStreamSynchronize(c->st));
     if (c->gx_fwd) (void)hipGraphExecDestroy(c->gx_fwd);
-    if (c->gx_bwd) (void)hipGraphExecDestroy(c->gx_bwd);
-    c->gx_fwd = c->gx_bwd = nullptr;
+    c->gx_fwd = nullptr;
+    drop_bwd(c);
     c->shots_h = sd;
     c->nshots = nshots;
     c->kmax = kmax;
@@ -644,8 +658,8 @@ extern "C" int fwi_hip_set_kernels(fwi_hip_ctx *c, int fwd_kind)
 {
     if (!c || fwd_kind < 0 || fwd_kind > 1) return set_err(c, FWI_HIP_EINVAL, "set_kernels: bad argument");
     if (c->gx_fwd) (void)hipGraphExecDestroy(c->gx_fwd);
-    if (c->gx_bwd) (void)hipGraphExecDestroy(c->gx_bwd);
-    c->gx_fwd = c->gx_bwd = nullptr;
+    c->gx_fwd = nullptr;
+    drop_bwd(c);
     c->fwd_kind = fwd_kind;
     return FWI_HIP_OK;
 }
@@ -653,8 +667,7 @@ extern "C" int fwi_hip_set_kernels(fwi_hip_ctx *c, int fwd_kind)
 extern "C" int fwi_hip_set_rec_group(fwi_hip_ctx *c, int shots)
 {
     if (!c || shots < 1) return set_err(c, FWI_HIP_EINVAL, "set_rec_group: need shots >= 1");
-    if (c->gx_bwd) (void)hipGraphExecDestroy(c->gx_bwd);
-    c->gx_bwd = nullptr;
+    drop_bwd(c);
     c->rec_group = shots;
     c->rec_lists_for = -1;
     return FWI_HIP_OK;
@@ -663,12 +676,8 @@ extern "C" int fwi_hip_set_rec_group(fwi_hip_ctx *c, int shots)
 extern "C" int fwi_hip_set_imaging(fwi_hip_ctx *c, int jump, int k0, int k1)
 {
     if (!c || jump < 1 || k0 < 0 || k1 < k0) return set_err(c, FWI_HIP_EINVAL, "set_imaging: need jump >= 1, 0 <= k0 <= k1");
-    if ((jump != c->img_jump || k0 != c->img_k0 || k1 != c->img_k1) && c->gx_bwd) {
-        // the z-marching kernels take their arguments by value: a captured
-        // backward graph holds the old window
-        (void)hipGraphExecDestroy(c->gx_bwd);
-        c->gx_bwd = nullptr;
-    }
+    if (jump != c->img_jump || k0 != c->img_k0 || k1 != c->img_k1)
+        drop_bwd(c); // the z-marching kernels take their arguments by value: a captured graph holds the old window
     c->img_jump = jump;
     c->img_k0 = k0;
     c->img_k1 = k1;
@@ -679,9 +688,17 @@ extern "C" int fwi_hip_set_graph(fwi_hip_ctx *c, int steps)
 {
     if (!c || steps < 0) return set_err(c, FWI_HIP_EINVAL, "set_graph: bad argument");
     if (c->gx_fwd) (void)hipGraphExecDestroy(c->gx_fwd);
-    if (c->gx_bwd) (void)hipGraphExecDestroy(c->gx_bwd);
-    c->gx_fwd = c->gx_bwd = nullptr;
+    c->gx_fwd = nullptr;
+    drop_bwd(c);
     c->graph_steps = steps;
+    return FWI_HIP_OK;
+}
+
+extern "C" int fwi_hip_set_bwd_chunk(fwi_hip_ctx *c, int shots)
+{
+    if (!c || shots < 0) return set_err(c, FWI_HIP_EINVAL, "set_bwd_chunk: need shots >= 0");
+    drop_bwd(c);
+    c->bwd_chunk = shots;
     return FWI_HIP_OK;
 }
 
@@ -845,7 +862,9 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
         // interior for every shot of the group
         const int ntz = (g.nz + FWI_REC_ROWS - 1) / FWI_REC_ROWS, ntiles = g.bntx * ntz;
         std::vector<int> lr[2];
-        for (int grp = 0; grp < ngr; grp++)
+        for (int q = 0; q < 2; q++) c->rec_pre[q].assign(ngr + 1, 0);
+        for (int grp = 0; grp < ngr; grp++) {
+            for (int q = 0; q < 2; q++) c->rec_pre[q][grp] = (int)lr[q].size();
             for (int t = 0; t < ntiles; t++) {
                 const int bx0 = (t % g.bntx) * FWI_TX, z0 = (t / g.bntx) * FWI_REC_ROWS;
                 bool in = true;
@@ -853,6 +872,8 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
                     in = fwi_rec_tile_interior(g, c->shots_h[s], bx0, z0);
                 lr[in ? 0 : 1].push_back(grp * ntiles + t);
             }
+        }
+        for (int q = 0; q < 2; q++) c->rec_pre[q][ngr] = (int)lr[q].size();
         for (int q = 0; q < 2; q++) {
             c->n_rec[q] = (int)lr[q].size();
             if (c->n_rec[q])
@@ -864,44 +885,70 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
     }
     const int nb_adj = g.ntx * g.ntz * c->ngroups;
     const int nb_rec = g.bntx * g.bntz * c->ngroups;
-    auto launch_adj = [&](const int *kb, int koff) {
-        return tile ? fwi_launch_adj_tile(&a, ad, c->tl_adj, c->n_adj, kb, koff, pair_of(c))
-                    : fwi_launch_adj(&a, kb, koff, nb_adj, c->st);
-    };
-    auto launch_rec = [&](const int *kb, int koff) {
-        return tile ? fwi_launch_rec_tile(&r, rd, c->tl_rec, c->n_rec, kb, koff, rgs, pair_of(c))
-                    : fwi_launch_rec(&r, kb, koff, nb_rec, c->st);
-    };
+    // shot chunks, each a whole backward pass over its own slice of the
+    // strip and item lists (shot-major and group-major), so that a chunk's
+    // working set (mu, p and the FP64 images of its shots) can stay cached
+    const int cs = (tile && c->bwd_chunk > 0 && c->bwd_chunk < c->nshots) ? ((c->bwd_chunk + rgs - 1) / rgs) * rgs
+                                                                          : c->nshots;
+    const int nch = (c->nshots + cs - 1) / cs;
+    if ((int)c->gx_bwd.size() != nch) {
+        drop_bwd(c);
+        c->gx_bwd.assign(nch, nullptr);
+    }
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
     const int B = c->graph_steps;
-    if (B > 0 && !c->timing) {
-        if (!c->gx_bwd) {
-            rc = capture(c, &c->gx_bwd, [&]() {
-                hipError_t e = hipSuccess;
-                for (int j = 0; j < B && e == hipSuccess; j++) {
-                    e = launch_adj(c->kbase, -j);
-                    if (e == hipSuccess) e = launch_rec(c->kbase, -j);
-                }
-                if (e == hipSuccess) e = fwi_launch_advance(c->kbase, -B, c->st);
-                return e;
-            });
-            if (rc) return rc;
-        }
-        HIPCHK(c, fwi_launch_set_int(c->kbase, c->kmax - 1, c->st));
-        for (int blk = 0; blk < (c->kmax + B - 1) / B; blk++) HIPCHK(c, hipGraphLaunch(c->gx_bwd, c->st));
-    } else {
-        for (int k = c->kmax - 1; k >= 0; k--) {
-            hipEvent_t e0, e1, e2;
-            if ((rc = tick(c, &e0))) return rc;
-            HIPCHK(c, launch_adj(nullptr, k));
-            if ((rc = tick(c, &e1))) return rc;
-            HIPCHK(c, launch_rec(nullptr, k));
-            if ((rc = tick(c, &e2))) return rc;
-            if (c->timing) {
-                marks.push_back({1, {e0, e1}});
-                marks.push_back({2, {e1, e2}});
+    for (int ch = 0; ch < nch; ch++) {
+        const int s0 = ch * cs, s1 = std::min(s0 + cs, c->nshots);
+        int kmax = 0;
+        for (int s = s0; s < s1; s++) kmax = std::max(kmax, c->shots_h[s].nt);
+        const int *ta[2], *tr[2];
+        int na[2], nr[2];
+        if (tile) {
+            const int g0 = s0 / rgs, g1 = (s1 + rgs - 1) / rgs;
+            for (int q = 0; q < 2; q++) {
+                ta[q] = c->tl_adj[q] + c->adj_pre[q][s0];
+                na[q] = c->adj_pre[q][s1] - c->adj_pre[q][s0];
+                tr[q] = c->tl_rec[q] + c->rec_pre[q][g0];
+                nr[q] = c->rec_pre[q][g1] - c->rec_pre[q][g0];
             }
-            if (marks.size() >= 512 && (rc = collect_timing(c, marks))) return rc;
+        }
+        auto launch_adj = [&](const int *kb, int koff) {
+            return tile ? fwi_launch_adj_tile(&a, ad, ta, na, kb, koff, pair_of(c))
+                        : fwi_launch_adj(&a, kb, koff, nb_adj, c->st);
+        };
+        auto launch_rec = [&](const int *kb, int koff) {
+            return tile ? fwi_launch_rec_tile(&r, rd, tr, nr, kb, koff, rgs, pair_of(c))
+                        : fwi_launch_rec(&r, kb, koff, nb_rec, c->st);
+        };
+        if (B > 0 && !c->timing) {
+            if (!c->gx_bwd[ch]) {
+                rc = capture(c, &c->gx_bwd[ch], [&]() {
+                    hipError_t e = hipSuccess;
+                    for (int j = 0; j < B && e == hipSuccess; j++) {
+                        e = launch_adj(c->kbase, -j);
+                        if (e == hipSuccess) e = launch_rec(c->kbase, -j);
+                    }
+                    if (e == hipSuccess) e = fwi_launch_advance(c->kbase, -B, c->st);
+                    return e;
+                });
+                if (rc) return rc;
+            }
+            HIPCHK(c, fwi_launch_set_int(c->kbase, kmax - 1, c->st));
+            for (int blk = 0; blk < (kmax + B - 1) / B; blk++) HIPCHK(c, hipGraphLaunch(c->gx_bwd[ch], c->st));
+        } else {
+            for (int k = kmax - 1; k >= 0; k--) {
+                hipEvent_t e0, e1, e2;
+                if ((rc = tick(c, &e0))) return rc;
+                HIPCHK(c, launch_adj(nullptr, k));
+                if ((rc = tick(c, &e1))) return rc;
+                HIPCHK(c, launch_rec(nullptr, k));
+                if ((rc = tick(c, &e2))) return rc;
+                if (c->timing) {
+                    marks.push_back({1, {e0, e1}});
+                    marks.push_back({2, {e1, e2}});
+                }
+                if (marks.size() >= 512 && (rc = collect_timing(c, marks))) return rc;
+            }
         }
     }
     if ((rc = collect_timing(c, marks))) return rc;

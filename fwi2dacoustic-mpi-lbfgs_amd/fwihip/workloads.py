@@ -185,3 +185,20 @@ def config2(nshots=64, nt=4000, seed=2026):
     vel = layered_faulted(g, seed)
     shots = surface_shots(g, nshots, nt, seed=seed + 1)
     return g, vel, shots
+
+
+def config3(nshots=96, nt_range=(2000, 6000), seed=2026):
+    """Uneven batch: 1536x400, dx=dz=10 m, dt=0.8 ms, npml=20, 96 shots with
+    per-shot nt drawn uniformly from nt_range, receiver apertures of 256-1536
+    contiguous columns centred on each source, Ricker peak frequency drawn
+    uniformly from 5-15 Hz, on the seeded layered/faulted model."""
+    g = Geometry(nx=1536, nz=400, npml=20, dx=10.0, dz=10.0, dt=0.8e-3)
+    vel = layered_faulted(g, seed)
+    rng = np.random.default_rng(seed + 2)
+    shots = []
+    for j in range(nshots):
+        nt = int(rng.integers(nt_range[0], nt_range[1] + 1))
+        aperture = int(rng.integers(256, 1537))
+        s = surface_shots(g, nshots, nt, seed=seed + 1 + j, fmin=5.0, fmax=15.0, aperture=aperture)[j]
+        shots.append(s)
+    return g, vel, shots

@@ -117,6 +117,13 @@ int fwi_hip_set_kernels(fwi_hip_ctx *ctx, int kind);
  * (k - k0) % jump == 0.  Default: every step (jump 1, k0 0, k1 INT_MAX). */
 int fwi_hip_set_imaging(fwi_hip_ctx *ctx, int jump, int k0, int k1);
 
+/* Shots per backward pass of the tile kernels (0, the default: all shots in
+ * one pass).  With n > 0 the adjoint pass runs the backward time loop once
+ * per chunk of n shots (rounded up to whole reconstruction groups), so that
+ * a chunk's working set can stay in the device's last-level cache; the
+ * image is bit-identical to the single pass.                               */
+int fwi_hip_set_bwd_chunk(fwi_hip_ctx *ctx, int shots);
+
 /* Shots per workgroup of the tile reconstruction + imaging kernel (default
  * 8): the group's image sum meets the FP64 accumulator once per step.     */
 int fwi_hip_set_rec_group(fwi_hip_ctx *ctx, int shots);
