@@ -113,7 +113,17 @@ struct FwiFwdArgs {
     float *xh, *xd, *zh, *zd;
     float *syn;
     float *pstrip, *xstrip;
+    // segment recompute of the checkpointed backward pass: X^k of every
+    // physical cell at xseg[(k - xseg_k0) * xseg_step + s nx nz + i nx + j]
+    float *xseg;
+    long long xseg_step;
+    int xseg_k0, pad2_;
 };
+
+// forward tile kernel modes
+#define FWI_FWD_FRAMES 0   // record traces, save the boundary frames (boundary-saving reconstruction)
+#define FWI_FWD_TRACES 1   // record traces only (checkpointed forward pass)
+#define FWI_FWD_XSEG 2     // store X on the physical grid only (segment recompute)
 
 struct FwiAdjArgs {
     FwiGeomDev g;

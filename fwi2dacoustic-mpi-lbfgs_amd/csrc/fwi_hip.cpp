@@ -21,7 +21,14 @@
 extern "C" {
 hipError_t fwi_launch_fwd(const FwiFwdArgs *a, const int *kbase, int koff, int nblocks, hipStream_t st);
 hipError_t fwi_launch_fwd_tile(const FwiFwdArgs *a, const FwiFwdArgs *a_dev, const int *const *tl, const int *ntl,
-                               const int *kbase, int koff, const FwiPair &p);
+                               const int *kbase, int koff, const FwiPair &p, int mode);
+hipError_t fwi_launch_absmax(const float *x, long long n, int nitem, unsigned *bits, hipStream_t st);
+hipError_t fwi_launch_pack16(const float *x, long long n, int nitem, const unsigned *bits, unsigned short *dst,
+                             int fp16, hipStream_t st);
+hipError_t fwi_launch_unpack16(const unsigned short *src, long long n, int nitem, const unsigned *bits, float *x,
+                               int fp16, hipStream_t st);
+hipError_t fwi_launch_img_seg(const FwiRecArgs *a, const FwiRecArgs *a_dev, const float *xk, int k, int gsize,
+                              int ngr, hipStream_t st);
 hipError_t fwi_launch_adj(const FwiAdjArgs *a, const int *kbase, int koff, int nblocks, hipStream_t st);
 hipError_t fwi_launch_adj_tile(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, const int *const *tl, const int *ntl,
                                const int *kbase, int koff, const FwiPair &p);
@@ -74,7 +81,19 @@ struct fwi_hip_ctx {
     long long fs = 0, xh_b = 0, xd_b = 0, zh_b = 0, zd_b = 0;
     float *tab = nullptr;
     // frames and traces
-    float *pstrip = nullptr, *xstrip = nullptr;
+    float *pstrip = nullptr, *xstrip = nullptr;  // boundary frames, allocated by the first boundary-saving forward
+    // checkpointed backward pass (ckpt_every > 0): snapshots of p every
+    // ckpt_every steps in FP32/FP16/BF16 (per-shot max-abs scale for 16 bit)
+    // with the CPML memory in FP32, a second CPML set for the recompute,
+    // and X of one segment on the physical grid
+    int ckpt_every = 0, ckpt_storage = 0;
+    void *ck_store = nullptr;
+    unsigned *ck_scale = nullptr;
+    float *ck_cpml = nullptr, *ck_xseg = nullptr;
+    float *xh2 = nullptr, *xd2 = nullptr, *zh2 = nullptr, *zd2 = nullptr;
+    long long ck_nseg = 0, ck_K = 0, ck_S = 0;  // capacity of the snapshot buffers
+    FwiFwdArgs *fwd_rc_d = nullptr;
+    FwiFwdArgs fwd_rc_h;
     float *syn_tm = nullptr, *syn = nullptr, *obs = nullptr, *res = nullptr, *res_tm = nullptr;
     double *gacc = nullptr, *gk = nullptr, *grad = nullptr, *partial = nullptr;
     int npartial = 1024;
@@ -346,8 +365,6 @@ extern "C" int fwi_hip_create(const fwi_hip_desc *desc, fwi_hip_ctx **out)
         {(void **)&c->zh, sizeof(float) * S * 2 * c->zh_b},
         {(void **)&c->zd, sizeof(float) * S * 2 * c->zd_b},
         {(void **)&c->tab, sizeof(float) * 6 * (2 * g.npml + 1)},
-        {(void **)&c->pstrip, sizeof(float) * S * NT * g.sl},
-        {(void **)&c->xstrip, sizeof(float) * S * NT * g.sl},
         {(void **)&c->syn_tm, sizeof(float) * S * NT * NR},
         {(void **)&c->syn, sizeof(float) * S * NT * NR},
         {(void **)&c->obs, sizeof(float) * S * NT * NR},
@@ -369,6 +386,7 @@ extern "C" int fwi_hip_create(const fwi_hip_desc *desc, fwi_hip_ctx **out)
         {(void **)&c->tl_rec[1], sizeof(int) * S * g.bntx * ((g.nz + FWI_REC_ROWS - 1) / FWI_REC_ROWS)},
         {(void **)&c->adj_args_d, sizeof(FwiAdjArgs)},
         {(void **)&c->rec_args_d, sizeof(FwiRecArgs)},
+        {(void **)&c->fwd_rc_d, sizeof(FwiFwdArgs)},
     };
     for (auto &a : plan) {
         rc = dalloc(c, a.p, a.bytes);
@@ -595,6 +613,114 @@ static FwiBatchDev batch_dev(fwi_hip_ctx *c)
     return b;
 }
 
+// boundary frames of the boundary-saving reconstruction (max_shots x max_nt steps)
+static int ensure_frames(fwi_hip_ctx *c)
+{
+    if (c->pstrip) return FWI_HIP_OK;
+    const size_t n = sizeof(float) * (size_t)c->d.max_shots * c->d.max_nt * c->g.sl;
+    int rc = dalloc(c, (void **)&c->pstrip, n);
+    if (!rc) rc = dalloc(c, (void **)&c->xstrip, n);
+    if (rc) {
+        dfree(c, c->pstrip);
+        c->pstrip = nullptr;
+    }
+    return rc;
+}
+
+static long long cpml_floats(const fwi_hip_ctx *c, long long S)
+{
+    return S * 2 * (c->xh_b + c->xd_b + c->zh_b + c->zd_b);
+}
+
+// snapshot, scale and segment buffers for the current shots and nt
+static int ensure_ckpt(fwi_hip_ctx *c)
+{
+    const long long S = c->nshots, K = c->ckpt_every, nseg = (c->kmax + K - 1) / K;
+    const long long nb = (long long)c->g.nx * c->g.nz;
+    if (c->ck_store && nseg <= c->ck_nseg && S <= c->ck_S && K <= c->ck_K) return FWI_HIP_OK;
+    dfree(c, c->ck_store);
+    dfree(c, c->ck_scale);
+    dfree(c, c->ck_cpml);
+    dfree(c, c->ck_xseg);
+    c->ck_store = nullptr;
+    c->ck_scale = nullptr;
+    c->ck_cpml = c->ck_xseg = nullptr;
+    c->ck_nseg = c->ck_S = c->ck_K = 0;
+    int rc = FWI_HIP_OK;
+    if (!c->xh2) {
+        if ((rc = dalloc(c, (void **)&c->xh2, sizeof(float) * c->d.max_shots * 2 * c->xh_b)) ||
+            (rc = dalloc(c, (void **)&c->xd2, sizeof(float) * c->d.max_shots * 2 * c->xd_b)) ||
+            (rc = dalloc(c, (void **)&c->zh2, sizeof(float) * c->d.max_shots * 2 * c->zh_b)) ||
+            (rc = dalloc(c, (void **)&c->zd2, sizeof(float) * c->d.max_shots * 2 * c->zd_b)))
+            return rc;
+    }
+    const size_t es = c->ckpt_storage ? 2 : 4;
+    if ((rc = dalloc(c, &c->ck_store, es * nseg * 2 * S * c->fs)) ||
+        (rc = dalloc(c, (void **)&c->ck_scale, sizeof(unsigned) * nseg * 2 * S)) ||
+        (rc = dalloc(c, (void **)&c->ck_cpml, sizeof(float) * nseg * cpml_floats(c, S))) ||
+        (rc = dalloc(c, (void **)&c->ck_xseg, sizeof(float) * K * S * nb)))
+        return rc;
+    c->ck_nseg = nseg;
+    c->ck_S = S;
+    c->ck_K = K;
+    return FWI_HIP_OK;
+}
+
+// the forward state before step j K: p in P0/P1 and the CPML memory
+static int ck_save(fwi_hip_ctx *c, long long j)
+{
+    const long long S = c->nshots, n = S * c->fs;
+    float *cp = c->ck_cpml + j * cpml_floats(c, c->ck_S);
+    HIPCHK(c, hipMemcpyAsync(cp, c->xh, sizeof(float) * S * 2 * c->xh_b, hipMemcpyDeviceToDevice, c->st));
+    cp += S * 2 * c->xh_b;
+    HIPCHK(c, hipMemcpyAsync(cp, c->xd, sizeof(float) * S * 2 * c->xd_b, hipMemcpyDeviceToDevice, c->st));
+    cp += S * 2 * c->xd_b;
+    HIPCHK(c, hipMemcpyAsync(cp, c->zh, sizeof(float) * S * 2 * c->zh_b, hipMemcpyDeviceToDevice, c->st));
+    cp += S * 2 * c->zh_b;
+    HIPCHK(c, hipMemcpyAsync(cp, c->zd, sizeof(float) * S * 2 * c->zd_b, hipMemcpyDeviceToDevice, c->st));
+    if (c->ckpt_storage == 0) {
+        float *dst = (float *)c->ck_store + j * 2 * c->ck_S * c->fs;
+        HIPCHK(c, hipMemcpyAsync(dst, c->P0, sizeof(float) * n, hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(c, hipMemcpyAsync(dst + n, c->P1, sizeof(float) * n, hipMemcpyDeviceToDevice, c->st));
+        return FWI_HIP_OK;
+    }
+    unsigned *bits = c->ck_scale + j * 2 * c->ck_S;
+    unsigned short *dst = (unsigned short *)c->ck_store + j * 2 * c->ck_S * c->fs;
+    const int fp16 = c->ckpt_storage == 1;
+    HIPCHK(c, fwi_launch_fill_f32((float *)bits, 2 * S, 0.f, c->st));
+    HIPCHK(c, fwi_launch_absmax(c->P0, c->fs, (int)S, bits, c->st));
+    HIPCHK(c, fwi_launch_absmax(c->P1, c->fs, (int)S, bits + S, c->st));
+    HIPCHK(c, fwi_launch_pack16(c->P0, c->fs, (int)S, bits, dst, fp16, c->st));
+    HIPCHK(c, fwi_launch_pack16(c->P1, c->fs, (int)S, bits + S, dst + n, fp16, c->st));
+    return FWI_HIP_OK;
+}
+
+// back into P0/P1 and the recompute's CPML set
+static int ck_restore(fwi_hip_ctx *c, long long j)
+{
+    const long long S = c->nshots, n = S * c->fs;
+    const float *cp = c->ck_cpml + j * cpml_floats(c, c->ck_S);
+    HIPCHK(c, hipMemcpyAsync(c->xh2, cp, sizeof(float) * S * 2 * c->xh_b, hipMemcpyDeviceToDevice, c->st));
+    cp += S * 2 * c->xh_b;
+    HIPCHK(c, hipMemcpyAsync(c->xd2, cp, sizeof(float) * S * 2 * c->xd_b, hipMemcpyDeviceToDevice, c->st));
+    cp += S * 2 * c->xd_b;
+    HIPCHK(c, hipMemcpyAsync(c->zh2, cp, sizeof(float) * S * 2 * c->zh_b, hipMemcpyDeviceToDevice, c->st));
+    cp += S * 2 * c->zh_b;
+    HIPCHK(c, hipMemcpyAsync(c->zd2, cp, sizeof(float) * S * 2 * c->zd_b, hipMemcpyDeviceToDevice, c->st));
+    if (c->ckpt_storage == 0) {
+        const float *src = (const float *)c->ck_store + j * 2 * c->ck_S * c->fs;
+        HIPCHK(c, hipMemcpyAsync(c->P0, src, sizeof(float) * n, hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(c, hipMemcpyAsync(c->P1, src + n, sizeof(float) * n, hipMemcpyDeviceToDevice, c->st));
+        return FWI_HIP_OK;
+    }
+    const unsigned *bits = c->ck_scale + j * 2 * c->ck_S;
+    const unsigned short *src = (const unsigned short *)c->ck_store + j * 2 * c->ck_S * c->fs;
+    const int fp16 = c->ckpt_storage == 1;
+    HIPCHK(c, fwi_launch_unpack16(src, c->fs, (int)S, bits, c->P0, fp16, c->st));
+    HIPCHK(c, fwi_launch_unpack16(src + n, c->fs, (int)S, bits + S, c->P1, fp16, c->st));
+    return FWI_HIP_OK;
+}
+
 static int zero_cpml(fwi_hip_ctx *c)
 {
     const long long S = c->nshots;
@@ -694,6 +820,29 @@ extern "C" int fwi_hip_set_graph(fwi_hip_ctx *c, int steps)
     return FWI_HIP_OK;
 }
 
+extern "C" int fwi_hip_set_checkpointing(fwi_hip_ctx *c, int every, int storage)
+{
+    if (!c || every < 0 || storage < 0 || storage > 2)
+        return set_err(c, FWI_HIP_EINVAL, "set_checkpointing: need every >= 0 and storage 0 (FP32), 1 (FP16) or 2 (BF16)");
+    if (c->gx_fwd) (void)hipGraphExecDestroy(c->gx_fwd);
+    c->gx_fwd = nullptr;
+    drop_bwd(c);
+    if (storage != c->ckpt_storage || every != c->ckpt_every) {
+        dfree(c, c->ck_store);
+        dfree(c, c->ck_scale);
+        dfree(c, c->ck_cpml);
+        dfree(c, c->ck_xseg);
+        c->ck_store = nullptr;
+        c->ck_scale = nullptr;
+        c->ck_cpml = c->ck_xseg = nullptr;
+        c->ck_nseg = c->ck_S = c->ck_K = 0;
+    }
+    c->ckpt_every = every;
+    c->ckpt_storage = storage;
+    c->have_fwd = false;
+    return FWI_HIP_OK;
+}
+
 extern "C" int fwi_hip_set_bwd_chunk(fwi_hip_ctx *c, int shots)
 {
     if (!c || shots < 0) return set_err(c, FWI_HIP_EINVAL, "set_bwd_chunk: need shots >= 0");
@@ -708,9 +857,14 @@ extern "C" int fwi_hip_forward(fwi_hip_ctx *c, float *traces)
     if (!c->have_model || c->nshots == 0) return set_err(c, FWI_HIP_ESTATE, "forward: set_model and set_shots first");
     const FwiGeomDev &g = c->g;
     const long long S = c->nshots;
+    const bool ck = c->ckpt_every > 0;
+    if (ck && c->fwd_kind != 0) return set_err(c, FWI_HIP_EINVAL, "forward: checkpointing needs the tile kernels (set_kernels 0)");
+    int rc = ck ? ensure_ckpt(c) : ensure_frames(c);
+    if (rc) return rc;
+    const int fmode = ck ? FWI_FWD_TRACES : FWI_FWD_FRAMES;
     HIPCHK(c, fwi_launch_fill_f32(c->P0, S * c->fs, 0.f, c->st));
     HIPCHK(c, fwi_launch_fill_f32(c->P1, S * c->fs, 0.f, c->st));
-    int rc = zero_cpml(c);
+    rc = zero_cpml(c);
     if (rc) return rc;
     FwiFwdArgs a;
     memset(&a, 0, sizeof(a));
@@ -723,21 +877,29 @@ extern "C" int fwi_hip_forward(fwi_hip_ctx *c, float *traces)
     a.zh = c->zh;
     a.zd = c->zd;
     a.syn = c->syn_tm;
-    a.pstrip = c->pstrip;
-    a.xstrip = c->xstrip;
+    a.pstrip = ck ? nullptr : c->pstrip;
+    a.xstrip = ck ? nullptr : c->xstrip;
     c->fwd_args_h = a;  // host copy stays alive while the stream reads it
     HIPCHK(c, hipMemcpyAsync(c->fwd_args_d, &c->fwd_args_h, sizeof(FwiFwdArgs), hipMemcpyHostToDevice, c->st));
     const FwiFwdArgs *ad = c->fwd_args_d;
     const int nblocks = g.ntx * g.ntz * c->ngroups;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
     const int B = c->graph_steps;
-    if (B > 0 && !c->timing) {
+    if (ck) {
+        // a snapshot before every ckpt_every-th step, plain launches between
+        const int K = c->ckpt_every;
+        for (long long j = 0; j * K < c->kmax; j++) {
+            if ((rc = ck_save(c, j))) return rc;
+            for (int k = (int)(j * K); k < std::min((long long)c->kmax, (j + 1) * K); k++)
+                HIPCHK(c, fwi_launch_fwd_tile(&a, ad, c->tl_fwd, c->n_fwd, nullptr, k, pair_of(c), fmode));
+        }
+    } else if (B > 0 && !c->timing) {
         if (!c->gx_fwd) {
             rc = capture(c, &c->gx_fwd, [&]() {
                 hipError_t e = hipSuccess;
                 for (int j = 0; j < B && e == hipSuccess; j++)
                     e = c->fwd_kind ? fwi_launch_fwd(&a, c->kbase, j, nblocks, c->st)
-                                    : fwi_launch_fwd_tile(&a, ad, c->tl_fwd, c->n_fwd, c->kbase, j, pair_of(c));
+                                    : fwi_launch_fwd_tile(&a, ad, c->tl_fwd, c->n_fwd, c->kbase, j, pair_of(c), fmode);
                 if (e == hipSuccess) e = fwi_launch_advance(c->kbase, B, c->st);
                 return e;
             });
@@ -750,7 +912,7 @@ extern "C" int fwi_hip_forward(fwi_hip_ctx *c, float *traces)
             hipEvent_t e0, e1;
             if ((rc = tick(c, &e0))) return rc;
             HIPCHK(c, c->fwd_kind ? fwi_launch_fwd(&a, nullptr, k, nblocks, c->st)
-                                  : fwi_launch_fwd_tile(&a, ad, c->tl_fwd, c->n_fwd, nullptr, k, pair_of(c)));
+                                  : fwi_launch_fwd_tile(&a, ad, c->tl_fwd, c->n_fwd, nullptr, k, pair_of(c), fmode));
             if ((rc = tick(c, &e1))) return rc;
             if (c->timing) marks.push_back({0, {e0, e1}});
             if (marks.size() >= 512 && (rc = collect_timing(c, marks))) return rc;
@@ -885,6 +1047,46 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
     }
     const int nb_adj = g.ntx * g.ntz * c->ngroups;
     const int nb_rec = g.bntx * g.bntz * c->ngroups;
+    if (c->ckpt_every > 0) {
+        if (!tile) return set_err(c, FWI_HIP_EINVAL, "adjoint: checkpointing needs the tile kernels (set_kernels 0)");
+        if (!c->ck_store) return set_err(c, FWI_HIP_ESTATE, "adjoint: no snapshots (forward with checkpointing first)");
+        // per segment from the last: restore its snapshot, recompute its
+        // steps storing X, then run its adjoint steps imaging mu X
+        const long long nbb = (long long)g.nx * g.nz, K = c->ckpt_every;
+        FwiFwdArgs f;
+        memset(&f, 0, sizeof(f));
+        f.g = g;
+        f.b = a.b;
+        f.P0 = c->P0;
+        f.P1 = c->P1;
+        f.xh = c->xh2;
+        f.xd = c->xd2;
+        f.zh = c->zh2;
+        f.zd = c->zd2;
+        f.xseg = c->ck_xseg;
+        f.xseg_step = S * nbb;
+        c->fwd_rc_h = f;
+        HIPCHK(c, hipMemcpyAsync(c->fwd_rc_d, &c->fwd_rc_h, sizeof(FwiFwdArgs), hipMemcpyHostToDevice, c->st));
+        int *k0_dev = &c->fwd_rc_d->xseg_k0;  // device address of the segment's first step
+        for (long long j = (c->kmax + K - 1) / K - 1; j >= 0; j--) {
+            const int k0 = (int)(j * K), k1 = (int)std::min((long long)c->kmax, (j + 1) * K);
+            if ((rc = ck_restore(c, j))) return rc;
+            HIPCHK(c, fwi_launch_set_int(k0_dev, k0, c->st));
+            for (int k = k0; k < k1; k++)
+                HIPCHK(c, fwi_launch_fwd_tile(&f, c->fwd_rc_d, c->tl_fwd, c->n_fwd, nullptr, k, pair_of(c),
+                                              FWI_FWD_XSEG));
+            for (int k = k1 - 1; k >= k0; k--) {
+                HIPCHK(c, fwi_launch_adj_tile(&a, ad, c->tl_adj, c->n_adj, nullptr, k, pair_of(c)));
+                HIPCHK(c, fwi_launch_img_seg(&r, rd, c->ck_xseg + (long long)(k - k0) * S * nbb, k, rgs, ngr, c->st));
+            }
+        }
+        HIPCHK(c, fwi_launch_grad_finalize(c->gacc, ngr, nb, c->vel, c->gk, c->grad, c->st));
+        if (grad) HIPCHK(c, hipMemcpyAsync(grad, c->grad, sizeof(double) * nb, hipMemcpyDeviceToHost, c->st));
+        if (gk) HIPCHK(c, hipMemcpyAsync(gk, c->gk, sizeof(double) * nb, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        c->have_fwd = false;
+        return FWI_HIP_OK;
+    }
     // shot chunks, each a whole backward pass over its own slice of the
     // strip and item lists (shot-major and group-major), so that a chunk's
     // working set (mu, p and the FP64 images of its shots) can stay cached
@@ -1000,6 +1202,7 @@ extern "C" int fwi_hip_read_state(fwi_hip_ctx *c, int what, int shot, float *out
                                    g.pnz, hipMemcpyDeviceToHost, c->st));
     } else {
         const FwiShotDev &d = c->shots_h[shot];
+        if (!c->pstrip) return set_err(c, FWI_HIP_ESTATE, "read_state: no boundary frames (checkpointed forward)");
         const float *src = (what == 4 ? c->pstrip : c->xstrip) + d.strip_off;
         HIPCHK(c, hipMemcpyAsync(out, src, sizeof(float) * (size_t)d.nt * g.sl, hipMemcpyDeviceToHost, c->st));
     }

@@ -20,7 +20,9 @@
 // Arithmetic follows the reference expression by expression (built with
 // -ffp-contract=off): float stencils, the free-surface rows and the leapfrog
 // sum evaluated in double as C evaluates the reference's double literals.
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <type_traits>
 #include "fwi_dev.h"
 
@@ -628,7 +630,7 @@ __device__ __forceinline__ XCoef fwd_ux_coef(const FwiGeomDev &g, const float *t
     return c;
 }
 
-template <bool EDGE>
+template <bool EDGE, int MODE>
 __device__ __forceinline__ void fwd_tile_rows(const FwiFwdArgs &A, const FwiShotDev &sh, const float *tab,
                                               float (*Ps)[HC], float (*Us)[HC], int s, int k, int x0, int r0,
                                               int lane)
@@ -840,13 +842,20 @@ __device__ __forceinline__ void fwd_tile_rows(const FwiFwdArgs &A, const FwiShot
         if (has_src && i == sh.isz) ux = (x == sh.isx) ? ux + sv : ux;
         const float X = ux + vz;
         const float pnew = leapfrog(p[d + 3], po[d], pk[d] * X);
+        if (MODE == FWI_FWD_XSEG) {
+            // interior strips lie inside the physical grid (fwi_fwd_strip_interior)
+            const int jx = x - npml;
+            if (!EDGE || (jx >= 0 && jx < g.nx && i < g.nz))
+                A.xseg[(long long)(k - A.xseg_k0) * A.xseg_step + ((long long)s * g.nz + i) * g.nx + jx] = X;
+        }
         if (!EDGE) {
             buf_st(pnew, rold, vo, i * rowb);
             continue;
         }
         if (x < pnx) buf_st(pnew, rold, vo, i * rowb);
-        if (has_recv && rix[d] >= 0) A.syn[sh.trace_off + (long long)k * sh.nr + rix[d]] = pnew;
-        if ((frame_rows || frame_cols) && bx >= 0 && bx < nx && i < nz &&
+        if (MODE != FWI_FWD_XSEG && has_recv && rix[d] >= 0)
+            A.syn[sh.trace_off + (long long)k * sh.nr + rix[d]] = pnew;
+        if (MODE == FWI_FWD_FRAMES && (frame_rows || frame_cols) && bx >= 0 && bx < nx && i < nz &&
             (i < 3 || i >= nz - 3 || bx < 3 || bx >= nx - 3)) {
             const long long so2 = sh.strip_off + (long long)k * g.sl;
             if (i < 3 || i >= nz - 3) {
@@ -873,7 +882,7 @@ __device__ __forceinline__ void fwd_tile_rows(const FwiFwdArgs &A, const FwiShot
 // runs the interior strips, the other the general strips: each list holds the
 // logical strips (s nsz + sz) ntx + tx of its class (fwi_fwd_strip_interior),
 // four consecutive entries per workgroup, one per wave.
-template <bool EDGE>
+template <bool EDGE, int MODE>
 __global__ __launch_bounds__(256) void k_fwd_tile(const FwiFwdArgs *__restrict__ Ap,
                                                   const FwiShotDev *__restrict__ shots,
                                                   const int *__restrict__ strips, int nst, const int *kbase, int koff)
@@ -897,7 +906,7 @@ __global__ __launch_bounds__(256) void k_fwd_tile(const FwiFwdArgs *__restrict__
     const int tx = l % g.ntx, rest = l / g.ntx, sz = rest % nsz, s = rest / nsz;
     const FwiShotDev sh = shots[s];
     if (k >= sh.nt) return;
-    fwd_tile_rows<EDGE>(A, sh, stab, Psh[w], Ush[w], s, k, tx * TX, sz * FR, lane);
+    fwd_tile_rows<EDGE, MODE>(A, sh, stab, Psh[w], Ush[w], s, k, tx * TX, sz * FR, lane);
 }
 
 // ============================================================================
@@ -2290,6 +2299,96 @@ __global__ void k_ref_read_bdr_lr(const float *lr, float *pb, int nx, int nz)
 }
 
 // ============================================================================
+// checkpointed backward pass: snapshots and the image from a recomputed segment
+// ============================================================================
+// max |x| of each of nitem blocks of n floats, into bits[item] as the bit
+// pattern of a non-negative float (ordered like the value, so an unsigned
+// atomicMax finds the maximum); bits must start at 0
+__global__ void k_absmax(const float *__restrict__ x, long long n, int nitem, unsigned *bits)
+{
+    const int item = blockIdx.y;
+    if (item >= nitem) return;
+    const float *xi = x + (long long)item * n;
+    float m = 0.f;
+    for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (long long)gridDim.x * blockDim.x)
+        m = fmaxf(m, fabsf(xi[j]));
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    __shared__ float wm[16];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) wm[wv] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float r = 0.f;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r = fmaxf(r, wm[w]);
+        atomicMax(bits + item, __float_as_uint(r));
+    }
+}
+
+// BF16 with round-to-nearest-even (finite inputs: the values are scaled into [-1, 1])
+__device__ __forceinline__ unsigned short f32_to_bf16(float f)
+{
+    const unsigned u = __float_as_uint(f);
+    return (unsigned short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float bf16_to_f32(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+
+// dst[item][j] = x[item][j] / scale[item] in FP16 (fp16 != 0) or BF16
+__global__ void k_pack16(const float *__restrict__ x, long long n, int nitem, const unsigned *__restrict__ bits,
+                         unsigned short *__restrict__ dst, int fp16)
+{
+    const int item = blockIdx.y;
+    if (item >= nitem) return;
+    const float sc = __uint_as_float(bits[item]);
+    const float inv = sc > 0.f ? 1.f / sc : 0.f;
+    const float *xi = x + (long long)item * n;
+    unsigned short *di = dst + (long long)item * n;
+    for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (long long)gridDim.x * blockDim.x) {
+        const float v = xi[j] * inv;
+        di[j] = fp16 ? __half_as_ushort(__float2half_rn(v)) : f32_to_bf16(v);
+    }
+}
+
+// x[item][j] = src[item][j] * scale[item]
+__global__ void k_unpack16(const unsigned short *__restrict__ src, long long n, int nitem,
+                           const unsigned *__restrict__ bits, float *__restrict__ x, int fp16)
+{
+    const int item = blockIdx.y;
+    if (item >= nitem) return;
+    const float sc = __uint_as_float(bits[item]);
+    const unsigned short *si = src + (long long)item * n;
+    float *xi = x + (long long)item * n;
+    for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (long long)gridDim.x * blockDim.x) {
+        const unsigned short h = si[j];
+        xi[j] = (fp16 ? __half2float(__ushort_as_half(h)) : bf16_to_f32(h)) * sc;
+    }
+}
+
+// the image of step k from a recomputed segment: for each physical cell and
+// shot group, gacc[grp] += sum over the group's live shots (FP64, fixed
+// order) of mu^{k+1} X^k, mu^{k+1} = M[(k+1) & 1]
+__global__ void k_img_seg(const FwiRecArgs *__restrict__ Ap, const float *__restrict__ xk, int k, int gsize,
+                          int ngr)
+{
+    const FwiRecArgs &A = *Ap;
+    const FwiGeomDev &g = A.g;
+    const long long nb = (long long)g.nx * g.nz;
+    const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int grp = blockIdx.y;
+    if (cell >= nb || grp >= ngr) return;
+    if (!fwi_image_step(A.img_jump, A.img_k0, A.img_k1, k)) return;
+    const int i = (int)(cell / g.nx), j = (int)(cell - (long long)i * g.nx);
+    const float *mu = (k & 1) ? A.M0 : A.M1;
+    const long long fo = (long long)i * g.ld + j + g.npml;
+    double acc = 0.0;
+    const int s1 = min((grp + 1) * gsize, A.b.nshots);
+    for (int s = grp * gsize; s < s1; ++s) {
+        if (k >= A.b.shots[s].nt) continue;
+        acc += (double)mu[(long long)s * A.b.fs + fo] * (double)xk[(long long)s * nb + cell];
+    }
+    A.gacc[(long long)grp * nb + cell] += acc;
+}
+
+// ============================================================================
 // host-side launchers (C linkage inside the library)
 // ============================================================================
 // The interior and general launches of a step are independent (disjoint
@@ -2315,6 +2414,22 @@ static hipError_t launch_pair(const FwiPair &p, bool has_int, bool has_gen, LI l
     return hipGetLastError();
 }
 
+template <int MODE>
+static hipError_t launch_fwd_tile_mode(const FwiFwdArgs *a, const FwiFwdArgs *a_dev, const int *const *tl,
+                                       const int *ntl, const int *kbase, int koff, const FwiPair &p)
+{
+    return launch_pair(
+        p, ntl[0] > 0, ntl[1] > 0,
+        [&](hipStream_t st) {
+            hipLaunchKernelGGL((k_fwd_tile<false, MODE>), dim3(((ntl[0] + 31) / 32) * 8), dim3(256), 0, st, a_dev,
+                               a->b.shots, tl[0], ntl[0], kbase, koff);
+        },
+        [&](hipStream_t st) {
+            hipLaunchKernelGGL((k_fwd_tile<true, MODE>), dim3(((ntl[1] + 31) / 32) * 8), dim3(256), 0, st, a_dev,
+                               a->b.shots, tl[1], ntl[1], kbase, koff);
+        });
+}
+
 extern "C" {
 
 hipError_t fwi_launch_fwd(const FwiFwdArgs *a, const int *kbase, int koff, int nblocks, hipStream_t st)
@@ -2325,18 +2440,11 @@ hipError_t fwi_launch_fwd(const FwiFwdArgs *a, const int *kbase, int koff, int n
 
 // tile lists: [0] interior tiles, [1] general tiles (device arrays and sizes)
 hipError_t fwi_launch_fwd_tile(const FwiFwdArgs *a, const FwiFwdArgs *a_dev, const int *const *tl, const int *ntl,
-                               const int *kbase, int koff, const FwiPair &p)
+                               const int *kbase, int koff, const FwiPair &p, int mode)
 {
-    return launch_pair(
-        p, ntl[0] > 0, ntl[1] > 0,
-        [&](hipStream_t st) {
-            hipLaunchKernelGGL(k_fwd_tile<false>, dim3(((ntl[0] + 31) / 32) * 8), dim3(256), 0, st, a_dev,
-                               a->b.shots, tl[0], ntl[0], kbase, koff);
-        },
-        [&](hipStream_t st) {
-            hipLaunchKernelGGL(k_fwd_tile<true>, dim3(((ntl[1] + 31) / 32) * 8), dim3(256), 0, st, a_dev,
-                               a->b.shots, tl[1], ntl[1], kbase, koff);
-        });
+    if (mode == FWI_FWD_TRACES) return launch_fwd_tile_mode<FWI_FWD_TRACES>(a, a_dev, tl, ntl, kbase, koff, p);
+    if (mode == FWI_FWD_XSEG) return launch_fwd_tile_mode<FWI_FWD_XSEG>(a, a_dev, tl, ntl, kbase, koff, p);
+    return launch_fwd_tile_mode<FWI_FWD_FRAMES>(a, a_dev, tl, ntl, kbase, koff, p);
 }
 
 hipError_t fwi_launch_adj_tile(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, const int *const *tl, const int *ntl,
@@ -2462,6 +2570,43 @@ hipError_t fwi_launch_ref_read_bdr(const float *ud, const float *lr, float *pb, 
 {
     hipLaunchKernelGGL(k_ref_read_bdr_ud, dim3((3 * nx + 255) / 256), dim3(256), 0, st, ud, pb, nx, nz);
     hipLaunchKernelGGL(k_ref_read_bdr_lr, dim3((3 * nz + 255) / 256), dim3(256), 0, st, lr, pb, nx, nz);
+    return hipGetLastError();
+}
+
+
+hipError_t fwi_launch_absmax(const float *x, long long n, int nitem, unsigned *bits, hipStream_t st)
+{
+    if (nitem <= 0 || n <= 0) return hipSuccess;
+    const long long per = (n + 255) / 256;
+    const int bx = (int)std::min<long long>(per, 1024);
+    hipLaunchKernelGGL(k_absmax, dim3(bx, nitem), dim3(256), 0, st, x, n, nitem, bits);
+    return hipGetLastError();
+}
+
+hipError_t fwi_launch_pack16(const float *x, long long n, int nitem, const unsigned *bits, unsigned short *dst,
+                             int fp16, hipStream_t st)
+{
+    if (nitem <= 0 || n <= 0) return hipSuccess;
+    const int bx = (int)std::min<long long>((n + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_pack16, dim3(bx, nitem), dim3(256), 0, st, x, n, nitem, bits, dst, fp16);
+    return hipGetLastError();
+}
+
+hipError_t fwi_launch_unpack16(const unsigned short *src, long long n, int nitem, const unsigned *bits, float *x,
+                               int fp16, hipStream_t st)
+{
+    if (nitem <= 0 || n <= 0) return hipSuccess;
+    const int bx = (int)std::min<long long>((n + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_unpack16, dim3(bx, nitem), dim3(256), 0, st, src, n, nitem, bits, x, fp16);
+    return hipGetLastError();
+}
+
+hipError_t fwi_launch_img_seg(const FwiRecArgs *a, const FwiRecArgs *a_dev, const float *xk, int k, int gsize,
+                              int ngr, hipStream_t st)
+{
+    const long long nb = (long long)a->g.nx * a->g.nz;
+    hipLaunchKernelGGL(k_img_seg, dim3((unsigned)((nb + 255) / 256), ngr), dim3(256), 0, st, a_dev, xk, k, gsize,
+                       ngr);
     return hipGetLastError();
 }
 

@@ -117,6 +117,20 @@ int fwi_hip_set_kernels(fwi_hip_ctx *ctx, int kind);
  * (k - k0) % jump == 0.  Default: every step (jump 1, k0 0, k1 INT_MAX). */
 int fwi_hip_set_imaging(fwi_hip_ctx *ctx, int jump, int k0, int k1);
 
+/* Checkpointed backward pass instead of the boundary-saving reconstruction
+ * (every = 0, the default).  With every = K > 0 the forward pass keeps a
+ * snapshot of p (both time levels) and of the CPML memory before every K-th
+ * step, and no boundary frames; the adjoint pass restores the snapshots
+ * segment by segment from the last, recomputes the segment's K forward
+ * steps storing X on the physical grid, and runs its adjoint steps imaging
+ * mu X from that store.  storage: 0 = FP32, 1 = FP16, 2 = BF16 snapshots
+ * of p, each shot's field scaled by its maximum |p| (the CPML memory stays
+ * FP32; all arithmetic stays FP32, the image FP64).  Extra device memory:
+ * ceil(nt/K) x 2 x shots x (pnz x ld) snapshot values + K x shots x nx x nz
+ * floats of X; the boundary frames (2 x 6 (nx + nz) floats per step and
+ * shot) are not allocated.  Tile kernels only.                              */
+int fwi_hip_set_checkpointing(fwi_hip_ctx *ctx, int every, int storage);
+
 /* Shots per backward pass of the tile kernels (0, the default: all shots in
  * one pass).  With n > 0 the adjoint pass runs the backward time loop once
  * per chunk of n shots (rounded up to whole reconstruction groups), so that
