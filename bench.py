@@ -46,7 +46,8 @@ def parse():
     ap.add_argument("--kernels", type=int, default=0, help="0: 2-D tile kernels, 1: z-marching kernels")
     ap.add_argument("--rec-group", type=int, default=0, help="shots per workgroup of the tile imaging kernel")
     ap.add_argument("--bwd-chunk", type=int, default=-1, help="shots per backward pass (0: all; -1: the default)")
-    ap.add_argument("--workload", default="batched", choices=("batched", "uneven"),
+    ap.add_argument("--ckpt-every", type=int, default=100, help="snapshot interval of --workload checkpoint")
+    ap.add_argument("--workload", default="batched", choices=("batched", "uneven", "checkpoint"),
                     help="batched: the 1024x320 64-shot gradient (the headline line); uneven: 1536x400, "
                          "96 shots with nt 2000-6000, apertures 256-1536, host Butterworth band-pass")
     ap.add_argument("--autotune", action="store_true",
@@ -189,6 +190,82 @@ def run_uneven(args):
     print(json.dumps(out))
 
 
+def run_checkpoint(args):
+    """Boundary-saving reconstruction against checkpointed recomputation with
+    FP32, FP16 and BF16 snapshots on the batched workload: shot-gradients/s
+    and device bytes of each, and the gradient error against the FP64 oracle
+    on a 2-shot subset with nt <= 1500 (one fixed residual)."""
+    import numpy as np
+    from fwihip import lib
+    from fwihip.workloads import config2, triangle_smooth, Shot
+
+    g, vel, shots = config2(nshots=args.shots, nt=args.nt)
+    start = triangle_smooth(vel, 25)
+    nr = max(s.nr for s in shots)
+    K = args.ckpt_every
+    modes = [(0, "fp32"), (K, "fp32"), (K, "fp16"), (K, "bf16")]
+    ref = lib.Context(g, max_shots=len(shots), max_nt=args.nt, max_nr=nr)
+    ref.set_model(vel)
+    ref.set_shots(shots)
+    obs = ref.forward()
+    ref.close()
+    rows = []
+    for every, storage in modes:
+        ctx = lib.Context(g, max_shots=len(shots), max_nt=args.nt, max_nr=nr)
+        ctx.set_checkpointing(every, storage)
+        ctx.set_model(start)
+        ctx.set_shots(shots)
+        ctx.set_observed(obs)
+
+        def step():
+            ctx.forward(want_traces=False)
+            ctx.misfit_l2()
+            ctx.adjoint(want=False)
+
+        for _ in range(args.warmup):
+            step()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        ctx.synchronize()
+        wall = time.perf_counter() - t0
+        rows.append(dict(method="boundary-saving" if every == 0 else f"checkpoint every {every}", storage=storage,
+                         shot_gradients_per_s=len(shots) * args.steps / wall,
+                         device_gbytes=ctx.bytes() / 1e9))
+        ctx.close()
+        print(json.dumps(rows[-1]), file=sys.stderr, flush=True)
+    # gradient error against the FP64 oracle on a small subset
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle
+    nt_acc = min(args.nt, 1500)
+    sub = [Shot(nt=nt_acc, isx=s.isx, isz=s.isz, wav=s.wav[:nt_acc], irx=s.irx, irz=s.irz) for s in shots[:2]]
+    rng = np.random.default_rng(5)
+    res = [rng.standard_normal((s.nr, s.nt)).astype(np.float32) for s in sub]
+    o64 = Oracle(np.float64)
+    t0 = time.perf_counter()
+    g64 = np.zeros((g.nz, g.nx))
+    for s, r in zip(sub, res):
+        f = o64.forward(g, start.astype(np.float64), s, want_strips=True)
+        g64 += o64.backward(g, start.astype(np.float64), s, r.astype(np.float64), f)["grad"]
+    t_oracle = time.perf_counter() - t0
+    for row, (every, storage) in zip(rows, modes):
+        ctx = lib.Context(g, max_shots=2, max_nt=nt_acc, max_nr=nr)
+        ctx.set_checkpointing(min(every, nt_acc) if every else 0, storage)
+        ctx.set_model(start)
+        ctx.set_shots(sub)
+        ctx.forward(want_traces=False)
+        ctx.set_residual(ctx.pack(res))
+        grad, _ = ctx.adjoint()
+        ctx.close()
+        row["grad_rel_l2_vs_fp64"] = float(np.linalg.norm(grad - g64) / np.linalg.norm(g64))
+    out = {"metric": "checkpointed vs boundary-saving backward pass at 1024x320, nt=%d, %d shots" % (args.nt, len(shots)),
+           "unit": "shot-gradients/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "rows": rows, "accuracy_sample": f"2 shots, nt={nt_acc}, one random residual, FP64 oracle ({t_oracle:.0f} s)",
+           "dtype": "f32 arithmetic, snapshot storage per row"}
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -208,6 +285,9 @@ def main():
         return
     if args.workload == "uneven":
         run_uneven(args)
+        return
+    if args.workload == "checkpoint":
+        run_checkpoint(args)
         return
     import numpy as np
     from fwihip import lib

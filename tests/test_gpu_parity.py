@@ -189,23 +189,26 @@ def test_backward_chunks_give_the_same_image(lib):
     chunk, each chunk its own time loop over its own nt) gives the image of
     the single pass bit for bit: every group accumulator sees the same
     updates in the same order."""
-    g, vel, shots = small_case(nshots=5, nt=230, seed=13)
+    g, vel, shots = small_case(nshots=3, nt=230, seed=13)
     rng = np.random.default_rng(14)
     res = [rng.standard_normal((s.nr, s.nt)).astype(np.float32) for s in shots]
-    out = []
-    for chunk, rg in ((0, 2), (2, 2), (3, 2), (1, 1), (4, 8)):
-        ctx = lib.Context(g, max_shots=5, max_nt=230, max_nr=100)
+
+    def image(chunk, rg):
+        ctx = lib.Context(g, max_shots=3, max_nt=230, max_nr=100)
         ctx.set_rec_group(rg)
         ctx.set_bwd_chunk(chunk)
         ctx.set_model(vel)
         ctx.set_shots(shots)
         ctx.forward(want_traces=False)
         ctx.set_residual(ctx.pack(res))
-        out.append((rg, ctx.adjoint()[1]))
+        gk = ctx.adjoint()[1]
         ctx.close()
-    np.testing.assert_array_equal(out[1][1], out[0][1])
-    np.testing.assert_array_equal(out[2][1], out[0][1])
-    assert rel(out[3][1], out[0][1]) < 1e-12 and rel(out[4][1], out[0][1]) < 1e-12
+        return gk
+
+    for rg, chunks in ((1, (1, 2)), (2, (1, 2, 3))):
+        ref = image(0, rg)
+        for ch in chunks:   # rg 2: chunk 1 rounds up to 2 -> [0, 2), [2, 3); chunk 3 is one pass
+            np.testing.assert_array_equal(image(ch, rg), ref)
 
 
 @pytest.mark.parametrize("kind", [0, 1])
