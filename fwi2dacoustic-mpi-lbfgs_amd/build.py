@@ -25,7 +25,7 @@ ARCH = os.environ.get("FWI_HIP_ARCH", "gfx950")
 HIP_FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-ffp-contract=off",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wall", "-Wno-unused-variable", "-Wno-unused-function", f"-I{INC}"]
-C_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-Wall", f"-I{INC}"]
+C_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-pthread", "-Wall", f"-I{INC}"]
 
 
 def run(cmd):
@@ -47,7 +47,7 @@ def build(verbose=True):
             run(["gcc", *C_FLAGS, "-c", path, "-o", obj])
             objs.append(obj)
     tmp = LIB + ".tmp"
-    run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread"])
     os.replace(tmp, LIB)
     # Second link of the same objects against the HIP runtime that PyTorch
     # bundles, so that one process holds a single HIP runtime when the
@@ -55,7 +55,7 @@ def build(verbose=True):
     tlib = torch_hip_dir()
     if tlib:
         tmp = LIB_TORCH + ".tmp"
-        run(["g++", "-shared", "-fPIC", "-o", tmp, *objs, f"-L{tlib}", "-l:libamdhip64.so",
+        run(["g++", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread", f"-L{tlib}", "-l:libamdhip64.so",
              f"-Wl,-rpath,{tlib}", "-Wl,--no-undefined"])
         os.replace(tmp, LIB_TORCH)
     return LIB

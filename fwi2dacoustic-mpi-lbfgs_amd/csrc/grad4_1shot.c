@@ -25,6 +25,8 @@
  */
 #include <limits.h>
 #include <math.h>
+#include <pthread.h>
+#include <unistd.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -40,6 +42,7 @@ static fwi_hip_ctx *g_ctx = NULL;
 static long g_key[8];
 static char g_msg[512] = "";
 static int g_res_mode = 0;
+static int g_ck_every = 0, g_ck_storage = 0, g_ck_applied = 0; /* checkpointing of the batched context */
 
 static int fail(int code, const char *fmt, ...);
 
@@ -103,6 +106,15 @@ int grad4_set_misfit(int kind, const void *params, float offset_min, float offse
     g_cfg.offsets = offset_min <= offset_max;
     g_cfg.omin = offset_min;
     g_cfg.omax = offset_max;
+    return FWI_HIP_OK;
+}
+
+int grad4_set_checkpointing(int every, int storage)
+{
+    if (every < 0 || storage < 0 || storage > 2) return FWI_HIP_EINVAL;
+    g_ck_every = every;
+    g_ck_storage = storage;
+    g_ck_applied = 0;
     return FWI_HIP_OK;
 }
 
@@ -172,6 +184,7 @@ static int ensure_ctx(const fwi_base *b, int nshots, int nt, int nr)
     int rc = fwi_hip_create(&d, &g_ctx);
     if (rc) return fail(rc, "fwi_hip_create: %s", fwi_hip_last_error(NULL));
     memcpy(g_key, key, sizeof(key));
+    g_ck_applied = 0;
     return FWI_HIP_OK;
 }
 
@@ -329,6 +342,58 @@ done:
     return rc;
 }
 
+/* the host misfit step of every shot, shots spread over a few threads
+ * (FWI_HOST_THREADS, default: the online cores, at most 16) */
+typedef struct {
+    fwi_plan **plans;
+    float *tr;
+    const long long *off;
+    int nshots, nt, nthreads, t, rc;
+    float dt;
+} MisfitJob;
+
+static void *misfit_worker(void *p)
+{
+    MisfitJob *j = (MisfitJob *)p;
+    for (int s = j->t; s < j->nshots && !j->rc; s += j->nthreads)
+        j->rc = host_misfit(j->plans[s], j->tr + j->off[s], j->nt, j->dt);
+    return NULL;
+}
+
+static int host_misfit_all(fwi_plan **plans, int nshots, float *tr, int nt, float dt)
+{
+    long long *off = (long long *)malloc(sizeof(long long) * (nshots + 1));
+    if (!off) return fail(FWI_HIP_ENOMEM, "host memory");
+    off[0] = 0;
+    for (int s = 0; s < nshots; s++) off[s + 1] = off[s] + (long long)plans[s]->msf->nr * nt;
+    int nth = 16;
+    const char *env = getenv("FWI_HOST_THREADS");
+    if (env && atoi(env) > 0) nth = atoi(env);
+    else {
+        const long cpu = sysconf(_SC_NPROCESSORS_ONLN);
+        if (cpu > 0 && cpu < nth) nth = (int)cpu;
+    }
+    if (nth > nshots) nth = nshots;
+    if (nth < 1) nth = 1;
+    if (nth > 64) nth = 64;
+    MisfitJob jobs[64];
+    pthread_t th[64];
+    int made[64];
+    int rc = FWI_HIP_OK;
+    for (int t = 0; t < nth; t++) {
+        jobs[t] = (MisfitJob){plans, tr, off, nshots, nt, nth, t, FWI_HIP_OK, dt};
+        made[t] = t > 0 && pthread_create(&th[t], NULL, misfit_worker, &jobs[t]) == 0;
+    }
+    for (int t = 0; t < nth; t++) /* the calling thread takes share 0 and any share without a thread */
+        if (!made[t]) misfit_worker(&jobs[t]);
+    for (int t = 1; t < nth; t++)
+        if (made[t]) pthread_join(th[t], NULL);
+    for (int t = 0; t < nth; t++)
+        if (jobs[t].rc && !rc) rc = jobs[t].rc;
+    free(off);
+    return rc;
+}
+
 int grad_all_shots(fwi_plan **plans, int nshots, int k)
 {
     (void)k;
@@ -347,6 +412,11 @@ int grad_all_shots(fwi_plan **plans, int nshots, int k)
     }
     int rc = ensure_ctx(b, nshots, nt, nrmax);
     if (rc) return rc;
+    if (!g_ck_applied) {
+        rc = fwi_hip_set_checkpointing(g_ctx, g_ck_every, g_ck_storage);
+        if (rc) return fail(rc, "set_checkpointing: %s", fwi_hip_last_error(g_ctx));
+        g_ck_applied = 1;
+    }
     rc = fwi_hip_set_model(g_ctx, &plans[0]->model->md[0][0]);
     if (rc) return fail(rc, "set_model: %s", fwi_hip_last_error(g_ctx));
     {
@@ -389,12 +459,8 @@ int grad_all_shots(fwi_plan **plans, int nshots, int k)
         rc = fwi_hip_forward(g_ctx, tr);
         if (rc) rc = fail(rc, "forward: %s", fwi_hip_last_error(g_ctx));
     }
+    if (!rc) rc = host_misfit_all(plans, nshots, tr, nt, b->dt);
     if (!rc) {
-        long long o = 0;
-        for (int s = 0; s < nshots && !rc; s++) {
-            rc = host_misfit(plans[s], tr + o, nt, b->dt);
-            o += (long long)plans[s]->msf->nr * nt;
-        }
         if (!rc) {
             rc = fwi_hip_set_residual(g_ctx, tr);
             if (rc) rc = fail(rc, "set_residual: %s", fwi_hip_last_error(g_ctx));

@@ -167,6 +167,12 @@ int grad4_set_misfit(int kind, const void *params, float offset_min, float offse
  * residual.  on = 0 switches it off (the default). */
 int grad4_set_bandpass(int on, int phase, int nphi, int nplo, float fhi, float flo);
 
+/* Checkpointed backward pass of the batched context (fwi_hip_set_checkpointing):
+ * every = 0 (default) boundary-saving reconstruction; storage 0/1/2 =
+ * FP32/FP16/BF16 snapshots.  The host misfit step runs the shots over
+ * FWI_HOST_THREADS threads (default: the online cores, at most 16). */
+int grad4_set_checkpointing(int every, int storage);
+
 /* First-break mute (fbt_muter, mute_fun.c:5-23) with msf->fbt, on the
  * observed and synthetic traces and on the residual.  on = 0: off. */
 int grad4_set_fbt_mute(int on, float alpha, float tune, int early_gone);
