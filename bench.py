@@ -238,7 +238,7 @@ def run_checkpoint(args):
         wall = time.perf_counter() - t0
         rows.append(dict(method="boundary-saving" if every == 0 else f"checkpoint every {every}", storage=storage,
                          shot_gradients_per_s=len(shots) * args.steps / wall,
-                         device_gbytes=ctx.bytes() / 1e9))
+                         device_gbytes=ctx.bytes / 1e9))
         ctx.close()
         print(json.dumps(rows[-1]), file=sys.stderr, flush=True)
     # gradient error against the FP64 oracle on a small subset
