@@ -22,11 +22,11 @@ extern "C" {
 hipError_t fwi_launch_fwd(const FwiFwdArgs *a, const int *kbase, int koff, int nblocks, hipStream_t st);
 hipError_t fwi_launch_fwd_tile(const FwiFwdArgs *a, const FwiFwdArgs *a_dev, const int *const *tl, const int *ntl,
                                const int *kbase, int koff, const FwiPair &p, int mode);
-hipError_t fwi_launch_absmax(const float *x, long long n, int nitem, unsigned *bits, hipStream_t st);
-hipError_t fwi_launch_pack16(const float *x, long long n, int nitem, const unsigned *bits, unsigned short *dst,
-                             int fp16, hipStream_t st);
-hipError_t fwi_launch_unpack16(const unsigned short *src, long long n, int nitem, const unsigned *bits, float *x,
-                               int fp16, hipStream_t st);
+hipError_t fwi_launch_absmax(const float *x, const float *y, long long n, int nitem, unsigned *bits, hipStream_t st);
+hipError_t fwi_launch_pack16(const float *x, const float *y, long long n, int nitem, const unsigned *bits,
+                             unsigned short *dst, int fp16, hipStream_t st);
+hipError_t fwi_launch_unpack16(const unsigned short *src, long long n, int nitem, const unsigned *bits,
+                               const float *base, float *x, int fp16, hipStream_t st);
 hipError_t fwi_launch_img_seg(const FwiRecArgs *a, const FwiRecArgs *a_dev, const float *xk, int k, int gsize,
                               int ngr, hipStream_t st);
 hipError_t fwi_launch_adj(const FwiAdjArgs *a, const int *kbase, int koff, int nblocks, hipStream_t st);
@@ -684,14 +684,21 @@ static int ck_save(fwi_hip_ctx *c, long long j)
         HIPCHK(c, hipMemcpyAsync(dst + n, c->P1, sizeof(float) * n, hipMemcpyDeviceToDevice, c->st));
         return FWI_HIP_OK;
     }
+    // 16 bit: p^k and the step difference p^k - p^{k-1}, each with its own
+    // per-shot scale.  Rounding the two time levels independently would leave
+    // their difference (the time derivative, a few per cent of p at a few
+    // tens of steps per period) with the rounding of p itself: tens of times
+    // the storage precision after the restart.
+    const long long k = j * c->ckpt_every;
+    float *pk = (k & 1) ? c->P1 : c->P0, *pm = (k & 1) ? c->P0 : c->P1;  // p^k, p^{k-1}
     unsigned *bits = c->ck_scale + j * 2 * c->ck_S;
     unsigned short *dst = (unsigned short *)c->ck_store + j * 2 * c->ck_S * c->fs;
     const int fp16 = c->ckpt_storage == 1;
     HIPCHK(c, fwi_launch_fill_f32((float *)bits, 2 * S, 0.f, c->st));
-    HIPCHK(c, fwi_launch_absmax(c->P0, c->fs, (int)S, bits, c->st));
-    HIPCHK(c, fwi_launch_absmax(c->P1, c->fs, (int)S, bits + S, c->st));
-    HIPCHK(c, fwi_launch_pack16(c->P0, c->fs, (int)S, bits, dst, fp16, c->st));
-    HIPCHK(c, fwi_launch_pack16(c->P1, c->fs, (int)S, bits + S, dst + n, fp16, c->st));
+    HIPCHK(c, fwi_launch_absmax(pk, nullptr, c->fs, (int)S, bits, c->st));
+    HIPCHK(c, fwi_launch_absmax(pk, pm, c->fs, (int)S, bits + S, c->st));
+    HIPCHK(c, fwi_launch_pack16(pk, nullptr, c->fs, (int)S, bits, dst, fp16, c->st));
+    HIPCHK(c, fwi_launch_pack16(pk, pm, c->fs, (int)S, bits + S, dst + n, fp16, c->st));
     return FWI_HIP_OK;
 }
 
@@ -713,11 +720,13 @@ static int ck_restore(fwi_hip_ctx *c, long long j)
         HIPCHK(c, hipMemcpyAsync(c->P1, src + n, sizeof(float) * n, hipMemcpyDeviceToDevice, c->st));
         return FWI_HIP_OK;
     }
+    const long long k = j * c->ckpt_every;
+    float *pk = (k & 1) ? c->P1 : c->P0, *pm = (k & 1) ? c->P0 : c->P1;
     const unsigned *bits = c->ck_scale + j * 2 * c->ck_S;
     const unsigned short *src = (const unsigned short *)c->ck_store + j * 2 * c->ck_S * c->fs;
     const int fp16 = c->ckpt_storage == 1;
-    HIPCHK(c, fwi_launch_unpack16(src, c->fs, (int)S, bits, c->P0, fp16, c->st));
-    HIPCHK(c, fwi_launch_unpack16(src + n, c->fs, (int)S, bits + S, c->P1, fp16, c->st));
+    HIPCHK(c, fwi_launch_unpack16(src, c->fs, (int)S, bits, nullptr, pk, fp16, c->st));
+    HIPCHK(c, fwi_launch_unpack16(src + n, c->fs, (int)S, bits + S, pk, pm, fp16, c->st));  // p^{k-1} = p^k - d
     return FWI_HIP_OK;
 }
 

@@ -2301,17 +2301,19 @@ __global__ void k_ref_read_bdr_lr(const float *lr, float *pb, int nx, int nz)
 // ============================================================================
 // checkpointed backward pass: snapshots and the image from a recomputed segment
 // ============================================================================
-// max |x| of each of nitem blocks of n floats, into bits[item] as the bit
-// pattern of a non-negative float (ordered like the value, so an unsigned
-// atomicMax finds the maximum); bits must start at 0
-__global__ void k_absmax(const float *__restrict__ x, long long n, int nitem, unsigned *bits)
+// max |x - y| (y == nullptr: max |x|) of each of nitem blocks of n floats,
+// into bits[item] as the bit pattern of a non-negative float (ordered like
+// the value, so an unsigned atomicMax finds the maximum); bits start at 0
+__global__ void k_absmax(const float *__restrict__ x, const float *__restrict__ y, long long n, int nitem,
+                         unsigned *bits)
 {
     const int item = blockIdx.y;
     if (item >= nitem) return;
     const float *xi = x + (long long)item * n;
+    const float *yi = y ? y + (long long)item * n : nullptr;
     float m = 0.f;
     for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (long long)gridDim.x * blockDim.x)
-        m = fmaxf(m, fabsf(xi[j]));
+        m = fmaxf(m, fabsf(yi ? xi[j] - yi[j] : xi[j]));
     for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
     __shared__ float wm[16];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -2332,34 +2334,39 @@ __device__ __forceinline__ unsigned short f32_to_bf16(float f)
 }
 __device__ __forceinline__ float bf16_to_f32(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
 
-// dst[item][j] = x[item][j] / scale[item] in FP16 (fp16 != 0) or BF16
-__global__ void k_pack16(const float *__restrict__ x, long long n, int nitem, const unsigned *__restrict__ bits,
-                         unsigned short *__restrict__ dst, int fp16)
+// dst[item][j] = (x - y)[item][j] / scale[item] (y == nullptr: x / scale) in
+// FP16 (fp16 != 0) or BF16
+__global__ void k_pack16(const float *__restrict__ x, const float *__restrict__ y, long long n, int nitem,
+                         const unsigned *__restrict__ bits, unsigned short *__restrict__ dst, int fp16)
 {
     const int item = blockIdx.y;
     if (item >= nitem) return;
     const float sc = __uint_as_float(bits[item]);
     const float inv = sc > 0.f ? 1.f / sc : 0.f;
     const float *xi = x + (long long)item * n;
+    const float *yi = y ? y + (long long)item * n : nullptr;
     unsigned short *di = dst + (long long)item * n;
     for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (long long)gridDim.x * blockDim.x) {
-        const float v = xi[j] * inv;
+        const float v = (yi ? xi[j] - yi[j] : xi[j]) * inv;
         di[j] = fp16 ? __half_as_ushort(__float2half_rn(v)) : f32_to_bf16(v);
     }
 }
 
-// x[item][j] = src[item][j] * scale[item]
+// x[item][j] = src[item][j] * scale[item], or base - that when base != nullptr
 __global__ void k_unpack16(const unsigned short *__restrict__ src, long long n, int nitem,
-                           const unsigned *__restrict__ bits, float *__restrict__ x, int fp16)
+                           const unsigned *__restrict__ bits, const float *__restrict__ base, float *__restrict__ x,
+                           int fp16)
 {
     const int item = blockIdx.y;
     if (item >= nitem) return;
     const float sc = __uint_as_float(bits[item]);
     const unsigned short *si = src + (long long)item * n;
+    const float *bi = base ? base + (long long)item * n : nullptr;
     float *xi = x + (long long)item * n;
     for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (long long)gridDim.x * blockDim.x) {
         const unsigned short h = si[j];
-        xi[j] = (fp16 ? __half2float(__ushort_as_half(h)) : bf16_to_f32(h)) * sc;
+        const float v = (fp16 ? __half2float(__ushort_as_half(h)) : bf16_to_f32(h)) * sc;
+        xi[j] = bi ? bi[j] - v : v;
     }
 }
 
@@ -2574,30 +2581,30 @@ hipError_t fwi_launch_ref_read_bdr(const float *ud, const float *lr, float *pb, 
 }
 
 
-hipError_t fwi_launch_absmax(const float *x, long long n, int nitem, unsigned *bits, hipStream_t st)
+hipError_t fwi_launch_absmax(const float *x, const float *y, long long n, int nitem, unsigned *bits, hipStream_t st)
 {
     if (nitem <= 0 || n <= 0) return hipSuccess;
     const long long per = (n + 255) / 256;
     const int bx = (int)std::min<long long>(per, 1024);
-    hipLaunchKernelGGL(k_absmax, dim3(bx, nitem), dim3(256), 0, st, x, n, nitem, bits);
+    hipLaunchKernelGGL(k_absmax, dim3(bx, nitem), dim3(256), 0, st, x, y, n, nitem, bits);
     return hipGetLastError();
 }
 
-hipError_t fwi_launch_pack16(const float *x, long long n, int nitem, const unsigned *bits, unsigned short *dst,
-                             int fp16, hipStream_t st)
+hipError_t fwi_launch_pack16(const float *x, const float *y, long long n, int nitem, const unsigned *bits,
+                             unsigned short *dst, int fp16, hipStream_t st)
 {
     if (nitem <= 0 || n <= 0) return hipSuccess;
     const int bx = (int)std::min<long long>((n + 255) / 256, 1024);
-    hipLaunchKernelGGL(k_pack16, dim3(bx, nitem), dim3(256), 0, st, x, n, nitem, bits, dst, fp16);
+    hipLaunchKernelGGL(k_pack16, dim3(bx, nitem), dim3(256), 0, st, x, y, n, nitem, bits, dst, fp16);
     return hipGetLastError();
 }
 
-hipError_t fwi_launch_unpack16(const unsigned short *src, long long n, int nitem, const unsigned *bits, float *x,
-                               int fp16, hipStream_t st)
+hipError_t fwi_launch_unpack16(const unsigned short *src, long long n, int nitem, const unsigned *bits,
+                               const float *base, float *x, int fp16, hipStream_t st)
 {
     if (nitem <= 0 || n <= 0) return hipSuccess;
     const int bx = (int)std::min<long long>((n + 255) / 256, 1024);
-    hipLaunchKernelGGL(k_unpack16, dim3(bx, nitem), dim3(256), 0, st, src, n, nitem, bits, x, fp16);
+    hipLaunchKernelGGL(k_unpack16, dim3(bx, nitem), dim3(256), 0, st, src, n, nitem, bits, base, x, fp16);
     return hipGetLastError();
 }
 

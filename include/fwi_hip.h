@@ -123,9 +123,10 @@ int fwi_hip_set_imaging(fwi_hip_ctx *ctx, int jump, int k0, int k1);
  * step, and no boundary frames; the adjoint pass restores the snapshots
  * segment by segment from the last, recomputes the segment's K forward
  * steps storing X on the physical grid, and runs its adjoint steps imaging
- * mu X from that store.  storage: 0 = FP32, 1 = FP16, 2 = BF16 snapshots
- * of p, each shot's field scaled by its maximum |p| (the CPML memory stays
- * FP32; all arithmetic stays FP32, the image FP64).  Extra device memory:
+ * mu X from that store.  storage: 0 = FP32, 1 = FP16, 2 = BF16 snapshots:
+ * in 16 bit, p^k and the step difference p^k - p^{k-1}, each shot's field
+ * scaled by its maximum magnitude (the CPML memory stays FP32; all
+ * arithmetic stays FP32, the image FP64).  Extra device memory:
  * ceil(nt/K) x 2 x shots x (pnz x ld) snapshot values + K x shots x nx x nz
  * floats of X; the boundary frames (2 x 6 (nx + nz) floats per step and
  * shot) are not allocated.  Tile kernels only.                              */
