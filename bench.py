@@ -46,6 +46,7 @@ def parse():
     ap.add_argument("--kernels", type=int, default=0, help="0: 2-D tile kernels, 1: z-marching kernels")
     ap.add_argument("--rec-group", type=int, default=0, help="shots per workgroup of the tile imaging kernel")
     ap.add_argument("--bwd-chunk", type=int, default=-1, help="shots per backward pass (0: all; -1: the default)")
+    ap.add_argument("--bwd-overlap", type=int, default=-1, help="adjoint beside imaging on two streams (1/0; -1: default)")
     ap.add_argument("--ckpt-every", type=int, default=100, help="snapshot interval of --workload checkpoint")
     ap.add_argument("--misfit", default="ncc", choices=("l2", "ncc", "eot"), help="misfit of --workload inversion")
     ap.add_argument("--iters", type=int, default=20, help="L-BFGS iterations of --workload inversion")
@@ -417,6 +418,8 @@ def main():
             ctx.set_rec_group(args.rec_group)
         if args.bwd_chunk >= 0:
             ctx.set_bwd_chunk(args.bwd_chunk)
+        if args.bwd_overlap >= 0:
+            ctx.set_bwd_overlap(args.bwd_overlap)
     ctx.set_model(vel)
     ctx.set_shots(shots)
     obs = ctx.forward()

@@ -68,6 +68,11 @@ struct fwi_hip_ctx {
     hipStream_t st2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     bool concurrent = false;
+    // backward pass: adjoint step k on st beside imaging step k+1 on st2
+    // (independent: the adjoint writes only the mu buffer the imaging step
+    // does not read, and never touches p or the images)
+    bool bwd_overlap = false;
+    hipEvent_t ev_fork2 = nullptr, ev_join2 = nullptr;
     char err[512] = "";
     size_t bytes = 0;
     std::vector<void *> allocs;
@@ -276,6 +281,8 @@ extern "C" void fwi_hip_destroy(fwi_hip_ctx *c)
     if (c->st2) (void)hipStreamDestroy(c->st2);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->ev_fork2) (void)hipEventDestroy(c->ev_fork2);
+    if (c->ev_join2) (void)hipEventDestroy(c->ev_join2);
     delete c;
 }
 
@@ -338,6 +345,8 @@ extern "C" int fwi_hip_create(const fwi_hip_desc *desc, fwi_hip_ctx **out)
         hipError_t e = hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork2, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join2, hipEventDisableTiming);
         if (e != hipSuccess) {
             rc = set_err(c, FWI_HIP_EHIP, "second stream: %s", hipGetErrorString(e));
             fwi_hip_destroy(c);
@@ -852,6 +861,14 @@ extern "C" int fwi_hip_set_checkpointing(fwi_hip_ctx *c, int every, int storage)
     return FWI_HIP_OK;
 }
 
+extern "C" int fwi_hip_set_bwd_overlap(fwi_hip_ctx *c, int on)
+{
+    if (!c) return set_err(c, FWI_HIP_EINVAL, "set_bwd_overlap: NULL context");
+    drop_bwd(c);
+    c->bwd_overlap = on != 0;
+    return FWI_HIP_OK;
+}
+
 extern "C" int fwi_hip_set_bwd_chunk(fwi_hip_ctx *c, int shots)
 {
     if (!c || shots < 0) return set_err(c, FWI_HIP_EINVAL, "set_bwd_chunk: need shots >= 0");
@@ -1131,13 +1148,31 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
             return tile ? fwi_launch_rec_tile(&r, rd, tr, nr, kb, koff, rgs, pair_of(c))
                         : fwi_launch_rec(&r, kb, koff, nb_rec, c->st);
         };
+        // overlapped form: adjoint k on st, imaging k+1 on st2, joined per step;
+        // k runs down to -1 so that the last pair images step 0 (kernels
+        // return for k < 0 and for k >= a shot's nt)
+        const bool ovl = tile && c->bwd_overlap && !c->timing;
+        auto pair_step = [&](const int *kb, int koff) {
+            FwiPair p1 = {c->st, nullptr, c->ev_fork, c->ev_join}, p2 = {c->st2, nullptr, c->ev_fork, c->ev_join};
+            hipError_t e = hipEventRecord(c->ev_fork2, c->st);
+            if (e == hipSuccess) e = hipStreamWaitEvent(c->st2, c->ev_fork2, 0);
+            if (e == hipSuccess) e = fwi_launch_rec_tile(&r, rd, tr, nr, kb, koff + 1, rgs, p2);
+            if (e == hipSuccess) e = fwi_launch_adj_tile(&a, ad, ta, na, kb, koff, p1);
+            if (e == hipSuccess) e = hipEventRecord(c->ev_join2, c->st2);
+            if (e == hipSuccess) e = hipStreamWaitEvent(c->st, c->ev_join2, 0);
+            return e;
+        };
         if (B > 0 && !c->timing) {
             if (!c->gx_bwd[ch]) {
                 rc = capture(c, &c->gx_bwd[ch], [&]() {
                     hipError_t e = hipSuccess;
                     for (int j = 0; j < B && e == hipSuccess; j++) {
-                        e = launch_adj(c->kbase, -j);
-                        if (e == hipSuccess) e = launch_rec(c->kbase, -j);
+                        if (ovl) {
+                            e = pair_step(c->kbase, -j);
+                        } else {
+                            e = launch_adj(c->kbase, -j);
+                            if (e == hipSuccess) e = launch_rec(c->kbase, -j);
+                        }
                     }
                     if (e == hipSuccess) e = fwi_launch_advance(c->kbase, -B, c->st);
                     return e;
@@ -1145,7 +1180,10 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
                 if (rc) return rc;
             }
             HIPCHK(c, fwi_launch_set_int(c->kbase, kmax - 1, c->st));
-            for (int blk = 0; blk < (kmax + B - 1) / B; blk++) HIPCHK(c, hipGraphLaunch(c->gx_bwd[ch], c->st));
+            const int nblk = ovl ? (kmax + B) / B : (kmax + B - 1) / B;
+            for (int blk = 0; blk < nblk; blk++) HIPCHK(c, hipGraphLaunch(c->gx_bwd[ch], c->st));
+        } else if (ovl) {
+            for (int k = kmax - 1; k >= -1; k--) HIPCHK(c, pair_step(nullptr, k));
         } else {
             for (int k = kmax - 1; k >= 0; k--) {
                 hipEvent_t e0, e1, e2;

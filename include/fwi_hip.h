@@ -132,6 +132,13 @@ int fwi_hip_set_imaging(fwi_hip_ctx *ctx, int jump, int k0, int k1);
  * shot) are not allocated.  Tile kernels only.                              */
 int fwi_hip_set_checkpointing(fwi_hip_ctx *ctx, int every, int storage);
 
+/* Backward pass of the tile kernels with the adjoint step k and the
+ * reconstruction + imaging step k+1 launched side by side on two streams
+ * (they touch disjoint data: the adjoint writes only the mu buffer the
+ * imaging step does not read).  The image is bit-identical.  on = 0: one
+ * stream, adjoint then imaging per step.                                    */
+int fwi_hip_set_bwd_overlap(fwi_hip_ctx *ctx, int on);
+
 /* Shots per backward pass of the tile kernels (0, the default: all shots in
  * one pass).  With n > 0 the adjoint pass runs the backward time loop once
  * per chunk of n shots (rounded up to whole reconstruction groups), so that
