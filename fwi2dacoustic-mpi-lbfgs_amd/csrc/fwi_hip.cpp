@@ -75,7 +75,7 @@ struct fwi_hip_ctx {
     hipEvent_t ev_fork2 = nullptr, ev_join2 = nullptr;
     char err[512] = "";
     size_t bytes = 0;
-    std::vector<void *> allocs;
+    std::vector<std::pair<void *, size_t>> allocs;  // live allocations and their sizes
 
     // model
     float *ekp = nullptr, *vel = nullptr;
@@ -175,15 +175,20 @@ static int dalloc(fwi_hip_ctx *c, void **p, size_t bytes)
         return set_err(c, FWI_HIP_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
     }
     c->bytes += bytes;
-    c->allocs.push_back(*p);
+    c->allocs.push_back({*p, bytes});
     return FWI_HIP_OK;
 }
 
 static void dfree(fwi_hip_ctx *c, void *p)
 {
     if (!p) return;
-    auto it = std::find(c->allocs.begin(), c->allocs.end(), p);
-    if (it != c->allocs.end()) c->allocs.erase(it);
+    auto it = std::find_if(c->allocs.begin(), c->allocs.end(), [p](const std::pair<void *, size_t> &a) {
+        return a.first == p;
+    });
+    if (it != c->allocs.end()) {
+        c->bytes -= it->second;
+        c->allocs.erase(it);
+    }
     (void)hipFree(p);
 }
 
@@ -275,7 +280,7 @@ extern "C" void fwi_hip_destroy(fwi_hip_ctx *c)
     if (c->st) (void)hipStreamSynchronize(c->st);
     if (c->gx_fwd) (void)hipGraphExecDestroy(c->gx_fwd);
     drop_bwd(c);
-    for (void *p : c->allocs) (void)hipFree(p);
+    for (auto &a : c->allocs) (void)hipFree(a.first);
     for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
     if (c->own_stream && c->st) (void)hipStreamDestroy(c->st);
     if (c->st2) (void)hipStreamDestroy(c->st2);
