@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--rec-group", type=int, default=0, help="shots per workgroup of the tile imaging kernel")
     ap.add_argument("--bwd-chunk", type=int, default=-1, help="shots per backward pass (0: all; -1: the default)")
     ap.add_argument("--bwd-overlap", type=int, default=-1, help="adjoint beside imaging on two streams (1/0; -1: default)")
+    ap.add_argument("--bwd-fused", type=int, default=-1,
+                    help="one fused adjoint + reconstruction + imaging kernel per backward step (1/0; -1: default 1)")
     ap.add_argument("--ckpt-every", type=int, default=100, help="snapshot interval of --workload checkpoint")
     ap.add_argument("--misfit", default="ncc", choices=("l2", "ncc", "eot"), help="misfit of --workload inversion")
     ap.add_argument("--iters", type=int, default=20, help="L-BFGS iterations of --workload inversion")
@@ -420,6 +422,8 @@ def main():
             ctx.set_bwd_chunk(args.bwd_chunk)
         if args.bwd_overlap >= 0:
             ctx.set_bwd_overlap(args.bwd_overlap)
+        if args.bwd_fused >= 0:
+            ctx.set_bwd_fused(args.bwd_fused)
     ctx.set_model(vel)
     ctx.set_shots(shots)
     obs = ctx.forward()
@@ -453,8 +457,16 @@ def main():
     kind = launch["kernels"] if launch else args.kernels
     names = (("k_fwd_tile", "k_adj_tile", "k_rec_tile") if kind == 0
              else ("k_fwd_step", "k_adj_step", "k_rec_step"))
-    alg = algorithmic_bytes(g, nshots)
-    mint = min_traffic_bytes(g, nshots, nr)
+    alg = list(algorithmic_bytes(g, nshots))
+    mint = list(min_traffic_bytes(g, nshots, nr))
+    fused = kind == 0 and ctx.kernel_time(2)[1] == 0
+    if fused:
+        # one fused backward kernel: both wavefields' cell-updates credited; its
+        # minimum traffic skips the re-read of mu^{k+1} and the second ekp read
+        b = g.nx * g.nz
+        names = ("k_fwd_tile", "k_bwd_tile")
+        alg = [alg[0], alg[1] + alg[2]]
+        mint = [mint[0], mint[1] + mint[2] - 4 * b * nshots - 4 * b]
     for w, name in enumerate(names):
         ms, n = ctx.kernel_time(w)
         avg = ms / max(n, 1) * 1e-3

@@ -35,6 +35,9 @@ hipError_t fwi_launch_adj_tile(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, con
 hipError_t fwi_launch_rec_tile(const FwiRecArgs *a, const FwiRecArgs *a_dev, const int *const *tl, const int *ntl,
                                const int *kbase, int koff, int gsize, const FwiPair &p);
 hipError_t fwi_launch_rec(const FwiRecArgs *a, const int *kbase, int koff, int nblocks, hipStream_t st);
+hipError_t fwi_launch_bwd_tile(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, const FwiRecArgs *r_dev,
+                               const int *const *tl, const int *ntl, const int *kbase, int koff, int gsize,
+                               hipStream_t st);
 hipError_t fwi_launch_advance(int *kbase, int delta, hipStream_t st);
 hipError_t fwi_launch_set_int(int *p, int v, hipStream_t st);
 hipError_t fwi_launch_fill_f32(float *p, long long n, float v, hipStream_t st);
@@ -131,6 +134,15 @@ struct fwi_hip_ctx {
     int *tl_fwd[2] = {nullptr, nullptr}, *tl_adj[2] = {nullptr, nullptr}, *tl_rec[2] = {nullptr, nullptr};
     int n_fwd[2] = {0, 0}, n_adj[2] = {0, 0}, n_rec[2] = {0, 0};
     int rec_lists_for = -1;  // rec_group the reconstruction lists were built for (-1: none)
+    // fused backward step (tile kernels) on the padded grid's tiles: [0] fused
+    // interior items (group, tile), group-major; [1] / [2] interior / general
+    // adjoint strips (shot, tile) of the other tiles, shot-major; [3] their
+    // reconstruction items (group, tile) with physical rows, group-major
+    bool bwd_fused = true;
+    int *tl_bwd[4] = {nullptr, nullptr, nullptr, nullptr};
+    int n_bwd[4] = {0, 0, 0, 0};
+    int bwd_lists_for = -1;  // rec_group the fused lists were built for (-1: none)
+    std::vector<int> bwd_pre[4];  // list entries before group g ([0], [3]) or shot s ([1], [2])
     hipGraphExec_t gx_fwd = nullptr;
     std::vector<hipGraphExec_t> gx_bwd; // one captured block of backward steps per shot chunk
     int bwd_chunk = 0;                  // shots per backward pass (0: all in one)
@@ -398,6 +410,10 @@ extern "C" int fwi_hip_create(const fwi_hip_desc *desc, fwi_hip_ctx **out)
         {(void **)&c->tl_adj[1], sizeof(int) * S * g.ntx * ((g.pnz + FWI_STRIP - 1) / FWI_STRIP)},
         {(void **)&c->tl_rec[0], sizeof(int) * S * g.bntx * ((g.nz + FWI_REC_ROWS - 1) / FWI_REC_ROWS)},
         {(void **)&c->tl_rec[1], sizeof(int) * S * g.bntx * ((g.nz + FWI_REC_ROWS - 1) / FWI_REC_ROWS)},
+        {(void **)&c->tl_bwd[0], sizeof(int) * S * g.ntx * ((g.pnz + FWI_STRIP - 1) / FWI_STRIP)},
+        {(void **)&c->tl_bwd[1], sizeof(int) * S * g.ntx * ((g.pnz + FWI_STRIP - 1) / FWI_STRIP)},
+        {(void **)&c->tl_bwd[2], sizeof(int) * S * g.ntx * ((g.pnz + FWI_STRIP - 1) / FWI_STRIP)},
+        {(void **)&c->tl_bwd[3], sizeof(int) * S * g.ntx * ((g.pnz + FWI_STRIP - 1) / FWI_STRIP)},
         {(void **)&c->adj_args_d, sizeof(FwiAdjArgs)},
         {(void **)&c->rec_args_d, sizeof(FwiRecArgs)},
         {(void **)&c->fwd_rc_d, sizeof(FwiFwdArgs)},
@@ -578,6 +594,7 @@ extern "C" int fwi_hip_set_shots(fwi_hip_ctx *c, int nshots, const fwi_hip_shot 
                                          c->st));
         }
         c->rec_lists_for = -1;
+        c->bwd_lists_for = -1;
     }
     HIPCHK(c, hipStreamSynchronize(c->st));
     if (c->gx_fwd) (void)hipGraphExecDestroy(c->gx_fwd);
@@ -819,6 +836,15 @@ extern "C" int fwi_hip_set_rec_group(fwi_hip_ctx *c, int shots)
     drop_bwd(c);
     c->rec_group = shots;
     c->rec_lists_for = -1;
+    c->bwd_lists_for = -1;
+    return FWI_HIP_OK;
+}
+
+extern "C" int fwi_hip_set_bwd_fused(fwi_hip_ctx *c, int on)
+{
+    if (!c) return set_err(c, FWI_HIP_EINVAL, "set_bwd_fused: NULL context");
+    drop_bwd(c);
+    c->bwd_fused = on != 0;
     return FWI_HIP_OK;
 }
 
@@ -1076,6 +1102,52 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
         HIPCHK(c, hipStreamSynchronize(c->st));
         c->rec_lists_for = rgs;
     }
+    const bool fused = tile && c->bwd_fused && c->ckpt_every == 0;
+    if (fused && c->bwd_lists_for != rgs) {
+        // a tile of the padded grid is fused for a shot group when it is
+        // interior for the adjoint and for the reconstruction of every shot of
+        // the group; the other tiles run as adjoint strips + reconstruction items
+        const int nsz = (g.pnz + FWI_STRIP - 1) / FWI_STRIP, ntiles = g.ntx * nsz;
+        std::vector<char> fz((size_t)ngr * ntiles);
+        std::vector<int> lb[4];
+        for (int q = 0; q < 4; q++) c->bwd_pre[q].assign((q == 1 || q == 2 ? c->nshots : ngr) + 1, 0);
+        for (int grp = 0; grp < ngr; grp++) {
+            c->bwd_pre[0][grp] = (int)lb[0].size();
+            c->bwd_pre[3][grp] = (int)lb[3].size();
+            for (int t = 0; t < ntiles; t++) {
+                const int x0 = (t % g.ntx) * FWI_TX, r0 = (t / g.ntx) * FWI_STRIP;
+                bool in = true;
+                for (int s = grp * rgs; s < std::min((grp + 1) * rgs, c->nshots) && in; s++)
+                    in = fwi_adj_strip_interior(g, c->shots_h[s], x0, r0) &&
+                         fwi_rec_tile_interior(g, c->shots_h[s], x0 - g.npml, r0);
+                fz[(size_t)grp * ntiles + t] = in;
+                if (in) lb[0].push_back(grp * ntiles + t);
+                else if (r0 < g.nz) lb[3].push_back(grp * ntiles + t);
+            }
+        }
+        c->bwd_pre[0][ngr] = (int)lb[0].size();
+        c->bwd_pre[3][ngr] = (int)lb[3].size();
+        for (int s = 0; s < c->nshots; s++) {
+            c->bwd_pre[1][s] = (int)lb[1].size();
+            c->bwd_pre[2][s] = (int)lb[2].size();
+            for (int t = 0; t < ntiles; t++) {
+                if (fz[(size_t)(s / rgs) * ntiles + t]) continue;
+                const int x0 = (t % g.ntx) * FWI_TX, sz = t / g.ntx;
+                lb[fwi_adj_strip_interior(g, c->shots_h[s], x0, sz * FWI_STRIP) ? 1 : 2].push_back(
+                    (s * nsz + sz) * g.ntx + t % g.ntx);
+            }
+        }
+        c->bwd_pre[1][c->nshots] = (int)lb[1].size();
+        c->bwd_pre[2][c->nshots] = (int)lb[2].size();
+        for (int q = 0; q < 4; q++) {
+            c->n_bwd[q] = (int)lb[q].size();
+            if (c->n_bwd[q])
+                HIPCHK(c, hipMemcpyAsync(c->tl_bwd[q], lb[q].data(), sizeof(int) * lb[q].size(), hipMemcpyHostToDevice,
+                                         c->st));
+        }
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        c->bwd_lists_for = rgs;
+    }
     const int nb_adj = g.ntx * g.ntz * c->ngroups;
     const int nb_rec = g.bntx * g.bntz * c->ngroups;
     if (c->ckpt_every > 0) {
@@ -1134,8 +1206,8 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
         const int s0 = ch * cs, s1 = std::min(s0 + cs, c->nshots);
         int kmax = 0;
         for (int s = s0; s < s1; s++) kmax = std::max(kmax, c->shots_h[s].nt);
-        const int *ta[2], *tr[2];
-        int na[2], nr[2];
+        const int *ta[2], *tr[2], *tb[4] = {nullptr, nullptr, nullptr, nullptr};
+        int na[2], nr[2], nbw[4] = {0, 0, 0, 0};
         if (tile) {
             const int g0 = s0 / rgs, g1 = (s1 + rgs - 1) / rgs;
             for (int q = 0; q < 2; q++) {
@@ -1143,6 +1215,14 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
                 na[q] = c->adj_pre[q][s1] - c->adj_pre[q][s0];
                 tr[q] = c->tl_rec[q] + c->rec_pre[q][g0];
                 nr[q] = c->rec_pre[q][g1] - c->rec_pre[q][g0];
+            }
+            if (fused) {
+                for (int q = 0; q < 4; q++) {
+                    const bool per_shot = q == 1 || q == 2;
+                    const int lo = per_shot ? s0 : g0, hi = per_shot ? s1 : g1;
+                    tb[q] = c->tl_bwd[q] + c->bwd_pre[q][lo];
+                    nbw[q] = c->bwd_pre[q][hi] - c->bwd_pre[q][lo];
+                }
             }
         }
         auto launch_adj = [&](const int *kb, int koff) {
@@ -1153,10 +1233,13 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
             return tile ? fwi_launch_rec_tile(&r, rd, tr, nr, kb, koff, rgs, pair_of(c))
                         : fwi_launch_rec(&r, kb, koff, nb_rec, c->st);
         };
+        auto launch_bwd = [&](const int *kb, int koff) {
+            return fwi_launch_bwd_tile(&a, ad, rd, tb, nbw, kb, koff, rgs, c->st);
+        };
         // overlapped form: adjoint k on st, imaging k+1 on st2, joined per step;
         // k runs down to -1 so that the last pair images step 0 (kernels
         // return for k < 0 and for k >= a shot's nt)
-        const bool ovl = tile && c->bwd_overlap && !c->timing;
+        const bool ovl = tile && c->bwd_overlap && !c->timing && !fused;
         auto pair_step = [&](const int *kb, int koff) {
             FwiPair p1 = {c->st, nullptr, c->ev_fork, c->ev_join}, p2 = {c->st2, nullptr, c->ev_fork, c->ev_join};
             hipError_t e = hipEventRecord(c->ev_fork2, c->st);
@@ -1172,7 +1255,9 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
                 rc = capture(c, &c->gx_bwd[ch], [&]() {
                     hipError_t e = hipSuccess;
                     for (int j = 0; j < B && e == hipSuccess; j++) {
-                        if (ovl) {
+                        if (fused) {
+                            e = launch_bwd(c->kbase, -j);
+                        } else if (ovl) {
                             e = pair_step(c->kbase, -j);
                         } else {
                             e = launch_adj(c->kbase, -j);
@@ -1189,6 +1274,15 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
             for (int blk = 0; blk < nblk; blk++) HIPCHK(c, hipGraphLaunch(c->gx_bwd[ch], c->st));
         } else if (ovl) {
             for (int k = kmax - 1; k >= -1; k--) HIPCHK(c, pair_step(nullptr, k));
+        } else if (fused) {
+            for (int k = kmax - 1; k >= 0; k--) {
+                hipEvent_t e0, e1;
+                if ((rc = tick(c, &e0))) return rc;
+                HIPCHK(c, launch_bwd(nullptr, k));
+                if ((rc = tick(c, &e1))) return rc;
+                if (c->timing) marks.push_back({1, {e0, e1}});
+                if (marks.size() >= 512 && (rc = collect_timing(c, marks))) return rc;
+            }
         } else {
             for (int k = kmax - 1; k >= 0; k--) {
                 hipEvent_t e0, e1, e2;

@@ -1303,10 +1303,13 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_adj_step(FwiAdjArgs A, c
 // x chain (C_d^T on the staged row, H_x^T + C_h^T at u nodes, G_x^T) through
 // wave-private LDS rows.  Same arithmetic and CPML ownership as k_adj_step.
 // ============================================================================
-template <bool EDGE>
+// FUSED: ekp of the rows comes from the caller (ek) and mu^{k+1} of the
+// rows is also left in keep (rows past the grid untouched), for the fused
+// backward step.
+template <bool EDGE, bool FUSED = false>
 __device__ __forceinline__ void adj_tile_rows(const FwiAdjArgs &A, const FwiShotDev &sh, const float *tab,
                                               float (*Ws)[HC], float (*Us)[HC], int s, int k, int x0, int r0,
-                                              int lane)
+                                              int lane, const float *ek = nullptr, float *keep = nullptr)
 {
     const FwiGeomDev &g = A.g;
     const int npml = g.npml, pnx = g.pnx, pnz = g.pnz, ld = g.ld, T = 2 * npml + 1;
@@ -1343,7 +1346,8 @@ __device__ __forceinline__ void adj_tile_rows(const FwiAdjArgs &A, const FwiShot
         const int i = r0 + d;
         const bool ok = !EDGE || i < pnz;
         mo[d] = ok ? buf_ld(rold, vo, i * rowb) : 0.f;
-        mk[d] = ok ? buf_ld(rk, vo, i * rowb) : 0.f;
+        if (FUSED) mk[d] = ek[d];
+        else mk[d] = ok ? buf_ld(rk, vo, i * rowb) : 0.f;
     }
 
     // chi after step k goes to buffer (k & 1); it was read from buffer ((k+1) & 1)
@@ -1577,6 +1581,7 @@ __device__ __forceinline__ void adj_tile_rows(const FwiAdjArgs &A, const FwiShot
         if (has_recv && rhas[d]) out = out + rsv[d];
         const float mnew = leapfrog(m[d + 3], mo[d], mk[d] * out);
         if (!EDGE || x < pnx) buf_st(mnew, rold, vo, i * rowb);
+        if (FUSED) keep[d] = mnew;
     }
 }
 
@@ -1879,15 +1884,17 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_rec_step(FwiRecArgs A, c
 #define RHP ((6 * RR + 63) / 64)
 static_assert(RR % 4 == 0 && 3 * RR <= 64, "reconstruction rows per wave");
 
-template <bool EDGE>
+// FUSED: mu^{k+1} of the rows comes from the caller (mu1).  bx0 may be
+// negative (tiles of the padded grid start npml columns left of the grid).
+template <bool EDGE, bool FUSED = false>
 __device__ __forceinline__ void rec_tile_shot(const FwiRecArgs &A, const FwiShotDev &sh, int s, int k, int bx0,
                                               int r0, int lane, float (*Ps)[HC], float (*Us)[HC], const float *pk,
-                                              double *acc)
+                                              double *acc, const float *mu1 = nullptr)
 {
     const FwiGeomDev &g = A.g;
     const int npml = g.npml, nx = g.nx, nz = g.nz, ld = g.ld;
     const int bx = bx0 + lane, fx = bx + npml;
-    const bool colok = !EDGE || bx < nx;
+    const bool colok = !EDGE || (bx >= 0 && bx < nx);
     const long long so = (long long)s * A.b.fs;
     const int cur = k & 1;
     const float *pn = (cur ? A.P1 : A.P0) + so;  // p^k
@@ -1919,7 +1926,8 @@ __device__ __forceinline__ void rec_tile_shot(const FwiRecArgs &A, const FwiShot
         const int i = r0 + d;
         const bool ok = !EDGE || i < nz;
         pq[d] = ok ? buf_ld(rq, vo, i * rowb) : 0.f;
-        pm[d] = ok ? buf_ld(rm, vo, i * rowb) : 0.f;
+        if (FUSED) pm[d] = mu1[d];
+        else pm[d] = ok ? buf_ld(rm, vo, i * rowb) : 0.f;
     }
 
     wave_sync();  // the previous shot's reads of Ps / Us are done
@@ -1988,8 +1996,10 @@ __device__ __forceinline__ void rec_tile_shot(const FwiRecArgs &A, const FwiShot
 
 
 // items (group, tile) of the lists built with fwi_rec_tile_interior (an item
-// is interior when the tile is interior for every shot of the group)
-template <bool EDGE>
+// is interior when the tile is interior for every shot of the group).
+// PADDED: the tiles are those of the padded grid (64 x FR from column 0, as
+// the fused backward step's), so bx0 = the tile's first column - npml.
+template <bool EDGE, bool PADDED = false>
 __device__ __forceinline__ void rec_tile_kernel(const FwiRecArgs *__restrict__ Ap, const FwiShotDev *__restrict__ shots,
                                                 const int *__restrict__ items, int nit, const int *kbase, int koff,
                                                 int gsize)
@@ -1999,20 +2009,21 @@ __device__ __forceinline__ void rec_tile_kernel(const FwiRecArgs *__restrict__ A
     const FwiRecArgs &A = *Ap;
     const FwiGeomDev &g = A.g;
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int ntz = (g.nz + RR - 1) / RR;
-    const int ntiles = g.bntx * ntz;
+    const int ncol = PADDED ? g.ntx : g.bntx;
+    const int ntz = PADDED ? (g.pnz + RR - 1) / RR : (g.nz + RR - 1) / RR;
+    const int ntiles = ncol * ntz;
     const int lb = xcd_remap(blockIdx.x, nit);
     if (lb < 0) return;
     const int l = items[lb];
     const int grp = l / ntiles, t = l - grp * ntiles;
-    const int tzi = t / g.bntx, txi = t - tzi * g.bntx;
-    const int bx0 = txi * TX, r0 = tzi * RR;
+    const int tzi = t / ncol, txi = t - tzi * ncol;
+    const int bx0 = txi * TX - (PADDED ? g.npml : 0), r0 = tzi * RR;
     // per wave: staged p rows and u rows while propagating, then its partial image (FP64)
     __shared__ __attribute__((aligned(16))) float lds[4][2][RR][HC];
     static_assert(sizeof(lds[0]) >= RR * TX * sizeof(double), "partial image fits the wave's rows");
     const int nx = g.nx, nz = g.nz, ld = g.ld, npml = g.npml;
     const int bx = bx0 + lane;
-    const bool colok = bx < nx;
+    const bool colok = bx >= 0 && bx < nx;
     // ekp of the tile's rows, shared by every shot
     float pk[RR];
     {
@@ -2064,6 +2075,90 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                                                       int koff, int gsize)
 {
     rec_tile_kernel<true>(Ap, shots, items, nit, kbase, koff, gsize);
+}
+// general items (group, padded tile) beside the fused backward step's interior ones
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_rec_tile_pad(const FwiRecArgs *__restrict__ Ap,
+                                                      const FwiShotDev *__restrict__ shots,
+                                                      const int *__restrict__ items, int nit, const int *kbase,
+                                                      int koff, int gsize)
+{
+    rec_tile_kernel<true, true>(Ap, shots, items, nit, kbase, koff, gsize);
+}
+
+// ============================================================================
+// fused backward step on interior items: adjoint step k, then reconstruction
+// + imaging step k, of a shot group on one 64 x FR tile of the padded grid
+// that is interior for both steps and every shot of the group.  The imaging
+// takes mu^{k+1} from the adjoint's registers and ekp is read once per tile
+// for the whole group, instead of mu^{k+1} re-read and ekp read per shot by
+// the separate kernels.  Each wave runs the shots w, w+4, ... of the group
+// and the four partial images are summed in wave order, as in
+// rec_tile_kernel, so every cell's image sum is the same bit for bit.  The
+// other tiles run as k_adj_tile strips, then k_rec_tile_pad items (on such
+// tiles the fused form ran slower than the two kernels: 2 waves per SIMD).
+// Items: group-major (group, row strip, tile column) of the padded grid.
+// ============================================================================
+// Occupancy: 96 VGPRs without spilling at 5 waves per SIMD (the default
+// schedule takes 115, 4 waves).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_bwd_tile_int(
+    const FwiAdjArgs *__restrict__ Aa, const FwiRecArgs *__restrict__ Ar, const FwiShotDev *__restrict__ shots,
+    const int *__restrict__ items, int nit, const int *kbase, int koff, int gsize)
+{
+    const int k = (kbase ? *kbase : 0) + koff;
+    if (k < 0) return;
+    const FwiAdjArgs &A = *Aa;
+    const FwiRecArgs &R = *Ar;
+    const FwiGeomDev &g = A.g;
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lb = xcd_remap(blockIdx.x, nit);
+    if (lb < 0) return;
+    // per wave: the adjoint's w' / u~ rows, then the reconstruction's p / u rows,
+    // then its partial image (FP64)
+    __shared__ __attribute__((aligned(16))) float lds[4][2][FR][HC];
+    static_assert(sizeof(lds[0]) >= FR * TX * sizeof(double), "partial image fits the wave's rows");
+    static_assert(FR == RR, "the adjoint and reconstruction rows of a tile coincide");
+    const int nsz = (g.pnz + FR - 1) / FR, ntiles = g.ntx * nsz;
+    const int l = items[lb];
+    const int grp = l / ntiles, t = l - grp * ntiles;
+    const int sz = t / g.ntx, tx = t - sz * g.ntx;
+    const int x0 = tx * TX, r0 = sz * FR;
+    const int npml = g.npml, nx = g.nx, ld = g.ld;
+    const int x = x0 + lane, bx = x - npml;
+    // ekp of the tile's rows, shared by every shot and both steps
+    float ek[FR];
+    {
+        const brsrc rk = buf_rsrc(A.b.ekp, (unsigned)(A.b.fs * sizeof(float)));
+#pragma unroll
+        for (int d = 0; d < FR; ++d) ek[d] = buf_ld(rk, x * 4, (r0 + d) * ld * 4);
+    }
+    double acc[FR];
+#pragma unroll
+    for (int d = 0; d < FR; ++d) acc[d] = 0.0;
+    const int s_lo = grp * gsize, s_hi = min(s_lo + gsize, A.b.nshots);
+    for (int s = s_lo + w; s < s_hi; s += 4) {
+        const FwiShotDev sh = shots[s];
+        if (k >= sh.nt) continue;
+        float mu1[FR];
+        wave_sync();  // the previous shot's reads of the wave's LDS rows are done
+        adj_tile_rows<false, true>(A, sh, nullptr, lds[w][0], lds[w][1], s, k, x0, r0, lane, ek, mu1);
+        rec_tile_shot<false, true>(R, sh, s, k, x0 - npml, r0, lane, lds[w][0], lds[w][1], ek, acc, mu1);
+    }
+    // steps outside the imaging window only propagate (uniform per launch)
+    if (!fwi_image_step(R.img_jump, R.img_k0, R.img_k1, k)) return;
+    wave_sync();
+    double(*red)[TX] = reinterpret_cast<double(*)[TX]>(&lds[w][0][0][0]);
+#pragma unroll
+    for (int d = 0; d < FR; ++d) red[d][lane] = acc[d];
+    __syncthreads();
+    double *gacc = R.gacc + (long long)grp * nx * g.nz;
+#pragma unroll
+    for (int e = 0; e < FR / 4; ++e) {
+        const int d = w * (FR / 4) + e, i = r0 + d;
+        double sum = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) sum += reinterpret_cast<double(*)[TX]>(&lds[q][0][0][0])[d][lane];
+        gacc[(long long)i * nx + bx] += sum;
+    }
 }
 
 // ============================================================================
@@ -2482,6 +2577,28 @@ hipError_t fwi_launch_rec_tile(const FwiRecArgs *a, const FwiRecArgs *a_dev, con
             hipLaunchKernelGGL(k_rec_tile_gen, dim3(((ntl[1] + 7) / 8) * 8), dim3(256), 0, st, a_dev, a->b.shots,
                                tl[1], ntl[1], kbase, koff, gsize);
         });
+}
+
+// one backward step: the fused interior items (tl[0]) beside the adjoint's
+// interior / general strips of the other tiles (tl[1], tl[2]), then the
+// reconstruction of the other tiles (tl[3]), which reads their mu^{k+1}
+hipError_t fwi_launch_bwd_tile(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, const FwiRecArgs *r_dev,
+                               const int *const *tl, const int *ntl, const int *kbase, int koff, int gsize,
+                               hipStream_t st)
+{
+    if (ntl[0] > 0)
+        hipLaunchKernelGGL(k_bwd_tile_int, dim3(((ntl[0] + 7) / 8) * 8), dim3(256), 0, st, a_dev, r_dev, a->b.shots,
+                           tl[0], ntl[0], kbase, koff, gsize);
+    if (ntl[1] > 0)
+        hipLaunchKernelGGL(k_adj_tile<false>, dim3(((ntl[1] + 31) / 32) * 8), dim3(256), 0, st, a_dev, a->b.shots,
+                           tl[1], ntl[1], kbase, koff);
+    if (ntl[2] > 0)
+        hipLaunchKernelGGL(k_adj_tile<true>, dim3(((ntl[2] + 31) / 32) * 8), dim3(256), 0, st, a_dev, a->b.shots,
+                           tl[2], ntl[2], kbase, koff);
+    if (ntl[3] > 0)
+        hipLaunchKernelGGL(k_rec_tile_pad, dim3(((ntl[3] + 7) / 8) * 8), dim3(256), 0, st, r_dev, a->b.shots, tl[3],
+                           ntl[3], kbase, koff, gsize);
+    return hipGetLastError();
 }
 
 hipError_t fwi_launch_adj(const FwiAdjArgs *a, const int *kbase, int koff, int nblocks, hipStream_t st)

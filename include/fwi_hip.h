@@ -132,11 +132,23 @@ int fwi_hip_set_imaging(fwi_hip_ctx *ctx, int jump, int k0, int k1);
  * shot) are not allocated.  Tile kernels only.                              */
 int fwi_hip_set_checkpointing(fwi_hip_ctx *ctx, int every, int storage);
 
-/* Backward pass of the tile kernels with the adjoint step k and the
- * reconstruction + imaging step k+1 launched side by side on two streams
- * (they touch disjoint data: the adjoint writes only the mu buffer the
- * imaging step does not read).  The image is bit-identical.  on = 0: one
- * stream, adjoint then imaging per step.                                    */
+/* Backward step of the tile kernels fused into one kernel (on = 1, the
+ * default): per shot group and 64 x 8 tile of the padded grid, adjoint step
+ * k then reconstruction + imaging step k, the imaging taking mu^{k+1} from
+ * the adjoint's registers and ekp read once per tile.  on = 0: the separate
+ * adjoint and reconstruction kernels.  The image is bit-identical.  The
+ * checkpointed backward pass (set_checkpointing) keeps separate kernels.
+ * One launch is one backward time step of the gradient driver grad4_1shot
+ * (declared at header.h:283, defined nowhere): the adjoint operator built
+ * from a_lap_opt_2d_pml (fwi_2d.c:761) and the reconstruction built from
+ * a_lap_opt_2d (fwi_2d.c:722) + read_bdr (fwi_2d.c:919).                    */
+int fwi_hip_set_bwd_fused(fwi_hip_ctx *ctx, int on);
+
+/* Backward pass of the separate tile kernels (set_bwd_fused 0) with the
+ * adjoint step k and the reconstruction + imaging step k+1 launched side by
+ * side on two streams (they touch disjoint data: the adjoint writes only the
+ * mu buffer the imaging step does not read).  The image is bit-identical.
+ * on = 0: one stream, adjoint then imaging per step.                        */
 int fwi_hip_set_bwd_overlap(fwi_hip_ctx *ctx, int on);
 
 /* Shots per backward pass of the tile kernels (0, the default: all shots in
@@ -155,8 +167,8 @@ int fwi_hip_synchronize(fwi_hip_ctx *ctx);
 
 /* Per-kernel timing with HIP events on the context stream (0 = off).
  * fwi_hip_kernel_time returns the summed milliseconds and launch count of
- * kernel `which` (0 forward step, 1 adjoint step, 2 reconstruction step)
- * since timing was switched on.                                            */
+ * kernel `which` (0 forward step, 1 adjoint step or, fused, the whole
+ * backward step, 2 reconstruction step) since timing was switched on.      */
 int fwi_hip_set_timing(fwi_hip_ctx *ctx, int on);
 int fwi_hip_kernel_time(fwi_hip_ctx *ctx, int which, double *ms, long long *launches);
 

@@ -223,6 +223,7 @@ def test_overlapped_backward_gives_the_same_image(lib, graph):
     def image(overlap):
         ctx = lib.Context(g, max_shots=3, max_nt=230, max_nr=100)
         ctx.set_graph(graph)
+        ctx.set_bwd_fused(False)
         ctx.set_bwd_overlap(overlap)
         ctx.set_model(vel)
         ctx.set_shots(shots)
@@ -239,6 +240,53 @@ def test_overlapped_backward_gives_the_same_image(lib, graph):
     for a, b in zip(got, ref):
         np.testing.assert_array_equal(a, b)
     np.testing.assert_array_equal(ref[0], ref[1])
+
+
+@pytest.mark.parametrize("graph", [100, 7, 0])
+def test_fused_backward_gives_the_same_image(lib, graph):
+    """One fused kernel per backward step (adjoint, then reconstruction and
+    imaging from the adjoint's registers) gives the image of the separate
+    kernels bit for bit: shot groups of 1, 2, 3 and 8 (a partial last
+    group), uneven nt, backward chunks, an imaging window, graph replays."""
+    g = Geometry(nx=190, nz=70, npml=10, dx=10.0, dz=10.0, dt=1e-3)
+    rng = np.random.default_rng(18)
+    vel = (2200 + 300 * rng.standard_normal((g.nz, g.nx))).clip(1500, 3200).astype(np.float32)
+    shots = []
+    for j in range(5):
+        irx = np.arange(j % 2, g.nx, 2, dtype=np.int32)
+        ntj = 230 - 29 * j
+        shots.append(Shot(nt=ntj, isx=15 + 40 * j, isz=2 + j, wav=ricker(ntj, g.dt, 20.0), irx=irx,
+                          irz=np.full(len(irx), 2 + (j % 3), np.int32)))
+    res = [rng.standard_normal((s.nr, s.nt)).astype(np.float32) for s in shots]
+    ctx = lib.Context(g, max_shots=5, max_nt=230, max_nr=100)
+    ctx.set_graph(graph)
+    ctx.set_model(vel)
+    ctx.set_shots(shots)
+
+    def image(fused, rg, chunk=0, window=(1, 0, 2**31 - 1)):
+        ctx.set_bwd_fused(fused)
+        ctx.set_rec_group(rg)
+        ctx.set_bwd_chunk(chunk)
+        ctx.set_imaging(*window)
+        out = []
+        for _ in range(2):                      # a second pass replays the captured graphs
+            ctx.forward(want_traces=False)
+            ctx.set_residual(ctx.pack(res))
+            grad, gk = ctx.adjoint()
+            out.append(gk)
+        return out
+
+    try:
+        for rg in (1, 2, 3, 8):
+            ref = image(False, rg)
+            assert np.abs(ref[0]).max() > 0
+            for a in ref[1:] + image(True, rg):
+                np.testing.assert_array_equal(a, ref[0], err_msg=f"group {rg}")
+        ref = image(False, 2, chunk=3, window=(3, 10, 200))
+        for a in image(True, 2, chunk=3, window=(3, 10, 200)):
+            np.testing.assert_array_equal(a, ref[0], err_msg="chunks + window")
+    finally:
+        ctx.close()
 
 
 @pytest.mark.parametrize("kind", [0, 1])
