@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--rec-group", type=int, default=0, help="shots per workgroup of the tile imaging kernel")
     ap.add_argument("--bwd-chunk", type=int, default=-1, help="shots per backward pass (0: all; -1: the default)")
     ap.add_argument("--bwd-overlap", type=int, default=-1, help="adjoint beside imaging on two streams (1/0; -1: default)")
+    ap.add_argument("--concurrent", type=int, default=-1,
+                    help="general strips on a second stream beside the interior launch (1/0; -1: default)")
     ap.add_argument("--bwd-fused", type=int, default=-1,
                     help="one fused adjoint + reconstruction + imaging kernel per backward step (1/0; -1: default 1)")
     ap.add_argument("--ckpt-every", type=int, default=100, help="snapshot interval of --workload checkpoint")
@@ -424,6 +426,8 @@ def main():
             ctx.set_bwd_overlap(args.bwd_overlap)
         if args.bwd_fused >= 0:
             ctx.set_bwd_fused(args.bwd_fused)
+        if args.concurrent >= 0:
+            ctx.set_concurrent(args.concurrent)
     ctx.set_model(vel)
     ctx.set_shots(shots)
     obs = ctx.forward()
@@ -450,29 +454,35 @@ def main():
     sg_per_s = nshots * args.steps / wall
     gcell = cells_shot * nshots * args.steps / wall / 1e9
 
-    # per-kernel durations (HIP events on the context stream), one extra step
-    ctx.set_timing(True)
-    step()
+    # per-step durations (HIP events on the context stream), one extra step
     kern = {}
     kind = launch["kernels"] if launch else args.kernels
     names = (("k_fwd_tile", "k_adj_tile", "k_rec_tile") if kind == 0
              else ("k_fwd_step", "k_adj_step", "k_rec_step"))
     alg = list(algorithmic_bytes(g, nshots))
     mint = list(min_traffic_bytes(g, nshots, nr))
-    fused = kind == 0 and ctx.kernel_time(2)[1] == 0
+    fused = kind == 0 and args.bwd_fused != 0 and args.workload == "batched"
     if fused:
-        # one fused backward kernel: both wavefields' cell-updates credited; its
-        # minimum traffic skips the re-read of mu^{k+1} and the second ekp read
+        # events around the whole forward and backward passes, run as in the
+        # timed steps (graphs, second stream): time per step of each pass.  The
+        # backward step credits both wavefields' cell-updates; its minimum
+        # traffic skips the re-read of mu^{k+1} and the second ekp read
+        ctx.set_timing(2)
         b = g.nx * g.nz
         names = ("k_fwd_tile", "k_bwd_tile")
         alg = [alg[0], alg[1] + alg[2]]
         mint = [mint[0], mint[1] + mint[2] - 4 * b * nshots - 4 * b]
+    else:
+        ctx.set_timing(1)  # an event pair around every launch
+    step()
     for w, name in enumerate(names):
         ms, n = ctx.kernel_time(w)
         avg = ms / max(n, 1) * 1e-3
         kern[name] = dict(launches=n, avg_us=avg * 1e6, algorithmic_bytes=alg[w], min_traffic_bytes=mint[w],
                           achieved_GBps=alg[w] / avg / 1e9 if avg > 0 else None)
-    ctx.set_timing(False)
+        if fused:
+            kern[name]["per"] = "time step (all launches of the step, graph replay)"
+    ctx.set_timing(0)
     dom = max(names, key=lambda k: kern[k]["avg_us"] * kern[k]["launches"])
     traffic = None
     if os.path.exists(PROFILE_TRAFFIC):

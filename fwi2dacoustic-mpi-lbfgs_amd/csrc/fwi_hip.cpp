@@ -37,7 +37,7 @@ hipError_t fwi_launch_rec_tile(const FwiRecArgs *a, const FwiRecArgs *a_dev, con
 hipError_t fwi_launch_rec(const FwiRecArgs *a, const int *kbase, int koff, int nblocks, hipStream_t st);
 hipError_t fwi_launch_bwd_tile(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, const FwiRecArgs *r_dev,
                                const int *const *tl, const int *ntl, const int *kbase, int koff, int gsize,
-                               hipStream_t st);
+                               const FwiPair &p);
 hipError_t fwi_launch_advance(int *kbase, int delta, hipStream_t st);
 hipError_t fwi_launch_set_int(int *p, int v, hipStream_t st);
 hipError_t fwi_launch_fill_f32(float *p, long long n, float v, hipStream_t st);
@@ -66,11 +66,12 @@ struct fwi_hip_ctx {
     bool own_stream = false;
     // second stream for the general strips of a step, which can run beside the
     // interior ones (default priority, like st); fork/join events order them.
-    // Off by default: on MI355X the overlap gains nothing at 1024x320 x 64 shots
-    // (both launches then share the memory system) and costs host calls.
+    // On by default: the general strips are latency bound (long dependent
+    // load chains at 4 waves per SIMD) and fill the interior launches' gaps
+    // and tails (596 -> 621 shot-gradients/s at 1024x320 x 64 shots, nt=400)
     hipStream_t st2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-    bool concurrent = false;
+    bool concurrent = true;
     // backward pass: adjoint step k on st beside imaging step k+1 on st2
     // (independent: the adjoint writes only the mu buffer the imaging step
     // does not read, and never touches p or the images)
@@ -147,8 +148,8 @@ struct fwi_hip_ctx {
     std::vector<hipGraphExec_t> gx_bwd; // one captured block of backward steps per shot chunk
     int bwd_chunk = 0;                  // shots per backward pass (0: all in one)
     std::vector<int> adj_pre[2], rec_pre[2]; // list entries before shot s / before group g
-    // timing
-    bool timing = false;
+    // timing: 0 off, 1 per launch (plain launches), 2 per pass (graphs as usual)
+    int timing = 0;
     double t_ms[3] = {0, 0, 0};
     long long t_n[3] = {0, 0, 0};
     std::vector<hipEvent_t> ev;
@@ -771,14 +772,41 @@ static int zero_cpml(fwi_hip_ctx *c)
     return FWI_HIP_OK;
 }
 
-// timing helpers: an event pair around a launch when timing is on
+// timing helpers: an event pair around a launch (timing 1)
 static int tick(fwi_hip_ctx *c, hipEvent_t *e)
 {
     *e = nullptr;
-    if (!c->timing) return FWI_HIP_OK;
+    if (c->timing != 1) return FWI_HIP_OK;
     HIPCHK(c, hipEventCreate(e));
     c->ev.push_back(*e);
     HIPCHK(c, hipEventRecord(*e, c->st));
+    return FWI_HIP_OK;
+}
+
+// an event pair around a whole pass (timing 2): t_ms[which] += its time,
+// t_n[which] += its time steps
+static int pass_begin(fwi_hip_ctx *c, hipEvent_t *e)
+{
+    *e = nullptr;
+    if (c->timing != 2) return FWI_HIP_OK;
+    HIPCHK(c, hipEventCreate(e));
+    HIPCHK(c, hipEventRecord(*e, c->st));
+    return FWI_HIP_OK;
+}
+static int pass_end(fwi_hip_ctx *c, int which, hipEvent_t e0, long long steps)
+{
+    if (!e0) return FWI_HIP_OK;
+    hipEvent_t e1 = nullptr;
+    float ms = 0.f;
+    hipError_t e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipEventRecord(e1, c->st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (e != hipSuccess) return set_err(c, FWI_HIP_EHIP, "pass timing: %s", hipGetErrorString(e));
+    c->t_ms[which] += ms;
+    c->t_n[which] += steps;
     return FWI_HIP_OK;
 }
 
@@ -837,6 +865,16 @@ extern "C" int fwi_hip_set_rec_group(fwi_hip_ctx *c, int shots)
     c->rec_group = shots;
     c->rec_lists_for = -1;
     c->bwd_lists_for = -1;
+    return FWI_HIP_OK;
+}
+
+extern "C" int fwi_hip_set_concurrent(fwi_hip_ctx *c, int on)
+{
+    if (!c) return set_err(c, FWI_HIP_EINVAL, "set_concurrent: NULL context");
+    if (c->gx_fwd) (void)hipGraphExecDestroy(c->gx_fwd);
+    c->gx_fwd = nullptr;
+    drop_bwd(c);
+    c->concurrent = on != 0;
     return FWI_HIP_OK;
 }
 
@@ -942,6 +980,8 @@ extern "C" int fwi_hip_forward(fwi_hip_ctx *c, float *traces)
     const int nblocks = g.ntx * g.ntz * c->ngroups;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
     const int B = c->graph_steps;
+    hipEvent_t pe0;
+    if ((rc = pass_begin(c, &pe0))) return rc;
     if (ck) {
         // a snapshot before every ckpt_every-th step, plain launches between
         const int K = c->ckpt_every;
@@ -950,7 +990,7 @@ extern "C" int fwi_hip_forward(fwi_hip_ctx *c, float *traces)
             for (int k = (int)(j * K); k < std::min((long long)c->kmax, (j + 1) * K); k++)
                 HIPCHK(c, fwi_launch_fwd_tile(&a, ad, c->tl_fwd, c->n_fwd, nullptr, k, pair_of(c), fmode));
         }
-    } else if (B > 0 && !c->timing) {
+    } else if (B > 0 && c->timing != 1) {
         if (!c->gx_fwd) {
             rc = capture(c, &c->gx_fwd, [&]() {
                 hipError_t e = hipSuccess;
@@ -971,10 +1011,11 @@ extern "C" int fwi_hip_forward(fwi_hip_ctx *c, float *traces)
             HIPCHK(c, c->fwd_kind ? fwi_launch_fwd(&a, nullptr, k, nblocks, c->st)
                                   : fwi_launch_fwd_tile(&a, ad, c->tl_fwd, c->n_fwd, nullptr, k, pair_of(c), fmode));
             if ((rc = tick(c, &e1))) return rc;
-            if (c->timing) marks.push_back({0, {e0, e1}});
+            if (c->timing == 1) marks.push_back({0, {e0, e1}});
             if (marks.size() >= 512 && (rc = collect_timing(c, marks))) return rc;
         }
     }
+    if ((rc = pass_end(c, 0, pe0, c->kmax))) return rc;
     if ((rc = collect_timing(c, marks))) return rc;
     for (int s = 0; s < c->nshots; s++) {
         const FwiShotDev &d = c->shots_h[s];
@@ -1150,6 +1191,8 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
     }
     const int nb_adj = g.ntx * g.ntz * c->ngroups;
     const int nb_rec = g.bntx * g.bntz * c->ngroups;
+    hipEvent_t pe0;
+    if ((rc = pass_begin(c, &pe0))) return rc;
     if (c->ckpt_every > 0) {
         if (!tile) return set_err(c, FWI_HIP_EINVAL, "adjoint: checkpointing needs the tile kernels (set_kernels 0)");
         if (!c->ck_store) return set_err(c, FWI_HIP_ESTATE, "adjoint: no snapshots (forward with checkpointing first)");
@@ -1183,6 +1226,7 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
                 HIPCHK(c, fwi_launch_img_seg(&r, rd, c->ck_xseg + (long long)(k - k0) * S * nbb, k, rgs, ngr, c->st));
             }
         }
+        if ((rc = pass_end(c, 1, pe0, c->kmax))) return rc;
         HIPCHK(c, fwi_launch_grad_finalize(c->gacc, ngr, nb, c->vel, c->gk, c->grad, c->st));
         if (grad) HIPCHK(c, hipMemcpyAsync(grad, c->grad, sizeof(double) * nb, hipMemcpyDeviceToHost, c->st));
         if (gk) HIPCHK(c, hipMemcpyAsync(gk, c->gk, sizeof(double) * nb, hipMemcpyDeviceToHost, c->st));
@@ -1234,12 +1278,12 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
                         : fwi_launch_rec(&r, kb, koff, nb_rec, c->st);
         };
         auto launch_bwd = [&](const int *kb, int koff) {
-            return fwi_launch_bwd_tile(&a, ad, rd, tb, nbw, kb, koff, rgs, c->st);
+            return fwi_launch_bwd_tile(&a, ad, rd, tb, nbw, kb, koff, rgs, pair_of(c));
         };
         // overlapped form: adjoint k on st, imaging k+1 on st2, joined per step;
         // k runs down to -1 so that the last pair images step 0 (kernels
         // return for k < 0 and for k >= a shot's nt)
-        const bool ovl = tile && c->bwd_overlap && !c->timing && !fused;
+        const bool ovl = tile && c->bwd_overlap && c->timing != 1 && !fused;
         auto pair_step = [&](const int *kb, int koff) {
             FwiPair p1 = {c->st, nullptr, c->ev_fork, c->ev_join}, p2 = {c->st2, nullptr, c->ev_fork, c->ev_join};
             hipError_t e = hipEventRecord(c->ev_fork2, c->st);
@@ -1250,7 +1294,7 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
             if (e == hipSuccess) e = hipStreamWaitEvent(c->st, c->ev_join2, 0);
             return e;
         };
-        if (B > 0 && !c->timing) {
+        if (B > 0 && c->timing != 1) {
             if (!c->gx_bwd[ch]) {
                 rc = capture(c, &c->gx_bwd[ch], [&]() {
                     hipError_t e = hipSuccess;
@@ -1280,7 +1324,7 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
                 if ((rc = tick(c, &e0))) return rc;
                 HIPCHK(c, launch_bwd(nullptr, k));
                 if ((rc = tick(c, &e1))) return rc;
-                if (c->timing) marks.push_back({1, {e0, e1}});
+                if (c->timing == 1) marks.push_back({1, {e0, e1}});
                 if (marks.size() >= 512 && (rc = collect_timing(c, marks))) return rc;
             }
         } else {
@@ -1291,7 +1335,7 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
                 if ((rc = tick(c, &e1))) return rc;
                 HIPCHK(c, launch_rec(nullptr, k));
                 if ((rc = tick(c, &e2))) return rc;
-                if (c->timing) {
+                if (c->timing == 1) {
                     marks.push_back({1, {e0, e1}});
                     marks.push_back({2, {e1, e2}});
                 }
@@ -1299,6 +1343,7 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
             }
         }
     }
+    if ((rc = pass_end(c, 1, pe0, c->kmax))) return rc;
     if ((rc = collect_timing(c, marks))) return rc;
     HIPCHK(c, fwi_launch_grad_finalize(c->gacc, ngr, nb, c->vel, c->gk, c->grad, c->st));
     if (grad) HIPCHK(c, hipMemcpyAsync(grad, c->grad, sizeof(double) * nb, hipMemcpyDeviceToHost, c->st));
@@ -1320,7 +1365,8 @@ extern "C" int fwi_hip_synchronize(fwi_hip_ctx *c)
 extern "C" int fwi_hip_set_timing(fwi_hip_ctx *c, int on)
 {
     if (!c) return set_err(c, FWI_HIP_EINVAL, "set_timing: NULL context");
-    c->timing = on != 0;
+    if (on < 0 || on > 2) return set_err(c, FWI_HIP_EINVAL, "set_timing: mode 0, 1 or 2");
+    c->timing = on;
     for (int i = 0; i < 3; i++) {
         c->t_ms[i] = 0;
         c->t_n[i] = 0;

@@ -2581,24 +2581,32 @@ hipError_t fwi_launch_rec_tile(const FwiRecArgs *a, const FwiRecArgs *a_dev, con
 
 // one backward step: the fused interior items (tl[0]) beside the adjoint's
 // interior / general strips of the other tiles (tl[1], tl[2]), then the
-// reconstruction of the other tiles (tl[3]), which reads their mu^{k+1}
+// reconstruction of the other tiles (tl[3]), which reads their mu^{k+1}.
+// With p.st2 the other tiles' chain runs on st2 beside the interior launch:
+// the two sides write disjoint cells of mu^{k+1}, p^{k-1} and the image and
+// read only the other side's previous-step fields.
 hipError_t fwi_launch_bwd_tile(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, const FwiRecArgs *r_dev,
                                const int *const *tl, const int *ntl, const int *kbase, int koff, int gsize,
-                               hipStream_t st)
+                               const FwiPair &p)
 {
-    if (ntl[0] > 0)
-        hipLaunchKernelGGL(k_bwd_tile_int, dim3(((ntl[0] + 7) / 8) * 8), dim3(256), 0, st, a_dev, r_dev, a->b.shots,
-                           tl[0], ntl[0], kbase, koff, gsize);
-    if (ntl[1] > 0)
-        hipLaunchKernelGGL(k_adj_tile<false>, dim3(((ntl[1] + 31) / 32) * 8), dim3(256), 0, st, a_dev, a->b.shots,
-                           tl[1], ntl[1], kbase, koff);
-    if (ntl[2] > 0)
-        hipLaunchKernelGGL(k_adj_tile<true>, dim3(((ntl[2] + 31) / 32) * 8), dim3(256), 0, st, a_dev, a->b.shots,
-                           tl[2], ntl[2], kbase, koff);
-    if (ntl[3] > 0)
-        hipLaunchKernelGGL(k_rec_tile_pad, dim3(((ntl[3] + 7) / 8) * 8), dim3(256), 0, st, r_dev, a->b.shots, tl[3],
-                           ntl[3], kbase, koff, gsize);
-    return hipGetLastError();
+    const bool has_gen = ntl[1] > 0 || ntl[2] > 0 || ntl[3] > 0;
+    return launch_pair(
+        p, ntl[0] > 0, has_gen,
+        [&](hipStream_t st) {
+            hipLaunchKernelGGL(k_bwd_tile_int, dim3(((ntl[0] + 7) / 8) * 8), dim3(256), 0, st, a_dev, r_dev,
+                               a->b.shots, tl[0], ntl[0], kbase, koff, gsize);
+        },
+        [&](hipStream_t st) {
+            if (ntl[1] > 0)
+                hipLaunchKernelGGL(k_adj_tile<false>, dim3(((ntl[1] + 31) / 32) * 8), dim3(256), 0, st, a_dev,
+                                   a->b.shots, tl[1], ntl[1], kbase, koff);
+            if (ntl[2] > 0)
+                hipLaunchKernelGGL(k_adj_tile<true>, dim3(((ntl[2] + 31) / 32) * 8), dim3(256), 0, st, a_dev,
+                                   a->b.shots, tl[2], ntl[2], kbase, koff);
+            if (ntl[3] > 0)
+                hipLaunchKernelGGL(k_rec_tile_pad, dim3(((ntl[3] + 7) / 8) * 8), dim3(256), 0, st, r_dev,
+                                   a->b.shots, tl[3], ntl[3], kbase, koff, gsize);
+        });
 }
 
 hipError_t fwi_launch_adj(const FwiAdjArgs *a, const int *kbase, int koff, int nblocks, hipStream_t st)

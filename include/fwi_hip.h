@@ -132,6 +132,12 @@ int fwi_hip_set_imaging(fwi_hip_ctx *ctx, int jump, int k0, int k1);
  * shot) are not allocated.  Tile kernels only.                              */
 int fwi_hip_set_checkpointing(fwi_hip_ctx *ctx, int every, int storage);
 
+/* Tile kernels: the general strips of a step (CPML, edges, free surface,
+ * frames, sources, receivers) on a second stream beside the interior launch
+ * (on = 1, the default), joined before the next step; off: one stream.
+ * Results are bit-identical.                                                */
+int fwi_hip_set_concurrent(fwi_hip_ctx *ctx, int on);
+
 /* Backward step of the tile kernels fused into one kernel (on = 1, the
  * default): per shot group and 64 x 8 tile of the padded grid, adjoint step
  * k then reconstruction + imaging step k, the imaging taking mu^{k+1} from
@@ -165,11 +171,15 @@ int fwi_hip_set_rec_group(fwi_hip_ctx *ctx, int shots);
 /* Wait for all work of the context. */
 int fwi_hip_synchronize(fwi_hip_ctx *ctx);
 
-/* Per-kernel timing with HIP events on the context stream (0 = off).
- * fwi_hip_kernel_time returns the summed milliseconds and launch count of
- * kernel `which` (0 forward step, 1 adjoint step or, fused, the whole
- * backward step, 2 reconstruction step) since timing was switched on.      */
-int fwi_hip_set_timing(fwi_hip_ctx *ctx, int on);
+/* Timing with HIP events on the context stream.  mode 0: off.  mode 1: an
+ * event pair around every launch, plain launches (no graphs, no second
+ * stream); fwi_hip_kernel_time returns the summed milliseconds and launch
+ * count of kernel `which` (0 forward step, 1 adjoint step or, fused, the
+ * whole backward step, 2 reconstruction step).  mode 2: an event pair
+ * around each whole pass, run as usual (graphs, second stream);
+ * fwi_hip_kernel_time returns the summed milliseconds and time steps of
+ * the forward (0) and backward (1) passes.                                  */
+int fwi_hip_set_timing(fwi_hip_ctx *ctx, int mode);
 int fwi_hip_kernel_time(fwi_hip_ctx *ctx, int which, double *ms, long long *launches);
 
 /* Device state for inspection (host copy, dense rows):

@@ -289,6 +289,40 @@ def test_fused_backward_gives_the_same_image(lib, graph):
         ctx.close()
 
 
+@pytest.mark.parametrize("graph", [100, 0])
+def test_concurrent_general_strips_give_the_same_results(lib, graph):
+    """General strips on a second stream beside the interior launch (forward
+    and both backward forms) give the one-stream traces and image bit for
+    bit, over graph replays."""
+    g, vel, shots = small_case(nshots=3, nt=230, seed=19)
+    rng = np.random.default_rng(20)
+    res = [rng.standard_normal((s.nr, s.nt)).astype(np.float32) for s in shots]
+    ctx = lib.Context(g, max_shots=3, max_nt=230, max_nr=100)
+    ctx.set_graph(graph)
+    ctx.set_rec_group(2)
+    ctx.set_model(vel)
+    ctx.set_shots(shots)
+
+    def run(concurrent, fused):
+        ctx.set_concurrent(concurrent)
+        ctx.set_bwd_fused(fused)
+        out = []
+        for _ in range(2):                      # a second pass replays the captured graphs
+            tr = ctx.forward()
+            ctx.set_residual(ctx.pack(res))
+            out.append((tr, ctx.adjoint()[1]))
+        return out
+
+    try:
+        for fused in (True, False):
+            ref = run(False, fused)
+            for tr, gk in run(True, fused) + ref[1:]:
+                np.testing.assert_array_equal(tr, ref[0][0])
+                np.testing.assert_array_equal(gk, ref[0][1])
+    finally:
+        ctx.close()
+
+
 @pytest.mark.parametrize("kind", [0, 1])
 def test_imaging_window_is_a_partition_of_the_image(lib, kind):
     """The image over every step equals the sum of the images over the even
