@@ -38,11 +38,18 @@
 #include "fwi_plan.h"
 #include "fwi_sigproc.h"
 
-static fwi_hip_ctx *g_ctx = NULL;
-static long g_key[8];
+/* process-wide contexts, re-created when the geometry or the capacity
+ * changes: slot 0 runs grad_all_shots and RWI's incident field, slot 1 RWI's
+ * total field */
+typedef struct {
+    fwi_hip_ctx *ctx;
+    long key[8];
+    int ck_applied; /* the checkpointing settings reached this context */
+} CtxSlot;
+static CtxSlot g_slot[2];
 static char g_msg[512] = "";
 static int g_res_mode = 0;
-static int g_ck_every = 0, g_ck_storage = 0, g_ck_applied = 0; /* checkpointing of the batched context */
+static int g_ck_every = 0, g_ck_storage = 0; /* checkpointing of the batched contexts */
 
 static int fail(int code, const char *fmt, ...);
 
@@ -114,7 +121,7 @@ int grad4_set_checkpointing(int every, int storage)
     if (every < 0 || storage < 0 || storage > 2) return FWI_HIP_EINVAL;
     g_ck_every = every;
     g_ck_storage = storage;
-    g_ck_applied = 0;
+    g_slot[0].ck_applied = g_slot[1].ck_applied = 0;
     return FWI_HIP_OK;
 }
 
@@ -157,34 +164,42 @@ static int fail(int code, const char *fmt, ...)
 
 const char *grad4_last_error(void) { return g_msg; }
 
-/* one context per process, re-created when the geometry or the capacity changes */
-static int ensure_ctx(const fwi_base *b, int nshots, int nt, int nr)
+/* the context of slot `which` for this geometry and capacity, with the
+ * checkpointing settings applied */
+static int ensure_ctx(int which, const fwi_base *b, int nshots, int nt, int nr)
 {
+    CtxSlot *sl = &g_slot[which];
     long key[8] = {(long)b->nx, (long)b->nz, (long)b->npml, (long)nshots, (long)nt, (long)nr, 0, 0};
     memcpy(&key[6], &b->dx, sizeof(float));
     memcpy((char *)&key[6] + 4, &b->dz, sizeof(float));
     memcpy(&key[7], &b->dt, sizeof(float));
-    if (g_ctx && memcmp(key, g_key, sizeof(key)) == 0) return FWI_HIP_OK;
-    if (g_ctx) fwi_hip_destroy(g_ctx);
-    g_ctx = NULL;
-    fwi_hip_desc d;
-    memset(&d, 0, sizeof(d));
-    d.grid.nx = (int)b->nx;
-    d.grid.nz = (int)b->nz;
-    d.grid.npml = (int)b->npml;
-    d.grid.dx = b->dx;
-    d.grid.dz = b->dz;
-    d.grid.dt = b->dt;
-    d.grid.max_f = 15.0f;    /* fwi_sys.h MAX_F  */
-    d.grid.max_v = 6000.0f;  /* fwi_sys.h MAX_V  */
-    d.grid.coef_r = 0.001f;  /* fwi_sys.h COEF_R */
-    d.max_shots = nshots;
-    d.max_nt = nt;
-    d.max_nr = nr;
-    int rc = fwi_hip_create(&d, &g_ctx);
-    if (rc) return fail(rc, "fwi_hip_create: %s", fwi_hip_last_error(NULL));
-    memcpy(g_key, key, sizeof(key));
-    g_ck_applied = 0;
+    if (!(sl->ctx && memcmp(key, sl->key, sizeof(key)) == 0)) {
+        if (sl->ctx) fwi_hip_destroy(sl->ctx);
+        sl->ctx = NULL;
+        fwi_hip_desc d;
+        memset(&d, 0, sizeof(d));
+        d.grid.nx = (int)b->nx;
+        d.grid.nz = (int)b->nz;
+        d.grid.npml = (int)b->npml;
+        d.grid.dx = b->dx;
+        d.grid.dz = b->dz;
+        d.grid.dt = b->dt;
+        d.grid.max_f = 15.0f;    /* fwi_sys.h MAX_F  */
+        d.grid.max_v = 6000.0f;  /* fwi_sys.h MAX_V  */
+        d.grid.coef_r = 0.001f;  /* fwi_sys.h COEF_R */
+        d.max_shots = nshots;
+        d.max_nt = nt;
+        d.max_nr = nr;
+        int rc = fwi_hip_create(&d, &sl->ctx);
+        if (rc) return fail(rc, "fwi_hip_create: %s", fwi_hip_last_error(NULL));
+        memcpy(sl->key, key, sizeof(key));
+        sl->ck_applied = 0;
+    }
+    if (!sl->ck_applied) {
+        int rc = fwi_hip_set_checkpointing(sl->ctx, g_ck_every, g_ck_storage);
+        if (rc) return fail(rc, "set_checkpointing: %s", fwi_hip_last_error(sl->ctx));
+        sl->ck_applied = 1;
+    }
     return FWI_HIP_OK;
 }
 
@@ -394,87 +409,135 @@ static int host_misfit_all(fwi_plan **plans, int nshots, float *tr, int nt, floa
     return rc;
 }
 
+/* checks shared by the drivers: data rate (fwi_2d.c:195), one grid */
+static int check_shots(const fwi_base *b, fwi_specs *const *grids, fwi_msf *const *msfs, int nshots, int *nrmax)
+{
+    if (b->free_surface != 1) return fail(FWI_HIP_EINVAL, "only the free-surface grid is supported");
+    *nrmax = 0;
+    for (int s = 0; s < nshots; s++) {
+        const fwi_msf *m = msfs[s];
+        if ((int)m->nr > *nrmax) *nrmax = (int)m->nr;
+        if ((m->nt - 1) * m->jump != b->nt - 1)
+            return fail(FWI_HIP_EINVAL, "shot %d: (nt0-1)*jump != nt-1 (%zu, %zu, %zu)", s, m->nt, m->jump, b->nt);
+        if (grids[s]->base->nx != b->nx || grids[s]->base->nz != b->nz)
+            return fail(FWI_HIP_EINVAL, "shot %d: grid differs from shot 0", s);
+    }
+    return FWI_HIP_OK;
+}
+
+/* imaging window from a plan's fwi_grad: every jump-th step with
+ * start_t <= k dt <= end_t (end_t <= 0: to the last step) */
+static int set_window(fwi_hip_ctx *ctx, const fwi_grad *gr, const fwi_base *b)
+{
+    const int jump = gr->jump > 0 ? (int)gr->jump : 1;
+    const int k0 = gr->start_t > 0.0f ? (int)ceil((double)gr->start_t / b->dt - 1e-9) : 0;
+    const int k1 = gr->end_t > 0.0f ? (int)floor((double)gr->end_t / b->dt + 1e-9) : INT_MAX;
+    const int rc = k1 >= k0 ? fwi_hip_set_imaging(ctx, jump, k0, k1) : FWI_HIP_EINVAL;
+    if (rc) return fail(rc, "imaging window: start_t=%g end_t=%g jump=%zu", gr->start_t, gr->end_t, gr->jump);
+    return FWI_HIP_OK;
+}
+
+/* the fwi_hip shot table of the plans' sources and receivers */
+typedef struct {
+    fwi_hip_shot *sh;
+    int **cells;
+    int n;
+    long long ntr; /* trace samples at the propagation rate */
+} ShotTable;
+
+static void shot_table_free(ShotTable *t)
+{
+    if (t->cells)
+        for (int s = 0; s < 2 * t->n; s++) free(t->cells[s]);
+    free(t->cells);
+    free(t->sh);
+    memset(t, 0, sizeof(*t));
+}
+
+static int shot_table(ShotTable *t, fwi_sou *const *sous, fwi_msf *const *msfs, int nshots, int nt)
+{
+    memset(t, 0, sizeof(*t));
+    t->n = nshots;
+    t->sh = (fwi_hip_shot *)calloc(nshots, sizeof(fwi_hip_shot));
+    t->cells = (int **)calloc(2 * nshots, sizeof(int *));
+    if (!t->sh || !t->cells) {
+        shot_table_free(t);
+        return fail(FWI_HIP_ENOMEM, "host memory");
+    }
+    for (int s = 0; s < nshots; s++) {
+        const fwi_sou *so = sous[s];
+        const int nr = (int)msfs[s]->nr;
+        t->cells[2 * s] = (int *)malloc(sizeof(int) * (nr ? nr : 1));
+        t->cells[2 * s + 1] = (int *)malloc(sizeof(int) * (nr ? nr : 1));
+        if (!t->cells[2 * s] || !t->cells[2 * s + 1]) {
+            shot_table_free(t);
+            return fail(FWI_HIP_ENOMEM, "host memory");
+        }
+        for (int r = 0; r < nr; r++) { /* ixyz_sr layout, fwi_2d.c:366-371 */
+            t->cells[2 * s][r] = so->ixyz_sr[2 + 2 * r];
+            t->cells[2 * s + 1][r] = so->ixyz_sr[2 + 2 * r + 1];
+        }
+        t->sh[s].nt = nt;
+        t->sh[s].isx = so->ixyz_sr[0];
+        t->sh[s].isz = so->ixyz_sr[1];
+        t->sh[s].nr = nr;
+        t->sh[s].irx = t->cells[2 * s];
+        t->sh[s].irz = t->cells[2 * s + 1];
+        t->sh[s].wav = so->wav0;
+        t->ntr += (long long)nr * nt;
+    }
+    return FWI_HIP_OK;
+}
+
+/* model, imaging window and shots of one context */
+static int ctx_setup(fwi_hip_ctx *ctx, const float *vel, const fwi_grad *gr, const fwi_base *b, const ShotTable *t)
+{
+    int rc = fwi_hip_set_model(ctx, vel);
+    if (rc) return fail(rc, "set_model: %s", fwi_hip_last_error(ctx));
+    if ((rc = set_window(ctx, gr, b))) return rc;
+    rc = fwi_hip_set_shots(ctx, t->n, t->sh);
+    if (rc) return fail(rc, "set_shots: %s", fwi_hip_last_error(ctx));
+    return FWI_HIP_OK;
+}
+
 int grad_all_shots(fwi_plan **plans, int nshots, int k)
 {
     (void)k;
     if (!plans || nshots < 1 || !plans[0]) return fail(FWI_HIP_EINVAL, "grad_all_shots: no plans");
     const fwi_base *b = plans[0]->grid->base;
-    if (b->free_surface != 1) return fail(FWI_HIP_EINVAL, "only the free-surface grid is supported");
     const int nx = (int)b->nx, nz = (int)b->nz, nt = (int)b->nt;
-    int nrmax = 0;
-    for (int s = 0; s < nshots; s++) {
-        const fwi_msf *m = plans[s]->msf;
-        if ((int)m->nr > nrmax) nrmax = (int)m->nr;
-        if ((m->nt - 1) * m->jump != b->nt - 1) /* fwi_2d.c:195 */
-            return fail(FWI_HIP_EINVAL, "shot %d: (nt0-1)*jump != nt-1 (%zu, %zu, %zu)", s, m->nt, m->jump, b->nt);
-        if (plans[s]->grid->base->nx != b->nx || plans[s]->grid->base->nz != b->nz)
-            return fail(FWI_HIP_EINVAL, "shot %d: grid differs from shot 0", s);
+    fwi_specs **grids = (fwi_specs **)malloc(sizeof(void *) * nshots);
+    fwi_msf **msfs = (fwi_msf **)malloc(sizeof(void *) * nshots);
+    fwi_sou **sous = (fwi_sou **)malloc(sizeof(void *) * nshots);
+    ShotTable t = {0};
+    float *tr = NULL;
+    double *grad = NULL;
+    int nrmax = 0, rc = (grids && msfs && sous) ? FWI_HIP_OK : fail(FWI_HIP_ENOMEM, "host memory");
+    for (int s = 0; !rc && s < nshots; s++) {
+        grids[s] = plans[s]->grid;
+        msfs[s] = plans[s]->msf;
+        sous[s] = plans[s]->sou;
     }
-    int rc = ensure_ctx(b, nshots, nt, nrmax);
-    if (rc) return rc;
-    if (!g_ck_applied) {
-        rc = fwi_hip_set_checkpointing(g_ctx, g_ck_every, g_ck_storage);
-        if (rc) return fail(rc, "set_checkpointing: %s", fwi_hip_last_error(g_ctx));
-        g_ck_applied = 1;
-    }
-    rc = fwi_hip_set_model(g_ctx, &plans[0]->model->md[0][0]);
-    if (rc) return fail(rc, "set_model: %s", fwi_hip_last_error(g_ctx));
-    {
-        /* imaging window from plans[0]->grad: every jump-th step with
-         * start_t <= k dt <= end_t (end_t <= 0: to the last step) */
-        const fwi_grad *gr = plans[0]->grad;
-        const int jump = gr->jump > 0 ? (int)gr->jump : 1;
-        const int k0 = gr->start_t > 0.0f ? (int)ceil((double)gr->start_t / b->dt - 1e-9) : 0;
-        const int k1 = gr->end_t > 0.0f ? (int)floor((double)gr->end_t / b->dt + 1e-9) : INT_MAX;
-        rc = k1 >= k0 ? fwi_hip_set_imaging(g_ctx, jump, k0, k1) : FWI_HIP_EINVAL;
-        if (rc) return fail(rc, "imaging window: start_t=%g end_t=%g jump=%zu", gr->start_t, gr->end_t, gr->jump);
-    }
-
-    fwi_hip_shot *sh = (fwi_hip_shot *)calloc(nshots, sizeof(fwi_hip_shot));
-    int **cells = (int **)calloc(2 * nshots, sizeof(int *));
-    long long ntr = 0;
-    for (int s = 0; s < nshots; s++) {
-        const fwi_sou *so = plans[s]->sou;
-        const int nr = (int)plans[s]->msf->nr;
-        cells[2 * s] = (int *)malloc(sizeof(int) * (nr ? nr : 1));
-        cells[2 * s + 1] = (int *)malloc(sizeof(int) * (nr ? nr : 1));
-        for (int r = 0; r < nr; r++) { /* ixyz_sr layout, fwi_2d.c:366-371 */
-            cells[2 * s][r] = so->ixyz_sr[2 + 2 * r];
-            cells[2 * s + 1][r] = so->ixyz_sr[2 + 2 * r + 1];
-        }
-        sh[s].nt = nt;
-        sh[s].isx = so->ixyz_sr[0];
-        sh[s].isz = so->ixyz_sr[1];
-        sh[s].nr = nr;
-        sh[s].irx = cells[2 * s];
-        sh[s].irz = cells[2 * s + 1];
-        sh[s].wav = so->wav0;
-        ntr += (long long)nr * nt;
-    }
-    float *tr = (float *)malloc(sizeof(float) * (ntr ? ntr : 1));
-    double *grad = (double *)malloc(sizeof(double) * nx * nz);
-    rc = fwi_hip_set_shots(g_ctx, nshots, sh);
-    if (rc) rc = fail(rc, "set_shots: %s", fwi_hip_last_error(g_ctx));
+    if (!rc) rc = check_shots(b, grids, msfs, nshots, &nrmax);
+    if (!rc) rc = ensure_ctx(0, b, nshots, nt, nrmax);
+    if (!rc) rc = shot_table(&t, sous, msfs, nshots, nt);
+    fwi_hip_ctx *ctx = g_slot[0].ctx;
+    if (!rc) rc = ctx_setup(ctx, &plans[0]->model->md[0][0], plans[0]->grad, b, &t);
     if (!rc) {
-        rc = fwi_hip_forward(g_ctx, tr);
-        if (rc) rc = fail(rc, "forward: %s", fwi_hip_last_error(g_ctx));
+        tr = (float *)malloc(sizeof(float) * (t.ntr ? t.ntr : 1));
+        grad = (double *)malloc(sizeof(double) * nx * nz);
+        if (!tr || !grad) rc = fail(FWI_HIP_ENOMEM, "host memory");
     }
+    if (!rc && (rc = fwi_hip_forward(ctx, tr))) rc = fail(rc, "forward: %s", fwi_hip_last_error(ctx));
     if (!rc) rc = host_misfit_all(plans, nshots, tr, nt, b->dt);
-    if (!rc) {
-        if (!rc) {
-            rc = fwi_hip_set_residual(g_ctx, tr);
-            if (rc) rc = fail(rc, "set_residual: %s", fwi_hip_last_error(g_ctx));
-        }
-    }
-    if (!rc) {
-        rc = fwi_hip_adjoint(g_ctx, grad, NULL);
-        if (rc) rc = fail(rc, "adjoint: %s", fwi_hip_last_error(g_ctx));
-    }
+    if (!rc && (rc = fwi_hip_set_residual(ctx, tr))) rc = fail(rc, "set_residual: %s", fwi_hip_last_error(ctx));
+    if (!rc && (rc = fwi_hip_adjoint(ctx, grad, NULL))) rc = fail(rc, "adjoint: %s", fwi_hip_last_error(ctx));
     if (!rc)
         for (long i = 0; i < (long)nx * nz; i++) plans[0]->grad->body[i] = (float)grad[i];
-    for (int s = 0; s < 2 * nshots; s++) free(cells[s]);
-    free(cells);
-    free(sh);
+    shot_table_free(&t);
+    free(grids);
+    free(msfs);
+    free(sous);
     free(tr);
     free(grad);
     return rc;
@@ -492,12 +555,107 @@ void grad4_1shot(fwi_plan *plan, int k)
 }
 
 /* ------------------------------------------------------------------------ */
+/* reflection waveform inversion, difference form (fwi_plan.h): the incident
+ * field in m0 on context slot 0, the perturbed field in m0 + m1 on slot 1 */
+int grad_all_shots_rfl(rwi_plan **plans, int nshots, int k, float *grad_m1)
+{
+    (void)k;
+    if (!plans || nshots < 1 || !plans[0]) return fail(FWI_HIP_EINVAL, "grad_all_shots_rfl: no plans");
+    const fwi_base *b = plans[0]->grid->base;
+    const int nx = (int)b->nx, nz = (int)b->nz, nt = (int)b->nt;
+    const long nb = (long)nx * nz;
+    fwi_specs **grids = (fwi_specs **)malloc(sizeof(void *) * nshots);
+    fwi_msf **msfs = (fwi_msf **)malloc(sizeof(void *) * nshots);
+    fwi_sou **sous = (fwi_sou **)malloc(sizeof(void *) * nshots);
+    fwi_plan *shell = (fwi_plan *)calloc(nshots, sizeof(fwi_plan)); /* the host misfit step's view */
+    fwi_plan **shells = (fwi_plan **)malloc(sizeof(void *) * nshots);
+    ShotTable t = {0};
+    float *tr0 = NULL, *tr1 = NULL, *vtot = NULL;
+    double *g0 = NULL, *g1 = NULL;
+    int nrmax = 0, rc = (grids && msfs && sous && shell && shells) ? FWI_HIP_OK : fail(FWI_HIP_ENOMEM, "host memory");
+    for (int s = 0; !rc && s < nshots; s++) {
+        grids[s] = plans[s]->grid;
+        msfs[s] = plans[s]->msf;
+        sous[s] = plans[s]->sou;
+        shell[s].grid = plans[s]->grid;
+        shell[s].msf = plans[s]->msf;
+        shell[s].sou = plans[s]->sou;
+        shell[s].grad = plans[s]->grad;
+        shells[s] = &shell[s];
+    }
+    if (!rc) rc = check_shots(b, grids, msfs, nshots, &nrmax);
+    if (!rc) rc = ensure_ctx(0, b, nshots, nt, nrmax);
+    if (!rc) rc = ensure_ctx(1, b, nshots, nt, nrmax);
+    if (!rc) rc = shot_table(&t, sous, msfs, nshots, nt);
+    if (!rc) {
+        tr0 = (float *)malloc(sizeof(float) * (t.ntr ? t.ntr : 1));
+        tr1 = (float *)malloc(sizeof(float) * (t.ntr ? t.ntr : 1));
+        vtot = (float *)malloc(sizeof(float) * nb);
+        g0 = (double *)malloc(sizeof(double) * nb);
+        g1 = (double *)malloc(sizeof(double) * nb);
+        if (!tr0 || !tr1 || !vtot || !g0 || !g1) rc = fail(FWI_HIP_ENOMEM, "host memory");
+    }
+    fwi_hip_ctx *c0 = g_slot[0].ctx, *c1 = g_slot[1].ctx;
+    if (!rc) {
+        const float *m0 = &plans[0]->model->m0[0][0], *m1 = &plans[0]->model->m1[0][0];
+        for (long i = 0; i < nb; i++) vtot[i] = m0[i] + m1[i];
+        rc = ctx_setup(c0, m0, plans[0]->grad, b, &t);
+        if (!rc) rc = ctx_setup(c1, vtot, plans[0]->grad, b, &t);
+    }
+    if (!rc && (rc = fwi_hip_forward(c0, tr0))) rc = fail(rc, "forward (incident): %s", fwi_hip_last_error(c0));
+    if (!rc && (rc = fwi_hip_forward(c1, tr1))) rc = fail(rc, "forward (perturbed): %s", fwi_hip_last_error(c1));
+    if (!rc) {
+        for (long long i = 0; i < t.ntr; i++) tr1[i] -= tr0[i]; /* the reflected data */
+        rc = host_misfit_all(shells, nshots, tr1, nt, b->dt); /* tr1 <- adjoint source r */
+    }
+    if (!rc && (rc = fwi_hip_set_residual(c1, tr1))) rc = fail(rc, "set_residual: %s", fwi_hip_last_error(c1));
+    if (!rc && (rc = fwi_hip_adjoint(c1, g1, NULL))) rc = fail(rc, "adjoint (perturbed): %s", fwi_hip_last_error(c1));
+    if (!rc) {
+        for (long long i = 0; i < t.ntr; i++) tr0[i] = -tr1[i];
+        if ((rc = fwi_hip_set_residual(c0, tr0))) rc = fail(rc, "set_residual: %s", fwi_hip_last_error(c0));
+    }
+    if (!rc && (rc = fwi_hip_adjoint(c0, g0, NULL))) rc = fail(rc, "adjoint (incident): %s", fwi_hip_last_error(c0));
+    if (!rc) {
+        for (long i = 0; i < nb; i++) plans[0]->grad->body[i] = (float)(g0[i] + g1[i]);
+        if (grad_m1)
+            for (long i = 0; i < nb; i++) grad_m1[i] = (float)g1[i];
+    }
+    shot_table_free(&t);
+    free(grids);
+    free(msfs);
+    free(sous);
+    free(shell);
+    free(shells);
+    free(tr0);
+    free(tr1);
+    free(vtot);
+    free(g0);
+    free(g1);
+    return rc;
+}
+
+void grad4_1shot_rfl(rwi_plan *plan, int k)
+{
+    rwi_plan *one[1] = {plan};
+    int rc = grad_all_shots_rfl(one, 1, k, NULL);
+    if (rc) {
+        fprintf(stderr, "grad4_1shot_rfl: %s\n", g_msg);
+        if (plan && plan->msf && plan->msf->val) plan->msf->val[0] = plan->msf->val[1] = NAN;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
 /* row-pointer arrays as alloc.c:3-16: a[n2][n1] over one zeroed slab */
 static float **alloc2d(size_t n1, size_t n2)
 {
     float **a = (float **)calloc(n2 ? n2 : 1, sizeof(float *));
     size_t n = n1 * n2;
     float *s = (float *)calloc(n ? n : 1, sizeof(float));
+    if (!a || !s) {
+        free(a);
+        free(s);
+        return NULL;
+    }
     for (size_t i = 0; i < n2; i++) a[i] = s + i * n1;
     if (n2 == 0) a[0] = s;
     return a;
@@ -510,15 +668,64 @@ static void free2d(float **a)
     free(a);
 }
 
-fwi_plan *fwi_plan_alloc(int nx, int nz, int npml, int nt, int jump, int nr, float dx, float dz, float dt)
+/* grid, misfit, source and gradient members shared by the FWI and RWI
+ * plans (fwi_2d.c:148-165,194-224,380; rwi_init the same, fwi_2d.c:402-515) */
+typedef struct {
+    fwi_specs *grid;
+    fwi_msf *msf;
+    fwi_sou *sou;
+    fwi_grad *grad;
+} PlanCommon;
+
+static void common_free(PlanCommon *c)
 {
-    if (nx < 1 || nz < 1 || npml < 0 || nt < 1 || jump < 1 || nr < 0 || (nt - 1) % jump != 0) return NULL;
-    fwi_plan *p = (fwi_plan *)calloc(1, sizeof(fwi_plan));
+    if (c->msf) {
+        free2d(c->msf->obs0);
+        free2d(c->msf->obs);
+        free2d(c->msf->syn);
+        free(c->msf->val);
+        free(c->msf->fbt);
+        free(c->msf->ifbt);
+        free(c->msf);
+    }
+    if (c->sou) {
+        free(c->sou->ixyz_sr);
+        free(c->sou->xyz_sr);
+        free(c->sou->wav0);
+        free(c->sou->wav);
+        free(c->sou);
+    }
+    if (c->grad) {
+        free(c->grad->body);
+        free(c->grad);
+    }
+    if (c->grid) {
+        free(c->grid->base);
+        free(c->grid->f);
+        free(c->grid->b);
+        free(c->grid);
+    }
+}
+
+static int common_alloc(PlanCommon *c, int nx, int nz, int npml, int nt, int jump, int nr, float dx, float dz,
+                        float dt)
+{
+    memset(c, 0, sizeof(*c));
+    if (nx < 1 || nz < 1 || npml < 0 || nt < 1 || jump < 1 || nr < 0 || (nt - 1) % jump != 0) return 0;
     const fwi_base base = {.dim = 2, .mm = 2, .npml = (size_t)npml, .nx = (size_t)nx, .nz = (size_t)nz,
                            .nt = (size_t)nt, .nt_mem = 1, .free_surface = 1, .dx = dx, .dz = dz, .dt = dt};
     fwi_base *bp = (fwi_base *)malloc(sizeof(fwi_base));
-    memcpy(bp, &base, sizeof(base));
     fwi_shape *gb = (fwi_shape *)calloc(1, sizeof(fwi_shape)), *gf = (fwi_shape *)calloc(1, sizeof(fwi_shape));
+    c->grid = (fwi_specs *)calloc(1, sizeof(fwi_specs));
+    if (!bp || !gb || !gf || !c->grid) {
+        free(bp);
+        free(gb);
+        free(gf);
+        free(c->grid);
+        c->grid = NULL;
+        return 0;
+    }
+    memcpy(bp, &base, sizeof(base));
     gb->mnx = gb->pnx = nx;            /* fwi_2d.c:148-152 */
     gb->mnz = gb->pnz = nz;
     gb->unx = nx + 1; gb->unz = nz;
@@ -527,37 +734,121 @@ fwi_plan *fwi_plan_alloc(int nx, int nz, int npml, int nt, int jump, int nr, flo
     gf->mnz = gf->pnz = nz + npml;
     gf->unx = gf->pnx + 1; gf->unz = gf->pnz;
     gf->vnx = gf->pnx; gf->vnz = gf->pnz + 1;
-    p->grid = (fwi_specs *)calloc(1, sizeof(fwi_specs));
-    p->grid->base = bp;
-    p->grid->f = gf;
-    p->grid->b = gb;
-    p->model = (fwi_model *)calloc(1, sizeof(fwi_model));
-    p->model->md = alloc2d(nx, nz);
-    p->model->kp = alloc2d(nx, nz);
-    p->model->ekp = alloc2d(gf->mnx, gf->mnz);
-    p->model->btry = (float *)calloc(nx, sizeof(float));
-    p->model->ibtry = (int *)calloc(nx, sizeof(int));
+    c->grid->base = bp;
+    c->grid->f = gf;
+    c->grid->b = gb;
     const int nt0 = (nt - 1) / jump + 1;
     const fwi_msf msf = {.nt = (size_t)nt0, .nr = (size_t)nr, .jump = (size_t)jump, .dt = dt * jump};
-    p->msf = (fwi_msf *)malloc(sizeof(fwi_msf));
-    memcpy(p->msf, &msf, sizeof(msf));
-    p->msf->obs0 = alloc2d(nt0, nr);
-    p->msf->obs = alloc2d(nt0, nr);
-    p->msf->syn = alloc2d(nt0, nr);
-    p->msf->val = (float *)calloc(2, sizeof(float));
-    p->msf->fbt = (float *)calloc(nr ? nr : 1, sizeof(float));
-    p->msf->ifbt = (int *)calloc(nr ? nr : 1, sizeof(int));
-    p->sou = (fwi_sou *)calloc(1, sizeof(fwi_sou));
-    p->sou->iarea = 1.0 / (dx * dz);
-    p->sou->ixyz_sr = (int *)calloc((size_t)(1 + nr) * 2, sizeof(int));
-    p->sou->xyz_sr = (float *)calloc((size_t)(1 + nr) * 2, sizeof(float));
-    p->sou->wav0 = (float *)calloc(nt, sizeof(float));
-    p->sou->wav = (float *)calloc(nt, sizeof(float));
+    c->msf = (fwi_msf *)malloc(sizeof(fwi_msf));
+    if (c->msf) {
+        memcpy(c->msf, &msf, sizeof(msf));
+        c->msf->obs0 = alloc2d(nt0, nr);
+        c->msf->obs = alloc2d(nt0, nr);
+        c->msf->syn = alloc2d(nt0, nr);
+        c->msf->val = (float *)calloc(2, sizeof(float));
+        c->msf->fbt = (float *)calloc(nr ? nr : 1, sizeof(float));
+        c->msf->ifbt = (int *)calloc(nr ? nr : 1, sizeof(int));
+    }
+    c->sou = (fwi_sou *)calloc(1, sizeof(fwi_sou));
+    if (c->sou) {
+        c->sou->iarea = 1.0 / (dx * dz);
+        c->sou->ixyz_sr = (int *)calloc((size_t)(1 + nr) * 2, sizeof(int));
+        c->sou->xyz_sr = (float *)calloc((size_t)(1 + nr) * 2, sizeof(float));
+        c->sou->wav0 = (float *)calloc(nt, sizeof(float));
+        c->sou->wav = (float *)calloc(nt, sizeof(float));
+    }
     const fwi_grad gr = {.jump = 1};
-    p->grad = (fwi_grad *)malloc(sizeof(fwi_grad));
-    memcpy(p->grad, &gr, sizeof(gr));
-    p->grad->body = (float *)calloc((size_t)nx * nz, sizeof(float));
+    c->grad = (fwi_grad *)malloc(sizeof(fwi_grad));
+    if (c->grad) {
+        memcpy(c->grad, &gr, sizeof(gr));
+        c->grad->body = (float *)calloc((size_t)nx * nz, sizeof(float));
+    }
+    const int ok = c->msf && c->msf->obs0 && c->msf->obs && c->msf->syn && c->msf->val && c->msf->fbt &&
+                   c->msf->ifbt && c->sou && c->sou->ixyz_sr && c->sou->xyz_sr && c->sou->wav0 && c->sou->wav &&
+                   c->grad && c->grad->body;
+    if (!ok) common_free(c);
+    return ok;
+}
+
+fwi_plan *fwi_plan_alloc(int nx, int nz, int npml, int nt, int jump, int nr, float dx, float dz, float dt)
+{
+    PlanCommon c;
+    if (!common_alloc(&c, nx, nz, npml, nt, jump, nr, dx, dz, dt)) return NULL;
+    fwi_plan *p = (fwi_plan *)calloc(1, sizeof(fwi_plan));
+    fwi_model *m = (fwi_model *)calloc(1, sizeof(fwi_model));
+    if (!p || !m) {
+        free(p);
+        free(m);
+        common_free(&c);
+        return NULL;
+    }
+    p->grid = c.grid;
+    p->msf = c.msf;
+    p->sou = c.sou;
+    p->grad = c.grad;
+    p->model = m;
+    m->md = alloc2d(nx, nz);
+    m->kp = alloc2d(nx, nz);
+    m->ekp = alloc2d(c.grid->f->mnx, c.grid->f->mnz);
+    m->btry = (float *)calloc(nx, sizeof(float));
+    m->ibtry = (int *)calloc(nx, sizeof(int));
+    if (!m->md || !m->kp || !m->ekp || !m->btry || !m->ibtry) {
+        fwi_plan_free(p);
+        return NULL;
+    }
     return p;
+}
+
+rwi_plan *rwi_plan_alloc(int nx, int nz, int npml, int nt, int jump, int nr, float dx, float dz, float dt)
+{
+    PlanCommon c;
+    if (!common_alloc(&c, nx, nz, npml, nt, jump, nr, dx, dz, dt)) return NULL;
+    rwi_plan *p = (rwi_plan *)calloc(1, sizeof(rwi_plan));
+    rwi_model *m = (rwi_model *)calloc(1, sizeof(rwi_model));
+    if (!p || !m) {
+        free(p);
+        free(m);
+        common_free(&c);
+        return NULL;
+    }
+    p->grid = c.grid;
+    p->msf = c.msf;
+    p->sou = c.sou;
+    p->grad = c.grad;
+    p->model = m;
+    const size_t fnx = c.grid->f->mnx, fnz = c.grid->f->mnz; /* rwi_init, fwi_2d.c:438-452 */
+    m->m0 = alloc2d(nx, nz);
+    m->m1 = alloc2d(nx, nz);
+    m->kp_0 = alloc2d(nx, nz);
+    m->kp_total = alloc2d(nx, nz);
+    m->ekp_0 = alloc2d(fnx, fnz);
+    m->ekp_total = alloc2d(fnx, fnz);
+    m->btry = (float *)calloc(nx, sizeof(float));
+    m->ibtry = (int *)calloc(nx, sizeof(int));
+    if (!m->m0 || !m->m1 || !m->kp_0 || !m->kp_total || !m->ekp_0 || !m->ekp_total || !m->btry || !m->ibtry) {
+        rwi_plan_free(p);
+        return NULL;
+    }
+    return p;
+}
+
+void rwi_plan_free(rwi_plan *p)
+{
+    if (!p) return;
+    if (p->model) {
+        free2d(p->model->m0);
+        free2d(p->model->m1);
+        free2d(p->model->kp_0);
+        free2d(p->model->kp_total);
+        free2d(p->model->ekp_0);
+        free2d(p->model->ekp_total);
+        free(p->model->btry);
+        free(p->model->ibtry);
+        free(p->model);
+    }
+    PlanCommon c = {p->grid, p->msf, p->sou, p->grad};
+    common_free(&c);
+    free(p);
 }
 
 int fwi_plan_set_grad(fwi_plan *p, float perc, float alpha_grad, int rectx1, int rectz1, int repeat1, int rectx2,
@@ -577,29 +868,15 @@ int fwi_plan_set_grad(fwi_plan *p, float perc, float alpha_grad, int rectx1, int
 void fwi_plan_free(fwi_plan *p)
 {
     if (!p) return;
-    free2d(p->model->md);
-    free2d(p->model->kp);
-    free2d(p->model->ekp);
-    free(p->model->btry);
-    free(p->model->ibtry);
-    free(p->model);
-    free2d(p->msf->obs0);
-    free2d(p->msf->obs);
-    free2d(p->msf->syn);
-    free(p->msf->val);
-    free(p->msf->fbt);
-    free(p->msf->ifbt);
-    free(p->msf);
-    free(p->sou->ixyz_sr);
-    free(p->sou->xyz_sr);
-    free(p->sou->wav0);
-    free(p->sou->wav);
-    free(p->sou);
-    free(p->grad->body);
-    free(p->grad);
-    free(p->grid->base);
-    free(p->grid->f);
-    free(p->grid->b);
-    free(p->grid);
+    if (p->model) {
+        free2d(p->model->md);
+        free2d(p->model->kp);
+        free2d(p->model->ekp);
+        free(p->model->btry);
+        free(p->model->ibtry);
+        free(p->model);
+    }
+    PlanCommon c = {p->grid, p->msf, p->sou, p->grad};
+    common_free(&c);
     free(p);
 }

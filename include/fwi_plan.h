@@ -6,8 +6,10 @@
  * order, types and constness kept, because MBotOpt.c-style callers pass
  * these structs by pointer): fwi_base, fwi_shape, fwi_specs, fwi_model,
  * fwi_msf, fwi_state, pml_base, pml_profile, fwi_fdcoe, fwi_sou, fwi_grad,
- * fwi_files_path, fwi_plan.  Only the GSL/FFTW/OpenMP includes of
- * header.h:7-14 are gone: nothing here needs them.
+ * fwi_files_path, fwi_plan, and the reflection-WI structs rwi_model,
+ * rwi_files_path, rwi_plan (header.h:53-58,144-156,187-220).  Only the
+ * GSL/FFTW/OpenMP includes of header.h:7-14 are gone: nothing here needs
+ * them.
  *
  * grad4_1shot is declared at header.h:283 and called at MBotOpt.c:55,99 but
  * has no body in the reference.  This library provides it on top of
@@ -55,6 +57,11 @@ typedef struct {
     float **md, **kp, **ekp, *btry, alpha_btry;
     int *ibtry;
 } fwi_model;
+
+typedef struct {
+    float **m0, **m1, **kp_total, **kp_0, **ekp_total, **ekp_0, *btry, alpha_btry;
+    int *ibtry;
+} rwi_model;
 
 typedef struct {
     const size_t nt, nr, jump, rectx, rectt, times;
@@ -111,6 +118,17 @@ typedef struct {
 } fwi_files_path;
 
 typedef struct {
+    char *common;
+    char *shots;
+    char *fbt;
+    char *xyz_sr;
+    char *init_mdl_0;
+    char *init_mdl_1;
+    char *btry;
+    char *wavelet;
+} rwi_files_path;
+
+typedef struct {
     fwi_specs *grid;
     fwi_model *model;
     fwi_state *fw;
@@ -128,6 +146,29 @@ typedef struct {
     };
 } fwi_plan;
 
+typedef struct {
+    fwi_specs *grid;
+    rwi_model *model;
+    fwi_state *fw_inc;
+    fwi_state *bw_inc;
+    fwi_state *fw_per;
+    fwi_state *bw_per;
+    fwi_sou *sou;
+    fwi_msf *msf;
+    pml_profile *pml_inc;
+    pml_profile *pml_per;
+    fwi_fdcoe *coe;
+    rwi_files_path *path;
+    fwi_grad *grad;
+    struct {
+        char path_bd0_wr[300];
+        char path_bd1_wr[300];
+        float ***bc_ud_inc, ***bc_lr_inc;
+        float ***bc_ud_per, ***bc_lr_per;
+        int offset;
+    };
+} rwi_plan;
+
 /* header.h:283 -- one shot: forward, L2 misfit, adjoint, gradient. */
 void grad4_1shot(fwi_plan *plan, int k);
 
@@ -137,7 +178,42 @@ void grad4_1shot(fwi_plan *plan, int k);
  * goes to plans[0]->grad->body.  Returns 0 or a FWI_HIP_E* code. */
 int grad_all_shots(fwi_plan **plans, int nshots, int k);
 
-/* Message of the last failure of grad4_1shot / grad_all_shots. */
+/* header.h:285 -- reflection waveform inversion, one shot.  Declared but
+ * not defined by the reference; rwi_init (fwi_2d.c:398-719) sets up its
+ * inputs: a background velocity m0, a second model m1, incident and
+ * perturbed states with their own CPML memory and frame buffers, one source
+ * and receiver set and one misfit.  This library's driver, the difference
+ * form of RWI:
+ *   p_inc   propagates in model->m0 (the incident field),
+ *   p_per   propagates in m0 + m1 (model->m1 is the velocity perturbation
+ *           that holds the reflectors),
+ *   syn     = R(p_per - p_inc) every jump steps (the reflected data),
+ *   misfit  the host misfit step of grad4_1shot (grad4_set_misfit and the
+ *           trace conditioning) on syn against msf->obs0 (reflection data),
+ *   grad    grad->body = dJ/dm0: the gradient through p_per (adjoint source
+ *           r, model m0 + m1) plus the gradient through p_inc (adjoint
+ *           source -r, model m0), both exact discrete gradients.
+ * Reads grid->base, model->m0/m1, sou (as grad4_1shot), msf->obs0; writes
+ * msf->syn, msf->val, grad->body.  kp_*, ekp_*, the states, the CPML
+ * profiles and the frame buffers of the plan are not used: the device
+ * holds its own. */
+void grad4_1shot_rfl(rwi_plan *plan, int k);
+
+/* Every shot of an RWI iteration in one batched run (two device contexts:
+ * the incident and the perturbed fields).  The model is plans[0]'s; the
+ * gradient summed over shots goes to plans[0]->grad->body, and when grad_m1
+ * is not NULL, dJ/dm1 (the gradient through p_per alone) to grad_m1[nz*nx].
+ * Returns 0 or a FWI_HIP_E* code (grad4_last_error). */
+int grad_all_shots_rfl(rwi_plan **plans, int nshots, int k, float *grad_m1);
+
+/* An RWI plan with what grad4_1shot_rfl reads and writes (the fields
+ * rwi_init fills from its files, fwi_2d.c:398-719, without the files and
+ * the host states).  Free with rwi_plan_free. */
+rwi_plan *rwi_plan_alloc(int nx, int nz, int npml, int nt, int jump, int nr, float dx, float dz, float dt);
+void rwi_plan_free(rwi_plan *plan);
+
+/* Message of the last failure of grad4_1shot / grad_all_shots (and of the
+ * RWI forms). */
 const char *grad4_last_error(void);
 
 /* The host misfit step of grad4_1shot / grad_all_shots (fwi_msf.h,

@@ -59,6 +59,25 @@ class FwiPlan(ctypes.Structure):
                 ("path", ctypes.c_void_p), ("grad", ctypes.POINTER(FwiGrad)), ("bd", BdState)]
 
 
+class RwiModel(ctypes.Structure):
+    _fields_ = [(n, PPF) for n in ("m0", "m1", "kp_total", "kp_0", "ekp_total", "ekp_0")] + \
+               [("btry", PF), ("alpha_btry", c_float), ("ibtry", PI)]
+
+
+class RwiBdState(ctypes.Structure):
+    _fields_ = [("path_bd0_wr", ctypes.c_char * 300), ("path_bd1_wr", ctypes.c_char * 300)] + \
+               [(n, ctypes.POINTER(PPF)) for n in ("bc_ud_inc", "bc_lr_inc", "bc_ud_per", "bc_lr_per")] + \
+               [("offset", c_int)]
+
+
+class RwiPlan(ctypes.Structure):
+    _fields_ = [("grid", ctypes.POINTER(FwiSpecs)), ("model", ctypes.POINTER(RwiModel))] + \
+               [(n, ctypes.c_void_p) for n in ("fw_inc", "bw_inc", "fw_per", "bw_per")] + \
+               [("sou", ctypes.POINTER(FwiSou)), ("msf", ctypes.POINTER(FwiMsf))] + \
+               [(n, ctypes.c_void_p) for n in ("pml_inc", "pml_per", "coe", "path")] + \
+               [("grad", ctypes.POINTER(FwiGrad)), ("bd", RwiBdState)]
+
+
 def lib():
     h = L.load()
     h.fwi_plan_alloc.restype = ctypes.POINTER(FwiPlan)
@@ -88,6 +107,14 @@ def lib():
     h.grad4_set_bandpass.restype = c_int
     h.grad4_set_fbt_mute.argtypes = [c_int, c_float, c_float, c_int]
     h.grad4_set_fbt_mute.restype = c_int
+    h.rwi_plan_alloc.restype = ctypes.POINTER(RwiPlan)
+    h.rwi_plan_alloc.argtypes = [c_int] * 6 + [c_float] * 3
+    h.rwi_plan_free.argtypes = [ctypes.POINTER(RwiPlan)]
+    h.rwi_plan_free.restype = None
+    h.grad4_1shot_rfl.argtypes = [ctypes.POINTER(RwiPlan), c_int]
+    h.grad4_1shot_rfl.restype = None
+    h.grad_all_shots_rfl.argtypes = [ctypes.POINTER(ctypes.POINTER(RwiPlan)), c_int, c_int, PF]
+    h.grad_all_shots_rfl.restype = c_int
     return h
 
 
@@ -111,6 +138,26 @@ def make_plan(h, g, shot, obs0, vel, jump=1):
     assert p, "fwi_plan_alloc failed"
     P = p.contents
     view(P.model.contents.md[0], (g.nz, g.nx))[:] = vel
+    sou = P.sou.contents
+    ix = np.ctypeslib.as_array(ctypes.cast(sou.ixyz_sr, ctypes.POINTER(c_int * (2 + 2 * shot.nr))).contents)
+    ix[0], ix[1] = shot.isx, shot.isz
+    ix[2::2] = shot.irx
+    ix[3::2] = shot.irz
+    view(sou.wav0, (shot.nt,))[:] = shot.wav
+    msf = P.msf.contents
+    assert msf.nt == nt0 and msf.nr == shot.nr and msf.jump == jump
+    view(msf.obs0[0], (shot.nr, nt0))[:] = obs0
+    return p
+
+
+def make_rwi_plan(h, g, shot, obs0, m0, m1, jump=1):
+    """An RWI plan as rwi_init would leave it (fwi_2d.c:398-719), without files."""
+    nt0 = obs0.shape[1]
+    p = h.rwi_plan_alloc(g.nx, g.nz, g.npml, shot.nt, jump, shot.nr, g.dx, g.dz, g.dt)
+    assert p, "rwi_plan_alloc failed"
+    P = p.contents
+    view(P.model.contents.m0[0], (g.nz, g.nx))[:] = m0
+    view(P.model.contents.m1[0], (g.nz, g.nx))[:] = m1
     sou = P.sou.contents
     ix = np.ctypeslib.as_array(ctypes.cast(sou.ixyz_sr, ctypes.POINTER(c_int * (2 + 2 * shot.nr))).contents)
     ix[0], ix[1] = shot.isx, shot.isz
