@@ -39,8 +39,10 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--shots", type=int, default=64)
-    ap.add_argument("--nt", type=int, default=4000)
+    ap.add_argument("--shots", type=int, default=None, help="shots (default: 64; 16 with --config 4; 96 uneven)")
+    ap.add_argument("--nt", type=int, default=None, help="time steps (default: 4000; 10000 with --config 4)")
+    ap.add_argument("--config", type=int, default=2, choices=(2, 4),
+                    help="batched workload: 2 = 1024x320 (the headline), 4 = 4096x1024, dx=dz=5 m, dt=0.4 ms")
     ap.add_argument("--tile-z", type=int, default=0)
     ap.add_argument("--group", type=int, default=0, help="shots per workgroup of the z-marching kernels")
     ap.add_argument("--kernels", type=int, default=0, help="0: 2-D tile kernels, 1: z-marching kernels")
@@ -68,7 +70,12 @@ def parse():
     ap.add_argument("--cpu-nt", type=int, default=1500, help="time steps of the CPU sample shot")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (HBM counter calibration for rocprofv3 --pmc)")
-    return ap.parse_args()
+    args = ap.parse_args()
+    if args.shots is None:
+        args.shots = 96 if args.workload == "uneven" else (16 if args.config == 4 else 64)
+    if args.nt is None:
+        args.nt = 10000 if args.config == 4 else 4000
+    return args
 
 
 def algorithmic_bytes(g, nshots):
@@ -132,7 +139,7 @@ def run_uneven(args):
     from fwihip import lib
     from fwihip.workloads import config3, triangle_smooth
 
-    g, vel, shots = config3(nshots=args.shots if args.shots != 64 else 96)
+    g, vel, shots = config3(nshots=args.shots)
     start = triangle_smooth(vel, 25)
     nr = max(s.nr for s in shots)
     ntmax = max(s.nt for s in shots)
@@ -401,9 +408,12 @@ def main():
         return
     import numpy as np
     from fwihip import lib
-    from fwihip.workloads import config2, triangle_smooth
+    from fwihip.workloads import config2, config4, triangle_smooth
 
-    g, vel, shots = config2(nshots=args.shots, nt=args.nt)
+    if args.config == 4:
+        g, vel, shots = config4(nshots=args.shots, nt=args.nt)
+    else:
+        g, vel, shots = config2(nshots=args.shots, nt=args.nt)
     start = triangle_smooth(vel, 25)
     nr = max(s.nr for s in shots)
     launch = None
@@ -485,7 +495,7 @@ def main():
     ctx.set_timing(0)
     dom = max(names, key=lambda k: kern[k]["avg_us"] * kern[k]["launches"])
     traffic = None
-    if os.path.exists(PROFILE_TRAFFIC):
+    if args.config == 2 and args.shots == 64 and os.path.exists(PROFILE_TRAFFIC):
         try:
             traffic = json.load(open(PROFILE_TRAFFIC)).get(dom)
         except Exception:
@@ -526,7 +536,7 @@ def main():
                           "scaled to the full-nt shot-gradient")
 
     out = {
-        "metric": "shot-gradients/s & Gcell-updates/s at 1024x320, nt=4000; % HBM peak",
+        "metric": f"shot-gradients/s & Gcell-updates/s at {g.nx}x{g.nz}, nt={args.nt}; % HBM peak",
         "value": sg_per_s,
         "unit": "shot-gradients/s",
         "n_gpus": 1,

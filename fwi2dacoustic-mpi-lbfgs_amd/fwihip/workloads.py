@@ -187,6 +187,16 @@ def config2(nshots=64, nt=4000, seed=2026):
     return g, vel, shots
 
 
+def config4(nshots=16, nt=10000, seed=2026):
+    """Large grid: 4096x1024, dx=dz=5 m, dt=0.4 ms, nt=10000, npml=20, 16
+    shots, 8-12 Hz Ricker, 4096 receivers at every column, on the seeded
+    layered/faulted model (4,317,984 propagation cells per shot)."""
+    g = Geometry(nx=4096, nz=1024, npml=20, dx=5.0, dz=5.0, dt=0.4e-3)
+    vel = layered_faulted(g, seed)
+    shots = surface_shots(g, nshots, nt, seed=seed + 1)
+    return g, vel, shots
+
+
 def config3(nshots=96, nt_range=(2000, 6000), seed=2026):
     """Uneven batch: 1536x400, dx=dz=10 m, dt=0.8 ms, npml=20, 96 shots with
     per-shot nt drawn uniformly from nt_range, receiver apertures of 256-1536
