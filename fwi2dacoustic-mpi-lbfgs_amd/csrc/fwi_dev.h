@@ -72,6 +72,39 @@ __host__ __device__ inline bool fwi_fwd_strip_interior(const FwiGeomDev &g, cons
     if (sh.rz_lo <= sh.rz_hi && sh.rz_lo < r1 && sh.rz_hi >= r0) return false;
     return true;
 }
+// Special rules a general strip can meet.  Each general body is compiled for
+// a mask of them (the rest of its code is left out); a strip runs in the
+// smallest compiled mask holding its class.  The conditions are those the
+// bodies test at run time (fwd_tile_rows / adj_tile_rows).
+#define FWI_CLS_X 1    // x CPML strips or grid edges within the x stencil
+#define FWI_CLS_Z 2    // free-surface rows, bottom CPML rows or the last rows
+#define FWI_CLS_RS 4   // receiver rows or the source cell
+#define FWI_CLS_FR 8   // frame rows or columns (forward)
+#define FWI_CLS_ALL 15
+__host__ __device__ inline int fwi_fwd_strip_class(const FwiGeomDev &g, const FwiShotDev &sh, int x0, int r0)
+{
+    const int npml = g.npml, FR = FWI_STRIP;
+    int c = 0;
+    if (!((x0 >= npml + 3) && (x0 + 66 <= g.pnx - npml - 1))) c |= FWI_CLS_X;
+    if (!(r0 >= 3 && r0 + FR + 1 < g.pnz - npml)) c |= FWI_CLS_Z;
+    if ((sh.rz_lo <= sh.rz_hi && sh.rz_lo < r0 + FR && sh.rz_hi >= r0) ||
+        (sh.isz >= r0 && sh.isz < r0 + FR && sh.isx >= x0 && sh.isx < x0 + FWI_TX))
+        c |= FWI_CLS_RS;
+    if ((r0 < 3 || (r0 + FR > g.nz - 3 && r0 < g.nz)) ||
+        ((x0 < npml + 3 && x0 + FWI_TX > npml) || (x0 < npml + g.nx && x0 + FWI_TX > npml + g.nx - 3)))
+        c |= FWI_CLS_FR;
+    return c;
+}
+__host__ __device__ inline int fwi_adj_strip_class(const FwiGeomDev &g, const FwiShotDev &sh, int x0, int r0)
+{
+    const int npml = g.npml, FR = FWI_STRIP;
+    int c = 0;
+    if (!((x0 >= npml + 3) && (x0 + 66 <= g.pnx - npml - 1))) c |= FWI_CLS_X;
+    if (!(r0 >= 6 && r0 + FR + 2 < g.pnz - npml)) c |= FWI_CLS_Z;
+    if (sh.rz_lo <= sh.rz_hi && sh.rz_lo < r0 + FR && sh.rz_hi >= r0) c |= FWI_CLS_RS;
+    return c;
+}
+
 // adjoint strip: no source term; the transposed free-surface rows reach node 4
 __host__ __device__ inline bool fwi_adj_strip_interior(const FwiGeomDev &g, const FwiShotDev &sh, int x0, int r0)
 {

@@ -630,7 +630,7 @@ __device__ __forceinline__ XCoef fwd_ux_coef(const FwiGeomDev &g, const float *t
     return c;
 }
 
-template <bool EDGE, int MODE>
+template <bool EDGE, int MODE, int CLS = FWI_CLS_ALL>
 __device__ __forceinline__ void fwd_tile_rows(const FwiFwdArgs &A, const FwiShotDev &sh, const float *tab,
                                               float (*Ps)[HC], float (*Us)[HC], int s, int k, int x0, int r0,
                                               int lane)
@@ -681,7 +681,8 @@ __device__ __forceinline__ void fwd_tile_rows(const FwiFwdArgs &A, const FwiShot
     const float *zd_o = A.zd + s * A.b.zd_s + cur * A.b.zd_b;
     float *zd_n = A.zd + s * A.b.zd_s + nxt * A.b.zd_b;
     // u nodes x0-1 .. x0+65 and p nodes x0 .. x0+63 clear of the x CPML and edges
-    const bool fastx = !EDGE || ((x0 >= npml + 3) && (x0 + 66 <= pnx - npml - 1));
+    // (each special rule is compiled in only when the body's mask CLS allows it)
+    const bool fastx = !EDGE || !(CLS & FWI_CLS_X) || ((x0 >= npml + 3) && (x0 + 66 <= pnx - npml - 1));
     const int ud0 = lane / 3;  // the extra-u-node pass: lane = 3 ud + (uc - 64)
     bool uok = lane < 3 * FR;
     if (EDGE) uok = uok && r0 + ud0 < pnz;
@@ -698,12 +699,13 @@ __device__ __forceinline__ void fwd_tile_rows(const FwiFwdArgs &A, const FwiShot
     const int n1 = npml + 1;
     // which special rules can occur in this strip at all (uniform): each is
     // tested once here instead of per row and node
-    const bool zplain = !EDGE || (r0 >= 3 && r0 + FR + 1 < pnz - npml);  // v nodes and vz rows: plain
-    const bool has_recv = EDGE && sh.rz_lo <= sh.rz_hi && sh.rz_lo < r0 + FR && sh.rz_hi >= r0;
-    const bool has_src = EDGE && sh.isz >= r0 && sh.isz < r0 + FR && sh.isx >= x0 && sh.isx < x0 + TX;
-    const bool frame_rows = EDGE && (r0 < 3 || (r0 + FR > g.nz - 3 && r0 < g.nz));
-    const bool frame_cols = EDGE && ((x0 < npml + 3 && x0 + TX > npml) ||
-                                     (x0 < npml + g.nx && x0 + TX > npml + g.nx - 3));
+    const bool zplain = !EDGE || !(CLS & FWI_CLS_Z) || (r0 >= 3 && r0 + FR + 1 < pnz - npml);  // plain v, vz
+    const bool has_recv = EDGE && (CLS & FWI_CLS_RS) && sh.rz_lo <= sh.rz_hi && sh.rz_lo < r0 + FR && sh.rz_hi >= r0;
+    const bool has_src =
+        EDGE && (CLS & FWI_CLS_RS) && sh.isz >= r0 && sh.isz < r0 + FR && sh.isx >= x0 && sh.isx < x0 + TX;
+    const bool frame_rows = EDGE && (CLS & FWI_CLS_FR) && (r0 < 3 || (r0 + FR > g.nz - 3 && r0 < g.nz));
+    const bool frame_cols = EDGE && (CLS & FWI_CLS_FR) &&
+                            ((x0 < npml + 3 && x0 + TX > npml) || (x0 < npml + g.nx && x0 + TX > npml + g.nx - 3));
     if (EDGE && !fastx) {
         cu = fwd_u_coef(g, tab, x0 - 1 + lane);
         cuh = fwd_u_coef(g, tab, x0 - 1 + uc);
@@ -881,7 +883,10 @@ __device__ __forceinline__ void fwd_tile_rows(const FwiFwdArgs &A, const FwiShot
 // latency per wave before its field loads can issue).  One instantiation
 // runs the interior strips, the other the general strips: each list holds the
 // logical strips (s nsz + sz) ntx + tx of its class (fwi_fwd_strip_interior),
-// four consecutive entries per workgroup, one per wave.
+// four consecutive entries per workgroup, one per wave.  A general strip whose
+// special rules are all on the x side or all on the z side (85 % of them at
+// 1024x320) runs the body compiled for just those rules: fewer instructions
+// than the body with every rule (-4 % on the general launch).
 template <bool EDGE, int MODE>
 __global__ __launch_bounds__(256) void k_fwd_tile(const FwiFwdArgs *__restrict__ Ap,
                                                   const FwiShotDev *__restrict__ shots,
@@ -906,7 +911,17 @@ __global__ __launch_bounds__(256) void k_fwd_tile(const FwiFwdArgs *__restrict__
     const int tx = l % g.ntx, rest = l / g.ntx, sz = rest % nsz, s = rest / nsz;
     const FwiShotDev sh = shots[s];
     if (k >= sh.nt) return;
-    fwd_tile_rows<EDGE, MODE>(A, sh, stab, Psh[w], Ush[w], s, k, tx * TX, sz * FR, lane);
+    if constexpr (EDGE) {
+        const int c = fwi_fwd_strip_class(g, sh, tx * TX, sz * FR);
+        if ((c & ~(FWI_CLS_X | FWI_CLS_FR)) == 0)
+            fwd_tile_rows<EDGE, MODE, FWI_CLS_X | FWI_CLS_FR>(A, sh, stab, Psh[w], Ush[w], s, k, tx * TX, sz * FR, lane);
+        else if ((c & ~(FWI_CLS_Z | FWI_CLS_FR)) == 0)
+            fwd_tile_rows<EDGE, MODE, FWI_CLS_Z | FWI_CLS_FR>(A, sh, stab, Psh[w], Ush[w], s, k, tx * TX, sz * FR, lane);
+        else
+            fwd_tile_rows<EDGE, MODE>(A, sh, stab, Psh[w], Ush[w], s, k, tx * TX, sz * FR, lane);
+    } else {
+        fwd_tile_rows<EDGE, MODE>(A, sh, stab, Psh[w], Ush[w], s, k, tx * TX, sz * FR, lane);
+    }
 }
 
 // ============================================================================
@@ -1306,7 +1321,7 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_adj_step(FwiAdjArgs A, c
 // FUSED: ekp of the rows comes from the caller (ek) and mu^{k+1} of the
 // rows is also left in keep (rows past the grid untouched), for the fused
 // backward step.
-template <bool EDGE, bool FUSED = false>
+template <bool EDGE, bool FUSED = false, int CLS = FWI_CLS_ALL>
 __device__ __forceinline__ void adj_tile_rows(const FwiAdjArgs &A, const FwiShotDev &sh, const float *tab,
                                               float (*Ws)[HC], float (*Us)[HC], int s, int k, int x0, int r0,
                                               int lane, const float *ek = nullptr, float *keep = nullptr)
@@ -1362,7 +1377,8 @@ __device__ __forceinline__ void adj_tile_rows(const FwiAdjArgs &A, const FwiShot
 
     // columns x0-3 .. x0+66 clear of the x CPML and of the grid edges: compact
     // x stencils (the form k_adj_step uses on such tiles)
-    const bool fastx = !EDGE || ((x0 >= npml + 3) && (x0 + 66 <= pnx - npml - 1));
+    // (each special rule is compiled in only when the body's mask CLS allows it)
+    const bool fastx = !EDGE || !(CLS & FWI_CLS_X) || ((x0 >= npml + 3) && (x0 + 66 <= pnx - npml - 1));
     const int ud0 = lane / 3;  // the extra-u-node pass: lane = 3 ud + (uc - 64)
     bool uok = lane < 3 * FR;
     if (EDGE) uok = uok && r0 + ud0 < pnz;
@@ -1380,8 +1396,8 @@ __device__ __forceinline__ void adj_tile_rows(const FwiAdjArgs &A, const FwiShot
     float gE = 0.f, gF = 0.f, gG = 0.f, gH = 0.f;
     const int n1 = npml + 1;
     // which special rules can occur in this strip at all (uniform)
-    const bool zplain = !EDGE || (r0 >= 6 && r0 + FR + 2 < pnz - npml);  // z chain: plain transposed stencils
-    const bool has_recv = EDGE && sh.rz_lo <= sh.rz_hi && sh.rz_lo < r0 + FR && sh.rz_hi >= r0;
+    const bool zplain = !EDGE || !(CLS & FWI_CLS_Z) || (r0 >= 6 && r0 + FR + 2 < pnz - npml);  // plain z chain
+    const bool has_recv = EDGE && (CLS & FWI_CLS_RS) && sh.rz_lo <= sh.rz_hi && sh.rz_lo < r0 + FR && sh.rz_hi >= r0;
     if (EDGE && !fastx) {
         cf = fwd_ux_coef(g, tab, x);
 #pragma unroll
@@ -1609,7 +1625,17 @@ __global__ __launch_bounds__(256) void k_adj_tile(const FwiAdjArgs *__restrict__
     const int tx = l % g.ntx, rest = l / g.ntx, sz = rest % nsz, s = rest / nsz;
     const FwiShotDev sh = shots[s];
     if (k >= sh.nt) return;
-    adj_tile_rows<EDGE>(A, sh, stab, Wsh[w], Ush[w], s, k, tx * TX, sz * FR, lane);
+    if constexpr (EDGE) {  // the body compiled for the strip's rules, as in k_fwd_tile
+        const int c = fwi_adj_strip_class(g, sh, tx * TX, sz * FR);
+        if ((c & ~FWI_CLS_X) == 0)
+            adj_tile_rows<EDGE, false, FWI_CLS_X>(A, sh, stab, Wsh[w], Ush[w], s, k, tx * TX, sz * FR, lane);
+        else if ((c & ~FWI_CLS_Z) == 0)
+            adj_tile_rows<EDGE, false, FWI_CLS_Z>(A, sh, stab, Wsh[w], Ush[w], s, k, tx * TX, sz * FR, lane);
+        else
+            adj_tile_rows<EDGE>(A, sh, stab, Wsh[w], Ush[w], s, k, tx * TX, sz * FR, lane);
+    } else {
+        adj_tile_rows<EDGE>(A, sh, stab, Wsh[w], Ush[w], s, k, tx * TX, sz * FR, lane);
+    }
 }
 
 // ============================================================================
