@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--bwd-overlap", type=int, default=-1, help="adjoint beside imaging on two streams (1/0; -1: default)")
     ap.add_argument("--concurrent", type=int, default=-1,
                     help="general strips on a second stream beside the interior launch (1/0; -1: default)")
+    ap.add_argument("--interior-wgs", type=int, default=-1,
+                    help="interior workgroups per CU beside the general strips (0: no cap; -1: default 3)")
     ap.add_argument("--bwd-fused", type=int, default=-1,
                     help="one fused adjoint + reconstruction + imaging kernel per backward step (1/0; -1: default 1)")
     ap.add_argument("--ckpt-every", type=int, default=100, help="snapshot interval of --workload checkpoint")
@@ -438,6 +440,8 @@ def main():
             ctx.set_bwd_fused(args.bwd_fused)
         if args.concurrent >= 0:
             ctx.set_concurrent(args.concurrent)
+        if args.interior_wgs >= 0:
+            ctx.set_interior_wgs(args.interior_wgs)
     ctx.set_model(vel)
     ctx.set_shots(shots)
     obs = ctx.forward()

@@ -186,7 +186,12 @@ __host__ __device__ inline bool fwi_image_step(int jump, int k0, int k1, int k)
 struct FwiPair {
     hipStream_t st, st2;
     hipEvent_t fork, join;
+    int lds_int;  // dynamic LDS bytes reserved per interior workgroup (occupancy cap)
 };
+// LDS of one workgroup of the tile kernels: interior (wave rows) and general
+// (wave rows + CPML tables)
+#define FWI_LDS_TILE (4 * 2 * FWI_STRIP * 72 * 4)
+#define FWI_LDS_GEN (FWI_LDS_TILE + 6 * (2 * 64 + 1) * 4)
 
 // per-step kernels with the reference's loop structure (unit parity)
 struct FwiRefArgs {

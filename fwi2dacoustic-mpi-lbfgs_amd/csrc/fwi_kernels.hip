@@ -2549,7 +2549,7 @@ static hipError_t launch_fwd_tile_mode(const FwiFwdArgs *a, const FwiFwdArgs *a_
     return launch_pair(
         p, ntl[0] > 0, ntl[1] > 0,
         [&](hipStream_t st) {
-            hipLaunchKernelGGL((k_fwd_tile<false, MODE>), dim3(((ntl[0] + 31) / 32) * 8), dim3(256), 0, st, a_dev,
+            hipLaunchKernelGGL((k_fwd_tile<false, MODE>), dim3(((ntl[0] + 31) / 32) * 8), dim3(256), p.lds_int, st, a_dev,
                                a->b.shots, tl[0], ntl[0], kbase, koff);
         },
         [&](hipStream_t st) {
@@ -2619,7 +2619,7 @@ hipError_t fwi_launch_bwd_tile(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, con
     return launch_pair(
         p, ntl[0] > 0, has_gen,
         [&](hipStream_t st) {
-            hipLaunchKernelGGL(k_bwd_tile_int, dim3(((ntl[0] + 7) / 8) * 8), dim3(256), 0, st, a_dev, r_dev,
+            hipLaunchKernelGGL(k_bwd_tile_int, dim3(((ntl[0] + 7) / 8) * 8), dim3(256), p.lds_int, st, a_dev, r_dev,
                                a->b.shots, tl[0], ntl[0], kbase, koff, gsize);
         },
         [&](hipStream_t st) {

@@ -8,6 +8,7 @@ forward + adjoint passes of each candidate setting and keeps the fastest:
     group        shots per workgroup of the z-marching kernels
     rec_group    shots per workgroup of the tile imaging kernel
     bwd_chunk    shots per backward pass (0: all in one pass)
+    interior_wgs interior workgroups per CU beside the general strips (0: no cap)
     graph        time steps per captured HIP graph block
 
 The probe runs the workload's own shots truncated to a few hundred steps, so
@@ -23,7 +24,7 @@ import numpy as np
 from . import lib as L
 from .workloads import Shot
 
-DEFAULT = dict(kernels=0, tile_z=0, group=0, rec_group=8, bwd_chunk=0, graph=100)
+DEFAULT = dict(kernels=0, tile_z=0, group=0, rec_group=8, bwd_chunk=0, interior_wgs=3, graph=100)
 
 
 def candidates(nshots):
@@ -33,11 +34,12 @@ def candidates(nshots):
     chunks = sorted({0} | {c for c in (16, 32) if c < nshots})
     for rg in (4, 8, 16):
         for bc in chunks:
-            for gs in (50, 100):
-                out.append(dict(kernels=0, tile_z=0, group=0, rec_group=rg, bwd_chunk=bc, graph=gs))
+            for iw in (0, 3):
+                for gs in (50, 100):
+                    out.append(dict(kernels=0, tile_z=0, group=0, rec_group=rg, bwd_chunk=bc, interior_wgs=iw, graph=gs))
     for tz in (16, 32):
         for sg in (2, 4):
-            out.append(dict(kernels=1, tile_z=tz, group=sg, rec_group=8, bwd_chunk=0, graph=100))
+            out.append(dict(kernels=1, tile_z=tz, group=sg, rec_group=8, bwd_chunk=0, interior_wgs=0, graph=100))
     return out
 
 
@@ -58,6 +60,7 @@ def apply(ctx, params):
     ctx.set_kernels(params["kernels"])
     ctx.set_rec_group(params["rec_group"])
     ctx.set_bwd_chunk(params["bwd_chunk"])
+    ctx.set_interior_wgs(params.get("interior_wgs", DEFAULT["interior_wgs"]))
     ctx.set_graph(params["graph"])
 
 

@@ -138,6 +138,12 @@ int fwi_hip_set_checkpointing(fwi_hip_ctx *ctx, int every, int storage);
  * Results are bit-identical.                                                */
 int fwi_hip_set_concurrent(fwi_hip_ctx *ctx, int on);
 
+/* With the general strips on the second stream: at most `wgs` workgroups of
+ * the interior launches (forward, fused backward) per CU, by reserving unused
+ * LDS in each, so that a general workgroup fits beside them (default 3; 0:
+ * no cap).  Results are bit-identical.                                      */
+int fwi_hip_set_interior_wgs(fwi_hip_ctx *ctx, int wgs);
+
 /* Backward step of the tile kernels fused into one kernel (on = 1, the
  * default): per shot group and 64 x 8 tile of the padded grid, adjoint step
  * k then reconstruction + imaging step k, the imaging taking mu^{k+1} from

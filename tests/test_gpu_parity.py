@@ -292,8 +292,8 @@ def test_fused_backward_gives_the_same_image(lib, graph):
 @pytest.mark.parametrize("graph", [100, 0])
 def test_concurrent_general_strips_give_the_same_results(lib, graph):
     """General strips on a second stream beside the interior launch (forward
-    and both backward forms) give the one-stream traces and image bit for
-    bit, over graph replays."""
+    and both backward forms, any cap on the interior workgroups per CU) give
+    the one-stream traces and image bit for bit, over graph replays."""
     g, vel, shots = small_case(nshots=3, nt=230, seed=19)
     rng = np.random.default_rng(20)
     res = [rng.standard_normal((s.nr, s.nt)).astype(np.float32) for s in shots]
@@ -319,6 +319,12 @@ def test_concurrent_general_strips_give_the_same_results(lib, graph):
             for tr, gk in run(True, fused) + ref[1:]:
                 np.testing.assert_array_equal(tr, ref[0][0])
                 np.testing.assert_array_equal(gk, ref[0][1])
+            for wgs in (0, 1, 2):               # interior occupancy caps (unused LDS reserved)
+                ctx.set_interior_wgs(wgs)
+                for tr, gk in run(True, fused):
+                    np.testing.assert_array_equal(tr, ref[0][0])
+                    np.testing.assert_array_equal(gk, ref[0][1])
+            ctx.set_interior_wgs(3)
     finally:
         ctx.close()
 
