@@ -1550,6 +1550,24 @@ extern "C" int fwi_hip_dfree(void *ptr)
     return FWI_HIP_OK;
 }
 
+extern "C" int fwi_hip_host_alloc(void **ptr, size_t bytes)
+{
+    if (!ptr) return set_err(nullptr, FWI_HIP_EINVAL, "host_alloc: NULL");
+    *ptr = nullptr;
+    hipError_t e = hipHostMalloc(ptr, bytes ? bytes : 4, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        *ptr = nullptr;
+        return set_err(nullptr, FWI_HIP_ENOMEM, "hipHostMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    }
+    return FWI_HIP_OK;
+}
+
+extern "C" int fwi_hip_host_free(void *ptr)
+{
+    if (ptr) HIPCHK0(hipHostFree(ptr));
+    return FWI_HIP_OK;
+}
+
 extern "C" int fwi_hip_memcpy(void *dst, const void *src, size_t bytes, int kind)
 {
     if (!dst || !src || kind < 1 || kind > 3) return set_err(nullptr, FWI_HIP_EINVAL, "memcpy: bad argument");

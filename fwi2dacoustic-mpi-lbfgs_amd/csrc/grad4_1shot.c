@@ -45,6 +45,8 @@ typedef struct {
     fwi_hip_ctx *ctx;
     long key[8];
     int ck_applied; /* the checkpointing settings reached this context */
+    float *tr;      /* page-locked trace buffer of the slot, kept across calls */
+    size_t tr_cap;  /* its size in floats */
 } CtxSlot;
 static CtxSlot g_slot[2];
 static char g_msg[512] = "";
@@ -164,6 +166,28 @@ static int fail(int code, const char *fmt, ...)
 
 const char *grad4_last_error(void) { return g_msg; }
 
+/* the page-locked trace buffer of slot `which`, at least n floats: the
+ * forward's traces come back and the adjoint source goes out through it at
+ * the pinned-copy rate, and its pages are touched once per process */
+static float *slot_traces(int which, size_t n)
+{
+    CtxSlot *sl = &g_slot[which];
+    if (n < 1) n = 1;
+    if (sl->tr_cap < n) {
+        if (sl->tr) fwi_hip_host_free(sl->tr);
+        sl->tr = NULL;
+        sl->tr_cap = 0;
+        void *p = NULL;
+        if (fwi_hip_host_alloc(&p, sizeof(float) * n)) {
+            fail(FWI_HIP_ENOMEM, "pinned host memory for %zu trace samples: %s", n, fwi_hip_last_error(NULL));
+            return NULL;
+        }
+        sl->tr = (float *)p;
+        sl->tr_cap = n;
+    }
+    return sl->tr;
+}
+
 /* the context of slot `which` for this geometry and capacity, with the
  * checkpointing settings applied */
 static int ensure_ctx(int which, const fwi_base *b, int nshots, int nt, int nr)
@@ -212,15 +236,40 @@ static float **slab_rows(float *a, int n1, int n2)
     return r;
 }
 
-/* The host misfit step of one shot.  t holds the shot's nr traces at the
- * propagation rate (nt samples each) and receives the adjoint source. */
-static int host_misfit(fwi_plan *pl, float *t, int nt, float dt_fine)
+/* per-thread working traces of the host misfit step (observed, synthetic,
+ * adjoint source), kept across shots and calls so that their pages are
+ * touched once per process, not once per shot */
+#define MISFIT_MAX_THREADS 64
+static struct {
+    float *ob, *sy, *rs;
+    size_t cap;
+} g_scr[MISFIT_MAX_THREADS];
+
+static int misfit_scratch(int tid, size_t n)
+{
+    if (n < 1) n = 1;
+    if (g_scr[tid].cap >= n) return FWI_HIP_OK;
+    free(g_scr[tid].ob);
+    free(g_scr[tid].sy);
+    free(g_scr[tid].rs);
+    g_scr[tid].ob = (float *)malloc(sizeof(float) * n);
+    g_scr[tid].sy = (float *)malloc(sizeof(float) * n);
+    g_scr[tid].rs = (float *)malloc(sizeof(float) * n);
+    g_scr[tid].cap = (g_scr[tid].ob && g_scr[tid].sy && g_scr[tid].rs) ? n : 0;
+    return g_scr[tid].cap ? FWI_HIP_OK : fail(FWI_HIP_ENOMEM, "host memory");
+}
+
+/* The host misfit step of one shot, on thread tid's working traces.  t holds
+ * the shot's nr traces at the propagation rate (nt samples each) and
+ * receives the adjoint source. */
+static int host_misfit(fwi_plan *pl, float *t, int nt, float dt_fine, int tid)
 {
     fwi_msf *m = pl->msf;
     const int nr = (int)m->nr, nt0 = (int)m->nt, jump = (int)m->jump;
     const size_t n = (size_t)nr * nt0;
-    float *ob = (float *)malloc(sizeof(float) * (n ? n : 1)), *sy = (float *)malloc(sizeof(float) * (n ? n : 1));
-    float *rs = (float *)calloc(n ? n : 1, sizeof(float));
+    if (misfit_scratch(tid, n)) return FWI_HIP_ENOMEM;
+    float *ob = g_scr[tid].ob, *sy = g_scr[tid].sy, *rs = g_scr[tid].rs;
+    memset(rs, 0, sizeof(float) * (n ? n : 1));
     float **obr = slab_rows(ob, nt0, nr), **syr = slab_rows(sy, nt0, nr), **rsr = slab_rows(rs, nt0, nr);
     unsigned char *drop = NULL; /* traces outside the offset window */
     int rc = FWI_HIP_OK;
@@ -348,9 +397,6 @@ static int host_misfit(fwi_plan *pl, float *t, int nt, float dt_fine)
     }
 done:
     free(drop);
-    free(ob);
-    free(sy);
-    free(rs);
     free(obr);
     free(syr);
     free(rsr);
@@ -371,7 +417,7 @@ static void *misfit_worker(void *p)
 {
     MisfitJob *j = (MisfitJob *)p;
     for (int s = j->t; s < j->nshots && !j->rc; s += j->nthreads)
-        j->rc = host_misfit(j->plans[s], j->tr + j->off[s], j->nt, j->dt);
+        j->rc = host_misfit(j->plans[s], j->tr + j->off[s], j->nt, j->dt, j->t);
     return NULL;
 }
 
@@ -390,10 +436,10 @@ static int host_misfit_all(fwi_plan **plans, int nshots, float *tr, int nt, floa
     }
     if (nth > nshots) nth = nshots;
     if (nth < 1) nth = 1;
-    if (nth > 64) nth = 64;
-    MisfitJob jobs[64];
-    pthread_t th[64];
-    int made[64];
+    if (nth > MISFIT_MAX_THREADS) nth = MISFIT_MAX_THREADS;
+    MisfitJob jobs[MISFIT_MAX_THREADS];
+    pthread_t th[MISFIT_MAX_THREADS];
+    int made[MISFIT_MAX_THREADS];
     int rc = FWI_HIP_OK;
     for (int t = 0; t < nth; t++) {
         jobs[t] = (MisfitJob){plans, tr, off, nshots, nt, nth, t, FWI_HIP_OK, dt};
@@ -524,9 +570,9 @@ int grad_all_shots(fwi_plan **plans, int nshots, int k)
     fwi_hip_ctx *ctx = g_slot[0].ctx;
     if (!rc) rc = ctx_setup(ctx, &plans[0]->model->md[0][0], plans[0]->grad, b, &t);
     if (!rc) {
-        tr = (float *)malloc(sizeof(float) * (t.ntr ? t.ntr : 1));
         grad = (double *)malloc(sizeof(double) * nx * nz);
-        if (!tr || !grad) rc = fail(FWI_HIP_ENOMEM, "host memory");
+        if (!grad) rc = fail(FWI_HIP_ENOMEM, "host memory");
+        else if (!(tr = slot_traces(0, (size_t)t.ntr))) rc = FWI_HIP_ENOMEM;
     }
     if (!rc && (rc = fwi_hip_forward(ctx, tr))) rc = fail(rc, "forward: %s", fwi_hip_last_error(ctx));
     if (!rc) rc = host_misfit_all(plans, nshots, tr, nt, b->dt);
@@ -538,7 +584,6 @@ int grad_all_shots(fwi_plan **plans, int nshots, int k)
     free(grids);
     free(msfs);
     free(sous);
-    free(tr);
     free(grad);
     return rc;
 }
@@ -588,8 +633,8 @@ int grad_all_shots_rfl(rwi_plan **plans, int nshots, int k, float *grad_m1)
     if (!rc) rc = ensure_ctx(1, b, nshots, nt, nrmax);
     if (!rc) rc = shot_table(&t, sous, msfs, nshots, nt);
     if (!rc) {
-        tr0 = (float *)malloc(sizeof(float) * (t.ntr ? t.ntr : 1));
-        tr1 = (float *)malloc(sizeof(float) * (t.ntr ? t.ntr : 1));
+        tr0 = slot_traces(0, (size_t)t.ntr);
+        tr1 = slot_traces(1, (size_t)t.ntr);
         vtot = (float *)malloc(sizeof(float) * nb);
         g0 = (double *)malloc(sizeof(double) * nb);
         g1 = (double *)malloc(sizeof(double) * nb);
@@ -626,8 +671,6 @@ int grad_all_shots_rfl(rwi_plan **plans, int nshots, int k, float *grad_m1)
     free(sous);
     free(shell);
     free(shells);
-    free(tr0);
-    free(tr1);
     free(vtot);
     free(g0);
     free(g1);

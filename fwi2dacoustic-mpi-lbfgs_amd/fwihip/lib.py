@@ -25,7 +25,7 @@ EXPORTS = (
     "fwi_hip_set_residual", "fwi_hip_adjoint", "fwi_hip_synchronize", "fwi_hip_set_timing",
     "fwi_hip_kernel_time", "fwi_hip_read_state", "fwi_hip_op_pml", "fwi_hip_op_interior",
     "fwi_hip_time_integral", "fwi_hip_write_bdr", "fwi_hip_read_bdr", "fwi_hip_cpml_coe",
-    "fwi_hip_set_graph", "fwi_hip_set_kernels", "fwi_hip_set_rec_group", "fwi_hip_set_imaging", "fwi_hip_set_bwd_chunk", "fwi_hip_set_checkpointing", "fwi_hip_set_bwd_overlap", "fwi_hip_set_bwd_fused", "fwi_hip_set_concurrent", "fwi_hip_set_interior_wgs", "fwi_hip_device_count", "fwi_hip_dmalloc", "fwi_hip_dfree", "fwi_hip_memcpy",
+    "fwi_hip_set_graph", "fwi_hip_set_kernels", "fwi_hip_set_rec_group", "fwi_hip_set_imaging", "fwi_hip_set_bwd_chunk", "fwi_hip_set_checkpointing", "fwi_hip_set_bwd_overlap", "fwi_hip_set_bwd_fused", "fwi_hip_set_concurrent", "fwi_hip_set_interior_wgs", "fwi_hip_device_count", "fwi_hip_dmalloc", "fwi_hip_dfree", "fwi_hip_host_alloc", "fwi_hip_host_free", "fwi_hip_memcpy",
 )
 
 

@@ -199,6 +199,11 @@ int fwi_hip_read_state(fwi_hip_ctx *ctx, int what, int shot, float *out);
 int fwi_hip_device_count(int *n);
 int fwi_hip_dmalloc(void **ptr, size_t bytes);
 int fwi_hip_dfree(void *ptr);
+/* Page-locked host memory (no counterpart in the reference): traces passed to
+ * fwi_hip_forward / fwi_hip_set_residual from such a buffer cross PCIe at the
+ * pinned-copy rate, without staging.  Freed with fwi_hip_host_free.       */
+int fwi_hip_host_alloc(void **ptr, size_t bytes);
+int fwi_hip_host_free(void *ptr);
 int fwi_hip_memcpy(void *dst, const void *src, size_t bytes, int kind);
 
 /* ---- per-step entry points on caller-owned device buffers (dense rows) --
