@@ -306,11 +306,18 @@ void orc_read_bdr(const real *ud, const real *lr, real *pb, int nx, int nz)
  * (u/v nodes -> p nodes) and C_d the recursion on the p nodes.  Its
  * transpose over space and time is D1^T C_h^T D2^T C_d^T, where C^T is the
  * same recursion run backward in time (chi = b chi + a x; y = x + chi).
- * D1^T and D2^T are formed by scattering every row of the forward
- * definitions above, so they are the transposes by construction.
+ * D1^T and D2^T gather, per node, the terms every row of the forward
+ * definitions above scatters to it, in the order a row-by-row scatter
+ * would add them, so they are the transposes by construction.
  * w[pnz][pnx] in; out[pnz][pnx] out; chi_* are the backward CPML states
  * (same shapes as the forward psi); tu[pnz][pnx+1], tv[pnz+1][pnx],
  * tw[pnz][pnx] are work arrays.                                          */
+/* memcpy of an n-element field split over the threads */
+static void par_copy(real *dst, const real *src, long n)
+{
+#pragma omp parallel for
+    for (long b = 0; b < n; b += 4096) memcpy(dst + b, src + b, sizeof(real) * (n - b < 4096 ? n - b : 4096));
+}
 void orc_op_pml_adj(const real *w, real *out, const real *coe, const real *tab,
                     int npml, int pnx, int pnz,
                     real *cu0, real *cu1, real *cv1, real *cux0, real *cux1, real *cvz1,
@@ -321,9 +328,12 @@ void orc_op_pml_adj(const real *w, real *out, const real *coe, const real *tab,
     const real iz840 = coe[6], iz24 = coe[7];
     const int n1 = npml + 1;
 
-    /* ---------------- x chain ---------------- */
-    memcpy(tw, w, sizeof(real) * pnx * pnz);
+    /* ---------------- x chain ----------------
+     * Every node is written by one thread with its terms in a fixed order,
+     * so the result does not depend on the thread count.                 */
+    par_copy(tw, w, (long)pnx * pnz);
     /* C_d^T on the ux nodes (transpose of fwi_2d.c:834-843) */
+#pragma omp parallel for
     for (int iz = 0; iz < pnz; iz++)
         for (int ix = 0; ix < npml; ix++) {
             cux0[iz * npml + ix] = TAB(1, 2 * ix + 1) * cux0[iz * npml + ix] + TAB(0, 2 * ix + 1) * tw[iz * pnx + ix];
@@ -331,23 +341,38 @@ void orc_op_pml_adj(const real *w, real *out, const real *coe, const real *tab,
             cux1[iz * npml + ix] = TAB(3, 2 * ix + 1) * cux1[iz * npml + ix] + TAB(2, 2 * ix + 1) * tw[iz * pnx + ix + pnx - npml];
             tw[iz * pnx + ix + pnx - npml] += cux1[iz * npml + ix];
         }
-    /* D2^T: scatter of ux rows (fwi_2d.c:821-832) */
-    memset(tu, 0, sizeof(real) * unx * pnz);
+    /* D2^T of the ux rows (fwi_2d.c:821-832).  Row ix of D2 writes
+     * ux[ix] from u[ix-1..ix+2]; its transpose, gathered per u node j, adds
+     * the terms of rows j-2, j-1, j, j+1 in that order, then the 2nd-order
+     * edge rows 0 and pnx-1.                                             */
+#pragma omp parallel for
     for (int iz = 0; iz < pnz; iz++) {
         real *t = tu + iz * unx;
         const real *g = tw + iz * pnx;
-        for (int ix = 1; ix < pnx - 1; ix++) {
-            t[ix + 1] += c0x * g[ix];
-            t[ix] -= c0x * g[ix];
-            t[ix + 2] += c1x * g[ix];
-            t[ix - 1] -= c1x * g[ix];
+        for (int j = 3; j <= pnx - 3; j++) {
+            real acc = 0;
+            acc += c1x * g[j - 2];
+            acc += c0x * g[j - 1];
+            acc -= c0x * g[j];
+            acc -= c1x * g[j + 1];
+            t[j] = acc;
         }
-        t[1] += ix_ * g[0];
-        t[0] -= ix_ * g[0];
-        t[unx - 1] += ix_ * g[pnx - 1];
-        t[unx - 2] -= ix_ * g[pnx - 1];
+        for (int j = 0; j < unx; j++) {
+            if (j == 3) j = pnx - 2; /* the nodes above */
+            real acc = 0;
+            if (j - 2 >= 1 && j - 2 <= pnx - 2) acc += c1x * g[j - 2];
+            if (j - 1 >= 1 && j - 1 <= pnx - 2) acc += c0x * g[j - 1];
+            if (j >= 1 && j <= pnx - 2) acc -= c0x * g[j];
+            if (j + 1 >= 1 && j + 1 <= pnx - 2) acc -= c1x * g[j + 1];
+            if (j == 1) acc += ix_ * g[0];
+            if (j == 0) acc -= ix_ * g[0];
+            if (j == unx - 1) acc += ix_ * g[pnx - 1];
+            if (j == unx - 2) acc -= ix_ * g[pnx - 1];
+            t[j] = acc;
+        }
     }
     /* C_h^T on the u nodes (transpose of fwi_2d.c:786-795) */
+#pragma omp parallel for
     for (int iz = 0; iz < pnz; iz++)
         for (int ix = 0; ix < npml + 1; ix++) {
             cu0[iz * n1 + ix] = TAB(1, 2 * ix) * cu0[iz * n1 + ix] + TAB(0, 2 * ix) * tu[iz * unx + ix];
@@ -355,77 +380,122 @@ void orc_op_pml_adj(const real *w, real *out, const real *coe, const real *tab,
             cu1[iz * n1 + ix] = TAB(3, 2 * ix) * cu1[iz * n1 + ix] + TAB(2, 2 * ix) * tu[iz * unx + ix + unx - npml - 1];
             tu[iz * unx + ix + unx - npml - 1] += cu1[iz * n1 + ix];
         }
-    /* D1^T: scatter of u rows (fwi_2d.c:775-784); rows 0 and unx-1 are zero */
-    memset(out, 0, sizeof(real) * pnx * pnz);
+    /* D1^T of the u rows (fwi_2d.c:775-784; u nodes 0 and unx-1 are zero):
+     * row ix of D1 writes u[ix] from p[ix-2..ix+1]; gathered per p node j,
+     * the terms of rows j-1, j, j+1, j+2 in that order, then the 2nd-order
+     * rows 1 and unx-2.                                                  */
+#pragma omp parallel for
     for (int iz = 0; iz < pnz; iz++) {
         real *o = out + iz * pnx;
         const real *t = tu + iz * unx;
-        for (int ix = 2; ix < unx - 2; ix++) {
-            o[ix] += c0x * t[ix];
-            o[ix - 1] -= c0x * t[ix];
-            o[ix + 1] += c1x * t[ix];
-            o[ix - 2] -= c1x * t[ix];
+        for (int j = 3; j <= pnx - 4; j++) {
+            real acc = 0;
+            acc += c1x * t[j - 1];
+            acc += c0x * t[j];
+            acc -= c0x * t[j + 1];
+            acc -= c1x * t[j + 2];
+            o[j] = acc;
         }
-        o[1] += ix_ * t[1];
-        o[0] -= ix_ * t[1];
-        o[pnx - 1] += ix_ * t[unx - 2];
-        o[pnx - 2] -= ix_ * t[unx - 2];
+        for (int j = 0; j < pnx; j++) {
+            if (j == 3) j = pnx - 3; /* the nodes above */
+            real acc = 0;
+            if (j - 1 >= 2 && j - 1 <= unx - 3) acc += c1x * t[j - 1];
+            if (j >= 2 && j <= unx - 3) acc += c0x * t[j];
+            if (j + 1 >= 2 && j + 1 <= unx - 3) acc -= c0x * t[j + 1];
+            if (j + 2 >= 2 && j + 2 <= unx - 3) acc -= c1x * t[j + 2];
+            if (j == 1) acc += ix_ * t[1];
+            if (j == 0) acc -= ix_ * t[1];
+            if (j == pnx - 1) acc += ix_ * t[unx - 2];
+            if (j == pnx - 2) acc -= ix_ * t[unx - 2];
+            o[j] = acc;
+        }
     }
 
     /* ---------------- z chain ---------------- */
-    memcpy(tw, w, sizeof(real) * pnx * pnz);
+    par_copy(tw, w, (long)pnx * pnz);
     /* C_d^T on the vz nodes (transpose of fwi_2d.c:857-862) */
+#pragma omp parallel for
     for (int iz = 0; iz < npml; iz++)
         for (int ix = 0; ix < pnx; ix++) {
             cvz1[iz * pnx + ix] = TAB(5, 2 * iz + 1) * cvz1[iz * pnx + ix] + TAB(4, 2 * iz + 1) * tw[(iz + pnz - npml) * pnx + ix];
             tw[(iz + pnz - npml) * pnx + ix] += cvz1[iz * pnx + ix];
         }
-    /* D2^T: scatter of vz rows (fwi_2d.c:846-855) */
-    memset(tv, 0, sizeof(real) * vnx * vnz);
-    for (int ix = 0; ix < pnx; ix++) {
-        const real g0 = tw[ix];
-        tv[0 * vnx + ix] += (-22. * iz24) * g0;
-        tv[1 * vnx + ix] += (17. * iz24) * g0;
-        tv[2 * vnx + ix] += (9. * iz24) * g0;
-        tv[3 * vnx + ix] += (-5. * iz24) * g0;
-        tv[4 * vnx + ix] += (1. * iz24) * g0;
-        const real gl = tw[(pnz - 1) * pnx + ix];
-        tv[(vnz - 1) * vnx + ix] += iz_ * gl;
-        tv[(vnz - 2) * vnx + ix] -= iz_ * gl;
-    }
-    for (int iz = 1; iz < pnz - 1; iz++)
-        for (int ix = 0; ix < pnx; ix++) {
-            const real g = tw[iz * pnx + ix];
-            tv[(iz + 1) * vnx + ix] += c0z * g;
-            tv[iz * vnx + ix] -= c0z * g;
-            tv[(iz + 2) * vnx + ix] += c1z * g;
-            tv[(iz - 1) * vnx + ix] -= c1z * g;
+    /* D2^T of the vz rows (fwi_2d.c:846-855): the free-surface row 0 reads
+     * v nodes 0..4 and the last row v nodes vnz-2, vnz-1; row iz in
+     * [1, pnz-2] reads v nodes iz-1..iz+2.  Gathered per v node r: the
+     * terms of rows 0 and pnz-1 first, then those of rows r-2, r-1, r, r+1. */
+    {
+        const double w0[5] = {-22., 17., 9., -5., 1.};
+#pragma omp parallel for
+        for (int r = 0; r < vnz; r++) {
+            real *t = tv + r * vnx;
+            if (r >= 5 && r <= pnz - 3) {
+                const real *ga = tw + (r - 2) * pnx, *gb = ga + pnx, *gc = gb + pnx, *gd = gc + pnx;
+                for (int ix = 0; ix < pnx; ix++) {
+                    real acc = 0;
+                    acc += c1z * ga[ix];
+                    acc += c0z * gb[ix];
+                    acc -= c0z * gc[ix];
+                    acc -= c1z * gd[ix];
+                    t[ix] = acc;
+                }
+                continue;
+            }
+            for (int ix = 0; ix < pnx; ix++) {
+                real acc = 0;
+                if (r <= 4) acc += (w0[r] * iz24) * tw[ix];
+                if (r == vnz - 1) acc += iz_ * tw[(pnz - 1) * pnx + ix];
+                if (r == vnz - 2) acc -= iz_ * tw[(pnz - 1) * pnx + ix];
+                if (r - 2 >= 1 && r - 2 <= pnz - 2) acc += c1z * tw[(r - 2) * pnx + ix];
+                if (r - 1 >= 1 && r - 1 <= pnz - 2) acc += c0z * tw[(r - 1) * pnx + ix];
+                if (r >= 1 && r <= pnz - 2) acc -= c0z * tw[r * pnx + ix];
+                if (r + 1 >= 1 && r + 1 <= pnz - 2) acc -= c1z * tw[(r + 1) * pnx + ix];
+                t[ix] = acc;
+            }
         }
+    }
     /* C_h^T on the v nodes (transpose of fwi_2d.c:812-819) */
+#pragma omp parallel for
     for (int iz = 0; iz < npml + 1; iz++)
         for (int ix = 0; ix < vnx; ix++) {
             cv1[iz * vnx + ix] = TAB(5, 2 * iz) * cv1[iz * vnx + ix] + TAB(4, 2 * iz) * tv[(iz + vnz - npml - 1) * vnx + ix];
             tv[(iz + vnz - npml - 1) * vnx + ix] += cv1[iz * vnx + ix];
         }
-    /* D1^T: scatter of v rows (fwi_2d.c:797-810); row vnz-1 is zero */
-    for (int ix = 0; ix < pnx; ix++) {
-        const real t0 = tv[ix], t1 = tv[vnx + ix], tl = tv[(vnz - 2) * vnx + ix];
-        real *o = out + ix;
-        o[0 * pnx] += (3675.0 * iz840) * t0 + (-1085.0 * iz840) * t1;
-        o[1 * pnx] += (-1225.0 * iz840) * t0 + (1015.0 * iz840) * t1;
-        o[2 * pnx] += (441.0 * iz840) * t0 + (-63.0 * iz840) * t1;
-        o[3 * pnx] += (-75.0 * iz840) * t0 + (5.0 * iz840) * t1;
-        o[(pnz - 1) * pnx] += iz_ * tl;
-        o[(pnz - 2) * pnx] -= iz_ * tl;
-    }
-    for (int iz = 2; iz < vnz - 2; iz++)
-        for (int ix = 0; ix < pnx; ix++) {
-            const real t = tv[iz * vnx + ix];
-            out[iz * pnx + ix] += c0z * t;
-            out[(iz - 1) * pnx + ix] -= c0z * t;
-            out[(iz + 1) * pnx + ix] += c1z * t;
-            out[(iz - 2) * pnx + ix] -= c1z * t;
+    /* D1^T of the v rows (fwi_2d.c:797-810; v node vnz-1 is zero): v rows
+     * 0 and 1 (free surface) read p rows 0..3, row vnz-2 rows pnz-2 and
+     * pnz-1, row iz in [2, vnz-3] rows iz-2..iz+1.  Gathered per p row r
+     * and added to the x chain's result: the free-surface and last-row
+     * terms first, then those of v rows r-1, r, r+1, r+2.                */
+    {
+        const double fs0[4] = {3675.0, -1225.0, 441.0, -75.0}, fs1[4] = {-1085.0, 1015.0, -63.0, 5.0};
+#pragma omp parallel for
+        for (int r = 0; r < pnz; r++) {
+            real *o = out + r * pnx;
+            if (r >= 4 && r <= pnz - 4) {
+                const real *ta = tv + (r - 1) * vnx, *tb = ta + vnx, *tc = tb + vnx, *td = tc + vnx;
+                for (int ix = 0; ix < pnx; ix++) {
+                    real acc = o[ix];
+                    acc += c1z * ta[ix];
+                    acc += c0z * tb[ix];
+                    acc -= c0z * tc[ix];
+                    acc -= c1z * td[ix];
+                    o[ix] = acc;
+                }
+                continue;
+            }
+            for (int ix = 0; ix < pnx; ix++) {
+                real acc = o[ix];
+                if (r <= 3) acc += (fs0[r] * iz840) * tv[ix] + (fs1[r] * iz840) * tv[vnx + ix];
+                if (r == pnz - 1) acc += iz_ * tv[(vnz - 2) * vnx + ix];
+                if (r == pnz - 2) acc -= iz_ * tv[(vnz - 2) * vnx + ix];
+                if (r - 1 >= 2 && r - 1 <= vnz - 3) acc += c1z * tv[(r - 1) * vnx + ix];
+                if (r >= 2 && r <= vnz - 3) acc += c0z * tv[r * vnx + ix];
+                if (r + 1 >= 2 && r + 1 <= vnz - 3) acc -= c0z * tv[(r + 1) * vnx + ix];
+                if (r + 2 >= 2 && r + 2 <= vnz - 3) acc -= c1z * tv[(r + 2) * vnx + ix];
+                o[ix] = acc;
+            }
         }
+    }
 }
 
 /* ------------------------------------------------------------------ */
@@ -641,6 +711,7 @@ int orc_backward(int nx, int nz, int npml, real dx, real dz, real dt,
         /* ---- adjoint: mu^{n+1} = 2 mu^{n+2} - mu^{n+3} + ekp (A^T mu^{n+2} + R) */
         orc_op_pml_adj(m1, aout, g.coe, g.tab, npml, pnx, pnz, cu0, cu1, cv1, cux0, cux1, cvz1, tu, tv, tw);
         for (int r = 0; r < nr; r++) aout[irz[r] * pnx + irx[r] + npml] += res[(size_t)r * nt + n];
+#pragma omp parallel for
         for (size_t i = 0; i < nf; i++) m2[i] = 2.0 * m1[i] - m2[i] + ekp[i] * aout[i];
         { real *t = m1; m1 = m2; m2 = t; } /* m1 = mu^{n+1}, m2 = mu^{n+2} */
         if (musrc) musrc[n] = m1[isz * pnx + isx + npml];
@@ -650,19 +721,23 @@ int orc_backward(int nx, int nz, int npml, real dx, real dz, real dt,
             memcpy(xb, xall + (size_t)n * nb, sizeof(real) * nb);
         } else {
             orc_op_interior(rn, bu, bv, bux, bvz, g.coe, nx, nz);
+#pragma omp parallel for
             for (int z = 3; z < nz - 3; z++)
                 for (int x = 3; x < nx - 3; x++) xb[z * nx + x] = bux[z * nx + x];
             if (isz >= 3 && isz < nz - 3 && isx >= 3 && isx < nx - 3) xb[isz * nx + isx] += wav[n] * iarea;
+#pragma omp parallel for
             for (int z = 3; z < nz - 3; z++)
                 for (int x = 3; x < nx - 3; x++) xb[z * nx + x] = xb[z * nx + x] + bvz[z * nx + x];
             orc_read_bdr(xstrip + (size_t)n * sl, xstrip + (size_t)n * sl + 2 * NB * nx, xb, nx, nz);
         }
         /* ---- zero-lag imaging */
+#pragma omp parallel for
         for (int z = 0; z < nz; z++)
             for (int x = 0; x < nx; x++)
                 gk[z * nx + x] += (double)m1[z * pnx + x + npml] * (double)xb[z * nx + x];
         /* ---- reconstruction p^{n-1} = 2 p^n - p^{n+1} + ekp X^n */
         if (!xall && n >= 1) {
+#pragma omp parallel for
             for (size_t i = 0; i < nb; i++) rm[i] = 2.0 * rn[i] - rp[i] + ekpb[i] * xb[i];
             if (n >= 2) orc_read_bdr(pstrip + (size_t)(n - 2) * sl, pstrip + (size_t)(n - 2) * sl + 2 * NB * nx, rm, nx, nz);
             else {

@@ -3,6 +3,13 @@ import sys
 
 import pytest
 
+# Under pytest-xdist each worker is its own process: give each a share of the
+# cores for the oracle's OpenMP loops (set before the library is loaded), so
+# that the workers' threads do not oversubscribe the machine.
+if os.environ.get("PYTEST_XDIST_WORKER") and "OMP_NUM_THREADS" not in os.environ:
+    _n = int(os.environ.get("PYTEST_XDIST_WORKER_COUNT", "1") or 1)
+    os.environ["OMP_NUM_THREADS"] = str(max(1, (os.cpu_count() or 1) // max(1, _n)))
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(ROOT, "fwi2dacoustic-mpi-lbfgs_amd")
 for p in (ROOT, PKG, os.path.join(ROOT, "oracle")):
