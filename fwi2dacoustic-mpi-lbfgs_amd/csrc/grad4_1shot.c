@@ -409,7 +409,7 @@ typedef struct {
     fwi_plan **plans;
     float *tr;
     const long long *off;
-    int nshots, nt, nthreads, t, rc;
+    int nshots, nthreads, t, rc;
     float dt;
 } MisfitJob;
 
@@ -417,16 +417,18 @@ static void *misfit_worker(void *p)
 {
     MisfitJob *j = (MisfitJob *)p;
     for (int s = j->t; s < j->nshots && !j->rc; s += j->nthreads)
-        j->rc = host_misfit(j->plans[s], j->tr + j->off[s], j->nt, j->dt, j->t);
+        j->rc = host_misfit(j->plans[s], j->tr + j->off[s], (int)j->plans[s]->grid->base->nt, j->dt, j->t);
     return NULL;
 }
 
-static int host_misfit_all(fwi_plan **plans, int nshots, float *tr, int nt, float dt)
+/* tr holds the shots' traces one after the other, shot s as nr x nt with
+ * its own plan's nt */
+static int host_misfit_all(fwi_plan **plans, int nshots, float *tr, float dt)
 {
     long long *off = (long long *)malloc(sizeof(long long) * (nshots + 1));
     if (!off) return fail(FWI_HIP_ENOMEM, "host memory");
     off[0] = 0;
-    for (int s = 0; s < nshots; s++) off[s + 1] = off[s] + (long long)plans[s]->msf->nr * nt;
+    for (int s = 0; s < nshots; s++) off[s + 1] = off[s] + (long long)plans[s]->msf->nr * (long long)plans[s]->grid->base->nt;
     int nth = 16;
     const char *env = getenv("FWI_HOST_THREADS");
     if (env && atoi(env) > 0) nth = atoi(env);
@@ -442,7 +444,7 @@ static int host_misfit_all(fwi_plan **plans, int nshots, float *tr, int nt, floa
     int made[MISFIT_MAX_THREADS];
     int rc = FWI_HIP_OK;
     for (int t = 0; t < nth; t++) {
-        jobs[t] = (MisfitJob){plans, tr, off, nshots, nt, nth, t, FWI_HIP_OK, dt};
+        jobs[t] = (MisfitJob){plans, tr, off, nshots, nth, t, FWI_HIP_OK, dt};
         made[t] = t > 0 && pthread_create(&th[t], NULL, misfit_worker, &jobs[t]) == 0;
     }
     for (int t = 0; t < nth; t++) /* the calling thread takes share 0 and any share without a thread */
@@ -456,17 +458,29 @@ static int host_misfit_all(fwi_plan **plans, int nshots, float *tr, int nt, floa
 }
 
 /* checks shared by the drivers: data rate (fwi_2d.c:195), one grid */
-static int check_shots(const fwi_base *b, fwi_specs *const *grids, fwi_msf *const *msfs, int nshots, int *nrmax)
+/* Each plan is one shot with its own time axis (the reference runs one rank
+ * per shot, each with its own fwi_base): shots may differ in nt and in
+ * their receivers, and share the grid, the propagation dt and the CPML. */
+static int check_shots(const fwi_base *b, fwi_specs *const *grids, fwi_msf *const *msfs, int nshots, int *nrmax,
+                       int *ntmax)
 {
     if (b->free_surface != 1) return fail(FWI_HIP_EINVAL, "only the free-surface grid is supported");
     *nrmax = 0;
+    *ntmax = 0;
     for (int s = 0; s < nshots; s++) {
         const fwi_msf *m = msfs[s];
+        const fwi_base *bs = grids[s]->base;
         if ((int)m->nr > *nrmax) *nrmax = (int)m->nr;
-        if ((m->nt - 1) * m->jump != b->nt - 1)
-            return fail(FWI_HIP_EINVAL, "shot %d: (nt0-1)*jump != nt-1 (%zu, %zu, %zu)", s, m->nt, m->jump, b->nt);
-        if (grids[s]->base->nx != b->nx || grids[s]->base->nz != b->nz)
-            return fail(FWI_HIP_EINVAL, "shot %d: grid differs from shot 0", s);
+        if ((int)bs->nt > *ntmax) *ntmax = (int)bs->nt;
+        if (bs->nt < 1 || (m->nt - 1) * m->jump != bs->nt - 1)
+            return fail(FWI_HIP_EINVAL, "shot %d: (nt0-1)*jump != nt-1 (%zu, %zu, %zu)", s, m->nt, m->jump, bs->nt);
+        /* the data rate must be jump x the propagation rate (fwi_2d.c:195): the
+         * filters and mutes run at msf.dt, the residual is resampled at base.dt */
+        if (fabs((double)m->dt - (double)bs->dt * (double)m->jump) > 1e-3)
+            return fail(FWI_HIP_EINVAL, "shot %d: msf.dt %g != jump %zu x base.dt %g", s, m->dt, m->jump, bs->dt);
+        if (bs->nx != b->nx || bs->nz != b->nz || bs->npml != b->npml || bs->dx != b->dx || bs->dz != b->dz ||
+            bs->dt != b->dt || bs->free_surface != b->free_surface)
+            return fail(FWI_HIP_EINVAL, "shot %d: grid (nx, nz, npml, dx, dz, dt) differs from shot 0", s);
     }
     return FWI_HIP_OK;
 }
@@ -500,7 +514,8 @@ static void shot_table_free(ShotTable *t)
     memset(t, 0, sizeof(*t));
 }
 
-static int shot_table(ShotTable *t, fwi_sou *const *sous, fwi_msf *const *msfs, int nshots, int nt)
+static int shot_table(ShotTable *t, fwi_specs *const *grids, fwi_sou *const *sous, fwi_msf *const *msfs,
+                      int nshots)
 {
     memset(t, 0, sizeof(*t));
     t->n = nshots;
@@ -523,6 +538,7 @@ static int shot_table(ShotTable *t, fwi_sou *const *sous, fwi_msf *const *msfs, 
             t->cells[2 * s][r] = so->ixyz_sr[2 + 2 * r];
             t->cells[2 * s + 1][r] = so->ixyz_sr[2 + 2 * r + 1];
         }
+        const int nt = (int)grids[s]->base->nt;
         t->sh[s].nt = nt;
         t->sh[s].isx = so->ixyz_sr[0];
         t->sh[s].isz = so->ixyz_sr[1];
@@ -551,22 +567,22 @@ int grad_all_shots(fwi_plan **plans, int nshots, int k)
     (void)k;
     if (!plans || nshots < 1 || !plans[0]) return fail(FWI_HIP_EINVAL, "grad_all_shots: no plans");
     const fwi_base *b = plans[0]->grid->base;
-    const int nx = (int)b->nx, nz = (int)b->nz, nt = (int)b->nt;
+    const int nx = (int)b->nx, nz = (int)b->nz;
     fwi_specs **grids = (fwi_specs **)malloc(sizeof(void *) * nshots);
     fwi_msf **msfs = (fwi_msf **)malloc(sizeof(void *) * nshots);
     fwi_sou **sous = (fwi_sou **)malloc(sizeof(void *) * nshots);
     ShotTable t = {0};
     float *tr = NULL;
     double *grad = NULL;
-    int nrmax = 0, rc = (grids && msfs && sous) ? FWI_HIP_OK : fail(FWI_HIP_ENOMEM, "host memory");
+    int nrmax = 0, ntmax = 0, rc = (grids && msfs && sous) ? FWI_HIP_OK : fail(FWI_HIP_ENOMEM, "host memory");
     for (int s = 0; !rc && s < nshots; s++) {
         grids[s] = plans[s]->grid;
         msfs[s] = plans[s]->msf;
         sous[s] = plans[s]->sou;
     }
-    if (!rc) rc = check_shots(b, grids, msfs, nshots, &nrmax);
-    if (!rc) rc = ensure_ctx(0, b, nshots, nt, nrmax);
-    if (!rc) rc = shot_table(&t, sous, msfs, nshots, nt);
+    if (!rc) rc = check_shots(b, grids, msfs, nshots, &nrmax, &ntmax);
+    if (!rc) rc = ensure_ctx(0, b, nshots, ntmax, nrmax);
+    if (!rc) rc = shot_table(&t, grids, sous, msfs, nshots);
     fwi_hip_ctx *ctx = g_slot[0].ctx;
     if (!rc) rc = ctx_setup(ctx, &plans[0]->model->md[0][0], plans[0]->grad, b, &t);
     if (!rc) {
@@ -575,7 +591,7 @@ int grad_all_shots(fwi_plan **plans, int nshots, int k)
         else if (!(tr = slot_traces(0, (size_t)t.ntr))) rc = FWI_HIP_ENOMEM;
     }
     if (!rc && (rc = fwi_hip_forward(ctx, tr))) rc = fail(rc, "forward: %s", fwi_hip_last_error(ctx));
-    if (!rc) rc = host_misfit_all(plans, nshots, tr, nt, b->dt);
+    if (!rc) rc = host_misfit_all(plans, nshots, tr, b->dt);
     if (!rc && (rc = fwi_hip_set_residual(ctx, tr))) rc = fail(rc, "set_residual: %s", fwi_hip_last_error(ctx));
     if (!rc && (rc = fwi_hip_adjoint(ctx, grad, NULL))) rc = fail(rc, "adjoint: %s", fwi_hip_last_error(ctx));
     if (!rc)
@@ -607,7 +623,7 @@ int grad_all_shots_rfl(rwi_plan **plans, int nshots, int k, float *grad_m1)
     (void)k;
     if (!plans || nshots < 1 || !plans[0]) return fail(FWI_HIP_EINVAL, "grad_all_shots_rfl: no plans");
     const fwi_base *b = plans[0]->grid->base;
-    const int nx = (int)b->nx, nz = (int)b->nz, nt = (int)b->nt;
+    const int nx = (int)b->nx, nz = (int)b->nz;
     const long nb = (long)nx * nz;
     fwi_specs **grids = (fwi_specs **)malloc(sizeof(void *) * nshots);
     fwi_msf **msfs = (fwi_msf **)malloc(sizeof(void *) * nshots);
@@ -617,7 +633,8 @@ int grad_all_shots_rfl(rwi_plan **plans, int nshots, int k, float *grad_m1)
     ShotTable t = {0};
     float *tr0 = NULL, *tr1 = NULL, *vtot = NULL;
     double *g0 = NULL, *g1 = NULL;
-    int nrmax = 0, rc = (grids && msfs && sous && shell && shells) ? FWI_HIP_OK : fail(FWI_HIP_ENOMEM, "host memory");
+    int nrmax = 0, ntmax = 0,
+        rc = (grids && msfs && sous && shell && shells) ? FWI_HIP_OK : fail(FWI_HIP_ENOMEM, "host memory");
     for (int s = 0; !rc && s < nshots; s++) {
         grids[s] = plans[s]->grid;
         msfs[s] = plans[s]->msf;
@@ -628,10 +645,10 @@ int grad_all_shots_rfl(rwi_plan **plans, int nshots, int k, float *grad_m1)
         shell[s].grad = plans[s]->grad;
         shells[s] = &shell[s];
     }
-    if (!rc) rc = check_shots(b, grids, msfs, nshots, &nrmax);
-    if (!rc) rc = ensure_ctx(0, b, nshots, nt, nrmax);
-    if (!rc) rc = ensure_ctx(1, b, nshots, nt, nrmax);
-    if (!rc) rc = shot_table(&t, sous, msfs, nshots, nt);
+    if (!rc) rc = check_shots(b, grids, msfs, nshots, &nrmax, &ntmax);
+    if (!rc) rc = ensure_ctx(0, b, nshots, ntmax, nrmax);
+    if (!rc) rc = ensure_ctx(1, b, nshots, ntmax, nrmax);
+    if (!rc) rc = shot_table(&t, grids, sous, msfs, nshots);
     if (!rc) {
         tr0 = slot_traces(0, (size_t)t.ntr);
         tr1 = slot_traces(1, (size_t)t.ntr);
@@ -651,7 +668,7 @@ int grad_all_shots_rfl(rwi_plan **plans, int nshots, int k, float *grad_m1)
     if (!rc && (rc = fwi_hip_forward(c1, tr1))) rc = fail(rc, "forward (perturbed): %s", fwi_hip_last_error(c1));
     if (!rc) {
         for (long long i = 0; i < t.ntr; i++) tr1[i] -= tr0[i]; /* the reflected data */
-        rc = host_misfit_all(shells, nshots, tr1, nt, b->dt); /* tr1 <- adjoint source r */
+        rc = host_misfit_all(shells, nshots, tr1, b->dt); /* tr1 <- adjoint source r */
     }
     if (!rc && (rc = fwi_hip_set_residual(c1, tr1))) rc = fail(rc, "set_residual: %s", fwi_hip_last_error(c1));
     if (!rc && (rc = fwi_hip_adjoint(c1, g1, NULL))) rc = fail(rc, "adjoint (perturbed): %s", fwi_hip_last_error(c1));

@@ -175,7 +175,10 @@ void grad4_1shot(fwi_plan *plan, int k);
 /* All shots of an iteration in one batched run (replaces the one-rank-per-
  * shot loop of MBotOpt.c:51).  The plans share the model of plans[0];
  * each plan gets its own syn and val; the gradient summed over the shots
- * goes to plans[0]->grad->body.  Returns 0 or a FWI_HIP_E* code. */
+ * goes to plans[0]->grad->body.  Like one rank per shot, each plan keeps its
+ * own time axis (grid->base->nt with its msf->nt, jump and dt) and its own
+ * receivers; the grid, the propagation dt and the CPML must be those of
+ * plans[0].  Returns 0 or a FWI_HIP_E* code. */
 int grad_all_shots(fwi_plan **plans, int nshots, int k);
 
 /* header.h:285 -- reflection waveform inversion, one shot.  Declared but

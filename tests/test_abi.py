@@ -94,5 +94,11 @@ def test_plan_layout_and_sampling_check():
     assert h.grad_all_shots(arr, 1, 0) == L.EINVAL
     assert b"nt0" in h.grad4_last_error()
     ctypes.c_size_t.from_address(ctypes.addressof(msf) + off).value = 21
+    # the data rate must be jump x the propagation rate, within 1e-3 s (fwi_2d.c:195)
+    assert abs(msf.dt - 2e-3) < 1e-9
+    msf.dt = 4e-3                            # 2 ms off, beyond the reference's 1e-3 s tolerance
+    assert h.grad_all_shots(arr, 1, 0) == L.EINVAL
+    assert b"msf.dt" in h.grad4_last_error()
+    msf.dt = 2e-3
     h.fwi_plan_free(p)
     assert not h.fwi_plan_alloc(30, 20, 5, 40, 2, 7, 10.0, 10.0, 1e-3)  # (nt-1) % jump != 0
