@@ -36,7 +36,7 @@ def run(cmd):
 def build(verbose=True):
     os.makedirs(BUILD, exist_ok=True)
     objs = []
-    for src in ("fwi_kernels.hip", "fwi_hip.cpp"):
+    for src in ("fwi_kernels.hip", "fwi_hip.cpp", "fwi_noalloc.hip"):
         obj = os.path.join(BUILD, src + ".o")
         run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)

@@ -1484,6 +1484,25 @@ extern "C" int fwi_hip_op_pml(const fwi_hip_grid *g, const float *p, float *u, f
     return FWI_HIP_OK;
 }
 
+// For the allocation-free entry point (fwi_noalloc.hip): the argument block of
+// the reference-structured a_lap_opt_2d_pml phases, the CPML tables on the
+// host (tab[6 (2 npml + 1)]) and the library's error message.
+extern "C" int fwi_hip_internal_op_pml_args(const fwi_hip_grid *g, FwiRefArgs *a, float *tab)
+{
+    if (!a || !tab) return set_err(nullptr, FWI_HIP_EINVAL, "op_pml: NULL argument block");
+    int rc = ref_args(g, *a, g ? g->nx + 2 * g->npml : 0, g ? g->nz + g->npml : 0);
+    if (rc) return rc;
+    std::vector<float> t;
+    cpml_tables(*g, t);
+    memcpy(tab, t.data(), sizeof(float) * t.size());
+    return FWI_HIP_OK;
+}
+
+extern "C" int fwi_hip_internal_err(int code, const char *msg)
+{
+    return set_err(nullptr, code, "%s", msg);
+}
+
 extern "C" int fwi_hip_op_interior(const fwi_hip_grid *g, const float *p, float *u, float *v, float *ux, float *vz,
                                    void *stream)
 {

@@ -24,7 +24,7 @@ EXPORTS = (
     "fwi_hip_set_shots", "fwi_hip_forward", "fwi_hip_set_observed", "fwi_hip_misfit_l2",
     "fwi_hip_set_residual", "fwi_hip_adjoint", "fwi_hip_synchronize", "fwi_hip_set_timing",
     "fwi_hip_kernel_time", "fwi_hip_read_state", "fwi_hip_op_pml", "fwi_hip_op_interior",
-    "fwi_hip_time_integral", "fwi_hip_write_bdr", "fwi_hip_read_bdr", "fwi_hip_cpml_coe",
+    "fwi_hip_time_integral", "fwi_hip_op_pml_noalloc", "fwi_hip_write_bdr", "fwi_hip_read_bdr", "fwi_hip_cpml_coe",
     "fwi_hip_set_graph", "fwi_hip_set_kernels", "fwi_hip_set_rec_group", "fwi_hip_set_imaging", "fwi_hip_set_bwd_chunk", "fwi_hip_set_checkpointing", "fwi_hip_set_bwd_overlap", "fwi_hip_set_bwd_fused", "fwi_hip_set_concurrent", "fwi_hip_set_interior_wgs", "fwi_hip_device_count", "fwi_hip_dmalloc", "fwi_hip_dfree", "fwi_hip_host_alloc", "fwi_hip_host_free", "fwi_hip_memcpy",
 )
 
@@ -304,6 +304,17 @@ def op_pml(geom, p, u, v, ux, vz, u0, u1, v1, ux0, ux1, vz1):
     gr = make_grid(geom)
     rc = lib.fwi_hip_op_pml(ctypes.byref(gr), *[_dp(t) for t in (p, u, v, ux, vz, u0, u1, v1, ux0, ux1, vz1)],
                             ctypes.c_void_p(0))
+    if rc:
+        raise FwiError(rc, lib.fwi_hip_last_error(None).decode())
+
+
+def op_pml_noalloc(geom, p, u, v, ux, vz, u0, u1, v1, ux0, ux1, vz1):
+    """a_lap_opt_2d_pml like op_pml, with the CPML tables in the library's
+    static device array instead of a fresh allocation."""
+    lib = load()
+    gr = make_grid(geom)
+    rc = lib.fwi_hip_op_pml_noalloc(ctypes.byref(gr), *[_dp(t) for t in (p, u, v, ux, vz, u0, u1, v1, ux0, ux1,
+                                                                          vz1)], ctypes.c_void_p(0))
     if rc:
         raise FwiError(rc, lib.fwi_hip_last_error(None).decode())
 

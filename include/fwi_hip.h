@@ -216,6 +216,17 @@ int fwi_hip_memcpy(void *dst, const void *src, size_t bytes, int kind);
 /* a_lap_opt_2d_pml, fwi_2d.c:761-864 (free surface) */
 int fwi_hip_op_pml(const fwi_hip_grid *g, const float *p, float *u, float *v, float *ux, float *vz,
                    float *u0, float *u1, float *v1, float *ux0, float *ux1, float *vz1, void *stream);
+/* fwi_hip_op_pml, allocating nothing.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The CPML
+ * tables go to a static device array of the library instead of a fresh
+ * allocation: 6 (2 npml + 1) floats, at most 3096 bytes (npml <= 64, larger
+ * npml is refused with FWI_HIP_EINVAL), one copy per GPU held until the
+ * process ends.  Every call on a GPU rewrites that array, so calls that
+ * overlap on one GPU (other streams or threads) race.  Same results as
+ * fwi_hip_op_pml, bit for bit; returns after enqueueing on `stream`.       */
+int fwi_hip_op_pml_noalloc(const fwi_hip_grid *g, const float *p, float *u, float *v, float *ux, float *vz,
+                           float *u0, float *u1, float *v1, float *ux0, float *ux1, float *vz1, void *stream);
 /* a_lap_opt_2d, fwi_2d.c:722-759, on the physical grid p[nz][nx] */
 int fwi_hip_op_interior(const fwi_hip_grid *g, const float *p, float *u, float *v, float *ux, float *vz,
                         void *stream);
