@@ -69,7 +69,11 @@ def parse():
                     help="time short passes of the candidate launch settings first and use the fastest")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-single", action="store_true", help="skip the one-shot-per-launch comparison")
-    ap.add_argument("--cpu-nt", type=int, default=1000, help="time steps of each CPU sample shot")
+    ap.add_argument("--cpu-nt", type=int, default=0, help="time steps of each CPU sample shot (0: the workload's nt)")
+    ap.add_argument("--cpu-shots", type=int, default=8, help="shots of the CPU sample, spread along the line")
+    ap.add_argument("--no-boundary", action="store_true",
+                    help="skip the same gradient through the drop-in grad_all_shots (host misfit, trace copies)")
+    ap.add_argument("--no-ckpt", action="store_true", help="skip the checkpointed (FP16/BF16 snapshot) backward pass")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (HBM counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -147,6 +151,76 @@ def cpu_baseline(g, vel, shots, nt_sample, nt_full, nsample=4):
     return dict(sg_per_s=1.0 / per_shot, shots=n, threads=threads, wall_s=t_fwd + t_msf + t_bwd,
                 fwd_gcell=f_cells / t_fwd / 1e9, bwd_gcell=b_cells / t_bwd / 1e9,
                 gcell=(f_cells + b_cells) / (t_fwd + t_msf + t_bwd) / 1e9)
+
+
+def boundary_leg(g, start, shots, obs_packed, steps, warmup):
+    """The same gradient through the drop-in boundary: grad_all_shots over one
+    reference-layout plan per shot (fwi_plan.h), with the model and the
+    observed data in host memory.  Each call uploads the model, runs the
+    forward pass, copies the traces to the host, runs the host misfit step
+    (L2 in FP64, threaded over shots), copies the adjoint source back, runs
+    the adjoint pass and copies the gradient to plans[0].  Returns the rate
+    and the mean wall time of each phase."""
+    import ctypes
+    import numpy as np
+    from fwihip import plans as PC
+    h = PC.lib()
+    plans, o = [], 0
+    for s in shots:
+        n = s.nr * s.nt
+        plans.append(PC.make_plan(h, g, s, obs_packed[o:o + n].reshape(s.nr, s.nt), start))
+        o += n
+    arr = (ctypes.POINTER(PC.FwiPlan) * len(plans))(*plans)
+    ph = (ctypes.c_double * 5)()
+    try:
+        for _ in range(warmup):
+            assert h.grad_all_shots(arr, len(plans), 0) == 0, h.grad4_last_error()
+        acc = np.zeros(5)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            assert h.grad_all_shots(arr, len(plans), 0) == 0, h.grad4_last_error()
+            h.grad4_phase_ms(ph, 5)
+            acc += np.array(ph[:])
+        wall = time.perf_counter() - t0
+    finally:
+        for p in plans:
+            h.fwi_plan_free(p)
+    acc /= steps
+    names = ("setup", "forward_and_trace_copy", "host_misfit", "residual_copy", "adjoint_and_gradient_copy")
+    return dict(value=len(shots) * steps / wall, unit="shot-gradients/s", ms_per_step=wall / steps * 1e3,
+                steps=steps, warmup=warmup, phases_ms={k: round(float(v), 2) for k, v in zip(names, acc)})
+
+
+def checkpoint_leg(g, vel, start, shots, obs_packed, nt, every, storage, steps, warmup):
+    """The headline step with the checkpointed backward pass: snapshots of
+    every shot every `every` steps in `storage` (per-shot max-abs scaling for
+    16 bit), segments recomputed in the adjoint pass, no boundary frames."""
+    from fwihip import lib
+    nr = max(s.nr for s in shots)
+    ctx = lib.Context(g, max_shots=len(shots), max_nt=nt, max_nr=nr)
+    try:
+        ctx.set_checkpointing(every, storage)
+        ctx.set_shots(shots)
+        ctx.set_observed(obs_packed)
+
+        def step():
+            ctx.set_model(start)
+            ctx.forward(want_traces=False)
+            ctx.misfit_l2()
+            ctx.adjoint()
+
+        for _ in range(warmup):
+            step()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        ctx.synchronize()
+        wall = time.perf_counter() - t0
+        return dict(storage=storage, every=every, value=len(shots) * steps / wall, unit="shot-gradients/s",
+                    ms_per_step=wall / steps * 1e3, device_gbytes=round(ctx.bytes / 1e9, 2))
+    finally:
+        ctx.close()
 
 
 def run_uneven(args):
@@ -315,9 +389,7 @@ def run_inversion(args):
     iteration."""
     import ctypes
     import numpy as np
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import plan_ctypes as PC
-    import host_ctypes as HC
+    from fwihip import plans as PC
     from fwihip import lib
     from fwihip.workloads import Geometry, layered_faulted, surface_shots, triangle_smooth
 
@@ -361,7 +433,7 @@ def run_inversion(args):
     kinds = {"l2": 0, "ncc": 1, "eot": 4}
     par = None
     if args.misfit == "eot":
-        par = HC.EotPar(0, 0, 500, None, None, None, float((8 * g.dt * jump) ** 2), 1e-6, 0.0)
+        par = PC.EotPar(0, 0, 500, None, None, None, float((8 * g.dt * jump) ** 2), 1e-6, 0.0)
     assert h.grad4_set_misfit(kinds[args.misfit], ctypes.byref(par) if par is not None else None, 1.0, 0.0) == 0
     storage = {"fp32": 0, "fp16": 1, "bf16": 2}[args.storage]
     every = args.ckpt_every if args.storage != "fp32" or args.ckpt_every else 0
@@ -551,15 +623,27 @@ def main():
         single = len(sub) / t1
         c1.close()
 
+    boundary = ckpt = None
+    if args.workload == "batched" and not args.autotune:
+        obs_host = obs
+        ctx.close()                        # its device memory is not needed any more
+        xs, xw = min(args.steps, 5), min(args.warmup, 1)   # the extra legs: a few steps each
+        if not args.no_boundary:
+            boundary = boundary_leg(g, start, shots, obs_host, xs, xw)
+            boundary["over_resident"] = boundary["value"] / sg_per_s
+        if not args.no_ckpt:
+            ckpt = [checkpoint_leg(g, vel, start, shots, obs_host, args.nt, 100, st, xs, xw)
+                    for st in ("fp16", "bf16")]
+
     cpu = None
     if not args.no_cpu:
-        c = cpu_baseline(g, start, shots, args.cpu_nt, args.nt)
+        c = cpu_baseline(g, start, shots, args.cpu_nt or args.nt, args.nt, args.cpu_shots)
         cpu = dict(value=c["sg_per_s"], unit="shot-gradients/s", cores=c["threads"], kind="port",
-                   sample=f"{c['shots']} shots spread along the line, nt={args.cpu_nt} each instead of {args.nt} "
-                          f"({c['wall_s']:.1f} s, FP32 oracle, OpenMP on {c['threads']} threads): forward "
+                   sample=f"{c['shots']} of the {nshots} shots, spread along the line, nt={args.cpu_nt or args.nt} "
+                          f"each ({c['wall_s']:.1f} s, FP32 oracle, OpenMP on {c['threads']} threads): forward "
                           f"{c['fwd_gcell']:.3f}, backward (adjoint + reconstruction + imaging) "
-                          f"{c['bwd_gcell']:.3f}, whole {c['gcell']:.3f} Gcell-updates/s; mean time per shot "
-                          "scaled to the full-nt shot-gradient",
+                          f"{c['bwd_gcell']:.3f}, whole {c['gcell']:.3f} Gcell-updates/s; shots are "
+                          "independent, so the mean time per shot gives the rate",
                    gcell_updates_per_s=c["gcell"], forward_gcell_updates_per_s=c["fwd_gcell"],
                    backward_gcell_updates_per_s=c["bwd_gcell"])
 
@@ -586,6 +670,10 @@ def main():
         "misfit": J,
         "roofline": roof,
         "kernels": kern,
+        "method": "boundary-saving reconstruction (frames in HBM), L2 misfit on the device with the observed "
+                  "data resident in HBM",
+        "drop_in_boundary": boundary,
+        "checkpointed": ckpt,
         "one_shot_per_launch_shot_gradients_per_s": single,
         "batched_over_single": (sg_per_s / single) if single else None,
         "cpu_baseline": cpu,

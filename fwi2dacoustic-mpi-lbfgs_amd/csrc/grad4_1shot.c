@@ -31,6 +31,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include "fwi_hip.h"
 #include "fwi_io.h"
@@ -165,6 +166,23 @@ static int fail(int code, const char *fmt, ...)
 }
 
 const char *grad4_last_error(void) { return g_msg; }
+
+/* wall-clock milliseconds of the phases of the last grad_all_shots call */
+#define GRAD4_NPHASE 5
+static double g_phase_ms[GRAD4_NPHASE];
+
+static double now_ms(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec * 1e3 + (double)ts.tv_nsec * 1e-6;
+}
+
+int grad4_phase_ms(double *ms, int n)
+{
+    for (int i = 0; ms && i < n && i < GRAD4_NPHASE; i++) ms[i] = g_phase_ms[i];
+    return GRAD4_NPHASE;
+}
 
 /* the page-locked trace buffer of slot `which`, at least n floats: the
  * forward's traces come back and the adjoint source goes out through it at
@@ -566,6 +584,9 @@ int grad_all_shots(fwi_plan **plans, int nshots, int k)
 {
     (void)k;
     if (!plans || nshots < 1 || !plans[0]) return fail(FWI_HIP_EINVAL, "grad_all_shots: no plans");
+    double tm[GRAD4_NPHASE + 1];
+    memset(g_phase_ms, 0, sizeof(g_phase_ms));
+    tm[0] = now_ms();
     const fwi_base *b = plans[0]->grid->base;
     const int nx = (int)b->nx, nz = (int)b->nz;
     fwi_specs **grids = (fwi_specs **)malloc(sizeof(void *) * nshots);
@@ -590,12 +611,18 @@ int grad_all_shots(fwi_plan **plans, int nshots, int k)
         if (!grad) rc = fail(FWI_HIP_ENOMEM, "host memory");
         else if (!(tr = slot_traces(0, (size_t)t.ntr))) rc = FWI_HIP_ENOMEM;
     }
+    tm[1] = now_ms();
     if (!rc && (rc = fwi_hip_forward(ctx, tr))) rc = fail(rc, "forward: %s", fwi_hip_last_error(ctx));
+    tm[2] = now_ms();
     if (!rc) rc = host_misfit_all(plans, nshots, tr, b->dt);
+    tm[3] = now_ms();
     if (!rc && (rc = fwi_hip_set_residual(ctx, tr))) rc = fail(rc, "set_residual: %s", fwi_hip_last_error(ctx));
+    tm[4] = now_ms();
     if (!rc && (rc = fwi_hip_adjoint(ctx, grad, NULL))) rc = fail(rc, "adjoint: %s", fwi_hip_last_error(ctx));
     if (!rc)
         for (long i = 0; i < (long)nx * nz; i++) plans[0]->grad->body[i] = (float)grad[i];
+    tm[5] = now_ms();
+    for (int i = 0; i < GRAD4_NPHASE; i++) g_phase_ms[i] = tm[i + 1] - tm[i];
     shot_table_free(&t);
     free(grids);
     free(msfs);

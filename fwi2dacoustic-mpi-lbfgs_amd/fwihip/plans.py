@@ -1,5 +1,7 @@
 """ctypes mirror of the plan structs in include/fwi_plan.h (the reference's
-header.h:21-185 layout), used by the tests of grad4_1shot / grad_all_shots."""
+header.h:21-185 layout) and of the drop-in driver's entry points
+(grad4_1shot, grad_all_shots, their settings), for Python callers of the
+drop-in boundary: the tests and the benchmark."""
 import ctypes
 
 import numpy as np
@@ -78,6 +80,13 @@ class RwiPlan(ctypes.Structure):
                [("grad", ctypes.POINTER(FwiGrad)), ("bd", RwiBdState)]
 
 
+class EotPar(ctypes.Structure):
+    """eot_msf_par (include/fwi_msf.h), the parameters of grad4_set_misfit's
+    entropic-OT misfit."""
+    _fields_ = [("nt", c_int), ("nr", c_int), ("max_iter", c_int), ("M", PPF), ("u", PF), ("v", PF),
+                ("eps", c_float), ("tor", c_float), ("dt", c_float)]
+
+
 def lib():
     h = L.load()
     h.fwi_plan_alloc.restype = ctypes.POINTER(FwiPlan)
@@ -89,6 +98,10 @@ def lib():
     h.grad_all_shots.argtypes = [ctypes.POINTER(ctypes.POINTER(FwiPlan)), c_int, c_int]
     h.grad_all_shots.restype = c_int
     h.grad4_last_error.restype = ctypes.c_char_p
+    h.grad4_phase_ms.argtypes = [ctypes.POINTER(ctypes.c_double), c_int]
+    h.grad4_phase_ms.restype = c_int
+    h.grad4_set_checkpointing.argtypes = [c_int, c_int]
+    h.grad4_set_checkpointing.restype = c_int
     h.fwi_read_f32.argtypes = [ctypes.c_char_p, PF, c_size_t, c_int]
     h.fwi_read_f32.restype = c_int
     h.fwi_write_f32.argtypes = [ctypes.c_char_p, PF, c_size_t]

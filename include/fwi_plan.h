@@ -219,6 +219,14 @@ void rwi_plan_free(rwi_plan *plan);
  * RWI forms). */
 const char *grad4_last_error(void);
 
+/* Wall-clock milliseconds of the phases of the last grad_all_shots (or
+ * grad4_1shot) call, written to ms[0..n-1] (no counterpart in the
+ * reference): [0] set-up (context, model, shots), [1] forward pass with the
+ * traces copied to the host, [2] host misfit step, [3] adjoint source
+ * copied to the device, [4] adjoint pass with the gradient copied back.
+ * Returns the number of phases (5). */
+int grad4_phase_ms(double *ms, int n);
+
 /* The host misfit step of grad4_1shot / grad_all_shots (fwi_msf.h,
  * fwi_sigproc.h), process-wide settings.  The reference leaves this step
  * inside its missing grad4_1shot; the misfit functions are its pool

@@ -616,6 +616,7 @@ int orc_forward(int nx, int nz, int npml, real dx, real dz, real dt,
         for (int r = 0; r < nr && syn; r++) syn[(size_t)r * nt + it] = s.pnew[irz[r] * pnx + irx[r] + npml];
         if (pstrip) frame_save_f(pstrip + (size_t)it * sl, s.pnew, &g);
         if (xstrip || xall) {
+#pragma omp parallel for
             for (int i = 0; i < pnx * pnz; i++) xf[i] = s.ux[i] + s.vz[i];
             if (xstrip) frame_save_f(xstrip + (size_t)it * sl, xf, &g);
             if (xall)

@@ -76,7 +76,7 @@ def test_plan_layout_and_sampling_check():
     """fwi_plan_alloc builds the reference's shapes (fwi_2d.c:148-165), the
     ctypes mirror of header.h reads them back, and grad_all_shots rejects
     (nt0-1)*jump != nt-1 (fwi_2d.c:195) before touching the device."""
-    import plan_ctypes as PC
+    from fwihip import plans as PC
     h = PC.lib()
     p = h.fwi_plan_alloc(30, 20, 5, 41, 2, 7, 10.0, 10.0, 1e-3)
     P = p.contents

@@ -11,7 +11,7 @@ import pytest
 from oracle import Oracle
 from fwihip import lib as L
 import host_ctypes as HC
-import plan_ctypes as PC
+from fwihip import plans as PC
 from test_gpu_grad4 import case
 
 pytestmark = pytest.mark.gpu

@@ -160,7 +160,7 @@ def test_config2_full_nt_forms_and_single_shots(lib):
 
 # ----------------------------------------------------------- 1536x400 uneven, drop-in
 def test_config3_uneven_apertures_bandpass_through_grad_all_shots(lib, o32, o64):
-    import plan_ctypes as PC
+    from fwihip import plans as PC
     h = PC.lib()
     PF = ctypes.POINTER(ctypes.c_float)
     for name in ("sfbandpass", "sfbandpass_adjoint"):

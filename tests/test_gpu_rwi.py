@@ -12,7 +12,7 @@ import pytest
 from oracle import Oracle
 from fwihip import lib as L
 from fwihip.workloads import Geometry, Shot, ricker
-import plan_ctypes as PC
+from fwihip import plans as PC
 
 pytestmark = pytest.mark.gpu
 

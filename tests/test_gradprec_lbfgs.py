@@ -12,7 +12,7 @@ import ctypes
 import numpy as np
 import pytest
 
-import plan_ctypes as PC
+from fwihip import plans as PC
 from fwihip import lib as L
 
 c_int, c_float = ctypes.c_int, ctypes.c_float

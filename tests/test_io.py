@@ -8,7 +8,7 @@ import os
 import numpy as np
 import pytest
 
-import plan_ctypes as PC
+from fwihip import plans as PC
 from fwihip import lib as L
 
 

@@ -6,7 +6,7 @@ import ctypes
 
 import numpy as np
 
-import plan_ctypes as PC
+from fwihip import plans as PC
 
 
 def test_rwi_plan_alloc_fields():
