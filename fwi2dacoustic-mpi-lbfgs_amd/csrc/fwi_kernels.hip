@@ -67,6 +67,43 @@ __device__ __forceinline__ void buf_st(float v, brsrc r, int voff, int soff)
 {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
 }
+// Row r of a field (rows [0, n), pitch rowb bytes) for the general strips, 0
+// outside the field.  The load is always issued, for an in-bounds row, and
+// the value is replaced afterwards: a load inside a branch of its own is
+// waited for before the branch ends, one memory latency per row.
+__device__ __forceinline__ float buf_ld_row(brsrc rs, int voff, int r, int n, int rowb)
+{
+    const bool in = r >= 0 && r < n;
+    const float v = buf_ld(rs, voff, (in ? r : 0) * rowb);
+    return in ? v : 0.f;
+}
+// element i of a, 0 when !in; the load is issued for element 0 then (a must
+// hold at least one element), for the reason above
+template <class T>
+__device__ __forceinline__ T ld_sel(const T *a, long long i, bool in, T zero)
+{
+    const T v = a[in ? i : 0];
+    return in ? v : zero;
+}
+
+// A uniform value held in a scalar register from here on.  Where a lane
+// condition picks between two fields of the parameter block (which lives in
+// device memory), the compiler would otherwise read the picked field per lane
+// with a vector load, and wait for it inside the divergent branch.
+__device__ __forceinline__ float uni(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+// the x-stencil constants of the propagation grid, in registers
+struct XConst {
+    int pnx, npml;
+    float ix, c0x, c1x;
+};
+__device__ __forceinline__ XConst xconst(const FwiGeomDev &g)
+{
+    return XConst{uni(g.pnx), uni(g.npml), uni(g.ix), uni(g.c0x), uni(g.c1x)};
+}
 
 // frame position of physical cell (bz, bx) inside one step of the strips
 __device__ __forceinline__ int frame_pos(int bz, int bx, int nx, int nz)
@@ -588,7 +625,7 @@ struct XCoef {
     int col;  // column of the node's CPML memory within its row (-1: none)
 };
 // u node j (between p[j-1] and p[j])
-__device__ __forceinline__ XCoef fwd_u_coef(const FwiGeomDev &g, const float *tab, int j)
+__device__ __forceinline__ XCoef fwd_u_coef(const XConst &g, const float *tab, int j)
 {
     const int pnx = g.pnx, npml = g.npml, T = 2 * npml + 1, n1 = npml + 1;
     XCoef c;
@@ -609,7 +646,7 @@ __device__ __forceinline__ XCoef fwd_u_coef(const FwiGeomDev &g, const float *ta
     return c;
 }
 // ux node i (p node)
-__device__ __forceinline__ XCoef fwd_ux_coef(const FwiGeomDev &g, const float *tab, int i)
+__device__ __forceinline__ XCoef fwd_ux_coef(const XConst &g, const float *tab, int i)
 {
     const int pnx = g.pnx, npml = g.npml, T = 2 * npml + 1;
     XCoef c;
@@ -651,7 +688,7 @@ __device__ __forceinline__ void fwd_tile_rows(const FwiFwdArgs &A, const FwiShot
 #pragma unroll
     for (int m = 0; m < FPR; ++m) {
         const int zr = r0 - 3 + m;
-        p[m] = (!EDGE || (zr >= 0 && zr < pnz)) ? buf_ld(rnow, vo, zr * rowb) : 0.f;
+        p[m] = EDGE ? buf_ld_row(rnow, vo, zr, pnz, rowb) : buf_ld(rnow, vo, zr * rowb);
     }
     // x halo of the FR own rows, HP loads
     XHalo<HP> xh;
@@ -667,9 +704,8 @@ __device__ __forceinline__ void fwd_tile_rows(const FwiFwdArgs &A, const FwiShot
 #pragma unroll
     for (int d = 0; d < FR; ++d) {
         const int i = r0 + d;
-        const bool ok = !EDGE || i < pnz;
-        po[d] = ok ? buf_ld(rold, vo, i * rowb) : 0.f;
-        pk[d] = ok ? buf_ld(rk, vo, i * rowb) : 0.f;
+        po[d] = EDGE ? buf_ld_row(rold, vo, i, pnz, rowb) : buf_ld(rold, vo, i * rowb);
+        pk[d] = EDGE ? buf_ld_row(rk, vo, i, pnz, rowb) : buf_ld(rk, vo, i * rowb);
     }
 
     const float *xh_o = A.xh + s * A.b.xh_s + cur * A.b.xh_b;
@@ -707,28 +743,31 @@ __device__ __forceinline__ void fwd_tile_rows(const FwiFwdArgs &A, const FwiShot
     const bool frame_cols = EDGE && (CLS & FWI_CLS_FR) &&
                             ((x0 < npml + 3 && x0 + TX > npml) || (x0 < npml + g.nx && x0 + TX > npml + g.nx - 3));
     if (EDGE && !fastx) {
-        cu = fwd_u_coef(g, tab, x0 - 1 + lane);
-        cuh = fwd_u_coef(g, tab, x0 - 1 + uc);
-        cx = fwd_ux_coef(g, tab, x);
+        const XConst xk = xconst(g);
+        cu = fwd_u_coef(xk, tab, x0 - 1 + lane);
+        cuh = fwd_u_coef(xk, tab, x0 - 1 + uc);
+        cx = fwd_ux_coef(xk, tab, x);
     }
     if (EDGE) {
         if (!zplain) {
 #pragma unroll
             for (int n = 0; n < FVN; ++n) {
                 const int j = r0 - 1 + n;
-                zho[n] = (j >= pnz - npml && j < pnz) ? zh_o[(j - (pnz - npml)) * ld + x] : 0.f;
+                zho[n] = ld_sel(zh_o, (long long)(j - (pnz - npml)) * ld + x, j >= pnz - npml && j < pnz, 0.f);
             }
 #pragma unroll
             for (int d = 0; d < FR; ++d) {
                 const int i = r0 + d;
-                zdo[d] = (i >= pnz - npml && i < pnz) ? zd_o[(i - (pnz - npml)) * ld + x] : 0.f;
+                zdo[d] = ld_sel(zd_o, (long long)(i - (pnz - npml)) * ld + x, i >= pnz - npml && i < pnz, 0.f);
             }
         }
 #pragma unroll
         for (int d = 0; d < FR; ++d) {
-            const int i = r0 + d;
+            // x strips: per lane, clamped index, no divergence; rows past the
+            // grid read the last row (their nodes are not evaluated)
+            const int i = min(r0 + d, pnz - 1);
             xuo[d] = xdo[d] = 0.f;
-            if (!fastx && i < pnz) {  // x strips: per lane, clamped index, no divergence
+            if (!fastx) {
                 xuo[d] = xh_o[i * 2 * n1 + (cu.col >= 0 ? cu.col : 0)];
                 xdo[d] = xd_o[i * 2 * npml + (cx.col >= 0 ? cx.col : 0)];
             }
@@ -745,10 +784,10 @@ __device__ __forceinline__ void fwd_tile_rows(const FwiFwdArgs &A, const FwiShot
                 slot[d] = in ? slot[d] : -1;
             }
 #pragma unroll
-            for (int d = 0; d < FR; ++d) rix[d] = (slot[d] >= 0) ? A.b.rmap[slot[d] * ld + x] : -1;
+            for (int d = 0; d < FR; ++d) rix[d] = ld_sel(A.b.rmap, (long long)slot[d] * ld + x, slot[d] >= 0, -1);
         }
         if (!fastx) xuho = xh_o[(r0 + ud) * 2 * n1 + (cuh.col >= 0 ? cuh.col : 0)];
-        if (has_src) sv = A.b.srcf[sh.src_off + k];
+        sv = A.b.srcf[sh.src_off + (has_src ? k : 0)];  // used only when has_src
     }
 
     // ---- stage the own rows (with halo) for the x chain
@@ -949,22 +988,22 @@ __device__ __forceinline__ float adj_filt_d(const FwiGeomDev &g, const float *ta
 }
 
 // rows of the second x derivative (p node i <- u nodes): taps u[i+1],u[i],u[i+2],u[i-1]
-__device__ __forceinline__ float hxA(const FwiGeomDev &g, int i)
+__device__ __forceinline__ float hxA(const XConst &g, int i)
 { return (i < 0 || i >= g.pnx) ? 0.f : ((i == 0 || i == g.pnx - 1) ? g.ix : g.c0x); }
-__device__ __forceinline__ float hxB(const FwiGeomDev &g, int i)
+__device__ __forceinline__ float hxB(const XConst &g, int i)
 { return (i < 0 || i >= g.pnx) ? 0.f : ((i == 0 || i == g.pnx - 1) ? -g.ix : -g.c0x); }
-__device__ __forceinline__ float hxC(const FwiGeomDev &g, int i)
+__device__ __forceinline__ float hxC(const XConst &g, int i)
 { return (i < 1 || i >= g.pnx - 1) ? 0.f : g.c1x; }
-__device__ __forceinline__ float hxD(const FwiGeomDev &g, int i)
+__device__ __forceinline__ float hxD(const XConst &g, int i)
 { return (i < 1 || i >= g.pnx - 1) ? 0.f : -g.c1x; }
 // rows of the first x derivative (u node j <- p nodes): taps p[j],p[j-1],p[j+1],p[j-2]
-__device__ __forceinline__ float gxE(const FwiGeomDev &g, int j)
+__device__ __forceinline__ float gxE(const XConst &g, int j)
 { return (j <= 0 || j >= g.pnx) ? 0.f : ((j == 1 || j == g.pnx - 1) ? g.ix : g.c0x); }
-__device__ __forceinline__ float gxF(const FwiGeomDev &g, int j)
+__device__ __forceinline__ float gxF(const XConst &g, int j)
 { return (j <= 0 || j >= g.pnx) ? 0.f : ((j == 1 || j == g.pnx - 1) ? -g.ix : -g.c0x); }
-__device__ __forceinline__ float gxG(const FwiGeomDev &g, int j)
+__device__ __forceinline__ float gxG(const XConst &g, int j)
 { return (j <= 1 || j >= g.pnx - 1) ? 0.f : g.c1x; }
-__device__ __forceinline__ float gxH(const FwiGeomDev &g, int j)
+__device__ __forceinline__ float gxH(const XConst &g, int j)
 { return (j <= 1 || j >= g.pnx - 1) ? 0.f : -g.c1x; }
 
 // H^T gather at u node j (wi = LDS index of w'[j]) followed by C^T of the u-node CPML
@@ -973,7 +1012,8 @@ __device__ __forceinline__ float adj_u_node(const FwiGeomDev &g, const float *ta
 {
     const int pnx = g.pnx, npml = g.npml, T = 2 * npml + 1, n1 = npml + 1;
     if (j < 0 || j > pnx) return 0.f;
-    float u = hxA(g, j - 1) * Ws[wi - 1] + hxB(g, j) * Ws[wi] + hxC(g, j - 2) * Ws[wi - 2] + hxD(g, j + 1) * Ws[wi + 1];
+    const XConst xk = xconst(g);
+    float u = hxA(xk, j - 1) * Ws[wi - 1] + hxB(xk, j) * Ws[wi] + hxC(xk, j - 2) * Ws[wi - 2] + hxD(xk, j + 1) * Ws[wi + 1];
     const bool own = (j >= x0) && (j < x0 + TX);
     if (j <= npml) {
         float c = xh_o[r * 2 * n1 + j];
@@ -1015,7 +1055,8 @@ __device__ __forceinline__ float adj_u_edge(const FwiGeomDev &g, const float *ta
     const int pnx = g.pnx, npml = g.npml, T = 2 * npml + 1;
     cn = 0.f;
     if (j < 0 || j > pnx) return 0.f;
-    float u = hxA(g, j - 1) * Ws[wi - 1] + hxB(g, j) * Ws[wi] + hxC(g, j - 2) * Ws[wi - 2] + hxD(g, j + 1) * Ws[wi + 1];
+    const XConst xk = xconst(g);
+    float u = hxA(xk, j - 1) * Ws[wi - 1] + hxB(xk, j) * Ws[wi] + hxC(xk, j - 2) * Ws[wi - 2] + hxD(xk, j + 1) * Ws[wi + 1];
     if (j <= npml) {
         cn = TB(1, 2 * j) * co + TB(0, 2 * j) * u;
         u = u + cn;
@@ -1079,8 +1120,8 @@ __device__ __forceinline__ void adj_xblock(const FwiGeomDev &g, const float *tab
             } else {
                 const int i = x0 + lane, ui = lane + 1;
                 box[d] = (i >= g.pnx) ? 0.f
-                                      : gxE(g, i) * U[ui] + gxF(g, i + 1) * U[ui + 1] + gxG(g, i - 1) * U[ui - 1] +
-                                            gxH(g, i + 2) * U[ui + 2];
+                                      : gxE(xconst(g), i) * U[ui] + gxF(xconst(g), i + 1) * U[ui + 1] +
+                                            gxG(xconst(g), i - 1) * U[ui - 1] + gxH(xconst(g), i + 2) * U[ui + 2];
             }
         }
     }
@@ -1343,7 +1384,7 @@ __device__ __forceinline__ void adj_tile_rows(const FwiAdjArgs &A, const FwiShot
 #pragma unroll
     for (int j = 0; j < FPR; ++j) {
         const int zr = r0 - 3 + j;
-        m[j] = (!EDGE || (zr >= 0 && zr < pnz)) ? buf_ld(rnow, vo, zr * rowb) : 0.f;
+        m[j] = EDGE ? buf_ld_row(rnow, vo, zr, pnz, rowb) : buf_ld(rnow, vo, zr * rowb);
     }
     XHalo<HP> xh;
     xhalo_plan(xh, lane, x0, FR);
@@ -1359,10 +1400,9 @@ __device__ __forceinline__ void adj_tile_rows(const FwiAdjArgs &A, const FwiShot
 #pragma unroll
     for (int d = 0; d < FR; ++d) {
         const int i = r0 + d;
-        const bool ok = !EDGE || i < pnz;
-        mo[d] = ok ? buf_ld(rold, vo, i * rowb) : 0.f;
+        mo[d] = EDGE ? buf_ld_row(rold, vo, i, pnz, rowb) : buf_ld(rold, vo, i * rowb);
         if (FUSED) mk[d] = ek[d];
-        else mk[d] = ok ? buf_ld(rk, vo, i * rowb) : 0.f;
+        else mk[d] = EDGE ? buf_ld_row(rk, vo, i, pnz, rowb) : buf_ld(rk, vo, i * rowb);
     }
 
     // chi after step k goes to buffer (k & 1); it was read from buffer ((k+1) & 1)
@@ -1399,22 +1439,25 @@ __device__ __forceinline__ void adj_tile_rows(const FwiAdjArgs &A, const FwiShot
     const bool zplain = !EDGE || !(CLS & FWI_CLS_Z) || (r0 >= 6 && r0 + FR + 2 < pnz - npml);  // plain z chain
     const bool has_recv = EDGE && (CLS & FWI_CLS_RS) && sh.rz_lo <= sh.rz_hi && sh.rz_lo < r0 + FR && sh.rz_hi >= r0;
     if (EDGE && !fastx) {
-        cf = fwd_ux_coef(g, tab, x);
+        const XConst xk = xconst(g);
+        cf = fwd_ux_coef(xk, tab, x);
 #pragma unroll
-        for (int q = 0; q < HP; ++q) cfh[q] = fwd_ux_coef(g, tab, xh.col[q]);
+        for (int q = 0; q < HP; ++q) cfh[q] = fwd_ux_coef(xk, tab, xh.col[q]);
         const int j = x0 - 1 + lane, jh = x0 - 1 + uc;
-        cu = fwd_u_coef(g, tab, j);
-        cuh = fwd_u_coef(g, tab, jh);
-        hA = hxA(g, j - 1); hB = hxB(g, j); hC = hxC(g, j - 2); hD = hxD(g, j + 1);
-        hhA = hxA(g, jh - 1); hhB = hxB(g, jh); hhC = hxC(g, jh - 2); hhD = hxD(g, jh + 1);
-        gE = gxE(g, x); gF = gxF(g, x + 1); gG = gxG(g, x - 1); gH = gxH(g, x + 2);
+        cu = fwd_u_coef(xk, tab, j);
+        cuh = fwd_u_coef(xk, tab, jh);
+        hA = hxA(xk, j - 1); hB = hxB(xk, j); hC = hxC(xk, j - 2); hD = hxD(xk, j + 1);
+        hhA = hxA(xk, jh - 1); hhB = hxB(xk, jh); hhC = hxC(xk, jh - 2); hhD = hxD(xk, jh + 1);
+        gE = gxE(xk, x); gF = gxF(xk, x + 1); gG = gxG(xk, x - 1); gH = gxH(xk, x + 2);
     }
     if (EDGE) {
 #pragma unroll
         for (int d = 0; d < FR; ++d) {
-            const int i = r0 + d;
+            // x strips: per lane, clamped index, no divergence; rows past the
+            // grid read the last row (their nodes are not evaluated)
+            const int i = min(r0 + d, pnz - 1);
             xdo[d] = xuo[d] = 0.f;
-            if (!fastx && i < pnz) {  // x strips: per lane, clamped index, no divergence
+            if (!fastx) {
                 xdo[d] = xd_o[i * 2 * npml + (cf.col >= 0 ? cf.col : 0)];
                 xuo[d] = xh_o[i * 2 * n1 + (cu.col >= 0 ? cu.col : 0)];
             }
@@ -1434,12 +1477,12 @@ __device__ __forceinline__ void adj_tile_rows(const FwiAdjArgs &A, const FwiShot
                 slot[d] = in ? slot[d] : -1;
             }
 #pragma unroll
-            for (int d = 0; d < FR; ++d) rix[d] = (slot[d] >= 0) ? A.b.rmap[slot[d] * ld + x] : -1;
+            for (int d = 0; d < FR; ++d) rix[d] = ld_sel(A.b.rmap, (long long)slot[d] * ld + x, slot[d] >= 0, -1);
 #pragma unroll
             for (int d = 0; d < FR; ++d) {
                 rhas[d] = rix[d] >= 0;
-                rsv[d] = (slot[d] >= 0) ? A.res[sh.trace_off + (long long)k * sh.nr + (rix[d] >= 0 ? rix[d] : 0)]
-                                        : 0.f;
+                rsv[d] = ld_sel(A.res, sh.trace_off + (long long)k * sh.nr + (rix[d] >= 0 ? rix[d] : 0),
+                                slot[d] >= 0, 0.f);
             }
         }
 #pragma unroll
@@ -1452,12 +1495,12 @@ __device__ __forceinline__ void adj_tile_rows(const FwiAdjArgs &A, const FwiShot
 #pragma unroll
             for (int j = 0; j < FPR; ++j) {
                 const int r = r0 - 3 + j;
-                zdo[j] = (r >= pnz - npml && r < pnz) ? zd_o[(r - (pnz - npml)) * ld + x] : 0.f;
+                zdo[j] = ld_sel(zd_o, (long long)(r - (pnz - npml)) * ld + x, r >= pnz - npml && r < pnz, 0.f);
             }
 #pragma unroll
             for (int n = 0; n < FVN; ++n) {
                 const int jn = r0 - 1 + n;
-                zho[n] = (jn >= pnz - npml && jn < pnz) ? zh_o[(jn - (pnz - npml)) * ld + x] : 0.f;
+                zho[n] = ld_sel(zh_o, (long long)(jn - (pnz - npml)) * ld + x, jn >= pnz - npml && jn < pnz, 0.f);
             }
         }
     }
@@ -1934,7 +1977,7 @@ __device__ __forceinline__ void rec_tile_shot(const FwiRecArgs &A, const FwiShot
 #pragma unroll
     for (int j = 0; j < RPR; ++j) {
         const int zr = r0 - 3 + j;
-        p[j] = (!EDGE || (zr >= 0 && zr < nz)) ? buf_ld(rn, vo, zr * rowb) : 0.f;
+        p[j] = EDGE ? buf_ld_row(rn, vo, zr, nz, rowb) : buf_ld(rn, vo, zr * rowb);
         if (EDGE && !colok) p[j] = 0.f;
     }
     XHalo<RHP> xh;
@@ -1950,10 +1993,9 @@ __device__ __forceinline__ void rec_tile_shot(const FwiRecArgs &A, const FwiShot
 #pragma unroll
     for (int d = 0; d < RR; ++d) {
         const int i = r0 + d;
-        const bool ok = !EDGE || i < nz;
-        pq[d] = ok ? buf_ld(rq, vo, i * rowb) : 0.f;
+        pq[d] = EDGE ? buf_ld_row(rq, vo, i, nz, rowb) : buf_ld(rq, vo, i * rowb);
         if (FUSED) pm[d] = mu1[d];
-        else pm[d] = ok ? buf_ld(rm, vo, i * rowb) : 0.f;
+        else pm[d] = EDGE ? buf_ld_row(rm, vo, i, nz, rowb) : buf_ld(rm, vo, i * rowb);
     }
 
     wave_sync();  // the previous shot's reads of Ps / Us are done
@@ -1992,9 +2034,10 @@ __device__ __forceinline__ void rec_tile_shot(const FwiRecArgs &A, const FwiShot
             const bool fr = colok && i < nz && !(i >= 3 && i < nz - 3 && bx >= 3 && bx < nx - 3);
             const int pos = fr ? frame_pos(i, bx, nx, nz) : 0;
             xsv[d] = A.xstrip[so2 + pos];
-            psv[d] = (k >= 2) ? A.pstrip[so0 + pos] : 0.f;
+            psv[d] = ld_sel(A.pstrip, so0 + pos, k >= 2, 0.f);
         }
-        if (sh.isz >= r0 && sh.isz < r0 + RR) fsrc = A.b.srcf[sh.src_off + k];
+        // used only on the source row
+        fsrc = A.b.srcf[sh.src_off + ((sh.isz >= r0 && sh.isz < r0 + RR) ? k : 0)];
     }
 #pragma unroll
     for (int d = 0; d < RR; ++d) {
@@ -2056,7 +2099,7 @@ __device__ __forceinline__ void rec_tile_kernel(const FwiRecArgs *__restrict__ A
         const brsrc rk = buf_rsrc(A.b.ekp, (unsigned)(A.b.fs * sizeof(float)));
         const int vo = (colok ? bx + npml : npml) * 4;
 #pragma unroll
-        for (int d = 0; d < RR; ++d) pk[d] = (r0 + d < nz) ? buf_ld(rk, vo, (r0 + d) * ld * 4) : 0.f;
+        for (int d = 0; d < RR; ++d) pk[d] = buf_ld_row(rk, vo, r0 + d, nz, ld * 4);
     }
     double acc[RR];
 #pragma unroll
