@@ -1953,12 +1953,17 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_rec_step(FwiRecArgs A, c
 #define RHP ((6 * RR + 63) / 64)
 static_assert(RR % 4 == 0 && 3 * RR <= 64, "reconstruction rows per wave");
 
-// FUSED: mu^{k+1} of the rows comes from the caller (mu1).  bx0 may be
-// negative (tiles of the padded grid start npml columns left of the grid).
+// The loads of one shot's reconstruction step on a tile: p^k (rows with a
+// 3-row halo, the x halo), p^{k+1} and, unless FUSED, mu^{k+1}.  They do not
+// depend on the adjoint step, so the fused backward step issues them before
+// the adjoint's arithmetic.  bx0 may be negative (tiles of the padded grid
+// start npml columns left of the grid).
+struct RecIn {
+    float p[RPR], hv[RHP], pq[RR], pm[RR];
+};
 template <bool EDGE, bool FUSED = false>
-__device__ __forceinline__ void rec_tile_shot(const FwiRecArgs &A, const FwiShotDev &sh, int s, int k, int bx0,
-                                              int r0, int lane, float (*Ps)[HC], float (*Us)[HC], const float *pk,
-                                              double *acc, const float *mu1 = nullptr)
+__device__ __forceinline__ void rec_tile_load(const FwiRecArgs &A, int s, int k, int bx0, int r0, int lane,
+                                              RecIn &in)
 {
     const FwiGeomDev &g = A.g;
     const int npml = g.npml, nx = g.nx, nz = g.nz, ld = g.ld;
@@ -1967,36 +1972,56 @@ __device__ __forceinline__ void rec_tile_shot(const FwiRecArgs &A, const FwiShot
     const long long so = (long long)s * A.b.fs;
     const int cur = k & 1;
     const float *pn = (cur ? A.P1 : A.P0) + so;  // p^k
-    float *pp = (cur ? A.P0 : A.P1) + so;        // p^{k+1} -> p^{k-1}
+    const float *pp = (cur ? A.P0 : A.P1) + so;  // p^{k+1}
     const float *mu = (cur ? A.M0 : A.M1) + so;  // mu^{k+1}
     const unsigned fbytes = (unsigned)(A.b.fs * sizeof(float));
     const brsrc rn = buf_rsrc(pn, fbytes), rq = buf_rsrc(pp, fbytes), rm = buf_rsrc(mu, fbytes);
     const int vo = (colok ? fx : npml) * 4, rowb = ld * 4;
-
-    float p[RPR], pq[RR], pm[RR];
 #pragma unroll
     for (int j = 0; j < RPR; ++j) {
         const int zr = r0 - 3 + j;
-        p[j] = EDGE ? buf_ld_row(rn, vo, zr, nz, rowb) : buf_ld(rn, vo, zr * rowb);
-        if (EDGE && !colok) p[j] = 0.f;
+        in.p[j] = EDGE ? buf_ld_row(rn, vo, zr, nz, rowb) : buf_ld(rn, vo, zr * rowb);
+        if (EDGE && !colok) in.p[j] = 0.f;
     }
     XHalo<RHP> xh;
     xhalo_plan(xh, lane, bx0, RR);
-    float hv[RHP];
 #pragma unroll
     for (int q = 0; q < RHP; ++q) {
         bool ok = xh.in[q];
         if (EDGE) ok = ok && r0 + xh.d[q] < nz && xh.col[q] >= 0 && xh.col[q] < nx;
-        hv[q] = buf_ld(rn, ok ? (xh.d[q] * ld + xh.col[q] + npml) * 4 : vo, r0 * rowb);
-        hv[q] = ok ? hv[q] : 0.f;
+        in.hv[q] = buf_ld(rn, ok ? (xh.d[q] * ld + xh.col[q] + npml) * 4 : vo, r0 * rowb);
+        in.hv[q] = ok ? in.hv[q] : 0.f;
     }
 #pragma unroll
     for (int d = 0; d < RR; ++d) {
         const int i = r0 + d;
-        pq[d] = EDGE ? buf_ld_row(rq, vo, i, nz, rowb) : buf_ld(rq, vo, i * rowb);
-        if (FUSED) pm[d] = mu1[d];
-        else pm[d] = EDGE ? buf_ld_row(rm, vo, i, nz, rowb) : buf_ld(rm, vo, i * rowb);
+        in.pq[d] = EDGE ? buf_ld_row(rq, vo, i, nz, rowb) : buf_ld(rq, vo, i * rowb);
+        if (!FUSED) in.pm[d] = EDGE ? buf_ld_row(rm, vo, i, nz, rowb) : buf_ld(rm, vo, i * rowb);
     }
+}
+
+// One shot's reconstruction + imaging step on a tile from the loads of
+// rec_tile_load.  FUSED: mu^{k+1} of the rows comes from the caller (mu1).
+template <bool EDGE, bool FUSED = false>
+__device__ __forceinline__ void rec_tile_shot(const FwiRecArgs &A, const FwiShotDev &sh, int s, int k, int bx0,
+                                              int r0, int lane, float (*Ps)[HC], float (*Us)[HC], const float *pk,
+                                              double *acc, const RecIn &in, const float *mu1 = nullptr)
+{
+    const FwiGeomDev &g = A.g;
+    const int npml = g.npml, nx = g.nx, nz = g.nz, ld = g.ld;
+    const int bx = bx0 + lane, fx = bx + npml;
+    const bool colok = !EDGE || (bx >= 0 && bx < nx);
+    const long long so = (long long)s * A.b.fs;
+    const int cur = k & 1;
+    float *pp = (cur ? A.P0 : A.P1) + so;        // p^{k+1} -> p^{k-1}
+    const unsigned fbytes = (unsigned)(A.b.fs * sizeof(float));
+    const brsrc rq = buf_rsrc(pp, fbytes);
+    const int vo = (colok ? fx : npml) * 4, rowb = ld * 4;
+    const float *p = in.p, *pq = in.pq;
+    const float *pm = FUSED ? mu1 : in.pm;
+    XHalo<RHP> xh;
+    xhalo_plan(xh, lane, bx0, RR);
+    const float *hv = in.hv;
 
     wave_sync();  // the previous shot's reads of Ps / Us are done
 #pragma unroll
@@ -2108,7 +2133,9 @@ __device__ __forceinline__ void rec_tile_kernel(const FwiRecArgs *__restrict__ A
     for (int s = s_lo + w; s < s_hi; s += 4) {
         const FwiShotDev sh = shots[s];
         if (k >= sh.nt) continue;
-        rec_tile_shot<EDGE>(A, sh, s, k, bx0, r0, lane, lds[w][0], lds[w][1], pk, acc);
+        RecIn rin;
+        rec_tile_load<EDGE>(A, s, k, bx0, r0, lane, rin);
+        rec_tile_shot<EDGE>(A, sh, s, k, bx0, r0, lane, lds[w][0], lds[w][1], pk, acc, rin);
     }
     // steps outside the imaging window only reconstruct (uniform per launch)
     if (!fwi_image_step(A.img_jump, A.img_k0, A.img_k1, k)) return;
@@ -2169,7 +2196,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 // ============================================================================
 // Occupancy: 96 VGPRs without spilling at 5 waves per SIMD (the default
 // schedule takes 115, 4 waves).
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_bwd_tile_int(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_bwd_tile_int(
     const FwiAdjArgs *__restrict__ Aa, const FwiRecArgs *__restrict__ Ar, const FwiShotDev *__restrict__ shots,
     const int *__restrict__ items, int nit, const int *kbase, int koff, int gsize)
 {
@@ -2207,10 +2234,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     for (int s = s_lo + w; s < s_hi; s += 4) {
         const FwiShotDev sh = shots[s];
         if (k >= sh.nt) continue;
+        // the reconstruction's loads first: they do not depend on the adjoint
+        // step, so they are in flight during its arithmetic
+        RecIn rin;
+        rec_tile_load<false, true>(R, s, k, x0 - npml, r0, lane, rin);
         float mu1[FR];
         wave_sync();  // the previous shot's reads of the wave's LDS rows are done
         adj_tile_rows<false, true>(A, sh, nullptr, lds[w][0], lds[w][1], s, k, x0, r0, lane, ek, mu1);
-        rec_tile_shot<false, true>(R, sh, s, k, x0 - npml, r0, lane, lds[w][0], lds[w][1], ek, acc, mu1);
+        rec_tile_shot<false, true>(R, sh, s, k, x0 - npml, r0, lane, lds[w][0], lds[w][1], ek, acc, rin, mu1);
     }
     // steps outside the imaging window only propagate (uniform per launch)
     if (!fwi_image_step(R.img_jump, R.img_k0, R.img_k1, k)) return;
