@@ -157,6 +157,7 @@ struct FwiFwdArgs {
 #define FWI_FWD_FRAMES 0   // record traces, save the boundary frames (boundary-saving reconstruction)
 #define FWI_FWD_TRACES 1   // record traces only (checkpointed forward pass)
 #define FWI_FWD_XSEG 2     // store X on the physical grid only (segment recompute)
+#define FWI_FWD_PLAIN 3    // record nothing (advancing a restored state)
 
 struct FwiAdjArgs {
     FwiGeomDev g;

@@ -59,6 +59,75 @@ hipError_t fwi_launch_ref_read_bdr(const float *ud, const float *lr, float *pb, 
 
 static char g_err[512] = "";
 
+// ---------------------------------------------------------------------------
+// Binomial checkpointing of the backward pass (the "revolve" schedule of
+// Griewank and Walther, ACM TOMS 26 (2000) 19-45).
+//
+// The backward pass needs the forward states S_k (p^k, p^{k-1} and the CPML
+// memory before forward step k) for k = t-1 down to 0.  A turn at k is one
+// forward step from S_k that gives X^k, then the adjoint step k and its
+// imaging.  With s snapshot slots, one of them holding S_a (the live state),
+// steps [a, b) are reversed as: advance j steps to S_{a+j}, snapshot it,
+// reverse [a+j, b) with s-1 slots, restore S_a, reverse [a, a+j) with s
+// slots; with one slot, restore S_a and advance to each step in turn.
+// beta(s, r) = C(s + r, s) steps can be reversed with s slots and no step
+// swept forward more than r times.  For l = b - a in (beta(s, r-1),
+// beta(s, r)] the splits j in [max(beta(s, r-2), l - beta(s-1, r)),
+// min(beta(s, r-1), l - beta(s-1, r-1))] (beta(s, -1) = 0) reach the least
+// number of advance steps, r l - beta(s+1, r-1); the schedule takes the
+// smallest.
+// ---------------------------------------------------------------------------
+enum { RV_ADVANCE = 0, RV_TAKESHOT = 1, RV_RESTORE = 2, RV_TURN = 3 };
+struct RvAct {
+    int op, a, b, slot;  // advance S_a -> S_b; snapshot / restore S_a in slot; turn at a
+};
+
+static long long rv_beta(long long s, long long r)
+{
+    const long long cap = 1LL << 61;
+    long long v = 1;  // C(s + i, i) for i = 0 .. r
+    for (long long i = 1; i <= r; i++) {
+        if (v > cap / (s + i)) return cap;
+        v = v * (s + i) / i;
+    }
+    return v;
+}
+
+static void rv_reverse(int a, int b, int slot_a, int s, int next_slot, std::vector<RvAct> &out)
+{
+    while (b > a) {
+        if (b - a == 1) {
+            out.push_back({RV_TURN, a, a + 1, -1});
+            return;
+        }
+        if (s <= 1) {  // S_a only: advance to every step from it, the last step first
+            for (int k = b - 1; k >= a; k--) {
+                if (k < b - 1) out.push_back({RV_RESTORE, a, a, slot_a});
+                if (k > a) out.push_back({RV_ADVANCE, a, k, -1});
+                out.push_back({RV_TURN, k, k + 1, -1});
+            }
+            return;
+        }
+        const long long l = b - a;
+        long long r = 0;
+        while (rv_beta(s, r) < l) r++;
+        const long long lo = std::max(r >= 2 ? rv_beta(s, r - 2) : 0LL, l - rv_beta(s - 1, r));
+        const int j = (int)std::max(1LL, lo);
+        out.push_back({RV_ADVANCE, a, a + j, -1});
+        out.push_back({RV_TAKESHOT, a + j, a + j, next_slot});
+        rv_reverse(a + j, b, next_slot, s - 1, next_slot + 1, out);
+        out.push_back({RV_RESTORE, a, a, slot_a});
+        b = a + j;  // then [a, a + j) with the same s slots (tail iteration)
+    }
+}
+
+// the schedule for t steps with `snaps` slots, S_0 in slot 0
+static void rv_schedule(int t, int snaps, std::vector<RvAct> &out)
+{
+    out.clear();
+    if (t >= 1) rv_reverse(0, t, 0, std::max(1, snaps), 1, out);
+}
+
 struct fwi_hip_ctx {
     fwi_hip_desc d;
     FwiGeomDev g;
@@ -107,6 +176,16 @@ struct fwi_hip_ctx {
     float *ck_cpml = nullptr, *ck_xseg = nullptr;
     float *xh2 = nullptr, *xd2 = nullptr, *zh2 = nullptr, *zd2 = nullptr;
     long long ck_nseg = 0, ck_K = 0, ck_S = 0;  // capacity of the snapshot buffers
+    // binomial checkpointing (set_revolve): rv_snaps slots (-1: from the HBM
+    // budget); the backward method the last forward pass prepared (0 frames,
+    // 1 uniform snapshots, 2 binomial) with its slots and schedule (the
+    // actions before the first turn ran during the forward pass); the
+    // forward steps the last adjoint pass recomputed
+    int rv_snaps = 0, bwd_method = 0, bwd_snaps = 0;
+    std::vector<RvAct> rv_sched;
+    int rv_sched_t = -1, rv_sched_s = -1;
+    size_t rv_prefix = 0;
+    long long recomputed = 0;
     FwiFwdArgs *fwd_rc_d = nullptr;
     FwiFwdArgs fwd_rc_h;
     float *syn_tm = nullptr, *syn = nullptr, *obs = nullptr, *res = nullptr, *res_tm = nullptr;
@@ -310,25 +389,22 @@ extern "C" void fwi_hip_destroy(fwi_hip_ctx *c)
     delete c;
 }
 
-extern "C" int fwi_hip_create(const fwi_hip_desc *desc, fwi_hip_ctx **out)
+static int check_desc(fwi_hip_ctx *c, const fwi_hip_desc *desc)
 {
-    if (!desc || !out) return set_err(nullptr, FWI_HIP_EINVAL, "fwi_hip_create: NULL argument");
-    *out = nullptr;
-    fwi_hip_ctx *c = new fwi_hip_ctx();
-    c->d = *desc;
-    const fwi_hip_grid &gr = desc->grid;
-    int rc = check_grid(c, gr);
+    int rc = check_grid(c, desc->grid);
     if (!rc && (desc->max_shots < 1 || desc->max_nt < 1 || desc->max_nr < 0))
         rc = set_err(c, FWI_HIP_EINVAL, "max_shots, max_nt must be >= 1 and max_nr >= 0");
     if (!rc && desc->tile_z != 0 && desc->tile_z < 8) rc = set_err(c, FWI_HIP_EINVAL, "tile_z must be 0 or >= 8");
     if (!rc && (desc->shots_per_group < 0 || desc->shots_per_group > FWI_MAXSG))
         rc = set_err(c, FWI_HIP_EINVAL, "shots_per_group must be 0..%d", FWI_MAXSG);
-    if (rc) {
-        snprintf(g_err, sizeof(g_err), "%s", c->err);
-        delete c;
-        return rc;
-    }
-    FwiGeomDev &g = c->g;
+    return rc;
+}
+
+// the device geometry of a descriptor (grid shapes, tiles, FD coefficients)
+static FwiGeomDev make_geom(const fwi_hip_desc *desc)
+{
+    const fwi_hip_grid &gr = desc->grid;
+    FwiGeomDev g;
     memset(&g, 0, sizeof(g));
     g.nx = gr.nx;
     g.nz = gr.nz;
@@ -352,6 +428,88 @@ extern "C" int fwi_hip_create(const fwi_hip_desc *desc, fwi_hip_ctx **out)
     g.c0x = co.c0x; g.c1x = co.c1x; g.ix = co.ix;
     g.c0z = co.c0z; g.c1z = co.c1z; g.iz = co.iz;
     g.iz840 = co.iz840; g.iz24 = co.iz24;
+    return g;
+}
+
+// strides of the per-shot field and CPML buffers of a geometry
+static void set_strides(fwi_hip_ctx *c)
+{
+    const FwiGeomDev &g = c->g;
+    const int n1 = g.npml + 1;
+    c->fs = (long long)g.pnz * g.ld;
+    c->xh_b = (long long)g.pnz * 2 * n1;
+    c->xd_b = (long long)g.pnz * 2 * g.npml;
+    c->zh_b = (long long)n1 * g.ld;
+    c->zd_b = (long long)g.npml * g.ld;
+}
+
+// every device buffer a context allocates when it is created, with its size
+// (the same table sizes the memory plan, fwi_hip_plan_memory)
+static std::vector<std::pair<void **, size_t>> create_table(fwi_hip_ctx *c)
+{
+    const fwi_hip_desc *desc = &c->d;
+    const fwi_hip_grid &gr = desc->grid;
+    const FwiGeomDev &g = c->g;
+    const long long S = desc->max_shots, NT = desc->max_nt, NR = desc->max_nr;
+    const int maxgroups = (int)S;  // image accumulators: at most one group per shot, whatever the grouping
+    const size_t strips = sizeof(int) * S * g.ntx * ((g.pnz + FWI_STRIP - 1) / FWI_STRIP);
+    const size_t items = sizeof(int) * S * g.bntx * ((g.nz + FWI_REC_ROWS - 1) / FWI_REC_ROWS);
+    return {
+        {(void **)&c->ekp, sizeof(float) * c->fs},
+        {(void **)&c->vel, sizeof(float) * gr.nx * gr.nz},
+        {(void **)&c->P0, sizeof(float) * S * c->fs},
+        {(void **)&c->P1, sizeof(float) * S * c->fs},
+        {(void **)&c->M0, sizeof(float) * S * c->fs},
+        {(void **)&c->M1, sizeof(float) * S * c->fs},
+        {(void **)&c->xh, sizeof(float) * S * 2 * c->xh_b},
+        {(void **)&c->xd, sizeof(float) * S * 2 * c->xd_b},
+        {(void **)&c->zh, sizeof(float) * S * 2 * c->zh_b},
+        {(void **)&c->zd, sizeof(float) * S * 2 * c->zd_b},
+        {(void **)&c->tab, sizeof(float) * 6 * (2 * g.npml + 1)},
+        {(void **)&c->syn_tm, sizeof(float) * S * NT * NR},
+        {(void **)&c->syn, sizeof(float) * S * NT * NR},
+        {(void **)&c->obs, sizeof(float) * S * NT * NR},
+        {(void **)&c->res, sizeof(float) * S * NT * NR},
+        {(void **)&c->res_tm, sizeof(float) * S * NT * NR},
+        {(void **)&c->gacc, sizeof(double) * maxgroups * gr.nx * gr.nz},
+        {(void **)&c->gk, sizeof(double) * gr.nx * gr.nz},
+        {(void **)&c->grad, sizeof(double) * gr.nx * gr.nz},
+        {(void **)&c->partial, sizeof(double) * c->npartial},
+        {(void **)&c->shots_d, sizeof(FwiShotDev) * S},
+        {(void **)&c->rrow, sizeof(int) * S * g.pnz},
+        {(void **)&c->kbase, sizeof(int) * 64},
+        {(void **)&c->fwd_args_d, sizeof(FwiFwdArgs)},
+        {(void **)&c->tl_fwd[0], strips},
+        {(void **)&c->tl_fwd[1], strips},
+        {(void **)&c->tl_adj[0], strips},
+        {(void **)&c->tl_adj[1], strips},
+        {(void **)&c->tl_rec[0], items},
+        {(void **)&c->tl_rec[1], items},
+        {(void **)&c->tl_bwd[0], strips},
+        {(void **)&c->tl_bwd[1], strips},
+        {(void **)&c->tl_bwd[2], strips},
+        {(void **)&c->tl_bwd[3], strips},
+        {(void **)&c->adj_args_d, sizeof(FwiAdjArgs)},
+        {(void **)&c->rec_args_d, sizeof(FwiRecArgs)},
+        {(void **)&c->fwd_rc_d, sizeof(FwiFwdArgs)},
+    };
+}
+
+extern "C" int fwi_hip_create(const fwi_hip_desc *desc, fwi_hip_ctx **out)
+{
+    if (!desc || !out) return set_err(nullptr, FWI_HIP_EINVAL, "fwi_hip_create: NULL argument");
+    *out = nullptr;
+    fwi_hip_ctx *c = new fwi_hip_ctx();
+    c->d = *desc;
+    const fwi_hip_grid &gr = desc->grid;
+    int rc = check_desc(c, desc);
+    if (rc) {
+        snprintf(g_err, sizeof(g_err), "%s", c->err);
+        delete c;
+        return rc;
+    }
+    c->g = make_geom(desc);
+    FwiGeomDev &g = c->g;
     c->sg = desc->shots_per_group ? desc->shots_per_group : std::min(FWI_MAXSG, desc->max_shots);
 
     if (desc->stream) {
@@ -382,55 +540,11 @@ extern "C" int fwi_hip_create(const fwi_hip_desc *desc, fwi_hip_ctx **out)
         }
     }
 
-    const long long S = desc->max_shots, NT = desc->max_nt, NR = desc->max_nr;
-    const int n1 = g.npml + 1;
-    c->fs = (long long)g.pnz * g.ld;
-    c->xh_b = (long long)g.pnz * 2 * n1;
-    c->xd_b = (long long)g.pnz * 2 * g.npml;
-    c->zh_b = (long long)n1 * g.ld;
-    c->zd_b = (long long)g.npml * g.ld;
-    const int maxgroups = (int)S;  // image accumulators: at most one group per shot, whatever the grouping
-    struct A { void **p; size_t bytes; } plan[] = {
-        {(void **)&c->ekp, sizeof(float) * c->fs},
-        {(void **)&c->vel, sizeof(float) * gr.nx * gr.nz},
-        {(void **)&c->P0, sizeof(float) * S * c->fs},
-        {(void **)&c->P1, sizeof(float) * S * c->fs},
-        {(void **)&c->M0, sizeof(float) * S * c->fs},
-        {(void **)&c->M1, sizeof(float) * S * c->fs},
-        {(void **)&c->xh, sizeof(float) * S * 2 * c->xh_b},
-        {(void **)&c->xd, sizeof(float) * S * 2 * c->xd_b},
-        {(void **)&c->zh, sizeof(float) * S * 2 * c->zh_b},
-        {(void **)&c->zd, sizeof(float) * S * 2 * c->zd_b},
-        {(void **)&c->tab, sizeof(float) * 6 * (2 * g.npml + 1)},
-        {(void **)&c->syn_tm, sizeof(float) * S * NT * NR},
-        {(void **)&c->syn, sizeof(float) * S * NT * NR},
-        {(void **)&c->obs, sizeof(float) * S * NT * NR},
-        {(void **)&c->res, sizeof(float) * S * NT * NR},
-        {(void **)&c->res_tm, sizeof(float) * S * NT * NR},
-        {(void **)&c->gacc, sizeof(double) * maxgroups * gr.nx * gr.nz},
-        {(void **)&c->gk, sizeof(double) * gr.nx * gr.nz},
-        {(void **)&c->grad, sizeof(double) * gr.nx * gr.nz},
-        {(void **)&c->partial, sizeof(double) * c->npartial},
-        {(void **)&c->shots_d, sizeof(FwiShotDev) * S},
-        {(void **)&c->rrow, sizeof(int) * S * g.pnz},
-        {(void **)&c->kbase, sizeof(int) * 64},
-        {(void **)&c->fwd_args_d, sizeof(FwiFwdArgs)},
-        {(void **)&c->tl_fwd[0], sizeof(int) * S * g.ntx * ((g.pnz + FWI_STRIP - 1) / FWI_STRIP)},
-        {(void **)&c->tl_fwd[1], sizeof(int) * S * g.ntx * ((g.pnz + FWI_STRIP - 1) / FWI_STRIP)},
-        {(void **)&c->tl_adj[0], sizeof(int) * S * g.ntx * ((g.pnz + FWI_STRIP - 1) / FWI_STRIP)},
-        {(void **)&c->tl_adj[1], sizeof(int) * S * g.ntx * ((g.pnz + FWI_STRIP - 1) / FWI_STRIP)},
-        {(void **)&c->tl_rec[0], sizeof(int) * S * g.bntx * ((g.nz + FWI_REC_ROWS - 1) / FWI_REC_ROWS)},
-        {(void **)&c->tl_rec[1], sizeof(int) * S * g.bntx * ((g.nz + FWI_REC_ROWS - 1) / FWI_REC_ROWS)},
-        {(void **)&c->tl_bwd[0], sizeof(int) * S * g.ntx * ((g.pnz + FWI_STRIP - 1) / FWI_STRIP)},
-        {(void **)&c->tl_bwd[1], sizeof(int) * S * g.ntx * ((g.pnz + FWI_STRIP - 1) / FWI_STRIP)},
-        {(void **)&c->tl_bwd[2], sizeof(int) * S * g.ntx * ((g.pnz + FWI_STRIP - 1) / FWI_STRIP)},
-        {(void **)&c->tl_bwd[3], sizeof(int) * S * g.ntx * ((g.pnz + FWI_STRIP - 1) / FWI_STRIP)},
-        {(void **)&c->adj_args_d, sizeof(FwiAdjArgs)},
-        {(void **)&c->rec_args_d, sizeof(FwiRecArgs)},
-        {(void **)&c->fwd_rc_d, sizeof(FwiFwdArgs)},
-    };
+    const long long S = desc->max_shots;
+    set_strides(c);
+    const auto plan = create_table(c);
     for (auto &a : plan) {
-        rc = dalloc(c, a.p, a.bytes);
+        rc = dalloc(c, a.first, a.second);
         if (rc) {
             char keep[512];
             snprintf(keep, sizeof(keep), "%s", c->err);
@@ -686,28 +800,18 @@ static long long cpml_floats(const fwi_hip_ctx *c, long long S)
     return S * 2 * (c->xh_b + c->xd_b + c->zh_b + c->zd_b);
 }
 
+static int ensure_cpml2(fwi_hip_ctx *c);
+static void free_ckpt(fwi_hip_ctx *c);
+
 // snapshot, scale and segment buffers for the current shots and nt
 static int ensure_ckpt(fwi_hip_ctx *c)
 {
     const long long S = c->nshots, K = c->ckpt_every, nseg = (c->kmax + K - 1) / K;
     const long long nb = (long long)c->g.nx * c->g.nz;
-    if (c->ck_store && nseg <= c->ck_nseg && S <= c->ck_S && K <= c->ck_K) return FWI_HIP_OK;
-    dfree(c, c->ck_store);
-    dfree(c, c->ck_scale);
-    dfree(c, c->ck_cpml);
-    dfree(c, c->ck_xseg);
-    c->ck_store = nullptr;
-    c->ck_scale = nullptr;
-    c->ck_cpml = c->ck_xseg = nullptr;
-    c->ck_nseg = c->ck_S = c->ck_K = 0;
-    int rc = FWI_HIP_OK;
-    if (!c->xh2) {
-        if ((rc = dalloc(c, (void **)&c->xh2, sizeof(float) * c->d.max_shots * 2 * c->xh_b)) ||
-            (rc = dalloc(c, (void **)&c->xd2, sizeof(float) * c->d.max_shots * 2 * c->xd_b)) ||
-            (rc = dalloc(c, (void **)&c->zh2, sizeof(float) * c->d.max_shots * 2 * c->zh_b)) ||
-            (rc = dalloc(c, (void **)&c->zd2, sizeof(float) * c->d.max_shots * 2 * c->zd_b)))
-            return rc;
-    }
+    if (c->ck_store && c->ck_K == K && nseg <= c->ck_nseg && S <= c->ck_S) return FWI_HIP_OK;
+    free_ckpt(c);
+    int rc = ensure_cpml2(c);
+    if (rc) return rc;
     const size_t es = c->ckpt_storage ? 2 : 4;
     if ((rc = dalloc(c, &c->ck_store, es * nseg * 2 * S * c->fs)) ||
         (rc = dalloc(c, (void **)&c->ck_scale, sizeof(unsigned) * nseg * 2 * S)) ||
@@ -720,18 +824,22 @@ static int ensure_ckpt(fwi_hip_ctx *c)
     return FWI_HIP_OK;
 }
 
-// the forward state before step j K: p in P0/P1 and the CPML memory
-static int ck_save(fwi_hip_ctx *c, long long j)
+// The forward state before step k (p^k, p^{k-1} in P0/P1 by parity, and the
+// CPML memory of the forward pass, or of the recompute when `recompute`) into
+// snapshot slot j.
+static int ck_save_at(fwi_hip_ctx *c, long long j, long long k, bool recompute)
 {
     const long long S = c->nshots, n = S * c->fs;
     float *cp = c->ck_cpml + j * cpml_floats(c, c->ck_S);
-    HIPCHK(c, hipMemcpyAsync(cp, c->xh, sizeof(float) * S * 2 * c->xh_b, hipMemcpyDeviceToDevice, c->st));
+    const float *xh = recompute ? c->xh2 : c->xh, *xd = recompute ? c->xd2 : c->xd;
+    const float *zh = recompute ? c->zh2 : c->zh, *zd = recompute ? c->zd2 : c->zd;
+    HIPCHK(c, hipMemcpyAsync(cp, xh, sizeof(float) * S * 2 * c->xh_b, hipMemcpyDeviceToDevice, c->st));
     cp += S * 2 * c->xh_b;
-    HIPCHK(c, hipMemcpyAsync(cp, c->xd, sizeof(float) * S * 2 * c->xd_b, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(cp, xd, sizeof(float) * S * 2 * c->xd_b, hipMemcpyDeviceToDevice, c->st));
     cp += S * 2 * c->xd_b;
-    HIPCHK(c, hipMemcpyAsync(cp, c->zh, sizeof(float) * S * 2 * c->zh_b, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(cp, zh, sizeof(float) * S * 2 * c->zh_b, hipMemcpyDeviceToDevice, c->st));
     cp += S * 2 * c->zh_b;
-    HIPCHK(c, hipMemcpyAsync(cp, c->zd, sizeof(float) * S * 2 * c->zd_b, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(cp, zd, sizeof(float) * S * 2 * c->zd_b, hipMemcpyDeviceToDevice, c->st));
     if (c->ckpt_storage == 0) {
         float *dst = (float *)c->ck_store + j * 2 * c->ck_S * c->fs;
         HIPCHK(c, hipMemcpyAsync(dst, c->P0, sizeof(float) * n, hipMemcpyDeviceToDevice, c->st));
@@ -743,7 +851,6 @@ static int ck_save(fwi_hip_ctx *c, long long j)
     // their difference (the time derivative, a few per cent of p at a few
     // tens of steps per period) with the rounding of p itself: tens of times
     // the storage precision after the restart.
-    const long long k = j * c->ckpt_every;
     float *pk = (k & 1) ? c->P1 : c->P0, *pm = (k & 1) ? c->P0 : c->P1;  // p^k, p^{k-1}
     unsigned *bits = c->ck_scale + j * 2 * c->ck_S;
     unsigned short *dst = (unsigned short *)c->ck_store + j * 2 * c->ck_S * c->fs;
@@ -756,8 +863,8 @@ static int ck_save(fwi_hip_ctx *c, long long j)
     return FWI_HIP_OK;
 }
 
-// back into P0/P1 and the recompute's CPML set
-static int ck_restore(fwi_hip_ctx *c, long long j)
+// snapshot slot j (taken before step k) back into P0/P1 and the recompute's CPML set
+static int ck_restore_at(fwi_hip_ctx *c, long long j, long long k)
 {
     const long long S = c->nshots, n = S * c->fs;
     const float *cp = c->ck_cpml + j * cpml_floats(c, c->ck_S);
@@ -774,13 +881,123 @@ static int ck_restore(fwi_hip_ctx *c, long long j)
         HIPCHK(c, hipMemcpyAsync(c->P1, src + n, sizeof(float) * n, hipMemcpyDeviceToDevice, c->st));
         return FWI_HIP_OK;
     }
-    const long long k = j * c->ckpt_every;
     float *pk = (k & 1) ? c->P1 : c->P0, *pm = (k & 1) ? c->P0 : c->P1;
     const unsigned *bits = c->ck_scale + j * 2 * c->ck_S;
     const unsigned short *src = (const unsigned short *)c->ck_store + j * 2 * c->ck_S * c->fs;
     const int fp16 = c->ckpt_storage == 1;
     HIPCHK(c, fwi_launch_unpack16(src, c->fs, (int)S, bits, nullptr, pk, fp16, c->st));
     HIPCHK(c, fwi_launch_unpack16(src + n, c->fs, (int)S, bits + S, pk, pm, fp16, c->st));  // p^{k-1} = p^k - d
+    return FWI_HIP_OK;
+}
+
+// uniform snapshots: slot j holds the state before step j K
+static int ck_save(fwi_hip_ctx *c, long long j) { return ck_save_at(c, j, j * c->ckpt_every, false); }
+static int ck_restore(fwi_hip_ctx *c, long long j) { return ck_restore_at(c, j, j * c->ckpt_every); }
+
+// ---- binomial checkpointing: buffers and the memory plan
+#define FWI_RV_MAX_SNAPS 128
+
+static int ensure_cpml2(fwi_hip_ctx *c)
+{
+    if (c->xh2) return FWI_HIP_OK;
+    int rc;
+    if ((rc = dalloc(c, (void **)&c->xh2, sizeof(float) * c->d.max_shots * 2 * c->xh_b)) ||
+        (rc = dalloc(c, (void **)&c->xd2, sizeof(float) * c->d.max_shots * 2 * c->xd_b)) ||
+        (rc = dalloc(c, (void **)&c->zh2, sizeof(float) * c->d.max_shots * 2 * c->zh_b)) ||
+        (rc = dalloc(c, (void **)&c->zd2, sizeof(float) * c->d.max_shots * 2 * c->zd_b)))
+        return rc;
+    return FWI_HIP_OK;
+}
+
+static void free_ckpt(fwi_hip_ctx *c)
+{
+    dfree(c, c->ck_store);
+    dfree(c, c->ck_scale);
+    dfree(c, c->ck_cpml);
+    dfree(c, c->ck_xseg);
+    c->ck_store = nullptr;
+    c->ck_scale = nullptr;
+    c->ck_cpml = c->ck_xseg = nullptr;
+    c->ck_nseg = c->ck_S = c->ck_K = 0;
+}
+
+static void free_frames(fwi_hip_ctx *c)
+{
+    dfree(c, c->pstrip);
+    dfree(c, c->xstrip);
+    c->pstrip = c->xstrip = nullptr;
+}
+
+// bytes of each backward method's buffers, at the context's capacity
+static size_t frames_bytes(const fwi_hip_ctx *c)
+{
+    return 2 * sizeof(float) * (size_t)c->d.max_shots * (size_t)c->d.max_nt * (size_t)c->g.sl;
+}
+static size_t rv_slot_bytes(const fwi_hip_ctx *c, int storage)
+{
+    const size_t S = c->d.max_shots, es = storage ? 2 : 4;
+    return es * 2 * S * (size_t)c->fs + sizeof(float) * (size_t)cpml_floats(c, S) +
+           (storage ? sizeof(unsigned) * 2 * S : 0);
+}
+// the recompute's CPML set and one step of X
+static size_t rv_fixed_bytes(const fwi_hip_ctx *c)
+{
+    const size_t S = c->d.max_shots;
+    return sizeof(float) * (size_t)cpml_floats(c, S) + sizeof(float) * S * (size_t)c->g.nx * c->g.nz;
+}
+
+// The backward method within the HBM budget beside `base` bytes: the boundary
+// frames when they fit (or without a budget), else binomial checkpoints with
+// as many slots as fit (at most FWI_RV_MAX_SNAPS, plus the slot of the state
+// before the last step).
+static int plan_core(fwi_hip_ctx *c, size_t base, int storage, int *method, int *snaps)
+{
+    const size_t B = c->d.hbm_budget, fr = frames_bytes(c);
+    *method = 0;
+    *snaps = 0;
+    if (!B || base + fr <= B) return FWI_HIP_OK;
+    const size_t fixed = rv_fixed_bytes(c), slot = rv_slot_bytes(c, storage);
+    const long long n = (B > base + fixed) ? (long long)((B - base - fixed) / slot) : 0;
+    const long long sn = std::min<long long>(n - 1, FWI_RV_MAX_SNAPS);
+    if (sn < 2)
+        return set_err(c, FWI_HIP_ENOMEM,
+                       "HBM budget %zu bytes: beside the context's %zu, neither the boundary frames (%zu) nor 3 "
+                       "snapshots (%zu each, + %zu) fit",
+                       B, base, fr, slot, fixed);
+    *method = 2;
+    *snaps = (int)sn;
+    return FWI_HIP_OK;
+}
+
+// device bytes held beyond what the backward method decides
+static size_t base_bytes(const fwi_hip_ctx *c)
+{
+    size_t held = c->bytes;
+    for (const auto &a : c->allocs)
+        if (a.first == c->pstrip || a.first == c->xstrip || a.first == c->ck_store || a.first == c->ck_scale ||
+            a.first == c->ck_cpml || a.first == c->ck_xseg || a.first == c->xh2 || a.first == c->xd2 ||
+            a.first == c->zh2 || a.first == c->zd2)
+            held -= a.second;
+    return held;
+}
+
+// slots (and the one-step X buffer) of the binomial schedule, at capacity
+static int ensure_slots(fwi_hip_ctx *c, long long nslots)
+{
+    const long long S = c->d.max_shots, nb = (long long)c->g.nx * c->g.nz;
+    if (c->ck_store && c->ck_K == 1 && nslots <= c->ck_nseg && S <= c->ck_S) return FWI_HIP_OK;
+    free_ckpt(c);
+    int rc = ensure_cpml2(c);
+    if (rc) return rc;
+    const size_t es = c->ckpt_storage ? 2 : 4;
+    if ((rc = dalloc(c, &c->ck_store, es * nslots * 2 * S * c->fs)) ||
+        (rc = dalloc(c, (void **)&c->ck_scale, sizeof(unsigned) * nslots * 2 * S)) ||
+        (rc = dalloc(c, (void **)&c->ck_cpml, sizeof(float) * nslots * cpml_floats(c, S))) ||
+        (rc = dalloc(c, (void **)&c->ck_xseg, sizeof(float) * S * nb)))
+        return rc;
+    c->ck_nseg = nslots;
+    c->ck_S = S;
+    c->ck_K = 1;
     return FWI_HIP_OK;
 }
 
@@ -958,7 +1175,74 @@ extern "C" int fwi_hip_set_checkpointing(fwi_hip_ctx *c, int every, int storage)
     }
     c->ckpt_every = every;
     c->ckpt_storage = storage;
+    c->rv_snaps = 0;
     c->have_fwd = false;
+    return FWI_HIP_OK;
+}
+
+extern "C" int fwi_hip_set_revolve(fwi_hip_ctx *c, int snaps, int storage)
+{
+    if (!c) return set_err(c, FWI_HIP_EINVAL, "set_revolve: NULL context");
+    if (snaps < -1 || snaps > FWI_RV_MAX_SNAPS || storage < 0 || storage > 2)
+        return set_err(c, FWI_HIP_EINVAL, "set_revolve: snaps must be -1..%d, storage 0 (FP32), 1 (FP16) or 2 (BF16)",
+                       FWI_RV_MAX_SNAPS);
+    if (storage != c->ckpt_storage) free_ckpt(c);
+    c->rv_snaps = snaps;
+    c->ckpt_storage = storage;
+    if (snaps != 0) c->ckpt_every = 0;
+    c->have_fwd = false;  // the next adjoint needs a forward pass with this plan
+    return FWI_HIP_OK;
+}
+
+extern "C" int fwi_hip_backward_plan(const fwi_hip_ctx *c, int *method, int *snaps, long long *recomputed)
+{
+    if (!c) return set_err(nullptr, FWI_HIP_EINVAL, "backward_plan: NULL context");
+    if (method) *method = c->bwd_method;
+    if (snaps) *snaps = c->bwd_snaps;
+    if (recomputed) *recomputed = c->recomputed;
+    return FWI_HIP_OK;
+}
+
+extern "C" int fwi_hip_plan_memory(const fwi_hip_desc *desc, int storage, int *method, int *snaps, size_t *bytes)
+{
+    if (!desc || !method || !snaps || storage < 0 || storage > 2)
+        return set_err(nullptr, FWI_HIP_EINVAL, "plan_memory: bad argument");
+    fwi_hip_ctx c;  // sizes only: no device call
+    c.d = *desc;
+    int rc = check_desc(&c, desc);
+    if (rc) return rc;
+    c.g = make_geom(desc);
+    set_strides(&c);
+    size_t base = 0;
+    for (const auto &a : create_table(&c)) base += a.second;
+    // set_shots: the source samples and one receiver row per shot
+    base += sizeof(float) * (size_t)desc->max_shots * desc->max_nt + sizeof(int) * (size_t)desc->max_shots * c.g.ld;
+    rc = plan_core(&c, base, storage, method, snaps);
+    if (rc) {
+        snprintf(g_err, sizeof(g_err), "%s", c.err);
+        return rc;
+    }
+    if (bytes)
+        *bytes = base + (*method == 0 ? frames_bytes(&c) : rv_fixed_bytes(&c) + (*snaps + 1) * rv_slot_bytes(&c, storage));
+    return FWI_HIP_OK;
+}
+
+extern "C" int fwi_hip_revolve_schedule(int nt, int snaps, int *acts, int cap, int *nacts, long long *forward_steps)
+{
+    if (nt < 1 || snaps < 1 || cap < 0 || (cap > 0 && !acts) || !nacts)
+        return set_err(nullptr, FWI_HIP_EINVAL, "revolve_schedule: bad argument");
+    std::vector<RvAct> v;
+    rv_schedule(nt, snaps, v);
+    long long fsteps = 0;
+    for (const RvAct &x : v) fsteps += x.op == RV_ADVANCE ? x.b - x.a : (x.op == RV_TURN ? 1 : 0);
+    for (size_t i = 0; i < v.size() && i < (size_t)cap; i++) {
+        acts[4 * i] = v[i].op;
+        acts[4 * i + 1] = v[i].a;
+        acts[4 * i + 2] = v[i].b;
+        acts[4 * i + 3] = v[i].slot;
+    }
+    *nacts = (int)v.size();
+    if (forward_steps) *forward_steps = fsteps;
     return FWI_HIP_OK;
 }
 
@@ -984,10 +1268,34 @@ extern "C" int fwi_hip_forward(fwi_hip_ctx *c, float *traces)
     if (!c->have_model || c->nshots == 0) return set_err(c, FWI_HIP_ESTATE, "forward: set_model and set_shots first");
     const FwiGeomDev &g = c->g;
     const long long S = c->nshots;
-    const bool ck = c->ckpt_every > 0;
+    // backward method: 1 uniform snapshots, 2 binomial checkpoints (fixed
+    // slots, or from the HBM budget), else 0 boundary frames
+    int method = c->ckpt_every > 0 ? 1 : 0, snaps = 0, rc = FWI_HIP_OK;
+    if (method == 0 && c->rv_snaps > 0) {
+        method = 2;
+        snaps = c->rv_snaps;
+    } else if (method == 0 && c->rv_snaps < 0) {
+        if ((rc = plan_core(c, base_bytes(c), c->ckpt_storage, &method, &snaps))) return rc;
+    }
+    const bool ck = method != 0;
     if (ck && c->fwd_kind != 0) return set_err(c, FWI_HIP_EINVAL, "forward: checkpointing needs the tile kernels (set_kernels 0)");
-    int rc = ck ? ensure_ckpt(c) : ensure_frames(c);
+    if (method == 0) {
+        free_ckpt(c);
+        rc = ensure_frames(c);
+    } else {
+        free_frames(c);
+        rc = method == 1 ? ensure_ckpt(c) : ensure_slots(c, snaps + 1);
+    }
     if (rc) return rc;
+    c->bwd_method = method;
+    c->bwd_snaps = snaps;
+    if (method == 2 && (c->rv_sched_t != c->kmax || c->rv_sched_s != snaps)) {
+        rv_schedule(c->kmax, snaps, c->rv_sched);
+        c->rv_prefix = 0;
+        while (c->rv_prefix < c->rv_sched.size() && c->rv_sched[c->rv_prefix].op != RV_TURN) c->rv_prefix++;
+        c->rv_sched_t = c->kmax;
+        c->rv_sched_s = snaps;
+    }
     const int fmode = ck ? FWI_FWD_TRACES : FWI_FWD_FRAMES;
     HIPCHK(c, fwi_launch_fill_f32(c->P0, S * c->fs, 0.f, c->st));
     HIPCHK(c, fwi_launch_fill_f32(c->P1, S * c->fs, 0.f, c->st));
@@ -1014,13 +1322,29 @@ extern "C" int fwi_hip_forward(fwi_hip_ctx *c, float *traces)
     const int B = c->graph_steps;
     hipEvent_t pe0;
     if ((rc = pass_begin(c, &pe0))) return rc;
-    if (ck) {
+    if (method == 1) {
         // a snapshot before every ckpt_every-th step, plain launches between
         const int K = c->ckpt_every;
         for (long long j = 0; j * K < c->kmax; j++) {
             if ((rc = ck_save(c, j))) return rc;
             for (int k = (int)(j * K); k < std::min((long long)c->kmax, (j + 1) * K); k++)
                 HIPCHK(c, fwi_launch_fwd_tile(&a, ad, c->tl_fwd, c->n_fwd, nullptr, k, pair_of(c), fmode));
+        }
+    } else if (method == 2) {
+        // the binomial schedule's first sweep: S_0 in slot 0, its snapshots in
+        // step order, and the state before the last step in slot `snaps`
+        // (where the adjoint pass's first turn starts)
+        size_t q = 0;
+        for (int k = 0; k < c->kmax; k++) {
+            if (k == 0 && (rc = ck_save_at(c, 0, 0, false))) return rc;
+            for (; q < c->rv_prefix; q++) {
+                const RvAct &x = c->rv_sched[q];
+                if (x.op == RV_ADVANCE) continue;
+                if (x.a != k) break;
+                if ((rc = ck_save_at(c, x.slot, k, false))) return rc;
+            }
+            if (k == c->kmax - 1 && (rc = ck_save_at(c, snaps, k, false))) return rc;
+            HIPCHK(c, fwi_launch_fwd_tile(&a, ad, c->tl_fwd, c->n_fwd, nullptr, k, pair_of(c), fmode));
         }
     } else if (B > 0 && c->timing != 1) {
         if (!c->gx_fwd) {
@@ -1225,7 +1549,7 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
     const int nb_rec = g.bntx * g.bntz * c->ngroups;
     hipEvent_t pe0;
     if ((rc = pass_begin(c, &pe0))) return rc;
-    if (c->ckpt_every > 0) {
+    if (c->bwd_method != 0) {
         if (!tile) return set_err(c, FWI_HIP_EINVAL, "adjoint: checkpointing needs the tile kernels (set_kernels 0)");
         if (!c->ck_store) return set_err(c, FWI_HIP_ESTATE, "adjoint: no snapshots (forward with checkpointing first)");
         // per segment from the last: restore its snapshot, recompute its
@@ -1246,9 +1570,36 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
         c->fwd_rc_h = f;
         HIPCHK(c, hipMemcpyAsync(c->fwd_rc_d, &c->fwd_rc_h, sizeof(FwiFwdArgs), hipMemcpyHostToDevice, c->st));
         int *k0_dev = &c->fwd_rc_d->xseg_k0;  // device address of the segment's first step
-        for (long long j = (c->kmax + K - 1) / K - 1; j >= 0; j--) {
+        long long rec = 0;
+        if (c->bwd_method == 2) {
+            // the rest of the binomial schedule, from the state before the last step
+            if ((rc = ck_restore_at(c, c->bwd_snaps, c->kmax - 1))) return rc;
+            for (size_t i = c->rv_prefix; i < c->rv_sched.size(); i++) {
+                const RvAct &x = c->rv_sched[i];
+                if (x.op == RV_ADVANCE) {
+                    for (int k = x.a; k < x.b; k++)
+                        HIPCHK(c, fwi_launch_fwd_tile(&f, c->fwd_rc_d, c->tl_fwd, c->n_fwd, nullptr, k, pair_of(c),
+                                                      FWI_FWD_PLAIN));
+                    rec += x.b - x.a;
+                } else if (x.op == RV_TAKESHOT) {
+                    if ((rc = ck_save_at(c, x.slot, x.a, true))) return rc;
+                } else if (x.op == RV_RESTORE) {
+                    if ((rc = ck_restore_at(c, x.slot, x.a))) return rc;
+                } else {  // turn: X^k from S_k, then the adjoint step k and its image
+                    const int k = x.a;
+                    HIPCHK(c, fwi_launch_set_int(k0_dev, k, c->st));
+                    HIPCHK(c, fwi_launch_fwd_tile(&f, c->fwd_rc_d, c->tl_fwd, c->n_fwd, nullptr, k, pair_of(c),
+                                                  FWI_FWD_XSEG));
+                    HIPCHK(c, fwi_launch_adj_tile(&a, ad, c->tl_adj, c->n_adj, nullptr, k, pair_of(c)));
+                    HIPCHK(c, fwi_launch_img_seg(&r, rd, c->ck_xseg, k, rgs, ngr, c->st));
+                    rec++;
+                }
+            }
+        }
+        for (long long j = c->bwd_method == 1 ? (c->kmax + K - 1) / K - 1 : -1; j >= 0; j--) {
             const int k0 = (int)(j * K), k1 = (int)std::min((long long)c->kmax, (j + 1) * K);
             if ((rc = ck_restore(c, j))) return rc;
+            rec += k1 - k0;
             HIPCHK(c, fwi_launch_set_int(k0_dev, k0, c->st));
             for (int k = k0; k < k1; k++)
                 HIPCHK(c, fwi_launch_fwd_tile(&f, c->fwd_rc_d, c->tl_fwd, c->n_fwd, nullptr, k, pair_of(c),
@@ -1258,6 +1609,7 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
                 HIPCHK(c, fwi_launch_img_seg(&r, rd, c->ck_xseg + (long long)(k - k0) * S * nbb, k, rgs, ngr, c->st));
             }
         }
+        c->recomputed = rec;
         if ((rc = pass_end(c, 1, pe0, c->kmax))) return rc;
         HIPCHK(c, fwi_launch_grad_finalize(c->gacc, ngr, nb, c->vel, c->gk, c->grad, c->st));
         if (grad) HIPCHK(c, hipMemcpyAsync(grad, c->grad, sizeof(double) * nb, hipMemcpyDeviceToHost, c->st));

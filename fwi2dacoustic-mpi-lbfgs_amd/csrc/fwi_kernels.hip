@@ -736,7 +736,9 @@ __device__ __forceinline__ void fwd_tile_rows(const FwiFwdArgs &A, const FwiShot
     // which special rules can occur in this strip at all (uniform): each is
     // tested once here instead of per row and node
     const bool zplain = !EDGE || !(CLS & FWI_CLS_Z) || (r0 >= 3 && r0 + FR + 1 < pnz - npml);  // plain v, vz
-    const bool has_recv = EDGE && (CLS & FWI_CLS_RS) && sh.rz_lo <= sh.rz_hi && sh.rz_lo < r0 + FR && sh.rz_hi >= r0;
+    constexpr bool TRACES = MODE == FWI_FWD_FRAMES || MODE == FWI_FWD_TRACES;  // receivers recorded
+    const bool has_recv =
+        TRACES && EDGE && (CLS & FWI_CLS_RS) && sh.rz_lo <= sh.rz_hi && sh.rz_lo < r0 + FR && sh.rz_hi >= r0;
     const bool has_src =
         EDGE && (CLS & FWI_CLS_RS) && sh.isz >= r0 && sh.isz < r0 + FR && sh.isx >= x0 && sh.isx < x0 + TX;
     const bool frame_rows = EDGE && (CLS & FWI_CLS_FR) && (r0 < 3 || (r0 + FR > g.nz - 3 && r0 < g.nz));
@@ -894,7 +896,7 @@ __device__ __forceinline__ void fwd_tile_rows(const FwiFwdArgs &A, const FwiShot
             continue;
         }
         if (x < pnx) buf_st(pnew, rold, vo, i * rowb);
-        if (MODE != FWI_FWD_XSEG && has_recv && rix[d] >= 0)
+        if (TRACES && has_recv && rix[d] >= 0)
             A.syn[sh.trace_off + (long long)k * sh.nr + rix[d]] = pnew;
         if (MODE == FWI_FWD_FRAMES && (frame_rows || frame_cols) && bx >= 0 && bx < nx && i < nz &&
             (i < 3 || i >= nz - 3 || bx < 3 || bx >= nx - 3)) {
@@ -2646,6 +2648,7 @@ hipError_t fwi_launch_fwd_tile(const FwiFwdArgs *a, const FwiFwdArgs *a_dev, con
 {
     if (mode == FWI_FWD_TRACES) return launch_fwd_tile_mode<FWI_FWD_TRACES>(a, a_dev, tl, ntl, kbase, koff, p);
     if (mode == FWI_FWD_XSEG) return launch_fwd_tile_mode<FWI_FWD_XSEG>(a, a_dev, tl, ntl, kbase, koff, p);
+    if (mode == FWI_FWD_PLAIN) return launch_fwd_tile_mode<FWI_FWD_PLAIN>(a, a_dev, tl, ntl, kbase, koff, p);
     return launch_fwd_tile_mode<FWI_FWD_FRAMES>(a, a_dev, tl, ntl, kbase, koff, p);
 }
 

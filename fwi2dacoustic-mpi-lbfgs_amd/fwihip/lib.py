@@ -25,7 +25,7 @@ EXPORTS = (
     "fwi_hip_set_residual", "fwi_hip_adjoint", "fwi_hip_synchronize", "fwi_hip_set_timing",
     "fwi_hip_kernel_time", "fwi_hip_read_state", "fwi_hip_op_pml", "fwi_hip_op_interior",
     "fwi_hip_time_integral", "fwi_hip_op_pml_noalloc", "fwi_hip_write_bdr", "fwi_hip_read_bdr", "fwi_hip_cpml_coe",
-    "fwi_hip_set_graph", "fwi_hip_set_kernels", "fwi_hip_set_rec_group", "fwi_hip_set_imaging", "fwi_hip_set_bwd_chunk", "fwi_hip_set_checkpointing", "fwi_hip_set_bwd_overlap", "fwi_hip_set_bwd_fused", "fwi_hip_set_concurrent", "fwi_hip_set_interior_wgs", "fwi_hip_device_count", "fwi_hip_dmalloc", "fwi_hip_dfree", "fwi_hip_host_alloc", "fwi_hip_host_free", "fwi_hip_memcpy",
+    "fwi_hip_set_graph", "fwi_hip_set_kernels", "fwi_hip_set_rec_group", "fwi_hip_set_imaging", "fwi_hip_set_bwd_chunk", "fwi_hip_set_checkpointing", "fwi_hip_set_revolve", "fwi_hip_backward_plan", "fwi_hip_plan_memory", "fwi_hip_revolve_schedule", "fwi_hip_set_bwd_overlap", "fwi_hip_set_bwd_fused", "fwi_hip_set_concurrent", "fwi_hip_set_interior_wgs", "fwi_hip_device_count", "fwi_hip_dmalloc", "fwi_hip_dfree", "fwi_hip_host_alloc", "fwi_hip_host_free", "fwi_hip_memcpy",
 )
 
 
@@ -193,6 +193,19 @@ class Context:
         code = {"fp32": 0, "fp16": 1, "bf16": 2}.get(storage, -1)
         self._chk(self.lib.fwi_hip_set_checkpointing(self.h, int(every), code))
 
+    def set_revolve(self, snaps, storage="fp32"):
+        """Binomial checkpointing with `snaps` slots (-1: chosen from the HBM
+        budget, 0: off), snapshots stored as "fp32", "fp16" or "bf16"."""
+        code = {"fp32": 0, "fp16": 1, "bf16": 2}.get(storage, -1)
+        self._chk(self.lib.fwi_hip_set_revolve(self.h, int(snaps), code))
+
+    def backward_plan(self):
+        """(method, snaps, recomputed forward steps): method 0 boundary frames,
+        1 uniform snapshots, 2 binomial checkpoints."""
+        m, sn, rec = ctypes.c_int(), ctypes.c_int(), ctypes.c_longlong()
+        self._chk(self.lib.fwi_hip_backward_plan(self.h, ctypes.byref(m), ctypes.byref(sn), ctypes.byref(rec)))
+        return m.value, sn.value, rec.value
+
     def set_concurrent(self, on):
         """General strips on a second stream beside the interior launch."""
         self._chk(self.lib.fwi_hip_set_concurrent(self.h, int(bool(on))))
@@ -248,6 +261,35 @@ def cpml_coe(npml, dx, dt, max_f=MAX_F, max_v=MAX_V, coef_r=COEF_R):
     if rc:
         raise FwiError(rc, lib.fwi_hip_last_error(None).decode())
     return a, b
+
+
+def plan_memory(geom, max_shots, max_nt, max_nr, hbm_budget, storage="fp32"):
+    """(method, snaps, bytes) a context takes under set_revolve(-1) within the
+    budget (no device call)."""
+    lib = load()
+    d = Desc(make_grid(geom), max_shots, max_nt, max_nr, 0, 0, hbm_budget, None)
+    m, sn, b = ctypes.c_int(), ctypes.c_int(), ctypes.c_size_t()
+    code = {"fp32": 0, "fp16": 1, "bf16": 2}[storage]
+    rc = lib.fwi_hip_plan_memory(ctypes.byref(d), code, ctypes.byref(m), ctypes.byref(sn), ctypes.byref(b))
+    if rc:
+        raise FwiError(rc, lib.fwi_hip_last_error(None).decode())
+    return m.value, sn.value, b.value
+
+
+def revolve_schedule(nt, snaps):
+    """The binomial schedule: (actions as an (n, 4) array of op, a, b, slot;
+    forward steps).  op 0 advance a -> b, 1 snapshot at a, 2 restore at a,
+    3 turn at a."""
+    lib = load()
+    n, fs = ctypes.c_int(), ctypes.c_longlong()
+    rc = lib.fwi_hip_revolve_schedule(int(nt), int(snaps), None, 0, ctypes.byref(n), ctypes.byref(fs))
+    if rc:
+        raise FwiError(rc, lib.fwi_hip_last_error(None).decode())
+    acts = np.zeros((n.value, 4), np.int32)
+    rc = lib.fwi_hip_revolve_schedule(int(nt), int(snaps), _p(acts), n.value, ctypes.byref(n), ctypes.byref(fs))
+    if rc:
+        raise FwiError(rc, lib.fwi_hip_last_error(None).decode())
+    return acts, fs.value
 
 
 def device_count():

@@ -132,6 +132,45 @@ int fwi_hip_set_imaging(fwi_hip_ctx *ctx, int jump, int k0, int k1);
  * shot) are not allocated.  Tile kernels only.                              */
 int fwi_hip_set_checkpointing(fwi_hip_ctx *ctx, int every, int storage);
 
+/* Binomial checkpointing of the backward pass, in the manner of Revolve
+ * (Griewank and Walther's schedule), instead of the boundary frames or the
+ * uniform snapshots of set_checkpointing.  snaps > 0: that many snapshot
+ * slots of the forward state (p at both time levels and the CPML memory of
+ * every shot; FP32, or FP16/BF16 as set_checkpointing stores them), one of
+ * them the initial state, plus one slot for the state before the last step.
+ * The forward pass takes the schedule's first snapshots; the adjoint pass
+ * restores and advances states so that each step's X is recomputed from the
+ * nearest snapshot, with the fewest forward steps the binomial bound allows
+ * (about r nt for nt <= C(snaps + r, snaps)).  snaps = -1: chosen from the
+ * HBM budget of the descriptor at each forward pass: the boundary frames
+ * when they fit beside the context, else as many slots as fit (at most 128;
+ * FWI_HIP_ENOMEM below 3).  snaps = 0: off.  Tile kernels only.  Calling
+ * set_checkpointing switches this off, and this switches off its uniform
+ * snapshots.  A new forward pass is needed before the next adjoint.         */
+int fwi_hip_set_revolve(fwi_hip_ctx *ctx, int snaps, int storage);
+
+/* The backward pass the last forward prepared: *method 0 boundary frames,
+ * 1 uniform snapshots, 2 binomial checkpoints with *snaps slots; and the
+ * forward steps the last adjoint pass recomputed.  Any pointer may be NULL. */
+int fwi_hip_backward_plan(const fwi_hip_ctx *ctx, int *method, int *snaps, long long *recomputed);
+
+/* The backward method a context with descriptor d takes under
+ * set_revolve(ctx, -1, storage) within d->hbm_budget, and the device bytes
+ * it then holds, for max_shots shots of max_nt steps with one receiver row
+ * each (no device call): *method 0 (boundary frames) or 2 (binomial, *snaps
+ * slots).  FWI_HIP_ENOMEM when neither fits.  No counterpart in the
+ * reference, which saves its frames to disk (write_bdr, fwi_2d.c:882-918). */
+int fwi_hip_plan_memory(const fwi_hip_desc *d, int storage, int *method, int *snaps, size_t *bytes);
+
+/* The binomial schedule for nt steps and snaps slots (no device call):
+ * *nacts actions, the first min(*nacts, cap) written to acts as (op, a, b,
+ * slot): op 0 advance the state from step a to step b, 1 snapshot the state
+ * at step a into slot, 2 restore slot (the state at step a), 3 turn at a (a
+ * forward step from the state at a, then the adjoint step a).  The state at
+ * step 0 is in slot 0.  *forward_steps: the forward steps of the advances
+ * and turns.                                                                 */
+int fwi_hip_revolve_schedule(int nt, int snaps, int *acts, int cap, int *nacts, long long *forward_steps);
+
 /* Tile kernels: the general strips of a step (CPML, edges, free surface,
  * frames, sources, receivers) on a second stream beside the interior launch
  * (on = 1, the default), joined before the next step; off: one stream.
