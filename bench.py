@@ -191,15 +191,20 @@ def boundary_leg(g, start, shots, obs_packed, steps, warmup):
                 steps=steps, warmup=warmup, phases_ms={k: round(float(v), 2) for k, v in zip(names, acc)})
 
 
-def checkpoint_leg(g, vel, start, shots, obs_packed, nt, every, storage, steps, warmup):
-    """The headline step with the checkpointed backward pass: snapshots of
-    every shot every `every` steps in `storage` (per-shot max-abs scaling for
-    16 bit), segments recomputed in the adjoint pass, no boundary frames."""
+def checkpoint_leg(g, vel, start, shots, obs_packed, nt, every, storage, steps, warmup, snaps=0):
+    """The headline step with a checkpointed backward pass instead of the
+    boundary frames: snapshots of every shot every `every` steps, or (snaps >
+    0) the binomial schedule with `snaps` snapshot slots, stored in `storage`
+    (per-shot max-abs scaling for 16 bit), states recomputed in the adjoint
+    pass."""
     from fwihip import lib
     nr = max(s.nr for s in shots)
     ctx = lib.Context(g, max_shots=len(shots), max_nt=nt, max_nr=nr)
     try:
-        ctx.set_checkpointing(every, storage)
+        if snaps:
+            ctx.set_revolve(snaps, storage)
+        else:
+            ctx.set_checkpointing(every, storage)
         ctx.set_shots(shots)
         ctx.set_observed(obs_packed)
 
@@ -217,8 +222,13 @@ def checkpoint_leg(g, vel, start, shots, obs_packed, nt, every, storage, steps, 
             step()
         ctx.synchronize()
         wall = time.perf_counter() - t0
-        return dict(storage=storage, every=every, value=len(shots) * steps / wall, unit="shot-gradients/s",
-                    ms_per_step=wall / steps * 1e3, device_gbytes=round(ctx.bytes / 1e9, 2))
+        out = dict(storage=storage, value=len(shots) * steps / wall, unit="shot-gradients/s",
+                   ms_per_step=wall / steps * 1e3, device_gbytes=round(ctx.bytes / 1e9, 2))
+        if snaps:
+            out.update(method="binomial", snaps=snaps, recomputed_forward_steps=ctx.backward_plan()[2])
+        else:
+            out.update(method="uniform snapshots", every=every)
+        return out
     finally:
         ctx.close()
 
@@ -317,16 +327,21 @@ def run_checkpoint(args):
     start = triangle_smooth(vel, 25)
     nr = max(s.nr for s in shots)
     K = args.ckpt_every
-    modes = [(0, "fp32"), (K, "fp32"), (K, "fp16"), (K, "bf16")]
+    # (every, storage, binomial slots)
+    modes = [(0, "fp32", 0), (K, "fp32", 0), (K, "fp16", 0), (K, "bf16", 0), (0, "fp32", 32), (0, "fp16", 32),
+             (0, "fp16", 128)]
     ref = lib.Context(g, max_shots=len(shots), max_nt=args.nt, max_nr=nr)
     ref.set_model(vel)
     ref.set_shots(shots)
     obs = ref.forward()
     ref.close()
     rows = []
-    for every, storage in modes:
+    for every, storage, snaps in modes:
         ctx = lib.Context(g, max_shots=len(shots), max_nt=args.nt, max_nr=nr)
-        ctx.set_checkpointing(every, storage)
+        if snaps:
+            ctx.set_revolve(snaps, storage)
+        else:
+            ctx.set_checkpointing(every, storage)
         ctx.set_model(start)
         ctx.set_shots(shots)
         ctx.set_observed(obs)
@@ -344,9 +359,10 @@ def run_checkpoint(args):
             step()
         ctx.synchronize()
         wall = time.perf_counter() - t0
-        rows.append(dict(method="boundary-saving" if every == 0 else f"checkpoint every {every}", storage=storage,
-                         shot_gradients_per_s=len(shots) * args.steps / wall,
-                         device_gbytes=ctx.bytes / 1e9))
+        rows.append(dict(method=(f"binomial, {snaps} slots" if snaps else
+                                 ("boundary-saving" if every == 0 else f"checkpoint every {every}")),
+                         storage=storage, shot_gradients_per_s=len(shots) * args.steps / wall,
+                         device_gbytes=ctx.bytes / 1e9, recomputed_forward_steps=ctx.backward_plan()[2]))
         ctx.close()
         print(json.dumps(rows[-1]), file=sys.stderr, flush=True)
     # gradient error against the FP64 oracle on a small subset
@@ -354,8 +370,14 @@ def run_checkpoint(args):
     from oracle import Oracle
     nt_acc = min(args.nt, 1500)
     sub = [Shot(nt=nt_acc, isx=s.isx, isz=s.isz, wav=s.wav[:nt_acc], irx=s.irx, irz=s.irz) for s in shots[:2]]
-    rng = np.random.default_rng(5)
-    res = [rng.standard_normal((s.nr, s.nt)).astype(np.float32) for s in sub]
+    # the L2 residual of the start model against the data of the true model
+    cr = lib.Context(g, max_shots=2, max_nt=nt_acc, max_nr=nr)
+    cr.set_shots(sub)
+    cr.set_model(vel)
+    ob = cr.split(cr.forward())
+    cr.set_model(start)
+    res = [(sy - o).astype(np.float32) for sy, o in zip(cr.split(cr.forward()), ob)]
+    cr.close()
     o64 = Oracle(np.float64)
     t0 = time.perf_counter()
     g64 = np.zeros((g.nz, g.nx))
@@ -363,9 +385,12 @@ def run_checkpoint(args):
         f = o64.forward(g, start.astype(np.float64), s, want_strips=True)
         g64 += o64.backward(g, start.astype(np.float64), s, r.astype(np.float64), f)["grad"]
     t_oracle = time.perf_counter() - t0
-    for row, (every, storage) in zip(rows, modes):
+    for row, (every, storage, snaps) in zip(rows, modes):
         ctx = lib.Context(g, max_shots=2, max_nt=nt_acc, max_nr=nr)
-        ctx.set_checkpointing(min(every, nt_acc) if every else 0, storage)
+        if snaps:
+            ctx.set_revolve(snaps, storage)
+        else:
+            ctx.set_checkpointing(min(every, nt_acc) if every else 0, storage)
         ctx.set_model(start)
         ctx.set_shots(sub)
         ctx.forward(want_traces=False)
@@ -375,7 +400,8 @@ def run_checkpoint(args):
         row["grad_rel_l2_vs_fp64"] = float(np.linalg.norm(grad - g64) / np.linalg.norm(g64))
     out = {"metric": "checkpointed vs boundary-saving backward pass at 1024x320, nt=%d, %d shots" % (args.nt, len(shots)),
            "unit": "shot-gradients/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-           "rows": rows, "accuracy_sample": f"2 shots, nt={nt_acc}, one random residual, FP64 oracle ({t_oracle:.0f} s)",
+           "rows": rows, "accuracy_sample": f"2 shots, nt={nt_acc}, the L2 residual of the start model, FP64 oracle "
+                                            f"({t_oracle:.0f} s)",
            "dtype": "f32 arithmetic, snapshot storage per row"}
     print(json.dumps(out))
 
@@ -634,6 +660,7 @@ def main():
         if not args.no_ckpt:
             ckpt = [checkpoint_leg(g, vel, start, shots, obs_host, args.nt, 100, st, xs, xw)
                     for st in ("fp16", "bf16")]
+            ckpt.append(checkpoint_leg(g, vel, start, shots, obs_host, args.nt, 0, "fp16", xs, xw, snaps=32))
 
     cpu = None
     if not args.no_cpu:

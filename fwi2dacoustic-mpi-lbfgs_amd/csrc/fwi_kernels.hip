@@ -2812,8 +2812,12 @@ hipError_t fwi_launch_ref_read_bdr(const float *ud, const float *lr, float *pb, 
 hipError_t fwi_launch_absmax(const float *x, const float *y, long long n, int nitem, unsigned *bits, hipStream_t st)
 {
     if (nitem <= 0 || n <= 0) return hipSuccess;
+    // few blocks per item, each reducing a long stretch first: the items'
+    // maxima sit side by side in one cache line, so every block's atomicMax
+    // on them is serialised (1024 blocks per item made a 16-bit snapshot of
+    // 64 shots take about 0.5 ms)
     const long long per = (n + 255) / 256;
-    const int bx = (int)std::min<long long>(per, 1024);
+    const int bx = (int)std::min<long long>(per, 32);
     hipLaunchKernelGGL(k_absmax, dim3(bx, nitem), dim3(256), 0, st, x, y, n, nitem, bits);
     return hipGetLastError();
 }
