@@ -40,7 +40,7 @@ def build(verbose=True):
         obj = os.path.join(BUILD, src + ".o")
         run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
-    for src in ("grad4_1shot.c", "fwi_io.c", "fwi_sigproc.c", "fwi_msf.c", "fwi_gradprec.c", "fwi_lbfgs.c"):
+    for src in ("grad4_1shot.c", "fwi_io.c", "fwi_sigproc.c", "fwi_msf.c", "fwi_msf_rank.c", "fwi_gradprec.c", "fwi_lbfgs.c"):
         path = os.path.join(CSRC, src)
         if os.path.exists(path):
             obj = os.path.join(BUILD, src + ".o")

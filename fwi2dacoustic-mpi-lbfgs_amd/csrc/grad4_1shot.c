@@ -66,6 +66,8 @@ static struct {
     char mode[16], reg[16];
     float eot_eps, eot_tor;
     int eot_iter;
+    grad4_diw_par diw;
+    float rank_reg;
     int bp, bp_phase, bp_nphi, bp_nplo;
     float bp_fhi, bp_flo;
     int mute, mute_early;
@@ -107,6 +109,24 @@ int grad4_set_misfit(int kind, const void *params, float offset_min, float offse
         g_cfg.eot_eps = e->eps;
         g_cfg.eot_tor = e->tor;
         g_cfg.eot_iter = e->max_iter;
+        break;
+    }
+    case GRAD4_MSF_DIW: {
+        if (!params) return FWI_HIP_EINVAL;
+        const grad4_diw_par *d = (const grad4_diw_par *)params;
+        if (d->nl < 1 || d->nw < 1 || d->nl % 2 != 1 || d->nw % 2 != 1 || !(d->alpha_win > 0.0f)) {
+            fail(FWI_HIP_EINVAL, "grad4_set_misfit: DIW needs odd nl, nw >= 1 and alpha_win > 0");
+            return FWI_HIP_EINVAL;
+        }
+        g_cfg.diw = *d;
+        break;
+    }
+    case GRAD4_MSF_SOFTRANK: {
+        if (!params || !(*(const float *)params > 0.0f)) {
+            fail(FWI_HIP_EINVAL, "grad4_set_misfit: soft ranking needs a regularisation > 0");
+            return FWI_HIP_EINVAL;
+        }
+        g_cfg.rank_reg = *(const float *)params;
         break;
     }
     default:
@@ -277,6 +297,17 @@ static int misfit_scratch(int tid, size_t n)
     return g_scr[tid].cap ? FWI_HIP_OK : fail(FWI_HIP_ENOMEM, "host memory");
 }
 
+/* the working set of soft_ranking for n samples (NULL without memory) */
+static soft_ranking_vars *rank_vars(int n)
+{
+    soft_ranking_vars *v = (soft_ranking_vars *)calloc(1, sizeof(soft_ranking_vars));
+    if (!v) return NULL;
+    soft_ranking_init(n, v);
+    if (v->sorted && v->sol && v->soft_rank && v->hard_rank && v->perm && v->target) return v;
+    soft_ranking_free(v);
+    return NULL;
+}
+
 /* The host misfit step of one shot, on thread tid's working traces.  t holds
  * the shot's nr traces at the propagation rate (nt samples each) and
  * receives the adjoint source. */
@@ -369,8 +400,33 @@ static int host_misfit(fwi_plan *pl, float *t, int nt, float dt_fine, int tid)
         rc = entropy_ot_2d_trace_by_trace(obr, syr, &p, val, rsr);
         break;
     }
+    case GRAD4_MSF_DIW: {
+        const grad4_diw_par d = g_cfg.diw;
+        float range[2] = {g_cfg.omin, g_cfg.omax};
+        rc = lcc_diw_msf(obr, syr, sr, range, d.eps, nt0, nr, d.nl, d.nw, d.alpha_win, d.sim_barrier, d.lag_barrier,
+                         val, rsr);
+        break;
+    }
+    case GRAD4_MSF_SOFTRANK: {
+        soft_ranking_vars *par[2] = {rank_vars(nt0), rank_vars(nt0)};
+        const int have = par[0] && par[1];
+        if (have)
+            rc = soft_ranking_2d_trace_by_trace(obr, syr, sr, g_cfg.omin, g_cfg.omax, par, g_cfg.rank_reg, nt0, nr,
+                                                val, rsr);
+        else
+            rc = fail(FWI_HIP_ENOMEM, "host memory");
+        soft_ranking_free(par[0]);
+        soft_ranking_free(par[1]);
+        if (rc && have) rc = fail(rc, "misfit: %s", fwi_msf_rank_last_error());
+        if (rc) goto done;
+        break;
+    }
     default:
         rc = L2norm(obr, syr, nt0, nr, sr, g_cfg.omin, g_cfg.omax, val, rsr);
+    }
+    if (rc && g_cfg.kind == GRAD4_MSF_DIW) {
+        rc = fail(rc, "misfit: %s", fwi_msf_rank_last_error());
+        goto done;
     }
     if (rc) {
         rc = fail(rc, "misfit: %s", fwi_msf_last_error());

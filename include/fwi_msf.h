@@ -17,6 +17,8 @@
 #ifndef FWI_MSF_H
 #define FWI_MSF_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -111,6 +113,88 @@ int entropy_ot_log_2d_trace_by_trace(float **obs, float **syn, eot_msf_par *par,
                                      float *msf, float **res);
 
 const char *fwi_msf_last_error(void);
+
+/* ---------------------------------------------------------------------------
+ * Dynamic warping by local cross-correlation (lcc_diw.c, lcc_dtw.h).
+ * Cubes err / ddd are row pointers [it][il] with nl + 2 columns; columns 0 and
+ * nl + 1 hold -1e30 (never on a path), lags il = 1 .. nl stand for the shifts
+ * il - 1 - nl/2 samples of obs against syn.
+ */
+/* lcc_diw.c:8-19: win[i] = exp(-(i - n/2)^2 / (2 delta^2)), i = 0 .. n-1 */
+void gaussian_win(size_t n, float delta, float *win);
+/* lcc_diw.c:242-319: err[ir][it][il] = the normalised correlation of the
+ * gaussian_win(nw, alpha_win)-tapered windows of syn centred at it and of obs
+ * centred at it + il - 1 - nl/2, for it from the first to before the last
+ * sample of syn above eps; 0 elsewhere and on traces outside offset_range
+ * (sr_xyz = NULL or offset_range = NULL keeps every trace).  nw and nl must
+ * be odd (the reference exit()s otherwise); FWI_HIP_EINVAL here. */
+int lcc_err(float **obs, float **syn, float *sr_xyz, float offset_range[], float eps, int nt, int nr, int nl, int nw,
+            float alpha_win, float ***err);
+/* lcc_diw.c:30-115: accumulation of err along the best lag path ending
+ * (_fw) or starting (_bw) at each cell; 3-point: the lag moves by at most one
+ * sample per step; 5-point: by at most one per two steps. */
+void ac_3pts_fw(float **err, float **ddd, int nl, int nt);
+void ac_3pts_bw(float **err, float **ddd, int nl, int nt);
+void ac_5pts_fw(float **err, float **ddd, int nl, int nt);
+void ac_5pts_bw(float **err, float **ddd, int nl, int nt);
+/* lcc_diw.c:117-240: path[it] = the lag column of the best path (half
+ * columns in 5-point diagonals), from the best start of a backward
+ * accumulation (bt_fw_*) or the best end of a forward one (bt_bw_5pts). */
+void bt_fw_3pts(float **ddd, int nl, int nt, float *path);
+void bt_fw_5pts(float **ddd, float **err, int nl, int nt, float *path);
+void bt_bw_5pts(float **ddd, float **err, int nl, int nt, float *path);
+/* lcc_dtw.h:3 (declared only in the reference): per trace in the offset
+ * range, the 5-point path of the forward accumulation of lcc_err, lag
+ * tau(t) = path(t) - 1 - nl/2 samples; where the similarity on the path is
+ * >= sim_barrier and |tau| <= lag_barrier: misfit[0] += 1/2 tau^2 (samples^2),
+ * misfit[1] += 1/2 (syn(t) - obs(t + tau))^2 (obs linear between samples),
+ * res(t) = syn(t) - obs(t + tau(t)) (0 elsewhere), the gradient of misfit[1]
+ * with the path held fixed. */
+int lcc_diw_msf(float **obs, float **syn, float *sy_xy, float offset_range[], float eps, int nt, int nr, int nl,
+                int nw, float alpha_win, float sim_barrier, float lag_barrier, float *msf, float **res);
+
+/* ---------------------------------------------------------------------------
+ * Soft ranking (soft_ranking.c, soft_ranking.h), after Blondel et al., "Fast
+ * differentiable sorting and ranking" (ICML 2020).
+ */
+/* soft_ranking.h:7-11 (same values) */
+enum sortOrder_enum { SORT_ASCENDING = 0, SORT_DESCENDING = 1 };
+/* soft_ranking.h:27-33 (same layout) */
+typedef struct _soft_ranking_vars {
+    float *sorted, *sol;
+    float *soft_rank;
+    int *hard_rank, *perm, *target;
+} soft_ranking_vars;
+/* soft_ranking.c:21-66: perm (1-based) sorts a in the order, rank[i]
+ * (1-based) = the place of a[i] in it, sorted_a = a in that order; equal
+ * values keep their index order. */
+int sorting_flt_argsort(const int n, const float *a, const enum sortOrder_enum order, int *perm, int *rank,
+                        float *sorted_a);
+/* soft_ranking.c:68-121: sol = the non-increasing sequence nearest y in L2
+ * (pool adjacent violators, FP64 sums); target[first] = last index of each
+ * block and target[last] = its first. */
+int isotonic_l2(int n, float *y, float *sol, int *target);
+/* soft_ranking.c:123-146: the working arrays of n entries; free releases
+ * them and base itself (base must come from malloc). */
+void soft_ranking_init(int n, soft_ranking_vars *base);
+void soft_ranking_free(soft_ranking_vars *base);
+/* soft_ranking.h:54 (declared only): soft_rank = the Euclidean projection of
+ * a / reg onto the permutahedron of (n, ..., 1): tends to the ascending ranks
+ * (1 = smallest) as reg -> 0 and to (n + 1)/2 as reg -> inf.  hard_rank = the
+ * ascending ranks, perm / sorted = the descending order, sol / target = the
+ * isotonic solution and its blocks.  reg must be > 0. */
+int soft_ranking(int n, float *a, float regularization_strength, soft_ranking_vars *base);
+/* soft_ranking.h:55 (declared only): msf[0] = 1/2 |r(syn) - r(obs)|^2 with
+ * r = soft_ranking(., reg); res = its gradient with respect to syn. */
+int soft_ranking_msf(float *obs, float *syn, int n, float reg, soft_ranking_vars *obs_ranking,
+                     soft_ranking_vars *syn_ranking, float *msf, float *res);
+/* soft_ranking.h:56 (declared only): soft_ranking_msf per trace in [min, max]
+ * of offset (min > max keeps every trace), working sets par[0] (obs) and
+ * par[1] (syn) initialised for nt; msf[1] = the summed misfit, msf[0] =
+ * sum (obs - syn)^2, res its gradient (0 on the traces left out). */
+int soft_ranking_2d_trace_by_trace(float **obs, float **syn, float *sr_xyz, float min, float max,
+                                   soft_ranking_vars *par[], float reg, int nt, int nr, float msf[], float **res);
+const char *fwi_msf_rank_last_error(void);
 
 #ifdef __cplusplus
 }

@@ -232,7 +232,8 @@ int grad4_phase_ms(double *ms, int n);
  * inside its missing grad4_1shot; the misfit functions are its pool
  * (msf_pool.h, xcorr_msf.c, lsq_decon_cg.c, entropy_ot.h).
  *   kind: GRAD4_MSF_* below; params: par_xcorr1d * (XCORR1D), epsdf1 *
- *   (MF1D), eot_msf_par * (EOT: eps, tor, max_iter used), else ignored.
+ *   (MF1D), eot_msf_par * (EOT: eps, tor, max_iter used), grad4_diw_par *
+ *   (DIW), float * (SOFTRANK: the regularisation), else ignored.
  *   The nr/nt fields of params are set per shot from msf->nr and msf->nt.
  *   Traces whose source-receiver distance (sou->xyz_sr) lies outside
  *   [offset_min, offset_max] are left out; offset_min > offset_max keeps all.
@@ -247,6 +248,16 @@ int grad4_phase_ms(double *ms, int n);
 #define GRAD4_MSF_XCORR1D 2
 #define GRAD4_MSF_MF1D 3
 #define GRAD4_MSF_EOT 4
+#define GRAD4_MSF_DIW 5      /* lcc_diw_msf; params: grad4_diw_par * */
+#define GRAD4_MSF_SOFTRANK 6 /* soft_ranking_2d_trace_by_trace; params: float * (reg > 0) */
+/* the arguments of lcc_diw_msf (lcc_dtw.h:3) beyond the traces: odd window
+ * nw and lag count nl (samples), syn threshold eps, Gaussian width alpha_win
+ * (samples), and the similarity / |lag| (samples) limits of the samples
+ * that count */
+typedef struct _grad4_diw_par {
+    int nl, nw;
+    float eps, alpha_win, sim_barrier, lag_barrier;
+} grad4_diw_par;
 int grad4_set_misfit(int kind, const void *params, float offset_min, float offset_max);
 
 /* Band-pass (sfbandpass, butterworth_filter.c:127-200) of the observed and

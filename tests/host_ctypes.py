@@ -16,6 +16,9 @@ VP = ctypes.c_void_p
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 REF_SIGPROC = os.path.join(ROOT, "oracle", "_ref", "libref_sigproc.so")
+REF_RANK = os.path.join(ROOT, "oracle", "_ref", "libref_rank.so")
+PI = ctypes.POINTER(c_int)
+PPPF = ctypes.POINTER(PPF)
 
 
 class ParXcorr1d(ctypes.Structure):
@@ -36,6 +39,15 @@ class EotPar(ctypes.Structure):
 class EotLogPar(ctypes.Structure):
     _fields_ = [("bar", c_float), ("M", PPF), ("u", PF), ("v", PF), ("logp", PF), ("logq", PF),
                 ("delta_u", PF), ("delta_v", PF), ("vmax0", c_float)]
+
+
+class DiwPar(ctypes.Structure):
+    _fields_ = [("nl", c_int), ("nw", c_int), ("eps", c_float), ("alpha_win", c_float), ("sim_barrier", c_float),
+                ("lag_barrier", c_float)]
+
+
+class SoftRankingVars(ctypes.Structure):
+    _fields_ = [("sorted", PF), ("sol", PF), ("soft_rank", PF), ("hard_rank", PI), ("perm", PI), ("target", PI)]
 
 
 def _sig(f, argtypes, restype=c_int):
@@ -76,6 +88,34 @@ def lib():
     _sig(h.entropy_ot_2d_trace_by_trace, [PPF, PPF, ctypes.POINTER(EotPar), PF, PPF])
     _sig(h.entropy_ot_log_2d_trace_by_trace, [PPF, PPF, ctypes.POINTER(EotPar), ctypes.POINTER(EotLogPar), PF, PPF])
     _sig(h.fwi_msf_last_error, [], ctypes.c_char_p)
+    PSR = ctypes.POINTER(SoftRankingVars)
+    _sig(h.gaussian_win, [ctypes.c_size_t, c_float, PF], None)
+    _sig(h.lcc_err, [PPF, PPF, PF, PF, c_float, c_int, c_int, c_int, c_int, c_float, PPPF])
+    for name in ("ac_3pts_fw", "ac_3pts_bw", "ac_5pts_fw", "ac_5pts_bw"):
+        _sig(getattr(h, name), [PPF, PPF, c_int, c_int], None)
+    _sig(h.bt_fw_3pts, [PPF, c_int, c_int, PF], None)
+    _sig(h.bt_fw_5pts, [PPF, PPF, c_int, c_int, PF], None)
+    _sig(h.bt_bw_5pts, [PPF, PPF, c_int, c_int, PF], None)
+    _sig(h.lcc_diw_msf, [PPF, PPF, PF, PF, c_float, c_int, c_int, c_int, c_int, c_float, c_float, c_float, PF, PPF])
+    _sig(h.sorting_flt_argsort, [c_int, PF, c_int, PI, PI, PF])
+    _sig(h.isotonic_l2, [c_int, PF, PF, PI])
+    _sig(h.soft_ranking_init, [c_int, PSR], None)
+    _sig(h.soft_ranking_free, [PSR], None)
+    _sig(h.soft_ranking, [c_int, PF, c_float, PSR])
+    _sig(h.soft_ranking_msf, [PF, PF, c_int, c_float, PSR, PSR, PF, PF])
+    _sig(h.soft_ranking_2d_trace_by_trace, [PPF, PPF, PF, c_float, c_float, ctypes.POINTER(PSR), c_float, c_int, c_int,
+                                            PF, PPF])
+    _sig(h.fwi_msf_rank_last_error, [], ctypes.c_char_p)
+    return h
+
+
+def ref_rank_lib():
+    """The reference's soft_ranking.c (argsort, isotonic regression), or None."""
+    if not os.path.exists(REF_RANK):
+        return None
+    h = ctypes.CDLL(REF_RANK)
+    _sig(h.sorting_flt_argsort, [c_int, PF, c_int, PI, PI, PF])
+    _sig(h.isotonic_l2, [c_int, PF, PF, PI], None)
     return h
 
 

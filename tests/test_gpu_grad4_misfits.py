@@ -1,5 +1,5 @@
 """The host misfit step of grad4_1shot with each misfit of the pool (NCC,
-XCORR1D, MF1D, entropic OT, L2) and the trace conditioning (band-pass and
+XCORR1D, MF1D, entropic OT, dynamic warping, soft ranking, L2) and the trace conditioning (band-pass and
 first-break mute, with their transposes on the residual): the gradient the
 driver returns is the derivative of msf->val[1] with respect to the model,
 checked by central differences of val[1] along a smooth direction."""
@@ -36,18 +36,23 @@ def params(kind):
         return HC.Epsdf1(0, 0, 41, 0, 0, 300, b"middle", b"diff", 0.0, 0.3, 1.0)
     if kind == 4:
         return HC.EotPar(0, 0, 3000, None, None, None, 2e-4, 1e-9, 0.0)
+    if kind == 5:
+        return HC.DiwPar(11, 21, 1e-6, 5.0, 0.5, 100.0)
     return None
 
 
-@pytest.mark.parametrize("kind,name", [(0, "L2"), (1, "NCC"), (2, "XCORR1D"), (3, "MF1D"), (4, "EOT")])
+@pytest.mark.parametrize("kind,name", [(0, "L2"), (1, "NCC"), (2, "XCORR1D"), (3, "MF1D"), (4, "EOT"),
+                                       (6, "SOFTRANK")])
 @pytest.mark.parametrize("nt", [241, 240])
 def test_driver_gradient_is_the_derivative_of_the_misfit(h, kind, name, nt):
-    if kind in (3, 4) and nt == 240:
+    if kind >= 3 and nt == 240:
         pytest.skip("even nt only changes XCORR1D")
     g, vel, shot = case(nt=nt, seed=8)
     o = Oracle(np.float32)
     obs0 = o.forward(g, vel * 1.02, shot)["syn"]
     par = params(kind)
+    if kind == 6:   # soft ranks of samples a fiftieth of the largest amplitude apart
+        par = ctypes.c_float(float(np.abs(obs0).max()) / 50.0)
     # receivers beyond 400 m see no wave within nt steps: left out by offset, as
     # the reference's misfits do (xcorr_msf.c:69-78)
     assert h.grad4_set_misfit(kind, ctypes.byref(par) if par is not None else None, 0.0, 400.0) == 0
@@ -93,8 +98,46 @@ def test_driver_gradient_is_the_derivative_of_the_misfit(h, kind, name, nt):
     h.fwi_plan_free(p)
 
 
+def test_driver_diw_chains_the_host_residual(h):
+    """The warping misfit's residual is its derivative with the lag path held
+    (pinned in syn on the CPU); the path itself moves with the model, so a
+    difference quotient along the model does not pin it.  Here the driver must
+    return lcc_diw_msf's value on the synthetics it computed, and the gradient
+    of its residual: the FP64 oracle's adjoint of that residual."""
+    g, vel, shot = case(nt=241, seed=8)
+    o64 = Oracle(np.float64)
+    obs0 = Oracle(np.float32).forward(g, vel * 1.02, shot)["syn"]
+    par = params(5)
+    assert h.grad4_set_misfit(5, ctypes.byref(par), 1.0, 0.0) == 0
+    assert h.grad4_set_bandpass(0, 0, 0, 0, 0.0, 0.0) == 0     # no conditioning: the raw traces
+    assert h.grad4_set_fbt_mute(0, 0.0, 0.0, 0) == 0
+    p = PC.make_plan(h, g, shot, obs0, vel)
+    P = p.contents
+    PC.view(P.model.contents.md[0], (g.nz, g.nx))[:] = vel
+    h.grad4_1shot(p, 0)
+    val = PC.view(P.msf.contents.val, (2,)).copy()
+    body = PC.view(P.grad.contents.body, (g.nz, g.nx)).copy()
+    syn = PC.view(P.msf.contents.syn[0], (shot.nr, shot.nt)).copy()
+    h.fwi_plan_free(p)
+    hc = HC.lib()
+    res = np.zeros_like(syn)
+    mis = np.zeros(2, np.float32)
+    rng = np.array([1.0, 0.0], np.float32)
+    assert hc.lcc_diw_msf(HC.rows(obs0), HC.rows(syn), None, HC.fp(rng), par.eps, shot.nt, shot.nr, par.nl, par.nw,
+                          par.alpha_win, par.sim_barrier, par.lag_barrier, HC.fp(mis), HC.rows(res)) == 0
+    np.testing.assert_array_equal(val, mis)
+    assert mis[1] > 0 and np.abs(res).max() > 0
+    f = o64.forward(g, vel.astype(np.float64), shot, want_strips=True)
+    g64 = o64.backward(g, vel.astype(np.float64), shot, res.astype(np.float64), f)["grad"]
+    err = float(np.linalg.norm(body - g64) / np.linalg.norm(g64))
+    print(f"DIW: misfit {mis}, gradient vs FP64 oracle adjoint of the host residual {err:.2e}")
+    assert err < 1e-4
+
+
 def test_xcorr1d_leading_refused(h):
     par = HC.ParXcorr1d(0, 0, 0, b"leading")
     assert h.grad4_set_misfit(2, ctypes.byref(par), 1.0, 0.0) == 1
     assert h.grad4_set_misfit(9, None, 1.0, 0.0) == 1
+    assert h.grad4_set_misfit(5, ctypes.byref(HC.DiwPar(10, 21, 1e-6, 5.0, 0.5, 100.0)), 1.0, 0.0) == 1
+    assert h.grad4_set_misfit(6, ctypes.byref(ctypes.c_float(0.0)), 1.0, 0.0) == 1
     assert h.grad4_set_misfit(0, None, 1.0, 0.0) == 0
