@@ -8,6 +8,11 @@ disabled.  The C host driver (grad4_1shot.c), the input/resampling layer
 (fwi_io.c), the trace conditioning (fwi_sigproc.c), the misfit pool
 (fwi_msf.c), the gradient post-processing (fwi_gradprec.c) and the L-BFGS
 loop (fwi_lbfgs.c) are compiled with gcc.
+
+    python fwi2dacoustic-mpi-lbfgs_amd/build.py --sanitize
+
+also builds libfwi_hip_san.so, the host code under AddressSanitizer and
+UndefinedBehaviorSanitizer, for the CPU tests.
 """
 import os
 import subprocess
@@ -61,6 +66,51 @@ def build(verbose=True):
     return LIB
 
 
+# Sanitized build of the host code (CPU tests only): the host C files and the
+# host side of fwi_hip.cpp with AddressSanitizer and UndefinedBehaviorSanitizer
+# (every finding fatal), linked with the ordinary device objects into
+# libfwi_hip_san.so.  One toolchain (ROCm's clang and its shared sanitizer
+# runtime) for every instrumented object, so that one runtime serves the
+# process; the device code is not instrumented.
+CLANG = os.environ.get("FWI_CLANG", "/opt/rocm/llvm/bin/clang")
+LIB_SAN = os.path.join(HERE, "libfwi_hip_san.so")
+SAN = ["-fsanitize=address,undefined", "-fno-sanitize-recover=all", "-fno-omit-frame-pointer", "-g"]
+HOST_C = ("grad4_1shot.c", "fwi_io.c", "fwi_sigproc.c", "fwi_msf.c", "fwi_msf_rank.c", "fwi_gradprec.c",
+          "fwi_lbfgs.c")
+
+
+def sanitizer_runtime():
+    """the shared sanitizer runtime the instrumented objects link against"""
+    out = subprocess.run([CLANG, "-print-file-name=libclang_rt.asan-x86_64.so"], check=True,
+                         capture_output=True, text=True).stdout.strip()
+    return out if os.path.exists(out) else None
+
+
+def build_sanitized():
+    sb = os.path.join(BUILD, "san")
+    os.makedirs(sb, exist_ok=True)
+    objs = []
+    for src in ("fwi_kernels.hip", "fwi_noalloc.hip"):
+        obj = os.path.join(BUILD, src + ".o")
+        if not os.path.exists(obj):
+            run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    host_san = [f for flag in SAN for f in ("-Xarch_host", flag)]
+    obj = os.path.join(sb, "fwi_hip.cpp.o")
+    run([HIPCC, *HIP_FLAGS, *host_san, "-c", os.path.join(CSRC, "fwi_hip.cpp"), "-o", obj])
+    objs.append(obj)
+    for src in HOST_C:
+        obj = os.path.join(sb, src + ".o")
+        run([CLANG, "-O1", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-pthread", *SAN, f"-I{INC}", "-c",
+             os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    tmp = LIB_SAN + ".tmp"
+    run([CLANG + "++", "-shared", "-fPIC", *SAN, "-shared-libsan", "-o", tmp, *objs, "-L/opt/rocm/lib",
+         "-lamdhip64", "-lpthread", "-Wl,-rpath,/opt/rocm/lib"])
+    os.replace(tmp, LIB_SAN)
+    return LIB_SAN
+
+
 def torch_hip_dir():
     """Directory of torch's bundled libamdhip64.so (None without torch)."""
     try:
@@ -76,4 +126,6 @@ def torch_hip_dir():
 
 if __name__ == "__main__":
     build()
+    if "--sanitize" in sys.argv[1:]:
+        build_sanitized()
     sys.exit(0)

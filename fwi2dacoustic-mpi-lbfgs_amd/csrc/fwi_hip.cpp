@@ -577,6 +577,53 @@ extern "C" int fwi_hip_create(const fwi_hip_desc *desc, fwi_hip_ctx **out)
     return FWI_HIP_OK;
 }
 
+// Largest stable velocity of the leapfrog step p^{k+1} = 2 p^k - p^{k-1} +
+// v^2 dt^2 L p^k: stable while v^2 dt^2 lambda_max(-L) <= 4 (von Neumann).
+// -L splits into x and z parts whose largest eigenvalues add:
+//   x: the staggered pair (c0, c1) of fd_coe at the Nyquist wavenumber,
+//      4 (c0 - c1)^2 / dx^2 (the edge rows only lower it);
+//   z: a mode bound to the free-surface rows (the one-sided 1/840 and 1/24
+//      formulas of the top two v rows and the top vz row), 6.0195791 / dz^2,
+//      the largest eigenvalue of the 1-D operator -D_vz D_v built from those
+//      rows over the interior pair (it exceeds the interior 4 (c0 - c1)^2 by
+//      a factor 1.0955 and does not depend on the depth of the grid).
+static double stable_vmax(const fwi_hip_grid &g)
+{
+    const double c = 0.1129042e1 + 0.4301412e-1, lz_fs = 6.0195791;
+    const double lx = 4.0 * c * c / ((double)g.dx * g.dx), lz = lz_fs / ((double)g.dz * g.dz);
+    return 2.0 / ((double)g.dt * sqrt(lx + lz));
+}
+
+static int check_model(fwi_hip_ctx *c, const fwi_hip_grid &g, const float *vel, long long *cell)
+{
+    const long long n = (long long)g.nx * g.nz;
+    long long imax = 0;
+    for (long long i = 0; i < n; i++) {
+        if (!(vel[i] > 0) || !isfinite(vel[i])) {
+            if (cell) *cell = i;
+            return set_err(c, FWI_HIP_EINVAL, "model: velocity must be finite and > 0 (cell %lld: %g)", i,
+                           (double)vel[i]);
+        }
+        if (vel[i] > vel[imax]) imax = i;
+    }
+    const double vmax = stable_vmax(g);
+    if (vel[imax] > vmax) {
+        if (cell) *cell = imax;
+        return set_err(c, FWI_HIP_EINVAL,
+                       "model: velocity %g m/s at cell %lld is above the stability limit %g m/s of dt=%g, dx=%g, dz=%g",
+                       (double)vel[imax], imax, vmax, (double)g.dt, (double)g.dx, (double)g.dz);
+    }
+    if (cell) *cell = -1;
+    return FWI_HIP_OK;
+}
+
+extern "C" int fwi_hip_check_model(const fwi_hip_grid *g, const float *vel, long long *cell)
+{
+    if (!g || !vel || g->nx < 1 || g->nz < 1 || !(g->dx > 0) || !(g->dz > 0) || !(g->dt > 0))
+        return set_err(nullptr, FWI_HIP_EINVAL, "check_model: NULL argument or empty grid");
+    return check_model(nullptr, *g, vel, cell);
+}
+
 // model_ext (fwi_2d.c:67-107) of kp = vel*vel*dt*dt into the pitched f-grid
 extern "C" int fwi_hip_set_model(fwi_hip_ctx *c, const float *vel)
 {
@@ -584,9 +631,8 @@ extern "C" int fwi_hip_set_model(fwi_hip_ctx *c, const float *vel)
     const FwiGeomDev &g = c->g;
     const float dt = c->d.grid.dt;
     const int nx = g.nx, nz = g.nz, npml = g.npml;
-    for (long long i = 0; i < (long long)nx * nz; i++)
-        if (!(vel[i] > 0) || !isfinite(vel[i]))
-            return set_err(c, FWI_HIP_EINVAL, "set_model: velocity must be finite and > 0 (cell %lld)", i);
+    int rc = check_model(c, c->d.grid, vel, nullptr);
+    if (rc) return rc;
     std::vector<float> kp((size_t)nx * nz), ekp((size_t)c->fs, 0.f);
     for (long long i = 0; i < (long long)nx * nz; i++) kp[i] = vel[i] * vel[i] * dt * dt;
     for (int z = 0; z < g.pnz; z++) {

@@ -23,7 +23,7 @@ EXPORTS = (
     "fwi_hip_create", "fwi_hip_destroy", "fwi_hip_last_error", "fwi_hip_bytes", "fwi_hip_set_model",
     "fwi_hip_set_shots", "fwi_hip_forward", "fwi_hip_set_observed", "fwi_hip_misfit_l2",
     "fwi_hip_set_residual", "fwi_hip_adjoint", "fwi_hip_synchronize", "fwi_hip_set_timing",
-    "fwi_hip_kernel_time", "fwi_hip_read_state", "fwi_hip_op_pml", "fwi_hip_op_interior",
+    "fwi_hip_kernel_time", "fwi_hip_read_state", "fwi_hip_check_model", "fwi_hip_op_pml", "fwi_hip_op_interior",
     "fwi_hip_time_integral", "fwi_hip_op_pml_noalloc", "fwi_hip_write_bdr", "fwi_hip_read_bdr", "fwi_hip_cpml_coe",
     "fwi_hip_set_graph", "fwi_hip_set_kernels", "fwi_hip_set_rec_group", "fwi_hip_set_imaging", "fwi_hip_set_bwd_chunk", "fwi_hip_set_checkpointing", "fwi_hip_set_revolve", "fwi_hip_backward_plan", "fwi_hip_plan_memory", "fwi_hip_revolve_schedule", "fwi_hip_set_bwd_overlap", "fwi_hip_set_bwd_fused", "fwi_hip_set_concurrent", "fwi_hip_set_interior_wgs", "fwi_hip_device_count", "fwi_hip_dmalloc", "fwi_hip_dfree", "fwi_hip_host_alloc", "fwi_hip_host_free", "fwi_hip_memcpy",
 )
@@ -274,6 +274,20 @@ def plan_memory(geom, max_shots, max_nt, max_nr, hbm_budget, storage="fp32"):
     if rc:
         raise FwiError(rc, lib.fwi_hip_last_error(None).decode())
     return m.value, sn.value, b.value
+
+
+def check_model(geom, vel):
+    """The host model check of set_model (no device call): raises FwiError
+    (EINVAL) naming the first non-finite, non-positive or unstable cell."""
+    lib = load()
+    g = make_grid(geom)
+    v = np.ascontiguousarray(vel, np.float32)
+    if v.size != geom.nx * geom.nz:
+        raise ValueError(f"model has {v.size} cells, the grid {geom.nx * geom.nz}")
+    cell = ctypes.c_longlong()
+    rc = lib.fwi_hip_check_model(ctypes.byref(g), v.ctypes.data_as(ctypes.c_void_p), ctypes.byref(cell))
+    if rc:
+        raise FwiError(rc, lib.fwi_hip_last_error(None).decode())
 
 
 def revolve_schedule(nt, snaps):

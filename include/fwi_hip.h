@@ -76,6 +76,18 @@ size_t fwi_hip_bytes(const fwi_hip_ctx *ctx);           /* device bytes held */
  * the CPML by edge replication (model_ext, fwi_2d.c:67-107).                */
 int fwi_hip_set_model(fwi_hip_ctx *ctx, const float *vel);
 
+/* The host check fwi_hip_set_model runs first (no device work): every
+ * velocity finite and > 0, and none above the stability limit of the
+ * leapfrog step with the staggered pair of fwi_fdcoe (fwi_2d.c:302-316) and
+ * the free-surface rows (fwi_2d.c:805-807,848),
+ * v dt sqrt(4 (c0 - c1)^2 / dx^2 + 6.0195791 / dz^2) <= 2 (the second term
+ * is the largest eigenvalue of the z operator, a mode bound to the free
+ * surface).  The reference checks neither
+ * and runs an unstable model into overflow.  FWI_HIP_EINVAL names the first
+ * offending cell in fwi_hip_last_error(NULL) and *cell (-1 when the model
+ * passes; cell may be NULL). */
+int fwi_hip_check_model(const fwi_hip_grid *g, const float *vel, long long *cell);
+
 /* The shots of the next forward/adjoint pair (receiver cells must be unique
  * within a shot).  Traces of a batch are packed shot after shot, each
  * [nr][nt] with time contiguous (fwi_msf.syn, header.h:66).                 */

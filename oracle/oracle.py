@@ -26,7 +26,9 @@ class Oracle:
 
     def __init__(self, dtype=np.float32):
         self.dtype = np.dtype(dtype)
-        name = "liboracle_f32.so" if self.dtype == np.float32 else "liboracle_f64.so"
+        # FWI_ORACLE_VARIANT=san: the sanitizer build (make san)
+        var = os.environ.get("FWI_ORACLE_VARIANT", "")
+        name = f"liboracle{'_' + var if var else ''}_{'f32' if self.dtype == np.float32 else 'f64'}.so"
         path = os.path.join(_HERE, name)
         if not os.path.exists(path):
             build()
