@@ -157,10 +157,13 @@ def boundary_leg(g, start, shots, obs_packed, steps, warmup):
     """The same gradient through the drop-in boundary: grad_all_shots over one
     reference-layout plan per shot (fwi_plan.h), with the model and the
     observed data in host memory.  Each call uploads the model, runs the
-    forward pass, copies the traces to the host, runs the host misfit step
-    (L2 in FP64, threaded over shots), copies the adjoint source back, runs
-    the adjoint pass and copies the gradient to plans[0].  Returns the rate
-    and the mean wall time of each phase."""
+    forward pass with each block of 100 steps' traces copied to the host
+    while the next block propagates and the host misfit step (L2 in FP64,
+    threaded over shots) run on each block as it arrives, hands the adjoint
+    source over (uploaded block by block beside the adjoint steps), runs the
+    adjoint pass and copies the gradient to plans[0].  Returns the rate and
+    the mean wall time of each phase (the misfit of the plain L2 is inside
+    the forward phase, the residual upload inside the adjoint phase)."""
     import ctypes
     import numpy as np
     from fwihip import plans as PC
@@ -186,7 +189,7 @@ def boundary_leg(g, start, shots, obs_packed, steps, warmup):
         for p in plans:
             h.fwi_plan_free(p)
     acc /= steps
-    names = ("setup", "forward_and_trace_copy", "host_misfit", "residual_copy", "adjoint_and_gradient_copy")
+    names = ("setup", "forward_trace_copy_and_l2", "host_misfit_rest", "residual_hand_over", "adjoint_residual_copy_and_gradient")
     return dict(value=len(shots) * steps / wall, unit="shot-gradients/s", ms_per_step=wall / steps * 1e3,
                 steps=steps, warmup=warmup, phases_ms={k: round(float(v), 2) for k, v in zip(names, acc)})
 

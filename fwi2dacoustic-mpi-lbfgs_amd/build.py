@@ -9,10 +9,8 @@ disabled.  The C host driver (grad4_1shot.c), the input/resampling layer
 (fwi_msf.c), the gradient post-processing (fwi_gradprec.c) and the L-BFGS
 loop (fwi_lbfgs.c) are compiled with gcc.
 
-    python fwi2dacoustic-mpi-lbfgs_amd/build.py --sanitize
-
 also builds libfwi_hip_san.so, the host code under AddressSanitizer and
-UndefinedBehaviorSanitizer, for the CPU tests.
+UndefinedBehaviorSanitizer, for the CPU tests (--no-sanitize skips it).
 """
 import os
 import subprocess
@@ -126,6 +124,6 @@ def torch_hip_dir():
 
 if __name__ == "__main__":
     build()
-    if "--sanitize" in sys.argv[1:]:
+    if "--no-sanitize" not in sys.argv[1:]:
         build_sanitized()
     sys.exit(0)

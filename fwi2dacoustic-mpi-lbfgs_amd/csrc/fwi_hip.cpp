@@ -152,6 +152,11 @@ struct fwi_hip_ctx {
     // does not read, and never touches p or the images)
     bool bwd_overlap = false;
     hipEvent_t ev_fork2 = nullptr, ev_join2 = nullptr;
+    // host <-> device trace copies beside the steps (forward_tm, a residual
+    // from set_residual_tm): a copy stream and one event pair per graph block
+    hipStream_t stc = nullptr;
+    std::vector<hipEvent_t> blk_done, blk_copy;
+    const float *res_host = nullptr;  // time-major residual the next adjoint uploads
     char err[512] = "";
     size_t bytes = 0;
     std::vector<std::pair<void *, size_t>> allocs;  // live allocations and their sizes
@@ -386,6 +391,10 @@ extern "C" void fwi_hip_destroy(fwi_hip_ctx *c)
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     if (c->ev_fork2) (void)hipEventDestroy(c->ev_fork2);
     if (c->ev_join2) (void)hipEventDestroy(c->ev_join2);
+    if (c->stc) (void)hipStreamSynchronize(c->stc);
+    for (hipEvent_t e : c->blk_done) (void)hipEventDestroy(e);
+    for (hipEvent_t e : c->blk_copy) (void)hipEventDestroy(e);
+    if (c->stc) (void)hipStreamDestroy(c->stc);
     delete c;
 }
 
@@ -1070,6 +1079,52 @@ static int tick(fwi_hip_ctx *c, hipEvent_t *e)
 
 // an event pair around a whole pass (timing 2): t_ms[which] += its time,
 // t_n[which] += its time steps
+// the copy stream and n event pairs
+static int ensure_copy(fwi_hip_ctx *c, int n)
+{
+    if (!c->stc) HIPCHK(c, hipStreamCreateWithFlags(&c->stc, hipStreamNonBlocking));
+    while ((int)c->blk_done.size() < n) {
+        hipEvent_t a, b;
+        HIPCHK(c, hipEventCreateWithFlags(&a, hipEventDisableTiming));
+        c->blk_done.push_back(a);
+        HIPCHK(c, hipEventCreateWithFlags(&b, hipEventDisableTiming));
+        c->blk_copy.push_back(b);
+    }
+    return FWI_HIP_OK;
+}
+
+// rows [k0, k1) of every shot's time-major traces between host and device
+// (each shot's rows are contiguous at its trace offset; one strided copy when
+// the shots are packed alike)
+static hipError_t copy_rows(fwi_hip_ctx *c, float *host, float *dev, int k0, int k1, bool to_host, hipStream_t s)
+{
+    const hipMemcpyKind kind = to_host ? hipMemcpyDeviceToHost : hipMemcpyHostToDevice;
+    const FwiShotDev &d0 = c->shots_h[0];
+    bool alike = d0.nr > 0;
+    for (int i = 0; i < c->nshots && alike; i++) {
+        const FwiShotDev &d = c->shots_h[i];
+        alike = d.nt == d0.nt && d.nr == d0.nr && d.trace_off == (long long)i * d0.nt * d0.nr;
+    }
+    if (alike) {
+        const int k1c = std::min(k1, d0.nt);
+        if (k0 >= k1c) return hipSuccess;
+        const size_t pitch = sizeof(float) * (size_t)d0.nt * d0.nr, off = (size_t)k0 * d0.nr;
+        const size_t width = sizeof(float) * (size_t)(k1c - k0) * d0.nr;
+        return to_host ? hipMemcpy2DAsync(host + off, pitch, dev + off, pitch, width, c->nshots, kind, s)
+                       : hipMemcpy2DAsync(dev + off, pitch, host + off, pitch, width, c->nshots, kind, s);
+    }
+    for (int i = 0; i < c->nshots; i++) {
+        const FwiShotDev &d = c->shots_h[i];
+        const int k1c = std::min(k1, d.nt);
+        if (d.nr == 0 || k0 >= k1c) continue;
+        const size_t off = (size_t)d.trace_off + (size_t)k0 * d.nr, n = sizeof(float) * (size_t)(k1c - k0) * d.nr;
+        hipError_t e = to_host ? hipMemcpyAsync(host + off, dev + off, n, kind, s)
+                               : hipMemcpyAsync(dev + off, host + off, n, kind, s);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 static int pass_begin(fwi_hip_ctx *c, hipEvent_t *e)
 {
     *e = nullptr;
@@ -1308,7 +1363,9 @@ extern "C" int fwi_hip_set_bwd_chunk(fwi_hip_ctx *c, int shots)
     return FWI_HIP_OK;
 }
 
-extern "C" int fwi_hip_forward(fwi_hip_ctx *c, float *traces)
+// tm: traces go to the host time-major (as the device records them) and fn,
+// when given, sees each block of steps as soon as its rows have arrived
+static int forward_impl(fwi_hip_ctx *c, float *traces, bool tm, fwi_hip_block_fn fn, void *user)
 {
     if (!c) return set_err(c, FWI_HIP_EINVAL, "forward: NULL context");
     if (!c->have_model || c->nshots == 0) return set_err(c, FWI_HIP_ESTATE, "forward: set_model and set_shots first");
@@ -1343,6 +1400,7 @@ extern "C" int fwi_hip_forward(fwi_hip_ctx *c, float *traces)
         c->rv_sched_s = snaps;
     }
     const int fmode = ck ? FWI_FWD_TRACES : FWI_FWD_FRAMES;
+    bool pipe = false;  // trace blocks copied beside the steps
     HIPCHK(c, fwi_launch_fill_f32(c->P0, S * c->fs, 0.f, c->st));
     HIPCHK(c, fwi_launch_fill_f32(c->P1, S * c->fs, 0.f, c->st));
     rc = zero_cpml(c);
@@ -1405,7 +1463,18 @@ extern "C" int fwi_hip_forward(fwi_hip_ctx *c, float *traces)
             if (rc) return rc;
         }
         HIPCHK(c, fwi_launch_set_int(c->kbase, 0, c->st));
-        for (int blk = 0; blk < (c->kmax + B - 1) / B; blk++) HIPCHK(c, hipGraphLaunch(c->gx_fwd, c->st));
+        const int nblk = (c->kmax + B - 1) / B;
+        pipe = tm && traces;
+        if (pipe && (rc = ensure_copy(c, nblk))) return rc;
+        for (int blk = 0; blk < nblk; blk++) {
+            HIPCHK(c, hipGraphLaunch(c->gx_fwd, c->st));
+            if (!pipe) continue;
+            // the block's trace rows go to the host while the next block runs
+            HIPCHK(c, hipEventRecord(c->blk_done[blk], c->st));
+            HIPCHK(c, hipStreamWaitEvent(c->stc, c->blk_done[blk], 0));
+            HIPCHK(c, copy_rows(c, traces, c->syn_tm, blk * B, std::min(c->kmax, (blk + 1) * B), true, c->stc));
+            HIPCHK(c, hipEventRecord(c->blk_copy[blk], c->stc));
+        }
     } else {
         for (int k = 0; k < c->kmax; k++) {
             hipEvent_t e0, e1;
@@ -1424,12 +1493,36 @@ extern "C" int fwi_hip_forward(fwi_hip_ctx *c, float *traces)
         if (d.nr == 0) continue;
         HIPCHK(c, fwi_launch_transpose(c->syn_tm + d.trace_off, c->syn + d.trace_off, d.nt, d.nr, c->st));
     }
-    if (traces)
+    if (traces && !tm)
         HIPCHK(c, hipMemcpyAsync(traces, c->syn, sizeof(float) * c->ntrace, hipMemcpyDeviceToHost, c->st));
+    if (traces && tm && !pipe)
+        HIPCHK(c, hipMemcpyAsync(traces, c->syn_tm, sizeof(float) * c->ntrace, hipMemcpyDeviceToHost, c->st));
+    int frc = 0;  // the first nonzero return of fn (later blocks are not handed over)
+    if (pipe) {
+        const int B = c->graph_steps, nblk = (c->kmax + B - 1) / B;
+        for (int blk = 0; blk < nblk; blk++) {
+            HIPCHK(c, hipEventSynchronize(c->blk_copy[blk]));
+            if (fn && !frc) frc = fn(user, blk * B, std::min(c->kmax, (blk + 1) * B));
+        }
+    }
     HIPCHK(c, hipStreamSynchronize(c->st));
+    if (tm && !pipe && fn) frc = fn(user, 0, c->kmax);
     c->have_fwd = true;
     c->have_res = false;
+    c->res_host = nullptr;
+    if (frc) return set_err(c, frc, "forward: the trace block callback returned %d", frc);
     return FWI_HIP_OK;
+}
+
+extern "C" int fwi_hip_forward(fwi_hip_ctx *c, float *traces)
+{
+    return forward_impl(c, traces, false, nullptr, nullptr);
+}
+
+extern "C" int fwi_hip_forward_tm(fwi_hip_ctx *c, float *traces_tm, fwi_hip_block_fn fn, void *user)
+{
+    if (c && fn && !traces_tm) return set_err(c, FWI_HIP_EINVAL, "forward_tm: a callback needs the trace buffer");
+    return forward_impl(c, traces_tm, true, fn, user);
 }
 
 extern "C" int fwi_hip_set_observed(fwi_hip_ctx *c, const float *obs)
@@ -1455,6 +1548,7 @@ extern "C" int fwi_hip_misfit_l2(fwi_hip_ctx *c, double *misfit)
     for (double v : part) J += v;
     *misfit = J;
     c->have_res = true;
+    c->res_host = nullptr;
     return FWI_HIP_OK;
 }
 
@@ -1464,6 +1558,16 @@ extern "C" int fwi_hip_set_residual(fwi_hip_ctx *c, const float *res)
     if (c->nshots == 0) return set_err(c, FWI_HIP_ESTATE, "set_residual: set_shots first");
     HIPCHK(c, hipMemcpyAsync(c->res, res, sizeof(float) * c->ntrace, hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
+    c->have_res = true;
+    c->res_host = nullptr;
+    return FWI_HIP_OK;
+}
+
+extern "C" int fwi_hip_set_residual_tm(fwi_hip_ctx *c, const float *res_tm)
+{
+    if (!c || !res_tm) return set_err(c, FWI_HIP_EINVAL, "set_residual_tm: NULL argument");
+    if (c->nshots == 0) return set_err(c, FWI_HIP_ESTATE, "set_residual_tm: set_shots first");
+    c->res_host = res_tm;  // read by the next adjoint, block by block beside its steps
     c->have_res = true;
     return FWI_HIP_OK;
 }
@@ -1483,7 +1587,15 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
     HIPCHK(c, fwi_launch_fill_f64(c->gacc, (long long)ngr * nb, 0.0, c->st));
     int rc = zero_cpml(c);
     if (rc) return rc;
-    for (int s = 0; s < c->nshots; s++) {
+    // a time-major residual from the host (set_residual_tm) goes up block by
+    // block beside the steps in the graph form of a one-chunk boundary-saving
+    // pass, else whole before them; a receiver-major one is transposed here
+    const bool tm_res = c->res_host != nullptr;
+    const bool one_chunk = !(tile && c->bwd_chunk > 0 && c->bwd_chunk < c->nshots);
+    const bool pipe_res = tm_res && c->bwd_method == 0 && c->graph_steps > 0 && c->timing != 1 && one_chunk;
+    if (tm_res && !pipe_res)
+        HIPCHK(c, hipMemcpyAsync(c->res_tm, c->res_host, sizeof(float) * c->ntrace, hipMemcpyHostToDevice, c->st));
+    for (int s = 0; s < c->nshots && !tm_res; s++) {
         const FwiShotDev &d = c->shots_h[s];
         if (d.nr == 0) continue;
         HIPCHK(c, fwi_launch_transpose(c->res + d.trace_off, c->res_tm + d.trace_off, d.nr, d.nt, c->st));
@@ -1662,6 +1774,7 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
         if (gk) HIPCHK(c, hipMemcpyAsync(gk, c->gk, sizeof(double) * nb, hipMemcpyDeviceToHost, c->st));
         HIPCHK(c, hipStreamSynchronize(c->st));
         c->have_fwd = false;
+        c->res_host = nullptr;
         return FWI_HIP_OK;
     }
     // shot chunks, each a whole backward pass over its own slice of the
@@ -1745,7 +1858,18 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
             }
             HIPCHK(c, fwi_launch_set_int(c->kbase, kmax - 1, c->st));
             const int nblk = ovl ? (kmax + B) / B : (kmax + B - 1) / B;
-            for (int blk = 0; blk < nblk; blk++) HIPCHK(c, hipGraphLaunch(c->gx_bwd[ch], c->st));
+            if (pipe_res && (rc = ensure_copy(c, nblk))) return rc;
+            for (int blk = 0; blk < nblk; blk++) {
+                if (pipe_res) {
+                    // residual rows of the block's steps kmax-1-blk*B down to kmax-(blk+1)*B
+                    const int hi = kmax - blk * B, lo = std::max(0, hi - B);
+                    if (hi > 0)
+                        HIPCHK(c, copy_rows(c, const_cast<float *>(c->res_host), c->res_tm, lo, hi, false, c->stc));
+                    HIPCHK(c, hipEventRecord(c->blk_copy[blk], c->stc));
+                    HIPCHK(c, hipStreamWaitEvent(c->st, c->blk_copy[blk], 0));
+                }
+                HIPCHK(c, hipGraphLaunch(c->gx_bwd[ch], c->st));
+            }
         } else if (ovl) {
             for (int k = kmax - 1; k >= -1; k--) HIPCHK(c, pair_step(nullptr, k));
         } else if (fused) {
@@ -1782,6 +1906,7 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
     // the reconstruction ran the forward's final state back to the start:
     // another adjoint needs another forward
     c->have_fwd = false;
+    c->res_host = nullptr;
     return FWI_HIP_OK;
 }
 

@@ -311,7 +311,7 @@ static soft_ranking_vars *rank_vars(int n)
 /* The host misfit step of one shot, on thread tid's working traces.  t holds
  * the shot's nr traces at the propagation rate (nt samples each) and
  * receives the adjoint source. */
-static int host_misfit(fwi_plan *pl, float *t, int nt, float dt_fine, int tid)
+static int host_misfit(fwi_plan *pl, float *t, int nt, float dt_fine, int tid, int tm)
 {
     fwi_msf *m = pl->msf;
     const int nr = (int)m->nr, nt0 = (int)m->nt, jump = (int)m->jump;
@@ -326,10 +326,11 @@ static int host_misfit(fwi_plan *pl, float *t, int nt, float dt_fine, int tid)
         rc = fail(FWI_HIP_ENOMEM, "host memory");
         goto done;
     }
-    /* synthetic at the data rate (fwi_2d.c:195) */
+    /* synthetic at the data rate (fwi_2d.c:195); t is [nr][nt], or [nt][nr] when tm */
     for (int r = 0; r < nr; r++)
         for (int i0 = 0; i0 < nt0; i0++) {
-            const float v = t[(size_t)r * nt + (size_t)i0 * jump];
+            const size_t it = (size_t)i0 * jump;
+            const float v = tm ? t[it * nr + r] : t[(size_t)r * nt + it];
             m->syn[r][i0] = v;
             sy[(size_t)r * nt0 + i0] = v;
             ob[(size_t)r * nt0 + i0] = m->obs0[r][i0];
@@ -457,17 +458,27 @@ static int host_misfit(fwi_plan *pl, float *t, int nt, float dt_fine, int tid)
     /* adjoint source at the propagation rate */
     if (g_res_mode == 0 || jump == 1) {
         for (int r = 0; r < nr; r++) /* residual on the samples, zero between them */
-            for (int it = 0; it < nt; it++)
-                t[(size_t)r * nt + it] = (it % jump == 0) ? rs[(size_t)r * nt0 + it / jump] : 0.0f;
+            for (int it = 0; it < nt; it++) {
+                const float v = (it % jump == 0) ? rs[(size_t)r * nt0 + it / jump] : 0.0f;
+                t[tm ? (size_t)it * nr + r : (size_t)r * nt + it] = v;
+            }
     } else {
-        float **tr = slab_rows(t, nt, nr);
+        const size_t nfine = (size_t)nt * nr;
+        float *dst = tm ? (float *)malloc(sizeof(float) * (nfine > 0 ? nfine : 1)) : t;
+        float **tr = dst ? slab_rows(dst, nt, nr) : NULL;
         if (!tr) {
+            if (tm) free(dst);
             rc = fail(FWI_HIP_ENOMEM, "host memory");
             goto done;
         }
         int e = fwi_trcinp(rsr, nt0, m->dt, tr, nt, dt_fine, nr);
         free(tr);
         if (e) rc = fail(e, "trcinp: %s", fwi_io_last_error());
+        if (tm) {
+            for (int r = 0; !e && r < nr; r++)
+                for (int it = 0; it < nt; it++) t[(size_t)it * nr + r] = dst[(size_t)r * nt + it];
+            free(dst);
+        }
     }
 done:
     free(drop);
@@ -485,24 +496,22 @@ typedef struct {
     const long long *off;
     int nshots, nthreads, t, rc;
     float dt;
+    int tm;
 } MisfitJob;
 
 static void *misfit_worker(void *p)
 {
     MisfitJob *j = (MisfitJob *)p;
     for (int s = j->t; s < j->nshots && !j->rc; s += j->nthreads)
-        j->rc = host_misfit(j->plans[s], j->tr + j->off[s], (int)j->plans[s]->grid->base->nt, j->dt, j->t);
+        j->rc = host_misfit(j->plans[s], j->tr + j->off[s], (int)j->plans[s]->grid->base->nt, j->dt, j->t, j->tm);
     return NULL;
 }
 
 /* tr holds the shots' traces one after the other, shot s as nr x nt with
  * its own plan's nt */
-static int host_misfit_all(fwi_plan **plans, int nshots, float *tr, float dt)
+/* the misfit threads: FWI_HOST_THREADS, else the online cores, at most 16 */
+static int misfit_threads(int nshots)
 {
-    long long *off = (long long *)malloc(sizeof(long long) * (nshots + 1));
-    if (!off) return fail(FWI_HIP_ENOMEM, "host memory");
-    off[0] = 0;
-    for (int s = 0; s < nshots; s++) off[s + 1] = off[s] + (long long)plans[s]->msf->nr * (long long)plans[s]->grid->base->nt;
     int nth = 16;
     const char *env = getenv("FWI_HOST_THREADS");
     if (env && atoi(env) > 0) nth = atoi(env);
@@ -513,12 +522,23 @@ static int host_misfit_all(fwi_plan **plans, int nshots, float *tr, float dt)
     if (nth > nshots) nth = nshots;
     if (nth < 1) nth = 1;
     if (nth > MISFIT_MAX_THREADS) nth = MISFIT_MAX_THREADS;
+    return nth;
+}
+
+/* tm: the shots' traces are time-major ([nt][nr], fwi_hip_forward_tm) */
+static int host_misfit_all(fwi_plan **plans, int nshots, float *tr, float dt, int tm)
+{
+    long long *off = (long long *)malloc(sizeof(long long) * (nshots + 1));
+    if (!off) return fail(FWI_HIP_ENOMEM, "host memory");
+    off[0] = 0;
+    for (int s = 0; s < nshots; s++) off[s + 1] = off[s] + (long long)plans[s]->msf->nr * (long long)plans[s]->grid->base->nt;
+    const int nth = misfit_threads(nshots);
     MisfitJob jobs[MISFIT_MAX_THREADS];
     pthread_t th[MISFIT_MAX_THREADS];
     int made[MISFIT_MAX_THREADS];
     int rc = FWI_HIP_OK;
     for (int t = 0; t < nth; t++) {
-        jobs[t] = (MisfitJob){plans, tr, off, nshots, nth, t, FWI_HIP_OK, dt};
+        jobs[t] = (MisfitJob){plans, tr, off, nshots, nth, t, FWI_HIP_OK, dt, tm};
         made[t] = t > 0 && pthread_create(&th[t], NULL, misfit_worker, &jobs[t]) == 0;
     }
     for (int t = 0; t < nth; t++) /* the calling thread takes share 0 and any share without a thread */
@@ -636,6 +656,77 @@ static int ctx_setup(fwi_hip_ctx *ctx, const float *vel, const fwi_grad *gr, con
     return FWI_HIP_OK;
 }
 
+/* The L2 misfit of plain traces (no band-pass, no mute, data at the
+ * propagation rate) needs only the samples of one time step at a time, so it
+ * runs on each block of trace rows as fwi_hip_forward_tm hands it over,
+ * beside the propagation of the later blocks: msf->syn gets the rows, the
+ * rows become the residual syn - obs0 in place (0 on traces outside the
+ * offset window), and the per-shot sums of 1/2 res^2 (FP64) become val. */
+typedef struct {
+    fwi_plan **plans;
+    float *tr;
+    const long long *off;
+    int nshots, nth;
+    double *J;
+    unsigned char **drop; /* per shot: traces outside the offset window, or NULL */
+} L2Stream;
+
+typedef struct {
+    const L2Stream *st;
+    int k0, k1, t;
+} L2Job;
+
+static void *l2_rows(void *p)
+{
+    const L2Job *j = (const L2Job *)p;
+    const L2Stream *st = j->st;
+    for (int s = j->t; s < st->nshots; s += st->nth) {
+        fwi_msf *m = st->plans[s]->msf;
+        const int nr = (int)m->nr, k1 = j->k1 < (int)m->nt ? j->k1 : (int)m->nt;
+        const unsigned char *dr = st->drop ? st->drop[s] : NULL;
+        float *t = st->tr + st->off[s];
+        double acc = 0.0;
+        for (int k = j->k0; k < k1; k++) {
+            float *row = t + (size_t)k * nr;
+            for (int r = 0; r < nr; r++) {
+                const float v = row[r];
+                m->syn[r][k] = v;
+                const double d = (dr && dr[r]) ? 0.0 : (double)v - m->obs0[r][k];
+                row[r] = (float)d;
+                acc += 0.5 * d * d;
+            }
+        }
+        st->J[s] += acc;
+    }
+    return NULL;
+}
+
+static int l2_block(void *user, int k0, int k1)
+{
+    const L2Stream *st = (const L2Stream *)user;
+    L2Job jobs[MISFIT_MAX_THREADS];
+    pthread_t th[MISFIT_MAX_THREADS];
+    int made[MISFIT_MAX_THREADS];
+    for (int t = 0; t < st->nth; t++) {
+        jobs[t] = (L2Job){st, k0, k1, t};
+        made[t] = t > 0 && pthread_create(&th[t], NULL, l2_rows, &jobs[t]) == 0;
+    }
+    for (int t = 0; t < st->nth; t++)
+        if (!made[t]) l2_rows(&jobs[t]);
+    for (int t = 1; t < st->nth; t++)
+        if (made[t]) pthread_join(th[t], NULL);
+    return 0;
+}
+
+/* whether the batch's misfit step is the plain L2 of l2_block */
+static int l2_streams(fwi_plan **plans, int nshots)
+{
+    if (g_cfg.kind != GRAD4_MSF_L2 || g_cfg.bp || g_cfg.mute) return 0;
+    for (int s = 0; s < nshots; s++)
+        if (plans[s]->msf->jump != 1 || plans[s]->msf->nt != plans[s]->grid->base->nt) return 0;
+    return 1;
+}
+
 int grad_all_shots(fwi_plan **plans, int nshots, int k)
 {
     (void)k;
@@ -667,18 +758,54 @@ int grad_all_shots(fwi_plan **plans, int nshots, int k)
         if (!grad) rc = fail(FWI_HIP_ENOMEM, "host memory");
         else if (!(tr = slot_traces(0, (size_t)t.ntr))) rc = FWI_HIP_ENOMEM;
     }
+    /* traces time-major in blocks beside the propagation; the plain L2
+     * misfit runs on each block as it arrives, any other after the pass */
+    const int stream_l2 = !rc && l2_streams(plans, nshots);
+    L2Stream l2 = {plans, tr, NULL, nshots, misfit_threads(nshots), NULL, NULL};
+    long long *toff = NULL; /* packed trace offsets */
+    if (stream_l2) {
+        toff = (long long *)malloc(sizeof(long long) * (nshots + 1));
+        if (toff) {
+            toff[0] = 0;
+            for (int s = 0; s < nshots; s++)
+                toff[s + 1] = toff[s] + (long long)plans[s]->msf->nr * (long long)plans[s]->grid->base->nt;
+        }
+        l2.off = toff;
+        l2.J = (double *)calloc(nshots, sizeof(double));
+        if (!toff) rc = fail(FWI_HIP_ENOMEM, "host memory");
+        l2.drop = g_cfg.offsets ? (unsigned char **)calloc(nshots, sizeof(unsigned char *)) : NULL;
+        if (!l2.J || (g_cfg.offsets && !l2.drop)) rc = fail(FWI_HIP_ENOMEM, "host memory");
+        for (int s = 0; !rc && l2.drop && s < nshots; s++) {
+            const int nr = (int)plans[s]->msf->nr;
+            const float *sr = plans[s]->sou->xyz_sr;
+            if (!(l2.drop[s] = (unsigned char *)calloc(nr ? nr : 1, 1))) rc = fail(FWI_HIP_ENOMEM, "host memory");
+            for (int r = 0; !rc && r < nr; r++) {
+                const double dx = (double)sr[2 + 2 * r] - sr[0], dz = (double)sr[3 + 2 * r] - sr[1];
+                const double d = sqrt(dx * dx + dz * dz);
+                l2.drop[s][r] = d < g_cfg.omin || d > g_cfg.omax;
+            }
+        }
+    }
     tm[1] = now_ms();
-    if (!rc && (rc = fwi_hip_forward(ctx, tr))) rc = fail(rc, "forward: %s", fwi_hip_last_error(ctx));
+    if (!rc && (rc = fwi_hip_forward_tm(ctx, tr, stream_l2 ? l2_block : NULL, &l2)))
+        rc = fail(rc, "forward: %s", fwi_hip_last_error(ctx));
     tm[2] = now_ms();
-    if (!rc) rc = host_misfit_all(plans, nshots, tr, b->dt);
+    if (!rc && stream_l2)
+        for (int s = 0; s < nshots; s++) plans[s]->msf->val[0] = plans[s]->msf->val[1] = (float)l2.J[s];
+    else if (!rc)
+        rc = host_misfit_all(plans, nshots, tr, b->dt, 1);
     tm[3] = now_ms();
-    if (!rc && (rc = fwi_hip_set_residual(ctx, tr))) rc = fail(rc, "set_residual: %s", fwi_hip_last_error(ctx));
+    if (!rc && (rc = fwi_hip_set_residual_tm(ctx, tr))) rc = fail(rc, "set_residual: %s", fwi_hip_last_error(ctx));
     tm[4] = now_ms();
     if (!rc && (rc = fwi_hip_adjoint(ctx, grad, NULL))) rc = fail(rc, "adjoint: %s", fwi_hip_last_error(ctx));
     if (!rc)
         for (long i = 0; i < (long)nx * nz; i++) plans[0]->grad->body[i] = (float)grad[i];
     tm[5] = now_ms();
     for (int i = 0; i < GRAD4_NPHASE; i++) g_phase_ms[i] = tm[i + 1] - tm[i];
+    for (int s = 0; l2.drop && s < nshots; s++) free(l2.drop[s]);
+    free(l2.drop);
+    free(l2.J);
+    free(toff);
     shot_table_free(&t);
     free(grids);
     free(msfs);
@@ -751,7 +878,7 @@ int grad_all_shots_rfl(rwi_plan **plans, int nshots, int k, float *grad_m1)
     if (!rc && (rc = fwi_hip_forward(c1, tr1))) rc = fail(rc, "forward (perturbed): %s", fwi_hip_last_error(c1));
     if (!rc) {
         for (long long i = 0; i < t.ntr; i++) tr1[i] -= tr0[i]; /* the reflected data */
-        rc = host_misfit_all(shells, nshots, tr1, b->dt); /* tr1 <- adjoint source r */
+        rc = host_misfit_all(shells, nshots, tr1, b->dt, 0); /* tr1 <- adjoint source r */
     }
     if (!rc && (rc = fwi_hip_set_residual(c1, tr1))) rc = fail(rc, "set_residual: %s", fwi_hip_last_error(c1));
     if (!rc && (rc = fwi_hip_adjoint(c1, g1, NULL))) rc = fail(rc, "adjoint (perturbed): %s", fwi_hip_last_error(c1));

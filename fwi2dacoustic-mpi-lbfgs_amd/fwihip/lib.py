@@ -21,7 +21,7 @@ MAX_F, MAX_V, COEF_R = 15.0, 6000.0, 0.001
 
 EXPORTS = (
     "fwi_hip_create", "fwi_hip_destroy", "fwi_hip_last_error", "fwi_hip_bytes", "fwi_hip_set_model",
-    "fwi_hip_set_shots", "fwi_hip_forward", "fwi_hip_set_observed", "fwi_hip_misfit_l2",
+    "fwi_hip_set_shots", "fwi_hip_forward", "fwi_hip_forward_tm", "fwi_hip_set_residual_tm", "fwi_hip_set_observed", "fwi_hip_misfit_l2",
     "fwi_hip_set_residual", "fwi_hip_adjoint", "fwi_hip_synchronize", "fwi_hip_set_timing",
     "fwi_hip_kernel_time", "fwi_hip_read_state", "fwi_hip_check_model", "fwi_hip_op_pml", "fwi_hip_op_interior",
     "fwi_hip_time_integral", "fwi_hip_op_pml_noalloc", "fwi_hip_write_bdr", "fwi_hip_read_bdr", "fwi_hip_cpml_coe",
@@ -83,6 +83,9 @@ def make_grid(g, max_f=MAX_F, max_v=MAX_V, coef_r=COEF_R):
 
 def _p(a):
     return ctypes.c_void_p(a.ctypes.data) if a is not None else ctypes.c_void_p(0)
+
+
+BLOCK_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int)
 
 
 class Context:
@@ -154,6 +157,30 @@ class Context:
         self._chk(self.lib.fwi_hip_forward(self.h, _p(out)))
         return out
 
+    def forward_tm(self, callback=None, out=None):
+        """fwi_hip_forward_tm: the traces time-major (split_tm), and
+        callback(k0, k1) -> int for each block of rows as it arrives (the
+        packed array is readable and writable inside the callback)."""
+        out = np.empty(self.ntrace(), np.float32) if out is None else out
+        assert out.dtype == np.float32 and out.size == self.ntrace() and out.flags.c_contiguous
+        fn = BLOCK_FN(lambda user, k0, k1: int(callback(k0, k1))) if callback is not None else None
+        self._chk(self.lib.fwi_hip_forward_tm(self.h, _p(out), fn, None))
+        return out
+
+    def split_tm(self, packed):
+        out, o = [], 0
+        for s in self.shots:
+            out.append(packed[o:o + s.nr * s.nt].reshape(s.nt, s.nr))
+            o += s.nr * s.nt
+        return out
+
+    def set_residual_tm(self, res_tm_packed):
+        """fwi_hip_set_residual_tm: the array is kept until the next adjoint."""
+        res = np.ascontiguousarray(res_tm_packed, np.float32)
+        assert res.size == self.ntrace()
+        self._res_tm = res
+        self._chk(self.lib.fwi_hip_set_residual_tm(self.h, _p(res)))
+
     def set_observed(self, obs_packed):
         obs = np.ascontiguousarray(obs_packed, np.float32)
         assert obs.size == self.ntrace()
@@ -173,7 +200,10 @@ class Context:
         g = self.geom
         grad = np.empty((g.nz, g.nx), np.float64) if want else None
         gk = np.empty((g.nz, g.nx), np.float64) if want else None
-        self._chk(self.lib.fwi_hip_adjoint(self.h, _p(grad), _p(gk)))
+        try:
+            self._chk(self.lib.fwi_hip_adjoint(self.h, _p(grad), _p(gk)))
+        finally:
+            self._res_tm = None
         return grad, gk
 
     def synchronize(self):

@@ -97,6 +97,21 @@ int fwi_hip_set_shots(fwi_hip_ctx *ctx, int nshots, const fwi_hip_shot *shots);
  * traces stay on the device; copied to `traces` (host) when non-NULL.       */
 int fwi_hip_forward(fwi_hip_ctx *ctx, float *traces);
 
+/* fwi_hip_forward with the traces handed over time-major, each shot
+ * [nt][nr] at the same packed offset (the layout the steps record), in
+ * blocks: in the graph form of a boundary-saving pass each block of
+ * graph-steps steps goes to traces_tm (page-locked memory, fwi_hip_host_alloc,
+ * lets the copy run beside the next block) while later blocks propagate, and
+ * fn(user, k0, k1), when given, runs on the calling thread as soon as rows
+ * k0 <= k < k1 of every shot are on the host, blocks in increasing order;
+ * other forms hand over all rows once, fn(user, 0, kmax).  A nonzero return
+ * of fn stops further calls; the pass completes and that value is returned.
+ * No counterpart in the reference, whose grad4_1shot holds its traces on the
+ * host from the start: this is the pipelined form of fwi_hip_forward for the
+ * host misfit step (grad_all_shots).  traces_tm may be NULL only without fn. */
+typedef int (*fwi_hip_block_fn)(void *user, int k0, int k1);
+int fwi_hip_forward_tm(fwi_hip_ctx *ctx, float *traces_tm, fwi_hip_block_fn fn, void *user);
+
 /* Observed traces (host, packed like the synthetics), kept on the device. */
 int fwi_hip_set_observed(fwi_hip_ctx *ctx, const float *obs);
 /* L2 misfit on the device: residual = syn - obs, *misfit = 1/2 sum res^2
@@ -104,6 +119,12 @@ int fwi_hip_set_observed(fwi_hip_ctx *ctx, const float *obs);
 int fwi_hip_misfit_l2(fwi_hip_ctx *ctx, double *misfit);
 /* Adjoint source computed elsewhere (host misfit pool), packed like obs. */
 int fwi_hip_set_residual(fwi_hip_ctx *ctx, const float *res);
+/* The adjoint source time-major, packed like fwi_hip_forward_tm's traces.
+ * Not copied here: the next fwi_hip_adjoint uploads it, block by block
+ * beside its steps in the graph form of a one-chunk boundary-saving pass
+ * (else whole, first), so res_tm must stay valid and unchanged until that
+ * call returns (page-locked memory lets the copies overlap the steps).    */
+int fwi_hip_set_residual_tm(fwi_hip_ctx *ctx, const float *res_tm);
 
 /* Back-propagation of the residual (exact transpose of the forward),
  * source-field reconstruction from the saved frames, zero-lag imaging

@@ -21,7 +21,7 @@ def test_cpu_suite_under_address_and_undefined_sanitizers():
     sys.path.insert(0, PKG)
     import build as B
     if not all(os.path.exists(p) for p in [LIB_SAN, *ORC_SAN]):
-        pytest.skip("sanitized libraries not built (fwi2dacoustic-mpi-lbfgs_amd/build.py --sanitize; make -C oracle san)")
+        pytest.skip("sanitized libraries not built (fwi2dacoustic-mpi-lbfgs_amd/build.py; make -C oracle san)")
     rt = B.sanitizer_runtime()
     if rt is None:
         pytest.skip("no shared sanitizer runtime")
