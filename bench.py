@@ -619,14 +619,15 @@ def main():
             kern[name]["per"] = "time step (all launches of the step, graph replay)"
     ctx.set_timing(0)
     dom = max(names, key=lambda k: kern[k]["avg_us"] * kern[k]["launches"])
-    traffic = None
+    traffic, traffic_kind = None, None
     if args.config == 2 and args.shots == 64 and os.path.exists(PROFILE_TRAFFIC):
         try:
-            traffic = json.load(open(PROFILE_TRAFFIC)).get(dom)
+            prof = json.load(open(PROFILE_TRAFFIC))
+            traffic, traffic_kind = prof.get(dom), prof.get("unit")
         except Exception:
             traffic = None
     roof = dict(bound="hbm", kernel=dom, achieved=kern[dom]["achieved_GBps"], peak=HBM_PEAK / 1e9, unit="GB/s",
-                frac=kern[dom]["achieved_GBps"] / (HBM_PEAK / 1e9), traffic=traffic)
+                frac=kern[dom]["achieved_GBps"] / (HBM_PEAK / 1e9), traffic=traffic, traffic_kind=traffic_kind)
 
     single = None
     if not args.no_single:
