@@ -8,6 +8,8 @@
  *   fwi_plan_read_inputs           the file reads and index arithmetic of
  *                                  fwi_init (fwi_2d.c:320-371) with the
  *                                  file names of fwi_files_path.h:7-18
+ *   rwi_plan_read_inputs           the same for rwi_init (fwi_2d.c:637-688),
+ *                                  with the two models m0, m1
  *   fwi_trcinp                     trcinp (spline_interp.c:19-38): natural
  *                                  cubic spline from nt0 samples at dt0 to
  *                                  nt samples at dt, end samples copied
@@ -71,18 +73,16 @@ int fwi_write_f32(const char *path, const float *src, size_t n)
     return FWI_HIP_OK;
 }
 
-int fwi_plan_read_inputs(fwi_plan *p, const char *common, int ishot, const char *wavelet, int shots_avail,
-                         int fbt_avail, float wav_scale)
+/* The reads and index arithmetic fwi_init (fwi_2d.c:320-371) and rwi_init
+ * (fwi_2d.c:637-688) share: data, geometry, first breaks, sea bottom and
+ * wavelet, then the bottom/first-break indices and the source and receiver
+ * cells. */
+static int read_shot_inputs(const fwi_base *b, fwi_msf *m, fwi_sou *so, float *btry, int *ibtry, const char *common,
+                            int ishot, const char *wavelet, int shots_avail, int fbt_avail, float wav_scale)
 {
-    if (!p || !common || !p->grid || !p->model || !p->msf || !p->sou)
-        return io_fail(FWI_HIP_EINVAL, "read_inputs: NULL argument");
-    const fwi_base *b = p->grid->base;
-    fwi_msf *m = p->msf;
-    fwi_sou *so = p->sou;
-    fwi_model *md = p->model;
-    const size_t nx = b->nx, nz = b->nz, nr = m->nr, nt0 = m->nt, dim = b->dim;
+    const size_t nx = b->nx, nr = m->nr, nt0 = m->nt, dim = b->dim;
     if (dim != 2) return io_fail(FWI_HIP_EINVAL, "read_inputs: dim=%zu (2-D only)", dim);
-    if (!md->btry || !md->ibtry || !m->fbt || !m->ifbt)
+    if (!btry || !ibtry || !m->fbt || !m->ifbt)
         return io_fail(FWI_HIP_EINVAL, "read_inputs: plan lacks btry/ibtry/fbt/ifbt (use fwi_plan_alloc)");
     char path[600];
     int rc;
@@ -98,10 +98,8 @@ int fwi_plan_read_inputs(fwi_plan *p, const char *common, int ishot, const char 
         snprintf(path, sizeof(path), "%sinput/fbt_%d.bin", common, ishot);
         if ((rc = fwi_read_f32(path, m->fbt, nr, 1))) return rc;
     }
-    snprintf(path, sizeof(path), "%sinput/initial_model.bin", common);
-    if ((rc = fwi_read_f32(path, &md->md[0][0], nz * nx, 1))) return rc;
     snprintf(path, sizeof(path), "%sinput/bottom.bin", common);
-    if ((rc = fwi_read_f32(path, md->btry, nx, 1))) return rc;
+    if ((rc = fwi_read_f32(path, btry, nx, 1))) return rc;
     if (wavelet)
         snprintf(path, sizeof(path), "%s", wavelet);
     else
@@ -110,7 +108,7 @@ int fwi_plan_read_inputs(fwi_plan *p, const char *common, int ishot, const char 
     for (size_t i = 0; i < b->nt; i++) so->wav0[i] *= wav_scale; /* fwi_2d.c:353 */
 
     /* fwi_2d.c:357-371: float quotients, C conversions and round() */
-    for (size_t i = 0; i < nx; i++) md->ibtry[i] = md->btry[i] / b->dz - 0.5;
+    for (size_t i = 0; i < nx; i++) ibtry[i] = btry[i] / b->dz - 0.5;
     for (size_t i = 0; i < nr; i++) m->ifbt[i] = m->fbt[i] / m->dt;
     so->ixyz_sr[0] = (int)round(so->xyz_sr[0] / b->dx);
     so->ixyz_sr[1] = (int)round(so->xyz_sr[1] / b->dz - 0.5);
@@ -119,6 +117,41 @@ int fwi_plan_read_inputs(fwi_plan *p, const char *common, int ishot, const char 
         so->ixyz_sr[dim + 2 * i + 1] = (int)round(so->xyz_sr[dim + 2 * i + 1] / b->dz - 0.5);
     }
     return FWI_HIP_OK;
+}
+
+int fwi_plan_read_inputs(fwi_plan *p, const char *common, int ishot, const char *wavelet, int shots_avail,
+                         int fbt_avail, float wav_scale)
+{
+    if (!p || !common || !p->grid || !p->model || !p->msf || !p->sou)
+        return io_fail(FWI_HIP_EINVAL, "read_inputs: NULL argument");
+    const fwi_base *b = p->grid->base;
+    fwi_model *md = p->model;
+    if (!md->md) return io_fail(FWI_HIP_EINVAL, "read_inputs: plan lacks the model");
+    char path[600];
+    snprintf(path, sizeof(path), "%sinput/initial_model.bin", common);
+    int rc = fwi_read_f32(path, &md->md[0][0], b->nz * b->nx, 1);
+    if (rc) return rc;
+    return read_shot_inputs(b, p->msf, p->sou, md->btry, md->ibtry, common, ishot, wavelet, shots_avail, fbt_avail,
+                            wav_scale);
+}
+
+int rwi_plan_read_inputs(rwi_plan *p, const char *common, int ishot, const char *init_mdl_0, const char *init_mdl_1,
+                         const char *wavelet, int shots_avail, int fbt_avail, float wav_scale)
+{
+    if (!p || !common || !p->grid || !p->model || !p->msf || !p->sou)
+        return io_fail(FWI_HIP_EINVAL, "rwi_read_inputs: NULL argument");
+    if (!init_mdl_0 || !init_mdl_1)
+        return io_fail(FWI_HIP_EINVAL, "rwi_read_inputs: the paths of the background (m0) and perturbation (m1) "
+                                       "models are required");
+    const fwi_base *b = p->grid->base;
+    rwi_model *md = p->model;
+    if (!md->m0 || !md->m1) return io_fail(FWI_HIP_EINVAL, "rwi_read_inputs: plan lacks m0/m1 (use rwi_plan_alloc)");
+    /* fwi_2d.c:667-668: m0, then m1, each [nz][nx] */
+    int rc = fwi_read_f32(init_mdl_0, &md->m0[0][0], b->nz * b->nx, 1);
+    if (!rc) rc = fwi_read_f32(init_mdl_1, &md->m1[0][0], b->nz * b->nx, 1);
+    if (rc) return rc;
+    return read_shot_inputs(b, p->msf, p->sou, md->btry, md->ibtry, common, ishot, wavelet, shots_avail, fbt_avail,
+                            wav_scale);
 }
 
 /* natural cubic spline through (x[i], y[i]), i < n: curvature c[i] (c = y''/2) */

@@ -109,6 +109,9 @@ def lib():
     h.fwi_plan_read_inputs.argtypes = [ctypes.POINTER(FwiPlan), ctypes.c_char_p, c_int, ctypes.c_char_p, c_int,
                                        c_int, c_float]
     h.fwi_plan_read_inputs.restype = c_int
+    h.rwi_plan_read_inputs.argtypes = [ctypes.POINTER(RwiPlan), ctypes.c_char_p, c_int, ctypes.c_char_p,
+                                       ctypes.c_char_p, ctypes.c_char_p, c_int, c_int, c_float]
+    h.rwi_plan_read_inputs.restype = c_int
     h.fwi_trcinp.argtypes = [PPF, c_int, c_float, PPF, c_int, c_float, c_int]
     h.fwi_trcinp.restype = c_int
     h.fwi_io_last_error.restype = ctypes.c_char_p

@@ -34,6 +34,16 @@ int fwi_write_f32(const char *path, const float *src, size_t n);
 int fwi_plan_read_inputs(fwi_plan *plan, const char *common, int ishot, const char *wavelet, int shots_avail,
                          int fbt_avail, float wav_scale);
 
+/* The input side of rwi_init (fwi_2d.c:637-688) for a plan of rwi_plan_alloc:
+ * the files of fwi_plan_read_inputs except the initial model, and the
+ * background and perturbation models m0[nz][nx], m1[nz][nx] from the paths
+ * init_mdl_0 and init_mdl_1 (required: the reference reads them through
+ * INIT_MDL_0 / INIT_MDL_1, which its fwi_files_path.h does not define); the
+ * same index arithmetic.  0 or FWI_HIP_EINVAL (message in
+ * fwi_io_last_error) where the reference exit()s.                            */
+int rwi_plan_read_inputs(rwi_plan *plan, const char *common, int ishot, const char *init_mdl_0,
+                         const char *init_mdl_1, const char *wavelet, int shots_avail, int fbt_avail, float wav_scale);
+
 /* trcinp, spline_interp.c:19-38: each of the howmany rows obs0[r][0..nt0) on
  * the abscissae i*dt0 is resampled to obs[r][0..nt) at i*dt by the natural
  * cubic spline; the end samples are copied (spline_interp.c:13).  Rows are

@@ -77,6 +77,49 @@ def test_plan_read_inputs_matches_fwi_init(h, tmp_path):
     h.fwi_plan_free(p)
 
 
+def test_rwi_plan_read_inputs_matches_rwi_init(h, tmp_path):
+    """rwi_init (fwi_2d.c:637-688) reads what fwi_init reads except the
+    initial model, plus the two models m0, m1: the RWI reader leaves every
+    shared field as the FWI reader does, and m0, m1 as written."""
+    nx, nz, npml, nt, jump, nr, dx, dz, dt = 30, 20, 5, 41, 2, 7, 12.5, 10.0, 1e-3
+    nt0 = (nt - 1) // jump + 1
+    rng = np.random.default_rng(12)
+    obs0, xyz, md, btry, wav, fbt = make_inputs(str(tmp_path), nx, nz, nt, nt0, nr, 4, rng, dx, dz)
+    m0 = rng.uniform(1500, 4500, (nz, nx)).astype(np.float32)
+    m1 = (0.05 * rng.standard_normal((nz, nx))).astype(np.float32)
+    f0, f1 = str(tmp_path / "m0.bin"), str(tmp_path / "m1.bin")
+    write(f0, m0)
+    write(f1, m1)
+    common = (str(tmp_path) + "/").encode()
+    pf = h.fwi_plan_alloc(nx, nz, npml, nt, jump, nr, dx, dz, dt)
+    pr = h.rwi_plan_alloc(nx, nz, npml, nt, jump, nr, dx, dz, dt)
+    try:
+        assert h.fwi_plan_read_inputs(pf, common, 4, None, 1, 1, 1.5) == 0, h.fwi_io_last_error()
+        assert h.rwi_plan_read_inputs(pr, common, 4, f0.encode(), f1.encode(), None, 1, 1, 1.5) == 0, \
+            h.fwi_io_last_error()
+        F, R = pf.contents, pr.contents
+        np.testing.assert_array_equal(PC.view(R.model.contents.m0[0], (nz, nx)), m0)
+        np.testing.assert_array_equal(PC.view(R.model.contents.m1[0], (nz, nx)), m1)
+        np.testing.assert_array_equal(PC.view(R.msf.contents.obs0[0], (nr, nt0)), obs0)
+        for name, n in (("xyz_sr", 2 * (nr + 1)), ("ixyz_sr", 2 * (nr + 1)), ("wav0", nt)):
+            np.testing.assert_array_equal(np.ctypeslib.as_array(getattr(R.sou.contents, name), (n,)),
+                                          np.ctypeslib.as_array(getattr(F.sou.contents, name), (n,)))
+        for name, n in (("btry", nx), ("ibtry", nx)):
+            np.testing.assert_array_equal(np.ctypeslib.as_array(getattr(R.model.contents, name), (n,)),
+                                          np.ctypeslib.as_array(getattr(F.model.contents, name), (n,)))
+        for name in ("fbt", "ifbt"):
+            np.testing.assert_array_equal(np.ctypeslib.as_array(getattr(R.msf.contents, name), (nr,)),
+                                          np.ctypeslib.as_array(getattr(F.msf.contents, name), (nr,)))
+        assert h.rwi_plan_read_inputs(pr, common, 4, None, f1.encode(), None, 1, 1, 1.5) == L.EINVAL
+        assert "m0" in h.fwi_io_last_error().decode()
+        os.remove(f1)
+        assert h.rwi_plan_read_inputs(pr, common, 4, f0.encode(), f1.encode(), None, 1, 1, 1.5) == L.EINVAL
+        assert "m1.bin" in h.fwi_io_last_error().decode()
+    finally:
+        h.fwi_plan_free(pf)
+        h.rwi_plan_free(pr)
+
+
 @pytest.mark.parametrize("case", ["missing", "long", "short"])
 def test_read_errors_return_codes(h, tmp_path, case):
     nx, nz, npml, nt, jump, nr = 12, 10, 4, 21, 2, 3
