@@ -53,7 +53,8 @@ extern "C" {
 hipError_t fwi_launch_ref_phase(const FwiRefArgs *a, int phase, hipStream_t st);
 hipError_t fwi_launch_ref_time_integral(float *pnew, const float *pnow, const float *pold, const float *pxx,
                                         const float *pzz, const float *ekp, long long n, hipStream_t st);
-hipError_t fwi_launch_ref_write_bdr(float *ud, float *lr, const float *pf, int nx, int nz, int npml, hipStream_t st);
+hipError_t fwi_launch_ref_write_bdr(float *ud, float *lr, const float *pf, int nx, int nz, int npml, int top,
+                                    hipStream_t st);
 hipError_t fwi_launch_ref_read_bdr(const float *ud, const float *lr, float *pb, int nx, int nz, hipStream_t st);
 }
 
@@ -2049,7 +2050,37 @@ extern "C" int fwi_hip_time_integral(float *pnew, const float *pnow, const float
 extern "C" int fwi_hip_write_bdr(const fwi_hip_grid *g, float *ud, float *lr, const float *pf, void *stream)
 {
     if (!g || !ud || !lr || !pf) return set_err(nullptr, FWI_HIP_EINVAL, "write_bdr: NULL");
-    HIPCHK0(fwi_launch_ref_write_bdr(ud, lr, pf, g->nx, g->nz, g->npml, (hipStream_t)stream));
+    HIPCHK0(fwi_launch_ref_write_bdr(ud, lr, pf, g->nx, g->nz, g->npml, 0, (hipStream_t)stream));
+    return FWI_HIP_OK;
+}
+
+// write_bdr_fs_no (fwi_2d.c:962-997): the CPML also above the physical grid
+extern "C" int fwi_hip_write_bdr_fs_no(const fwi_hip_grid *g, float *ud, float *lr, const float *pf, void *stream)
+{
+    if (!g || !ud || !lr || !pf) return set_err(nullptr, FWI_HIP_EINVAL, "write_bdr_fs_no: NULL");
+    HIPCHK0(fwi_launch_ref_write_bdr(ud, lr, pf, g->nx, g->nz, g->npml, g->npml, (hipStream_t)stream));
+    return FWI_HIP_OK;
+}
+
+// model_ext_fs_no (fwi_2d.c:29-65, host): model[n2][n1] into the middle of
+// emodel[en2][en1], the margins (en - n)/2 wide on every side filled with
+// the nearest edge or corner value
+extern "C" int fwi_hip_model_ext_fs_no(const float *model, int n1, int n2, float *emodel, int en1, int en2)
+{
+    if (!model || !emodel || n1 < 1 || n2 < 1 || en1 < n1 || en2 < n2)
+        return set_err(nullptr, FWI_HIP_EINVAL, "model_ext_fs_no: NULL argument or extended grid smaller");
+    if ((en1 - n1) % 2 || (en2 - n2) % 2)
+        return set_err(nullptr, FWI_HIP_EINVAL,
+                       "model_ext_fs_no: margins must be equal on both sides (en1 - n1 = %d, en2 - n2 = %d)", en1 - n1,
+                       en2 - n2);
+    const int g1 = (en1 - n1) / 2, g2 = (en2 - n2) / 2;
+    for (int i2 = 0; i2 < en2; i2++) {
+        const int s2 = std::min(std::max(i2 - g2, 0), n2 - 1);
+        for (int i1 = 0; i1 < en1; i1++) {
+            const int s1 = std::min(std::max(i1 - g1, 0), n1 - 1);
+            emodel[(size_t)i2 * en1 + i1] = model[(size_t)s2 * n1 + s1];
+        }
+    }
     return FWI_HIP_OK;
 }
 

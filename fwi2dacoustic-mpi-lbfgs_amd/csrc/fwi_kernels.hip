@@ -2459,19 +2459,20 @@ __global__ void k_ref_time_integral(float *pnew, const float *pnow, const float 
 }
 
 // write_bdr / read_bdr for one step (fwi_2d.c:891-910, 940-957)
-__global__ void k_ref_write_bdr(float *ud, float *lr, const float *pf, int nx, int nz, int npml)
+// top: rows of CPML above the physical grid (0 with the free surface)
+__global__ void k_ref_write_bdr(float *ud, float *lr, const float *pf, int nx, int nz, int npml, int top)
 {
     const int pnx_f = nx + 2 * npml;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < 3 * nx) {
         const int i = t / nx, j = t - i * nx;
-        ud[i * nx + j] = pf[i * pnx_f + j + npml];
-        ud[(i + 3) * nx + j] = pf[(nz - i - 1) * pnx_f + j + npml];
+        ud[i * nx + j] = pf[(i + top) * pnx_f + j + npml];
+        ud[(i + 3) * nx + j] = pf[(nz - i - 1 + top) * pnx_f + j + npml];
     }
     if (t < 3 * nz) {
         const int i = t / 3, j = t - i * 3;
-        lr[i * 6 + j] = pf[i * pnx_f + j + npml];
-        lr[i * 6 + j + 3] = pf[i * pnx_f + nx + npml - j - 1];
+        lr[i * 6 + j] = pf[(i + top) * pnx_f + j + npml];
+        lr[i * 6 + j + 3] = pf[(i + top) * pnx_f + nx + npml - j - 1];
     }
 }
 
@@ -2794,10 +2795,11 @@ hipError_t fwi_launch_ref_time_integral(float *pnew, const float *pnow, const fl
     return hipGetLastError();
 }
 
-hipError_t fwi_launch_ref_write_bdr(float *ud, float *lr, const float *pf, int nx, int nz, int npml, hipStream_t st)
+hipError_t fwi_launch_ref_write_bdr(float *ud, float *lr, const float *pf, int nx, int nz, int npml, int top,
+                                    hipStream_t st)
 {
     const int n = 3 * (nx > nz ? nx : nz);
-    hipLaunchKernelGGL(k_ref_write_bdr, dim3((n + 255) / 256), dim3(256), 0, st, ud, lr, pf, nx, nz, npml);
+    hipLaunchKernelGGL(k_ref_write_bdr, dim3((n + 255) / 256), dim3(256), 0, st, ud, lr, pf, nx, nz, npml, top);
     return hipGetLastError();
 }
 

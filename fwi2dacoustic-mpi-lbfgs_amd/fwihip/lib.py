@@ -24,7 +24,7 @@ EXPORTS = (
     "fwi_hip_set_shots", "fwi_hip_forward", "fwi_hip_forward_tm", "fwi_hip_set_residual_tm", "fwi_hip_set_observed", "fwi_hip_misfit_l2",
     "fwi_hip_set_residual", "fwi_hip_adjoint", "fwi_hip_synchronize", "fwi_hip_set_timing",
     "fwi_hip_kernel_time", "fwi_hip_read_state", "fwi_hip_check_model", "fwi_hip_op_pml", "fwi_hip_op_interior",
-    "fwi_hip_time_integral", "fwi_hip_op_pml_noalloc", "fwi_hip_write_bdr", "fwi_hip_read_bdr", "fwi_hip_cpml_coe",
+    "fwi_hip_time_integral", "fwi_hip_op_pml_noalloc", "fwi_hip_write_bdr", "fwi_hip_write_bdr_fs_no", "fwi_hip_model_ext_fs_no", "fwi_hip_read_bdr", "fwi_hip_cpml_coe",
     "fwi_hip_set_graph", "fwi_hip_set_kernels", "fwi_hip_set_rec_group", "fwi_hip_set_imaging", "fwi_hip_set_bwd_chunk", "fwi_hip_set_checkpointing", "fwi_hip_set_revolve", "fwi_hip_backward_plan", "fwi_hip_plan_memory", "fwi_hip_revolve_schedule", "fwi_hip_set_bwd_overlap", "fwi_hip_set_bwd_fused", "fwi_hip_set_concurrent", "fwi_hip_set_interior_wgs", "fwi_hip_device_count", "fwi_hip_dmalloc", "fwi_hip_dfree", "fwi_hip_host_alloc", "fwi_hip_host_free", "fwi_hip_memcpy",
 )
 
@@ -428,6 +428,26 @@ def write_bdr(geom, ud, lr, pf):
     rc = lib.fwi_hip_write_bdr(ctypes.byref(gr), _dp(ud), _dp(lr), _dp(pf), ctypes.c_void_p(0))
     if rc:
         raise FwiError(rc, lib.fwi_hip_last_error(None).decode())
+
+
+def write_bdr_fs_no(geom, ud, lr, pf):
+    lib = load()
+    gr = make_grid(geom)
+    rc = lib.fwi_hip_write_bdr_fs_no(ctypes.byref(gr), _dp(ud), _dp(lr), _dp(pf), ctypes.c_void_p(0))
+    if rc:
+        raise FwiError(rc, lib.fwi_hip_last_error(None).decode())
+
+
+def model_ext_fs_no(model, en1, en2):
+    """model[n2][n1] extended to [en2][en1] (host)."""
+    lib = load()
+    m = np.ascontiguousarray(model, np.float32)
+    out = np.empty((en2, en1), np.float32)
+    rc = lib.fwi_hip_model_ext_fs_no(m.ctypes.data_as(ctypes.c_void_p), m.shape[1], m.shape[0],
+                                     out.ctypes.data_as(ctypes.c_void_p), en1, en2)
+    if rc:
+        raise FwiError(rc, lib.fwi_hip_last_error(None).decode())
+    return out
 
 
 def read_bdr(geom, ud, lr, pb):

@@ -309,6 +309,17 @@ int fwi_hip_time_integral(float *pnew, const float *pnow, const float *pold, con
 int fwi_hip_write_bdr(const fwi_hip_grid *g, float *ud, float *lr, const float *pf, void *stream);
 /* one step of read_bdr (fwi_2d.c:940-957) into a physical-grid field pb[nz][nx] */
 int fwi_hip_read_bdr(const fwi_hip_grid *g, const float *ud, const float *lr, float *pb, void *stream);
+/* write_bdr_fs_no (fwi_2d.c:962-997): one step of write_bdr from a field
+ * without the free surface, pf[nz + 2 npml][nx + 2 npml] (CPML above the
+ * physical grid too), same strip layout.  The reference declares the
+ * matching propagator (a_lap_opt_2d_pml_fs_no, header.h:318-328) without a
+ * body; the propagators here keep the free surface. */
+int fwi_hip_write_bdr_fs_no(const fwi_hip_grid *g, float *ud, float *lr, const float *pf, void *stream);
+/* model_ext_fs_no (fwi_2d.c:29-65, host): model[n2][n1] into the middle of
+ * emodel[en2][en1] with margins (en1 - n1)/2 and (en2 - n2)/2 on every side
+ * filled with the nearest edge or corner value.  Uneven margins are refused
+ * (the reference reads past its model then). */
+int fwi_hip_model_ext_fs_no(const float *model, int n1, int n2, float *emodel, int en1, int en2);
 /* cpml_coe, fwi_2d.c:110-137 (host): a[2 npml + 1], b[2 npml + 1] */
 int fwi_hip_cpml_coe(float *a, float *b, int npml, float dx, float dt, float max_f, float max_v, float coef_r);
 
