@@ -641,6 +641,16 @@ extern "C" int fwi_hip_set_model(fwi_hip_ctx *c, const float *vel)
     const FwiGeomDev &g = c->g;
     const float dt = c->d.grid.dt;
     const int nx = g.nx, nz = g.nz, npml = g.npml;
+    // a model in device memory is checked and extended from a host copy
+    std::vector<float> staged;
+    hipPointerAttribute_t pa;
+    if (hipPointerGetAttributes(&pa, vel) == hipSuccess && pa.type == hipMemoryTypeDevice) {
+        staged.resize((size_t)nx * nz);
+        HIPCHK(c, hipMemcpy(staged.data(), vel, sizeof(float) * staged.size(), hipMemcpyDefault));
+        vel = staged.data();
+    } else {
+        (void)hipGetLastError();  // a plain host pointer is not an error
+    }
     int rc = check_model(c, c->d.grid, vel, nullptr);
     if (rc) return rc;
     std::vector<float> kp((size_t)nx * nz), ekp((size_t)c->fs, 0.f);
@@ -1099,7 +1109,7 @@ static int ensure_copy(fwi_hip_ctx *c, int n)
 // the shots are packed alike)
 static hipError_t copy_rows(fwi_hip_ctx *c, float *host, float *dev, int k0, int k1, bool to_host, hipStream_t s)
 {
-    const hipMemcpyKind kind = to_host ? hipMemcpyDeviceToHost : hipMemcpyHostToDevice;
+    const hipMemcpyKind kind = hipMemcpyDefault;  // the caller's buffer may be host or device memory
     const FwiShotDev &d0 = c->shots_h[0];
     bool alike = d0.nr > 0;
     for (int i = 0; i < c->nshots && alike; i++) {
@@ -1495,9 +1505,9 @@ static int forward_impl(fwi_hip_ctx *c, float *traces, bool tm, fwi_hip_block_fn
         HIPCHK(c, fwi_launch_transpose(c->syn_tm + d.trace_off, c->syn + d.trace_off, d.nt, d.nr, c->st));
     }
     if (traces && !tm)
-        HIPCHK(c, hipMemcpyAsync(traces, c->syn, sizeof(float) * c->ntrace, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipMemcpyAsync(traces, c->syn, sizeof(float) * c->ntrace, hipMemcpyDefault, c->st));
     if (traces && tm && !pipe)
-        HIPCHK(c, hipMemcpyAsync(traces, c->syn_tm, sizeof(float) * c->ntrace, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipMemcpyAsync(traces, c->syn_tm, sizeof(float) * c->ntrace, hipMemcpyDefault, c->st));
     int frc = 0;  // the first nonzero return of fn (later blocks are not handed over)
     if (pipe) {
         const int B = c->graph_steps, nblk = (c->kmax + B - 1) / B;
@@ -1530,7 +1540,7 @@ extern "C" int fwi_hip_set_observed(fwi_hip_ctx *c, const float *obs)
 {
     if (!c || !obs) return set_err(c, FWI_HIP_EINVAL, "set_observed: NULL argument");
     if (c->nshots == 0) return set_err(c, FWI_HIP_ESTATE, "set_observed: set_shots first");
-    HIPCHK(c, hipMemcpyAsync(c->obs, obs, sizeof(float) * c->ntrace, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->obs, obs, sizeof(float) * c->ntrace, hipMemcpyDefault, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     c->have_obs = true;
     return FWI_HIP_OK;
@@ -1557,7 +1567,7 @@ extern "C" int fwi_hip_set_residual(fwi_hip_ctx *c, const float *res)
 {
     if (!c || !res) return set_err(c, FWI_HIP_EINVAL, "set_residual: NULL argument");
     if (c->nshots == 0) return set_err(c, FWI_HIP_ESTATE, "set_residual: set_shots first");
-    HIPCHK(c, hipMemcpyAsync(c->res, res, sizeof(float) * c->ntrace, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->res, res, sizeof(float) * c->ntrace, hipMemcpyDefault, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     c->have_res = true;
     c->res_host = nullptr;
@@ -1595,7 +1605,7 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
     const bool one_chunk = !(tile && c->bwd_chunk > 0 && c->bwd_chunk < c->nshots);
     const bool pipe_res = tm_res && c->bwd_method == 0 && c->graph_steps > 0 && c->timing != 1 && one_chunk;
     if (tm_res && !pipe_res)
-        HIPCHK(c, hipMemcpyAsync(c->res_tm, c->res_host, sizeof(float) * c->ntrace, hipMemcpyHostToDevice, c->st));
+        HIPCHK(c, hipMemcpyAsync(c->res_tm, c->res_host, sizeof(float) * c->ntrace, hipMemcpyDefault, c->st));
     for (int s = 0; s < c->nshots && !tm_res; s++) {
         const FwiShotDev &d = c->shots_h[s];
         if (d.nr == 0) continue;
@@ -1771,8 +1781,8 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
         c->recomputed = rec;
         if ((rc = pass_end(c, 1, pe0, c->kmax))) return rc;
         HIPCHK(c, fwi_launch_grad_finalize(c->gacc, ngr, nb, c->vel, c->gk, c->grad, c->st));
-        if (grad) HIPCHK(c, hipMemcpyAsync(grad, c->grad, sizeof(double) * nb, hipMemcpyDeviceToHost, c->st));
-        if (gk) HIPCHK(c, hipMemcpyAsync(gk, c->gk, sizeof(double) * nb, hipMemcpyDeviceToHost, c->st));
+        if (grad) HIPCHK(c, hipMemcpyAsync(grad, c->grad, sizeof(double) * nb, hipMemcpyDefault, c->st));
+        if (gk) HIPCHK(c, hipMemcpyAsync(gk, c->gk, sizeof(double) * nb, hipMemcpyDefault, c->st));
         HIPCHK(c, hipStreamSynchronize(c->st));
         c->have_fwd = false;
         c->res_host = nullptr;
@@ -1901,8 +1911,8 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
     if ((rc = pass_end(c, 1, pe0, c->kmax))) return rc;
     if ((rc = collect_timing(c, marks))) return rc;
     HIPCHK(c, fwi_launch_grad_finalize(c->gacc, ngr, nb, c->vel, c->gk, c->grad, c->st));
-    if (grad) HIPCHK(c, hipMemcpyAsync(grad, c->grad, sizeof(double) * nb, hipMemcpyDeviceToHost, c->st));
-    if (gk) HIPCHK(c, hipMemcpyAsync(gk, c->gk, sizeof(double) * nb, hipMemcpyDeviceToHost, c->st));
+    if (grad) HIPCHK(c, hipMemcpyAsync(grad, c->grad, sizeof(double) * nb, hipMemcpyDefault, c->st));
+    if (gk) HIPCHK(c, hipMemcpyAsync(gk, c->gk, sizeof(double) * nb, hipMemcpyDefault, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     // the reconstruction ran the forward's final state back to the start:
     // another adjoint needs another forward

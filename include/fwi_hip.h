@@ -67,13 +67,19 @@ typedef struct {
     const float *wav;
 } fwi_hip_shot;
 
+/* Buffers of the caller (model, traces, observed data, residual, gradient)
+ * may be host memory (page-locked is fastest) or device memory of the
+ * context's GPU: the copies take either.  Every call below returns after its
+ * copies into the caller's buffers are complete; device buffers the caller
+ * fills on another stream must be complete before the call.                */
 int fwi_hip_create(const fwi_hip_desc *desc, fwi_hip_ctx **out);
 void fwi_hip_destroy(fwi_hip_ctx *ctx);
 const char *fwi_hip_last_error(const fwi_hip_ctx *ctx); /* ctx may be NULL */
 size_t fwi_hip_bytes(const fwi_hip_ctx *ctx);           /* device bytes held */
 
-/* Velocity model vel[nz][nx] (m/s, host).  ekp = vel^2 dt^2 extended into
- * the CPML by edge replication (model_ext, fwi_2d.c:67-107).                */
+/* Velocity model vel[nz][nx] (m/s; host or device memory, checked and
+ * extended from a host copy).  ekp = vel^2 dt^2 extended into the CPML by
+ * edge replication (model_ext, fwi_2d.c:67-107).                            */
 int fwi_hip_set_model(fwi_hip_ctx *ctx, const float *vel);
 
 /* The host check fwi_hip_set_model runs first (no device work): every

@@ -76,12 +76,24 @@ def main():
             return float((0.5 * ((simulate(v, eng) - obs).double() ** 2).sum()).item())
     gd = float(np.sum(grad * dv))
     vbase = torch.from_numpy(vtrue * 0.97).to(dev)
-    errs = []
-    for h in (1e-1, 3e-2):
-        fd = (J(vbase + h * dvt) - J(vbase - h * dvt)) / (2 * h)
+    J0 = J(vbase)
+    errs, rem = [], []
+    # Taylor ladder: the first-order remainder falls as h^2 down to the FP32
+    # rounding floor of the misfit; the central difference meets the gradient
+    for h in (1e-1, 1e-2, 1e-3, 1e-4):
+        jp, jm = J(vbase + h * dvt), J(vbase - h * dvt)
+        fd = (jp - jm) / (2 * h)
         errs.append(abs(fd - gd) / abs(gd))
-        print(f"h={h}: fd {fd:.6e} adjoint {gd:.6e} rel {errs[-1]:.2e}")
+        rem.append(abs(jp - J0 - h * gd))
+        print(f"h={h:.0e}: remainder {rem[-1]:.3e} fd {fd:.6e} adjoint {gd:.6e} rel {errs[-1]:.2e}")
+    assert 25 < rem[0] / rem[1] < 400
     assert min(errs) < 2e-3
+    # CPU tensors take the host-memory path through the same calls: the same
+    # gradient bit for bit as the device-pointer path above
+    vc = torch.from_numpy(vtrue * 0.97).requires_grad_(True)
+    (0.5 * ((simulate(vc, eng) - obs.cpu()) ** 2).sum()).backward()
+    assert vc.grad.device.type == "cpu" and v0.grad.device.type == dev.type
+    np.testing.assert_array_equal(vc.grad.numpy(), v0.grad.detach().cpu().numpy())
     eng.close()
     print("autograd ok")
 
