@@ -77,13 +77,30 @@ __device__ __forceinline__ float buf_ld_row(brsrc rs, int voff, int r, int n, in
     const float v = buf_ld(rs, voff, (in ? r : 0) * rowb);
     return in ? v : 0.f;
 }
+// A pointer read from the parameter block is generic to the compiler, so its
+// loads and stores become flat instructions, which count against the LDS
+// wait counter as well: a wait for the stencil's LDS rows then also waits for
+// every such load in flight.  Every buffer of the context is device memory,
+// so such pointers are taken into the global address space, where the
+// compiler emits global loads and stores.
+#define FWI_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ FWI_GLOBAL T *gbl(T *p)
+{
+    return (FWI_GLOBAL T *)p;
+}
 // element i of a, 0 when !in; the load is issued for element 0 then (a must
 // hold at least one element), for the reason above
 template <class T>
-__device__ __forceinline__ T ld_sel(const T *a, long long i, bool in, T zero)
+__device__ __forceinline__ T ld_sel(const FWI_GLOBAL T *a, long long i, bool in, T zero)
 {
     const T v = a[in ? i : 0];
     return in ? v : zero;
+}
+template <class T>
+__device__ __forceinline__ T ld_sel(const T *a, long long i, bool in, T zero)
+{
+    return ld_sel(gbl(a), i, in, zero);
 }
 
 // A uniform value held in a scalar register from here on.  Where a lane
@@ -146,7 +163,7 @@ __device__ __forceinline__ int halo_col(int x0, int lane)
 __device__ __forceinline__ void stage_tables(float *stab, const float *tab, int npml)
 {
     const int n = 6 * (2 * npml + 1);
-    for (int i = threadIdx.x; i < n; i += blockDim.x) stab[i] = tab[i];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) stab[i] = gbl(tab)[i];
     __syncthreads();
 }
 // 256-thread workgroups (the tile kernels): every load issued before the first
@@ -159,7 +176,7 @@ __device__ __forceinline__ void stage_tables_256(float *stab, const float *tab, 
 #pragma unroll
     for (int q = 0; q < NP; ++q) {
         const int i = threadIdx.x + 256 * q;
-        t[q] = tab[i < n ? i : 0];
+        t[q] = gbl(tab)[i < n ? i : 0];
     }
 #pragma unroll
     for (int q = 0; q < NP; ++q) {
@@ -403,7 +420,7 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_fwd_step(FwiFwdArgs A, c
             if (i >= sh.rz_lo && i <= sh.rz_hi) {
                 const int slot = A.b.rrow[sh.rrow_off + i];
                 if (slot >= 0) {
-                    const int ri = A.b.rmap[slot * ld + x];
+                    const int ri = gbl(A.b.rmap)[slot * ld + x];
                     if (ri >= 0) A.syn[sh.trace_off + (long long)k * sh.nr + ri] = pnew;
                 }
             }
@@ -708,14 +725,14 @@ __device__ __forceinline__ void fwd_tile_rows(const FwiFwdArgs &A, const FwiShot
         pk[d] = EDGE ? buf_ld_row(rk, vo, i, pnz, rowb) : buf_ld(rk, vo, i * rowb);
     }
 
-    const float *xh_o = A.xh + s * A.b.xh_s + cur * A.b.xh_b;
-    float *xh_n = A.xh + s * A.b.xh_s + nxt * A.b.xh_b;
-    const float *xd_o = A.xd + s * A.b.xd_s + cur * A.b.xd_b;
-    float *xd_n = A.xd + s * A.b.xd_s + nxt * A.b.xd_b;
-    const float *zh_o = A.zh + s * A.b.zh_s + cur * A.b.zh_b;
-    float *zh_n = A.zh + s * A.b.zh_s + nxt * A.b.zh_b;
-    const float *zd_o = A.zd + s * A.b.zd_s + cur * A.b.zd_b;
-    float *zd_n = A.zd + s * A.b.zd_s + nxt * A.b.zd_b;
+    const FWI_GLOBAL float *xh_o = gbl(A.xh) + s * A.b.xh_s + cur * A.b.xh_b;
+    FWI_GLOBAL float *xh_n = gbl(A.xh) + s * A.b.xh_s + nxt * A.b.xh_b;
+    const FWI_GLOBAL float *xd_o = gbl(A.xd) + s * A.b.xd_s + cur * A.b.xd_b;
+    FWI_GLOBAL float *xd_n = gbl(A.xd) + s * A.b.xd_s + nxt * A.b.xd_b;
+    const FWI_GLOBAL float *zh_o = gbl(A.zh) + s * A.b.zh_s + cur * A.b.zh_b;
+    FWI_GLOBAL float *zh_n = gbl(A.zh) + s * A.b.zh_s + nxt * A.b.zh_b;
+    const FWI_GLOBAL float *zd_o = gbl(A.zd) + s * A.b.zd_s + cur * A.b.zd_b;
+    FWI_GLOBAL float *zd_n = gbl(A.zd) + s * A.b.zd_s + nxt * A.b.zd_b;
     // u nodes x0-1 .. x0+65 and p nodes x0 .. x0+63 clear of the x CPML and edges
     // (each special rule is compiled in only when the body's mask CLS allows it)
     const bool fastx = !EDGE || !(CLS & FWI_CLS_X) || ((x0 >= npml + 3) && (x0 + 66 <= pnx - npml - 1));
@@ -782,14 +799,14 @@ __device__ __forceinline__ void fwd_tile_rows(const FwiFwdArgs &A, const FwiShot
             for (int d = 0; d < FR; ++d) {
                 const int i = r0 + d;
                 const bool in = i >= sh.rz_lo && i <= sh.rz_hi;
-                slot[d] = A.b.rrow[sh.rrow_off + (in ? i : 0)];
+                slot[d] = gbl(A.b.rrow)[sh.rrow_off + (in ? i : 0)];
                 slot[d] = in ? slot[d] : -1;
             }
 #pragma unroll
             for (int d = 0; d < FR; ++d) rix[d] = ld_sel(A.b.rmap, (long long)slot[d] * ld + x, slot[d] >= 0, -1);
         }
         if (!fastx) xuho = xh_o[(r0 + ud) * 2 * n1 + (cuh.col >= 0 ? cuh.col : 0)];
-        sv = A.b.srcf[sh.src_off + (has_src ? k : 0)];  // used only when has_src
+        sv = gbl(A.b.srcf)[sh.src_off + (has_src ? k : 0)];  // used only when has_src
     }
 
     // ---- stage the own rows (with halo) for the x chain
@@ -889,7 +906,7 @@ __device__ __forceinline__ void fwd_tile_rows(const FwiFwdArgs &A, const FwiShot
             // interior strips lie inside the physical grid (fwi_fwd_strip_interior)
             const int jx = x - npml;
             if (!EDGE || (jx >= 0 && jx < g.nx && i < g.nz))
-                A.xseg[(long long)(k - A.xseg_k0) * A.xseg_step + ((long long)s * g.nz + i) * g.nx + jx] = X;
+                gbl(A.xseg)[(long long)(k - A.xseg_k0) * A.xseg_step + ((long long)s * g.nz + i) * g.nx + jx] = X;
         }
         if (!EDGE) {
             buf_st(pnew, rold, vo, i * rowb);
@@ -897,19 +914,19 @@ __device__ __forceinline__ void fwd_tile_rows(const FwiFwdArgs &A, const FwiShot
         }
         if (x < pnx) buf_st(pnew, rold, vo, i * rowb);
         if (TRACES && has_recv && rix[d] >= 0)
-            A.syn[sh.trace_off + (long long)k * sh.nr + rix[d]] = pnew;
+            gbl(A.syn)[sh.trace_off + (long long)k * sh.nr + rix[d]] = pnew;
         if (MODE == FWI_FWD_FRAMES && (frame_rows || frame_cols) && bx >= 0 && bx < nx && i < nz &&
             (i < 3 || i >= nz - 3 || bx < 3 || bx >= nx - 3)) {
             const long long so2 = sh.strip_off + (long long)k * g.sl;
             if (i < 3 || i >= nz - 3) {
                 const int pos = (i < 3) ? i * nx + bx : (3 + nz - 1 - i) * nx + bx;
-                A.pstrip[so2 + pos] = pnew;
-                A.xstrip[so2 + pos] = X;
+                gbl(A.pstrip)[so2 + pos] = pnew;
+                gbl(A.xstrip)[so2 + pos] = X;
             }
             if (bx < 3 || bx >= nx - 3) {
                 const int pos = 6 * nx + i * 6 + ((bx < 3) ? bx : 3 + (nx - 1 - bx));
-                A.pstrip[so2 + pos] = pnew;
-                A.xstrip[so2 + pos] = X;
+                gbl(A.pstrip)[so2 + pos] = pnew;
+                gbl(A.xstrip)[so2 + pos] = X;
             }
         }
     }
@@ -1261,10 +1278,10 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_adj_step(FwiAdjArgs A, c
             }
             float out = q0 + oz;
             if (i >= sh.rz_lo && i <= sh.rz_hi) {
-                const int slot = A.b.rrow[sh.rrow_off + i];
+                const int slot = gbl(A.b.rrow)[sh.rrow_off + i];
                 if (slot >= 0) {
-                    const int ri = A.b.rmap[slot * ld + x];
-                    if (ri >= 0) out = out + A.res[sh.trace_off + (long long)k * sh.nr + ri];
+                    const int ri = gbl(A.b.rmap)[slot * ld + x];
+                    if (ri >= 0) out = out + gbl(A.res)[sh.trace_off + (long long)k * sh.nr + ri];
                 }
             }
             const long long o = (long long)i * ld + x;
@@ -1408,14 +1425,14 @@ __device__ __forceinline__ void adj_tile_rows(const FwiAdjArgs &A, const FwiShot
     }
 
     // chi after step k goes to buffer (k & 1); it was read from buffer ((k+1) & 1)
-    const float *xh_o = A.xh + s * A.b.xh_s + prv * A.b.xh_b;
-    float *xh_n = A.xh + s * A.b.xh_s + cur * A.b.xh_b;
-    const float *xd_o = A.xd + s * A.b.xd_s + prv * A.b.xd_b;
-    float *xd_n = A.xd + s * A.b.xd_s + cur * A.b.xd_b;
-    const float *zh_o = A.zh + s * A.b.zh_s + prv * A.b.zh_b;
-    float *zh_n = A.zh + s * A.b.zh_s + cur * A.b.zh_b;
-    const float *zd_o = A.zd + s * A.b.zd_s + prv * A.b.zd_b;
-    float *zd_n = A.zd + s * A.b.zd_s + cur * A.b.zd_b;
+    const FWI_GLOBAL float *xh_o = gbl(A.xh) + s * A.b.xh_s + prv * A.b.xh_b;
+    FWI_GLOBAL float *xh_n = gbl(A.xh) + s * A.b.xh_s + cur * A.b.xh_b;
+    const FWI_GLOBAL float *xd_o = gbl(A.xd) + s * A.b.xd_s + prv * A.b.xd_b;
+    FWI_GLOBAL float *xd_n = gbl(A.xd) + s * A.b.xd_s + cur * A.b.xd_b;
+    const FWI_GLOBAL float *zh_o = gbl(A.zh) + s * A.b.zh_s + prv * A.b.zh_b;
+    FWI_GLOBAL float *zh_n = gbl(A.zh) + s * A.b.zh_s + cur * A.b.zh_b;
+    const FWI_GLOBAL float *zd_o = gbl(A.zd) + s * A.b.zd_s + prv * A.b.zd_b;
+    FWI_GLOBAL float *zd_n = gbl(A.zd) + s * A.b.zd_s + cur * A.b.zd_b;
 
     // columns x0-3 .. x0+66 clear of the x CPML and of the grid edges: compact
     // x stencils (the form k_adj_step uses on such tiles)
@@ -1475,7 +1492,7 @@ __device__ __forceinline__ void adj_tile_rows(const FwiAdjArgs &A, const FwiShot
             for (int d = 0; d < FR; ++d) {
                 const int i = r0 + d;
                 const bool in = i >= sh.rz_lo && i <= sh.rz_hi;
-                slot[d] = A.b.rrow[sh.rrow_off + (in ? i : 0)];
+                slot[d] = gbl(A.b.rrow)[sh.rrow_off + (in ? i : 0)];
                 slot[d] = in ? slot[d] : -1;
             }
 #pragma unroll
@@ -1727,9 +1744,9 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_rec_step(FwiRecArgs A, c
     const float *mu = (cur ? A.M0 : A.M1) + so;  // mu^{k+1}
     const float *ekp = A.b.ekp;
     const float c0x = g.c0x, c1x = g.c1x, c0z = g.c0z, c1z = g.c1z;
-    double *gacc = A.gacc + (long long)grp * nx * nz;
+    FWI_GLOBAL double *gacc = gbl(A.gacc) + (long long)grp * nx * nz;
     const bool img_on = fwi_image_step(A.img_jump, A.img_k0, A.img_k1, k);  // uniform per launch
-    const float fsrc = active ? A.b.srcf[sh.src_off + k] : 0.f;
+    const float fsrc = active ? gbl(A.b.srcf)[sh.src_off + k] : 0.f;
     const int hq = halo_col(bx0, lane);
 
     float p0w = 0.f, p1w = 0.f, p2w = 0.f, p3w = 0.f;
@@ -1811,13 +1828,13 @@ __global__ __launch_bounds__(FWI_TX * FWI_MAXSG) void k_rec_step(FwiRecArgs A, c
                     const float vz = c0z * (v2w - v1w) + c1z * (v3w - v0w);
                     X = ux + vz;
                 } else {
-                    X = A.xstrip[so2 + frame_pos(i, bx, nx, nz)];
+                    X = gbl(A.xstrip)[so2 + frame_pos(i, bx, nx, nz)];
                 }
                 img = (double)mu_i * (double)X;
                 if (k >= 1) {
                     float pm;
                     if (inner) pm = leapfrog(p0w, pp_i, ekp_i * X);
-                    else pm = (k >= 2) ? A.pstrip[sh.strip_off + (long long)(k - 2) * g.sl + frame_pos(i, bx, nx, nz)]
+                    else pm = (k >= 2) ? gbl(A.pstrip)[sh.strip_off + (long long)(k - 2) * g.sl + frame_pos(i, bx, nx, nz)]
                                        : 0.f;
                     pp[(long long)i * ld + fx] = pm;
                 }
@@ -2060,11 +2077,11 @@ __device__ __forceinline__ void rec_tile_shot(const FwiRecArgs &A, const FwiShot
             const int i = r0 + d;
             const bool fr = colok && i < nz && !(i >= 3 && i < nz - 3 && bx >= 3 && bx < nx - 3);
             const int pos = fr ? frame_pos(i, bx, nx, nz) : 0;
-            xsv[d] = A.xstrip[so2 + pos];
+            xsv[d] = gbl(A.xstrip)[so2 + pos];
             psv[d] = ld_sel(A.pstrip, so0 + pos, k >= 2, 0.f);
         }
         // used only on the source row
-        fsrc = A.b.srcf[sh.src_off + ((sh.isz >= r0 && sh.isz < r0 + RR) ? k : 0)];
+        fsrc = gbl(A.b.srcf)[sh.src_off + ((sh.isz >= r0 && sh.isz < r0 + RR) ? k : 0)];
     }
 #pragma unroll
     for (int d = 0; d < RR; ++d) {
@@ -2147,7 +2164,7 @@ __device__ __forceinline__ void rec_tile_kernel(const FwiRecArgs *__restrict__ A
 #pragma unroll
     for (int d = 0; d < RR; ++d) red[d][lane] = acc[d];
     __syncthreads();
-    double *gacc = A.gacc + (long long)grp * nx * nz;
+    FWI_GLOBAL double *gacc = gbl(A.gacc) + (long long)grp * nx * nz;
 #pragma unroll
     for (int e = 0; e < RR / 4; ++e) {
         const int d = w * (RR / 4) + e, i = r0 + d;
@@ -2196,8 +2213,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 // tiles the fused form ran slower than the two kernels: 2 waves per SIMD).
 // Items: group-major (group, row strip, tile column) of the padded grid.
 // ============================================================================
-// Occupancy: 96 VGPRs without spilling at 5 waves per SIMD (the default
-// schedule takes 115, 4 waves).
+// Occupancy: 123 VGPRs without spilling at 4 waves per SIMD.  A cap of 112
+// (so that a 160-VGPR general adjoint wave fits beside three of these waves)
+// spills 8 VGPRs and costs more than the co-residency gains.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_bwd_tile_int(
     const FwiAdjArgs *__restrict__ Aa, const FwiRecArgs *__restrict__ Ar, const FwiShotDev *__restrict__ shots,
     const int *__restrict__ items, int nit, const int *kbase, int koff, int gsize)
@@ -2252,7 +2270,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 #pragma unroll
     for (int d = 0; d < FR; ++d) red[d][lane] = acc[d];
     __syncthreads();
-    double *gacc = R.gacc + (long long)grp * nx * g.nz;
+    FWI_GLOBAL double *gacc = gbl(R.gacc) + (long long)grp * nx * g.nz;
 #pragma unroll
     for (int e = 0; e < FR / 4; ++e) {
         const int d = w * (FR / 4) + e, i = r0 + d;
@@ -2590,7 +2608,7 @@ __global__ void k_img_seg(const FwiRecArgs *__restrict__ Ap, const float *__rest
         if (k >= A.b.shots[s].nt) continue;
         acc += (double)mu[(long long)s * A.b.fs + fo] * (double)xk[(long long)s * nb + cell];
     }
-    A.gacc[(long long)grp * nb + cell] += acc;
+    gbl(A.gacc)[(long long)grp * nb + cell] += acc;
 }
 
 // ============================================================================
