@@ -1693,14 +1693,19 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
         }
         c->bwd_pre[0][ngr] = (int)lb[0].size();
         c->bwd_pre[3][ngr] = (int)lb[3].size();
+        // adjoint strips of the other tiles: [1] those whose rules are all on
+        // the x side (interior strips included) or all on the z side with the
+        // receiver rows, [2] the corner strips that meet both (the bodies of
+        // k_adj_tile_split<true> / <false>)
         for (int s = 0; s < c->nshots; s++) {
             c->bwd_pre[1][s] = (int)lb[1].size();
             c->bwd_pre[2][s] = (int)lb[2].size();
             for (int t = 0; t < ntiles; t++) {
                 if (fz[(size_t)(s / rgs) * ntiles + t]) continue;
                 const int x0 = (t % g.ntx) * FWI_TX, sz = t / g.ntx;
-                lb[fwi_adj_strip_interior(g, c->shots_h[s], x0, sz * FWI_STRIP) ? 1 : 2].push_back(
-                    (s * nsz + sz) * g.ntx + t % g.ntx);
+                const int cls = fwi_adj_strip_class(g, c->shots_h[s], x0, sz * FWI_STRIP);
+                const bool corner = (cls & FWI_CLS_X) && (cls & (FWI_CLS_Z | FWI_CLS_RS));
+                lb[corner ? 2 : 1].push_back((s * nsz + sz) * g.ntx + t % g.ntx);
             }
         }
         c->bwd_pre[1][c->nshots] = (int)lb[1].size();

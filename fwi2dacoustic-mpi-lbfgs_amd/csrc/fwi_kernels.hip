@@ -1700,6 +1700,53 @@ __global__ __launch_bounds__(256) void k_adj_tile(const FwiAdjArgs *__restrict__
     }
 }
 
+// The adjoint strips beside the fused backward step's interior launch, split by
+// register need.  One launch holding every body takes the VGPRs of its largest
+// body and more (154 in one kernel, against 85 for the x-side body, 97 for the
+// z-side one and 133 for a strip that meets x-side and z-side rules at once),
+// and a 154-VGPR wave does not fit on a SIMD beside three 128-VGPR interior
+// waves.  LIGHT: strips whose rules are all on the x side (interior strips
+// included: every x rule then resolves to the compact form) or all on the z
+// side with the receiver rows (96 VGPRs).  Otherwise: the corner strips, every
+// rule compiled in.  The host puts each strip in the list of its kernel with
+// fwi_adj_strip_class.
+template <bool LIGHT>
+__global__ __launch_bounds__(256) void k_adj_tile_split(const FwiAdjArgs *__restrict__ Ap,
+                                                        const FwiShotDev *__restrict__ shots,
+                                                        const int *__restrict__ strips, int nst, const int *kbase,
+                                                        int koff)
+{
+    const int k = (kbase ? *kbase : 0) + koff;
+    if (k < 0) return;
+    const FwiAdjArgs &A = *Ap;
+    const FwiGeomDev &g = A.g;
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lb = xcd_remap(blockIdx.x, (nst + 3) >> 2);
+    if (lb < 0) return;
+    __shared__ float Wsh[4][FR][HC];
+    __shared__ float Ush[4][FR][HC];
+    __shared__ float stab[FWI_TABN];
+    stage_tables_256(stab, A.b.tab, g.npml);  // the only workgroup barrier
+    const int e = 4 * lb + w;
+    if (e >= nst) return;
+    const int nsz = (g.pnz + FR - 1) / FR, l = strips[e];
+    const int tx = l % g.ntx, rest = l / g.ntx, sz = rest % nsz, s = rest / nsz;
+    const FwiShotDev sh = shots[s];
+    if (k >= sh.nt) return;
+    if constexpr (LIGHT) {
+        const int c = fwi_adj_strip_class(g, sh, tx * TX, sz * FR);
+        if ((c & ~FWI_CLS_X) == 0)
+            adj_tile_rows<true, false, FWI_CLS_X>(A, sh, stab, Wsh[w], Ush[w], s, k, tx * TX, sz * FR, lane);
+        else if ((c & ~FWI_CLS_Z) == 0)
+            adj_tile_rows<true, false, FWI_CLS_Z>(A, sh, stab, Wsh[w], Ush[w], s, k, tx * TX, sz * FR, lane);
+        else
+            adj_tile_rows<true, false, FWI_CLS_Z | FWI_CLS_RS>(A, sh, stab, Wsh[w], Ush[w], s, k, tx * TX, sz * FR,
+                                                              lane);
+    } else {
+        adj_tile_rows<true>(A, sh, stab, Wsh[w], Ush[w], s, k, tx * TX, sz * FR, lane);
+    }
+}
+
 // ============================================================================
 // reconstruction (physical grid) + zero-lag imaging
 // ============================================================================
@@ -2702,8 +2749,9 @@ hipError_t fwi_launch_rec_tile(const FwiRecArgs *a, const FwiRecArgs *a_dev, con
 }
 
 // one backward step: the fused interior items (tl[0]) beside the adjoint's
-// interior / general strips of the other tiles (tl[1], tl[2]), then the
-// reconstruction of the other tiles (tl[3]), which reads their mu^{k+1}.
+// strips of the other tiles (tl[1]: those of k_adj_tile_split<true>, tl[2]:
+// the corner strips), then the reconstruction of the other tiles (tl[3]),
+// which reads their mu^{k+1}.
 // With p.st2 the other tiles' chain runs on st2 beside the interior launch:
 // the two sides write disjoint cells of mu^{k+1}, p^{k-1} and the image and
 // read only the other side's previous-step fields.
@@ -2720,11 +2768,11 @@ hipError_t fwi_launch_bwd_tile(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, con
         },
         [&](hipStream_t st) {
             if (ntl[1] > 0)
-                hipLaunchKernelGGL(k_adj_tile<false>, dim3(((ntl[1] + 31) / 32) * 8), dim3(256), 0, st, a_dev,
+                hipLaunchKernelGGL(k_adj_tile_split<true>, dim3(((ntl[1] + 31) / 32) * 8), dim3(256), 0, st, a_dev,
                                    a->b.shots, tl[1], ntl[1], kbase, koff);
             if (ntl[2] > 0)
-                hipLaunchKernelGGL(k_adj_tile<true>, dim3(((ntl[2] + 31) / 32) * 8), dim3(256), 0, st, a_dev,
-                                   a->b.shots, tl[2], ntl[2], kbase, koff);
+                hipLaunchKernelGGL(k_adj_tile_split<false>, dim3(((ntl[2] + 31) / 32) * 8), dim3(256), 0, st,
+                                   a_dev, a->b.shots, tl[2], ntl[2], kbase, koff);
             if (ntl[3] > 0)
                 hipLaunchKernelGGL(k_rec_tile_pad, dim3(((ntl[3] + 7) / 8) * 8), dim3(256), 0, st, r_dev,
                                    a->b.shots, tl[3], ntl[3], kbase, koff, gsize);
