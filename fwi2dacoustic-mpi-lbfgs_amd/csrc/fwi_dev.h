@@ -188,6 +188,10 @@ struct FwiPair {
     hipStream_t st, st2;
     hipEvent_t fork, join;
     int lds_int;  // dynamic LDS bytes reserved per interior workgroup (occupancy cap)
+    // third stream of the backward step (the corner adjoint strips beside the
+    // other general strips), joined into st2 before the reconstruction
+    hipStream_t st3;
+    hipEvent_t join3;
 };
 // LDS of one workgroup of the tile kernels: interior (wave rows) and general
 // (wave rows + CPML tables)

@@ -141,6 +141,11 @@ struct fwi_hip_ctx {
     // and tails (596 -> 621 shot-gradients/s at 1024x320 x 64 shots, nt=400)
     hipStream_t st2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // third stream of the fused backward step: its corner adjoint strips (the
+    // only general strips whose body does not fit beside three interior waves)
+    // run beside the other general strips instead of after them
+    hipStream_t st3 = nullptr;
+    hipEvent_t ev_join3 = nullptr;
     bool concurrent = true;
     // With the second stream, the interior launches of the forward and fused
     // backward steps reserve unused LDS so that at most int_wgs of their
@@ -388,6 +393,8 @@ extern "C" void fwi_hip_destroy(fwi_hip_ctx *c)
     for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
     if (c->own_stream && c->st) (void)hipStreamDestroy(c->st);
     if (c->st2) (void)hipStreamDestroy(c->st2);
+    if (c->st3) (void)hipStreamDestroy(c->st3);
+    if (c->ev_join3) (void)hipEventDestroy(c->ev_join3);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     if (c->ev_fork2) (void)hipEventDestroy(c->ev_fork2);
@@ -539,6 +546,8 @@ extern "C" int fwi_hip_create(const fwi_hip_desc *desc, fwi_hip_ctx **out)
             hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) == hipSuccess)
             c->lds_cu = lds;
         hipError_t e = hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->st3, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join3, hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork2, hipEventDisableTiming);
@@ -819,6 +828,8 @@ static FwiPair pair_of(const fwi_hip_ctx *c)
     p.fork = c->ev_fork;
     p.join = c->ev_join;
     p.lds_int = interior_lds(c);
+    p.st3 = c->concurrent ? c->st3 : nullptr;
+    p.join3 = c->ev_join3;
     return p;
 }
 
@@ -856,7 +867,8 @@ static int ensure_frames(fwi_hip_ctx *c)
     if (!rc) rc = dalloc(c, (void **)&c->xstrip, n);
     if (rc) {
         dfree(c, c->pstrip);
-        c->pstrip = nullptr;
+        dfree(c, c->xstrip);
+        c->pstrip = c->xstrip = nullptr;
     }
     return rc;
 }
@@ -1844,7 +1856,8 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
         // return for k < 0 and for k >= a shot's nt)
         const bool ovl = tile && c->bwd_overlap && c->timing != 1 && !fused;
         auto pair_step = [&](const int *kb, int koff) {
-            FwiPair p1 = {c->st, nullptr, c->ev_fork, c->ev_join, 0}, p2 = {c->st2, nullptr, c->ev_fork, c->ev_join, 0};
+            FwiPair p1 = {c->st, nullptr, c->ev_fork, c->ev_join, 0, nullptr, nullptr},
+                    p2 = {c->st2, nullptr, c->ev_fork, c->ev_join, 0, nullptr, nullptr};
             hipError_t e = hipEventRecord(c->ev_fork2, c->st);
             if (e == hipSuccess) e = hipStreamWaitEvent(c->st2, c->ev_fork2, 0);
             if (e == hipSuccess) e = fwi_launch_rec_tile(&r, rd, tr, nr, kb, koff + 1, rgs, p2);

@@ -71,10 +71,10 @@ __device__ __forceinline__ void buf_st(float v, brsrc r, int voff, int soff)
 // outside the field.  The load is always issued, for an in-bounds row, and
 // the value is replaced afterwards: a load inside a branch of its own is
 // waited for before the branch ends, one memory latency per row.
-__device__ __forceinline__ float buf_ld_row(brsrc rs, int voff, int r, int n, int rowb)
+__device__ __forceinline__ float buf_ld_row(brsrc rs, int voff, int r, int n, int rowb, int base = 0)
 {
     const bool in = r >= 0 && r < n;
-    const float v = buf_ld(rs, voff, (in ? r : 0) * rowb);
+    const float v = buf_ld(rs, voff, base + (in ? r : 0) * rowb);
     return in ? v : 0.f;
 }
 // A pointer read from the parameter block is generic to the compiler, so its
@@ -1424,15 +1424,18 @@ __device__ __forceinline__ void adj_tile_rows(const FwiAdjArgs &A, const FwiShot
         else mk[d] = EDGE ? buf_ld_row(rk, vo, i, pnz, rowb) : buf_ld(rk, vo, i * rowb);
     }
 
-    // chi after step k goes to buffer (k & 1); it was read from buffer ((k+1) & 1)
-    const FWI_GLOBAL float *xh_o = gbl(A.xh) + s * A.b.xh_s + prv * A.b.xh_b;
-    FWI_GLOBAL float *xh_n = gbl(A.xh) + s * A.b.xh_s + cur * A.b.xh_b;
-    const FWI_GLOBAL float *xd_o = gbl(A.xd) + s * A.b.xd_s + prv * A.b.xd_b;
-    FWI_GLOBAL float *xd_n = gbl(A.xd) + s * A.b.xd_s + cur * A.b.xd_b;
-    const FWI_GLOBAL float *zh_o = gbl(A.zh) + s * A.b.zh_s + prv * A.b.zh_b;
-    FWI_GLOBAL float *zh_n = gbl(A.zh) + s * A.b.zh_s + cur * A.b.zh_b;
-    const FWI_GLOBAL float *zd_o = gbl(A.zd) + s * A.b.zd_s + prv * A.b.zd_b;
-    FWI_GLOBAL float *zd_n = gbl(A.zd) + s * A.b.zd_s + cur * A.b.zd_b;
+    // chi after step k goes to buffer (k & 1); it was read from buffer ((k+1) & 1).
+    // One buffer resource per array over both buffers of the shot: the lane's
+    // column in the byte offset (fixed for the wave), the row and the buffer
+    // in the uniform offset, so no 64-bit address arithmetic per row.
+    const brsrc rxh = buf_rsrc(A.xh + s * A.b.xh_s, (unsigned)(8 * A.b.xh_b));
+    const brsrc rxd = buf_rsrc(A.xd + s * A.b.xd_s, (unsigned)(8 * A.b.xd_b));
+    const brsrc rzh = buf_rsrc(A.zh + s * A.b.zh_s, (unsigned)(8 * A.b.zh_b));
+    const brsrc rzd = buf_rsrc(A.zd + s * A.b.zd_s, (unsigned)(8 * A.b.zd_b));
+    const int xh_o = (int)(prv * A.b.xh_b) * 4, xh_n = (int)(cur * A.b.xh_b) * 4;  // byte offsets of the buffers
+    const int xd_o = (int)(prv * A.b.xd_b) * 4, xd_n = (int)(cur * A.b.xd_b) * 4;
+    const int zh_o = (int)(prv * A.b.zh_b) * 4, zh_n = (int)(cur * A.b.zh_b) * 4;
+    const int zd_o = (int)(prv * A.b.zd_b) * 4, zd_n = (int)(cur * A.b.zd_b) * 4;
 
     // columns x0-3 .. x0+66 clear of the x CPML and of the grid edges: compact
     // x stencils (the form k_adj_step uses on such tiles)
@@ -1477,8 +1480,8 @@ __device__ __forceinline__ void adj_tile_rows(const FwiAdjArgs &A, const FwiShot
             const int i = min(r0 + d, pnz - 1);
             xdo[d] = xuo[d] = 0.f;
             if (!fastx) {
-                xdo[d] = xd_o[i * 2 * npml + (cf.col >= 0 ? cf.col : 0)];
-                xuo[d] = xh_o[i * 2 * n1 + (cu.col >= 0 ? cu.col : 0)];
+                xdo[d] = buf_ld(rxd, (cf.col >= 0 ? cf.col : 0) * 4, xd_o + i * 2 * npml * 4);
+                xuo[d] = buf_ld(rxh, (cu.col >= 0 ? cu.col : 0) * 4, xh_o + i * 2 * n1 * 4);
             }
         }
 #pragma unroll
@@ -1507,19 +1510,21 @@ __device__ __forceinline__ void adj_tile_rows(const FwiAdjArgs &A, const FwiShot
 #pragma unroll
         for (int q = 0; q < HP; ++q) {
             xdho[q] = 0.f;
-            if (!fastx) xdho[q] = xd_o[(r0 + xh.d[q]) * 2 * npml + ((hok[q] && cfh[q].col >= 0) ? cfh[q].col : 0)];
+            if (!fastx)
+                xdho[q] = buf_ld(rxd, (xh.d[q] * 2 * npml + ((hok[q] && cfh[q].col >= 0) ? cfh[q].col : 0)) * 4,
+                                 xd_o + r0 * 2 * npml * 4);
         }
-        if (!fastx) xuho = xh_o[(r0 + ud) * 2 * n1 + (cuh.col >= 0 ? cuh.col : 0)];
+        if (!fastx) xuho = buf_ld(rxh, (ud * 2 * n1 + (cuh.col >= 0 ? cuh.col : 0)) * 4, xh_o + r0 * 2 * n1 * 4);
         if (!zplain) {
 #pragma unroll
             for (int j = 0; j < FPR; ++j) {
-                const int r = r0 - 3 + j;
-                zdo[j] = ld_sel(zd_o, (long long)(r - (pnz - npml)) * ld + x, r >= pnz - npml && r < pnz, 0.f);
+                const int q = r0 - 3 + j - (pnz - npml);
+                zdo[j] = buf_ld_row(rzd, x * 4, q, npml, ld * 4, zd_o);
             }
 #pragma unroll
             for (int n = 0; n < FVN; ++n) {
-                const int jn = r0 - 1 + n;
-                zho[n] = ld_sel(zh_o, (long long)(jn - (pnz - npml)) * ld + x, jn >= pnz - npml && jn < pnz, 0.f);
+                const int q = r0 - 1 + n - (pnz - npml);
+                zho[n] = buf_ld_row(rzh, x * 4, q, npml, ld * 4, zh_o);
             }
         }
     }
@@ -1531,7 +1536,7 @@ __device__ __forceinline__ void adj_tile_rows(const FwiAdjArgs &A, const FwiShot
         if (!fastx && r0 + d < pnz) {
             const float cn = cf.cb * xdo[d] + cf.ca * wm;
             wm = wm + cn;
-            if (cf.col >= 0) xd_n[(r0 + d) * 2 * npml + cf.col] = cn;  // x < pnx whenever col >= 0
+            if (cf.col >= 0) buf_st(cn, rxd, cf.col * 4, xd_n + (r0 + d) * 2 * npml * 4);  // x < pnx whenever col >= 0
         }
         Ws[d][lane + 3] = wm;
     }
@@ -1554,7 +1559,7 @@ __device__ __forceinline__ void adj_tile_rows(const FwiAdjArgs &A, const FwiShot
             const float cn = cu.cb * xuo[d] + cu.ca * u;
             u = u + cn;
             Us[d][lane] = u;
-            if (cu.col >= 0 && lane >= 1) xh_n[(r0 + d) * 2 * n1 + cu.col] = cn;  // owned: x0 <= j < x0+64
+            if (cu.col >= 0 && lane >= 1) buf_st(cn, rxh, cu.col * 4, xh_n + (r0 + d) * 2 * n1 * 4);  // owned: x0 <= j < x0+64
         }
     }
     {
@@ -1566,7 +1571,7 @@ __device__ __forceinline__ void adj_tile_rows(const FwiAdjArgs &A, const FwiShot
             u = hhA * W[uc + 1] + hhB * W[uc + 2] + hhC * W[uc] + hhD * W[uc + 3];
             const float cn = cuh.cb * xuho + cuh.ca * u;
             u = u + cn;
-            if (cuh.col >= 0 && uc == 64) xh_n[(r0 + ud) * 2 * n1 + cuh.col] = cn;  // owned: node x0+63
+            if (cuh.col >= 0 && uc == 64) buf_st(cn, rxh, (ud * 2 * n1 + cuh.col) * 4, xh_n + r0 * 2 * n1 * 4);  // owned: node x0+63
         }
         Us[ud][uok ? uc : HJ] = u;
     }
@@ -1583,7 +1588,7 @@ __device__ __forceinline__ void adj_tile_rows(const FwiAdjArgs &A, const FwiShot
             const int q = r - (pnz - npml);
             const float c = TB(5, 2 * q + 1) * zdo[j] + TB(4, 2 * q + 1) * wz;
             wz = wz + c;
-            if (j >= 3 && j < FR + 3) zd_n[q * ld + x] = c;  // own rows
+            if (j >= 3 && j < FR + 3) buf_st(c, rzd, x * 4, zd_n + q * ld * 4);  // own rows
         }
         a[j] = wz;
     }
@@ -1615,7 +1620,7 @@ __device__ __forceinline__ void adj_tile_rows(const FwiAdjArgs &A, const FwiShot
                 const int q = jn - (pnz - npml);
                 const float c = TB(5, 2 * q) * zho[n] + TB(4, 2 * q) * t;
                 t = t + c;
-                if (n >= 1 && n <= FR) zh_n[q * ld + x] = c;  // own nodes r0 .. r0+FR-1
+                if (n >= 1 && n <= FR) buf_st(c, rzh, x * 4, zh_n + q * ld * 4);  // own nodes r0 .. r0+FR-1
             }
         }
         tv[n] = t;
@@ -2751,32 +2756,55 @@ hipError_t fwi_launch_rec_tile(const FwiRecArgs *a, const FwiRecArgs *a_dev, con
 // one backward step: the fused interior items (tl[0]) beside the adjoint's
 // strips of the other tiles (tl[1]: those of k_adj_tile_split<true>, tl[2]:
 // the corner strips), then the reconstruction of the other tiles (tl[3]),
-// which reads their mu^{k+1}.
-// With p.st2 the other tiles' chain runs on st2 beside the interior launch:
-// the two sides write disjoint cells of mu^{k+1}, p^{k-1} and the image and
-// read only the other side's previous-step fields.
+// which reads their mu^{k+1}.  With p.st2 the other tiles' chain runs on st2
+// beside the interior launch: the two sides write disjoint cells of mu^{k+1},
+// p^{k-1} and the image and read only the other side's previous-step fields.
+// With p.st3 as well, the corner strips run on st3 beside the other adjoint
+// strips, joined into st2 before the reconstruction.
 hipError_t fwi_launch_bwd_tile(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, const FwiRecArgs *r_dev,
                                const int *const *tl, const int *ntl, const int *kbase, int koff, int gsize,
                                const FwiPair &p)
 {
+    auto launch_int = [&](hipStream_t st) {
+        hipLaunchKernelGGL(k_bwd_tile_int, dim3(((ntl[0] + 7) / 8) * 8), dim3(256), p.lds_int, st, a_dev, r_dev,
+                           a->b.shots, tl[0], ntl[0], kbase, koff, gsize);
+    };
+    auto launch_light = [&](hipStream_t st) {
+        hipLaunchKernelGGL(k_adj_tile_split<true>, dim3(((ntl[1] + 31) / 32) * 8), dim3(256), 0, st, a_dev,
+                           a->b.shots, tl[1], ntl[1], kbase, koff);
+    };
+    auto launch_corner = [&](hipStream_t st) {
+        hipLaunchKernelGGL(k_adj_tile_split<false>, dim3(((ntl[2] + 31) / 32) * 8), dim3(256), 0, st, a_dev,
+                           a->b.shots, tl[2], ntl[2], kbase, koff);
+    };
+    auto launch_rec = [&](hipStream_t st) {
+        hipLaunchKernelGGL(k_rec_tile_pad, dim3(((ntl[3] + 7) / 8) * 8), dim3(256), 0, st, r_dev, a->b.shots, tl[3],
+                           ntl[3], kbase, koff, gsize);
+    };
+    if (p.st2 && p.st3 && ntl[0] > 0 && ntl[1] > 0 && ntl[2] > 0) {
+        // one fork, two joins: st3 -> st2 before the reconstruction, st2 -> st
+        hipError_t e = hipEventRecord(p.fork, p.st);
+        if (e == hipSuccess) e = hipStreamWaitEvent(p.st2, p.fork, 0);
+        if (e == hipSuccess) e = hipStreamWaitEvent(p.st3, p.fork, 0);
+        if (e != hipSuccess) return e;
+        launch_corner(p.st3);
+        launch_light(p.st2);
+        launch_int(p.st);
+        e = hipEventRecord(p.join3, p.st3);
+        if (e == hipSuccess) e = hipStreamWaitEvent(p.st2, p.join3, 0);
+        if (e != hipSuccess) return e;
+        if (ntl[3] > 0) launch_rec(p.st2);
+        e = hipEventRecord(p.join, p.st2);
+        if (e == hipSuccess) e = hipStreamWaitEvent(p.st, p.join, 0);
+        if (e != hipSuccess) return e;
+        return hipGetLastError();
+    }
     const bool has_gen = ntl[1] > 0 || ntl[2] > 0 || ntl[3] > 0;
-    return launch_pair(
-        p, ntl[0] > 0, has_gen,
-        [&](hipStream_t st) {
-            hipLaunchKernelGGL(k_bwd_tile_int, dim3(((ntl[0] + 7) / 8) * 8), dim3(256), p.lds_int, st, a_dev, r_dev,
-                               a->b.shots, tl[0], ntl[0], kbase, koff, gsize);
-        },
-        [&](hipStream_t st) {
-            if (ntl[1] > 0)
-                hipLaunchKernelGGL(k_adj_tile_split<true>, dim3(((ntl[1] + 31) / 32) * 8), dim3(256), 0, st, a_dev,
-                                   a->b.shots, tl[1], ntl[1], kbase, koff);
-            if (ntl[2] > 0)
-                hipLaunchKernelGGL(k_adj_tile_split<false>, dim3(((ntl[2] + 31) / 32) * 8), dim3(256), 0, st,
-                                   a_dev, a->b.shots, tl[2], ntl[2], kbase, koff);
-            if (ntl[3] > 0)
-                hipLaunchKernelGGL(k_rec_tile_pad, dim3(((ntl[3] + 7) / 8) * 8), dim3(256), 0, st, r_dev,
-                                   a->b.shots, tl[3], ntl[3], kbase, koff, gsize);
-        });
+    return launch_pair(p, ntl[0] > 0, has_gen, launch_int, [&](hipStream_t st) {
+        if (ntl[1] > 0) launch_light(st);
+        if (ntl[2] > 0) launch_corner(st);
+        if (ntl[3] > 0) launch_rec(st);
+    });
 }
 
 hipError_t fwi_launch_adj(const FwiAdjArgs *a, const int *kbase, int koff, int nblocks, hipStream_t st)
