@@ -20,6 +20,10 @@
 #include <stdint.h>
 
 #define FWI_TX 64      // x cells per wave (one lane per column)
+// lane byte offset of a buffer load outside every field: the buffer
+// resources of the tile kernels cover one shot's field (or less), which the
+// host keeps below this size, so the resource's range check returns 0
+#define FWI_BUF_OUT 0x40000000
 #define FWI_MAXSG 4    // shots (waves) per workgroup
 
 struct FwiShotDev {

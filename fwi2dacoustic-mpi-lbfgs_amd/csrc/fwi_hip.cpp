@@ -369,6 +369,11 @@ static int check_grid(fwi_hip_ctx *c, const fwi_hip_grid &g)
     if (!(g.dx > 0) || !(g.dz > 0) || !(g.dt > 0)) return set_err(c, FWI_HIP_EINVAL, "dx, dz, dt must be > 0");
     if (!(g.max_v > 0) || !(g.coef_r > 0) || !(g.coef_r < 1) || !(g.max_f >= 0))
         return set_err(c, FWI_HIP_EINVAL, "CPML meta: need max_v > 0, 0 < coef_r < 1, max_f >= 0");
+    // one shot's padded field is addressed through a 32-bit buffer range
+    const long long pnx = (long long)g.nx + 2 * g.npml, pnz = (long long)g.nz + g.npml;
+    if (pnz * (FWI_TX * ((pnx + FWI_TX - 1) / FWI_TX)) * (long long)sizeof(float) >= FWI_BUF_OUT)
+        return set_err(c, FWI_HIP_EINVAL, "grid %dx%d: one shot's padded field must stay below %d bytes", g.nx, g.nz,
+                       FWI_BUF_OUT);
     return FWI_HIP_OK;
 }
 

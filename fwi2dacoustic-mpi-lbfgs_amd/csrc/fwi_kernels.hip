@@ -52,7 +52,9 @@ __device__ __forceinline__ int xcd_remap(int b, int n)
 // Raw buffer access: a 128-bit resource (base, size in bytes) in SGPRs, a
 // per-lane byte offset (the column, fixed for a wave) and a uniform byte
 // offset (the row), so a row costs one scalar add instead of a 64-bit vector
-// address.  Every access stays inside [0, bytes) by construction.
+// address.  The resource's range check covers every access: a load whose
+// lane offset lies at or beyond the size returns 0 without touching memory
+// (buf_ld_row uses that for rows outside a field).
 typedef __amdgpu_buffer_rsrc_t brsrc;
 #define FWI_BUF_WORD3 0x00020000  // raw 32-bit dwords, gfx9-family resource format
 __device__ __forceinline__ brsrc buf_rsrc(const void *base, unsigned bytes)
@@ -68,13 +70,25 @@ __device__ __forceinline__ void buf_st(float v, brsrc r, int voff, int soff)
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
 }
 // Row r of a field (rows [0, n), pitch rowb bytes) for the general strips, 0
-// outside the field.  The load is always issued, for an in-bounds row, and
-// the value is replaced afterwards: a load inside a branch of its own is
-// waited for before the branch ends, one memory latency per row.
+// outside the field.  The load is always issued (a load inside a branch of
+// its own is waited for before the branch ends, one memory latency per row);
+// for a row outside the field its lane offset is FWI_BUF_OUT, beyond any
+// resource size, so the range check returns 0.  Selecting 0 after the load
+// instead keeps one 64-bit lane mask per row alive until the loads return:
+// with every load of a strip issued first, that spilled 70 to 200 SGPRs of
+// the general bodies into VGPR lanes.
 __device__ __forceinline__ float buf_ld_row(brsrc rs, int voff, int r, int n, int rowb, int base = 0)
 {
     const bool in = r >= 0 && r < n;
-    const float v = buf_ld(rs, voff, base + (in ? r : 0) * rowb);
+    return buf_ld(rs, in ? voff : FWI_BUF_OUT, base + (in ? r : 0) * rowb);
+}
+// the same with 0 selected after the load: one VGPR for all rows' lane offset
+// (the reconstruction items, whose loads of two shots are in flight at once,
+// run out of VGPRs with an offset per row)
+__device__ __forceinline__ float buf_ld_row_sel(brsrc rs, int voff, int r, int n, int rowb)
+{
+    const bool in = r >= 0 && r < n;
+    const float v = buf_ld(rs, voff, (in ? r : 0) * rowb);
     return in ? v : 0.f;
 }
 // A pointer read from the parameter block is generic to the compiler, so its
@@ -2051,7 +2065,7 @@ __device__ __forceinline__ void rec_tile_load(const FwiRecArgs &A, int s, int k,
 #pragma unroll
     for (int j = 0; j < RPR; ++j) {
         const int zr = r0 - 3 + j;
-        in.p[j] = EDGE ? buf_ld_row(rn, vo, zr, nz, rowb) : buf_ld(rn, vo, zr * rowb);
+        in.p[j] = EDGE ? buf_ld_row_sel(rn, vo, zr, nz, rowb) : buf_ld(rn, vo, zr * rowb);
         if (EDGE && !colok) in.p[j] = 0.f;
     }
     XHalo<RHP> xh;
@@ -2066,8 +2080,8 @@ __device__ __forceinline__ void rec_tile_load(const FwiRecArgs &A, int s, int k,
 #pragma unroll
     for (int d = 0; d < RR; ++d) {
         const int i = r0 + d;
-        in.pq[d] = EDGE ? buf_ld_row(rq, vo, i, nz, rowb) : buf_ld(rq, vo, i * rowb);
-        if (!FUSED) in.pm[d] = EDGE ? buf_ld_row(rm, vo, i, nz, rowb) : buf_ld(rm, vo, i * rowb);
+        in.pq[d] = EDGE ? buf_ld_row_sel(rq, vo, i, nz, rowb) : buf_ld(rq, vo, i * rowb);
+        if (!FUSED) in.pm[d] = EDGE ? buf_ld_row_sel(rm, vo, i, nz, rowb) : buf_ld(rm, vo, i * rowb);
     }
 }
 
@@ -2195,7 +2209,7 @@ __device__ __forceinline__ void rec_tile_kernel(const FwiRecArgs *__restrict__ A
         const brsrc rk = buf_rsrc(A.b.ekp, (unsigned)(A.b.fs * sizeof(float)));
         const int vo = (colok ? bx + npml : npml) * 4;
 #pragma unroll
-        for (int d = 0; d < RR; ++d) pk[d] = buf_ld_row(rk, vo, r0 + d, nz, ld * 4);
+        for (int d = 0; d < RR; ++d) pk[d] = buf_ld_row_sel(rk, vo, r0 + d, nz, ld * 4);
     }
     double acc[RR];
 #pragma unroll
