@@ -819,8 +819,10 @@ extern "C" int fwi_hip_set_shots(fwi_hip_ctx *c, int nshots, const fwi_hip_shot 
 static int interior_lds(const fwi_hip_ctx *c)
 {
     if (!c->concurrent || c->int_wgs <= 0 || c->lds_cu <= 0) return 0;
+    // the least LDS per interior workgroup that keeps n + 1 of them off a CU,
+    // so that the rest holds as many general workgroups as it can
     const int n = c->int_wgs;
-    const int total = (int)((2LL * c->lds_cu) / (2 * n + 1)) & ~255;  // lds_cu / (n + 1/2)
+    const int total = (c->lds_cu / (n + 1) + 256) & ~255;
     if (total * n + FWI_LDS_GEN > c->lds_cu || total * (n + 1) <= c->lds_cu || total <= FWI_LDS_TILE) return 0;
     return total - FWI_LDS_TILE;
 }
