@@ -52,7 +52,12 @@ def parse():
     ap.add_argument("--concurrent", type=int, default=-1,
                     help="general strips on a second stream beside the interior launch (1/0; -1: default)")
     ap.add_argument("--interior-wgs", type=int, default=-1,
-                    help="interior workgroups per CU beside the general strips (0: no cap; -1: default 3)")
+                    help="interior workgroups per CU beside the general strips, both passes (0: no cap; -1: the "
+                         "defaults, 6 forward and 4 backward)")
+    ap.add_argument("--fwd-shape", type=int, nargs=2, default=None, metavar=("ROWS", "WGS"),
+                    help="forward pass: rows per wave of the tile kernels (4 or 8) and interior workgroups per CU")
+    ap.add_argument("--bwd-shape", type=int, nargs=2, default=None, metavar=("ROWS", "WGS"),
+                    help="backward pass: rows per wave of the tile kernels (4 or 8) and interior workgroups per CU")
     ap.add_argument("--bwd-fused", type=int, default=-1,
                     help="one fused adjoint + reconstruction + imaging kernel per backward step (1/0; -1: default 1)")
     ap.add_argument("--ckpt-every", type=int, default=100, help="snapshot interval of --workload checkpoint")
@@ -563,6 +568,10 @@ def main():
             ctx.set_concurrent(args.concurrent)
         if args.interior_wgs >= 0:
             ctx.set_interior_wgs(args.interior_wgs)
+        if args.fwd_shape:
+            ctx.set_pass_shape(0, *args.fwd_shape)
+        if args.bwd_shape:
+            ctx.set_pass_shape(1, *args.bwd_shape)
     ctx.set_model(vel)
     ctx.set_shots(shots)
     obs = ctx.forward()
@@ -695,7 +704,9 @@ def main():
                                f"nt={args.nt}, {nshots} shots x {nr} receivers, npml={g.npml}, free surface",
                    "nx": g.nx, "nz": g.nz, "nt": args.nt, "nshots": nshots, "nr": nr, "npml": g.npml,
                    "launch": launch or {"kernels": args.kernels, "rec_group": args.rec_group or 8,
-                                        "bwd_chunk": max(args.bwd_chunk, 0), "graph": 100}},
+                                        "bwd_chunk": max(args.bwd_chunk, 0), "graph": 100,
+                                        "fwd_shape": list(args.fwd_shape or (4, 6)),
+                                        "bwd_shape": list(args.bwd_shape or (4, 4))}},
         "gcell_updates_per_s": gcell,
         "hbm_frac_whole_job": gcell * 1e9 * BYTES_PER_CELL_UPDATE / HBM_PEAK,
         "misfit": J,

@@ -29,6 +29,14 @@ HIP_FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-ffp-contr
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wall", "-Wno-unused-variable", "-Wno-unused-function", f"-I{INC}"]
 C_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-pthread", "-Wall", f"-I{INC}"]
+# device objects: (source, object, extra flags).  The tile kernels are built
+# twice, with 8-row strips and with 4-row strips (launchers suffixed _s4).
+DEVICE_OBJS = (
+    ("fwi_kernels.hip", "fwi_kernels.hip.o", []),
+    ("fwi_kernels.hip", "fwi_kernels_s4.hip.o",
+     ["-DFWI_TZ2=16", "-DFWI_REC_ROWS=4", "-DFWI_VARIANT=_s4", "-DFWI_SECONDARY"]),
+    ("fwi_noalloc.hip", "fwi_noalloc.hip.o", []),
+)
 
 
 def run(cmd):
@@ -39,9 +47,9 @@ def run(cmd):
 def build(verbose=True):
     os.makedirs(BUILD, exist_ok=True)
     objs = []
-    for src in ("fwi_kernels.hip", "fwi_hip.cpp", "fwi_noalloc.hip"):
-        obj = os.path.join(BUILD, src + ".o")
-        run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+    for src, name, extra in DEVICE_OBJS + (("fwi_hip.cpp", "fwi_hip.cpp.o", []),):
+        obj = os.path.join(BUILD, name)
+        run([HIPCC, *HIP_FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     for src in ("grad4_1shot.c", "fwi_io.c", "fwi_sigproc.c", "fwi_msf.c", "fwi_msf_rank.c", "fwi_gradprec.c", "fwi_lbfgs.c"):
         path = os.path.join(CSRC, src)
@@ -88,10 +96,10 @@ def build_sanitized():
     sb = os.path.join(BUILD, "san")
     os.makedirs(sb, exist_ok=True)
     objs = []
-    for src in ("fwi_kernels.hip", "fwi_noalloc.hip"):
-        obj = os.path.join(BUILD, src + ".o")
+    for src, name, extra in DEVICE_OBJS:
+        obj = os.path.join(BUILD, name)
         if not os.path.exists(obj):
-            run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+            run([HIPCC, *HIP_FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     host_san = [f for flag in SAN for f in ("-Xarch_host", flag)]
     obj = os.path.join(sb, "fwi_hip.cpp.o")

@@ -49,12 +49,18 @@ struct FwiGeomDev {
     int pad_;
 };
 
+// Rows of a 2-D tile (4 waves x FWI_TZ2/4 output rows).  The tile kernels are
+// compiled twice: with 8-row strips (the default) and with 4-row strips (the
+// object built with FWI_TZ2 = 16, its launchers suffixed _s4); a context picks
+// one per pass (fwi_hip_set_pass_shape).  Host code passes the strip height
+// to the functions below; device code uses its own FWI_STRIP.
 #ifndef FWI_TZ2
-#define FWI_TZ2 32       // rows of a 2-D tile (4 waves x 8 output rows)
+#define FWI_TZ2 32
 #endif
 
 // rows per wave of the tile reconstruction kernel (its FP64 image
-// accumulators grow with it, so it is set apart from the forward/adjoint)
+// accumulators grow with it, so it is set apart from the forward/adjoint);
+// the fused backward step needs it equal to the strip height
 #ifndef FWI_REC_ROWS
 #define FWI_REC_ROWS 8
 #endif
@@ -67,9 +73,10 @@ struct FwiGeomDev {
 // functions.
 #define FWI_STRIP (FWI_TZ2 / 4)
 // forward strip at (x0, r0)
-__host__ __device__ inline bool fwi_fwd_strip_interior(const FwiGeomDev &g, const FwiShotDev &sh, int x0, int r0)
+__host__ __device__ inline bool fwi_fwd_strip_interior(const FwiGeomDev &g, const FwiShotDev &sh, int x0, int r0,
+                                                      int fr = FWI_STRIP)
 {
-    const int r1 = r0 + FWI_STRIP;
+    const int r1 = r0 + fr;
     if (x0 < g.npml + 3 || x0 + 66 > g.pnx - g.npml - 1) return false;  // u nodes x0-1 .. x0+65
     if (r0 < 3 || r1 > g.nz - 3) return false;                           // v nodes r0-1 .. r1+1
     if (sh.isz >= r0 && sh.isz < r1 && sh.isx >= x0 && sh.isx < x0 + FWI_TX) return false;
@@ -85,9 +92,10 @@ __host__ __device__ inline bool fwi_fwd_strip_interior(const FwiGeomDev &g, cons
 #define FWI_CLS_RS 4   // receiver rows or the source cell
 #define FWI_CLS_FR 8   // frame rows or columns (forward)
 #define FWI_CLS_ALL 15
-__host__ __device__ inline int fwi_fwd_strip_class(const FwiGeomDev &g, const FwiShotDev &sh, int x0, int r0)
+__host__ __device__ inline int fwi_fwd_strip_class(const FwiGeomDev &g, const FwiShotDev &sh, int x0, int r0,
+                                                   int fr = FWI_STRIP)
 {
-    const int npml = g.npml, FR = FWI_STRIP;
+    const int npml = g.npml, FR = fr;
     int c = 0;
     if (!((x0 >= npml + 3) && (x0 + 66 <= g.pnx - npml - 1))) c |= FWI_CLS_X;
     if (!(r0 >= 3 && r0 + FR + 1 < g.pnz - npml)) c |= FWI_CLS_Z;
@@ -99,9 +107,10 @@ __host__ __device__ inline int fwi_fwd_strip_class(const FwiGeomDev &g, const Fw
         c |= FWI_CLS_FR;
     return c;
 }
-__host__ __device__ inline int fwi_adj_strip_class(const FwiGeomDev &g, const FwiShotDev &sh, int x0, int r0)
+__host__ __device__ inline int fwi_adj_strip_class(const FwiGeomDev &g, const FwiShotDev &sh, int x0, int r0,
+                                                   int fr = FWI_STRIP)
 {
-    const int npml = g.npml, FR = FWI_STRIP;
+    const int npml = g.npml, FR = fr;
     int c = 0;
     if (!((x0 >= npml + 3) && (x0 + 66 <= g.pnx - npml - 1))) c |= FWI_CLS_X;
     if (!(r0 >= 6 && r0 + FR + 2 < g.pnz - npml)) c |= FWI_CLS_Z;
@@ -110,9 +119,10 @@ __host__ __device__ inline int fwi_adj_strip_class(const FwiGeomDev &g, const Fw
 }
 
 // adjoint strip: no source term; the transposed free-surface rows reach node 4
-__host__ __device__ inline bool fwi_adj_strip_interior(const FwiGeomDev &g, const FwiShotDev &sh, int x0, int r0)
+__host__ __device__ inline bool fwi_adj_strip_interior(const FwiGeomDev &g, const FwiShotDev &sh, int x0, int r0,
+                                                      int fr = FWI_STRIP)
 {
-    const int r1 = r0 + FWI_STRIP;
+    const int r1 = r0 + fr;
     if (x0 < g.npml + 3 || x0 + 66 > g.pnx - g.npml - 1) return false;
     if (r0 < 6 || r1 > g.nz - 3) return false;
     if (sh.rz_lo <= sh.rz_hi && sh.rz_lo < r1 && sh.rz_hi >= r0) return false;
@@ -120,12 +130,13 @@ __host__ __device__ inline bool fwi_adj_strip_interior(const FwiGeomDev &g, cons
 }
 // reconstruction tile: 64 x FWI_REC_ROWS physical cells at (bx0, z0), every
 // cell inner (3 or more from the edge) and the source elsewhere
-__host__ __device__ inline bool fwi_rec_tile_interior(const FwiGeomDev &g, const FwiShotDev &sh, int bx0, int z0)
+__host__ __device__ inline bool fwi_rec_tile_interior(const FwiGeomDev &g, const FwiShotDev &sh, int bx0, int z0,
+                                                     int rr = FWI_REC_ROWS)
 {
     if (bx0 < 3 || bx0 + 66 >= g.nx) return false;
-    if (z0 < 3 || z0 + FWI_REC_ROWS > g.nz - 3) return false;
+    if (z0 < 3 || z0 + rr > g.nz - 3) return false;
     const int sx = sh.isx - g.npml;
-    if (sh.isz >= z0 && sh.isz < z0 + FWI_REC_ROWS && sx >= bx0 && sx < bx0 + FWI_TX) return false;
+    if (sh.isz >= z0 && sh.isz < z0 + rr && sx >= bx0 && sx < bx0 + FWI_TX) return false;
     return true;
 }
 
@@ -197,10 +208,10 @@ struct FwiPair {
     hipStream_t st3;
     hipEvent_t join3;
 };
-// LDS of one workgroup of the tile kernels: interior (wave rows) and general
-// (wave rows + CPML tables)
-#define FWI_LDS_TILE (4 * 2 * FWI_STRIP * 72 * 4)
-#define FWI_LDS_GEN (FWI_LDS_TILE + 6 * (2 * 64 + 1) * 4)
+// LDS of one workgroup of the tile kernels with fr-row strips: interior (wave
+// rows) and general (wave rows + CPML tables)
+__host__ __device__ constexpr int fwi_lds_tile(int fr) { return 4 * 2 * fr * 72 * 4; }
+__host__ __device__ constexpr int fwi_lds_gen(int fr) { return fwi_lds_tile(fr) + 6 * (2 * 64 + 1) * 4; }
 
 // per-step kernels with the reference's loop structure (unit parity)
 struct FwiRefArgs {

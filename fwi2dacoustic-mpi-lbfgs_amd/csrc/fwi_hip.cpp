@@ -38,6 +38,16 @@ hipError_t fwi_launch_rec(const FwiRecArgs *a, const int *kbase, int koff, int n
 hipError_t fwi_launch_bwd_tile(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, const FwiRecArgs *r_dev,
                                const int *const *tl, const int *ntl, const int *kbase, int koff, int gsize,
                                const FwiPair &p);
+// the same tile kernels built with 4-row strips
+hipError_t fwi_launch_fwd_tile_s4(const FwiFwdArgs *a, const FwiFwdArgs *a_dev, const int *const *tl,
+                                  const int *ntl, const int *kbase, int koff, const FwiPair &p, int mode);
+hipError_t fwi_launch_adj_tile_s4(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, const int *const *tl,
+                                  const int *ntl, const int *kbase, int koff, const FwiPair &p);
+hipError_t fwi_launch_rec_tile_s4(const FwiRecArgs *a, const FwiRecArgs *a_dev, const int *const *tl,
+                                  const int *ntl, const int *kbase, int koff, int gsize, const FwiPair &p);
+hipError_t fwi_launch_bwd_tile_s4(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, const FwiRecArgs *r_dev,
+                                  const int *const *tl, const int *ntl, const int *kbase, int koff, int gsize,
+                                  const FwiPair &p);
 hipError_t fwi_launch_advance(int *kbase, int delta, hipStream_t st);
 hipError_t fwi_launch_set_int(int *p, int v, hipStream_t st);
 hipError_t fwi_launch_fill_f32(float *p, long long n, float v, hipStream_t st);
@@ -147,11 +157,18 @@ struct fwi_hip_ctx {
     hipStream_t st3 = nullptr;
     hipEvent_t ev_join3 = nullptr;
     bool concurrent = true;
+    // rows per wave of the tile kernels in the forward pass (and the
+    // recompute of the checkpointed backward pass) and in the backward pass:
+    // 8 or 4 (fwi_hip_set_pass_shape); the strip lists are built for them
+    int strip_fwd = 4, strip_bwd = 4;
     // With the second stream, the interior launches of the forward and fused
-    // backward steps reserve unused LDS so that at most int_wgs of their
+    // backward steps reserve unused LDS so that at most int_wgs_* of their
     // workgroups share a CU, leaving room for a general workgroup beside them
-    // (1024x320 x 64 shots, nt=400: 626 -> 658 shot-gradients/s at 3; 0: no cap)
-    int int_wgs = 3;
+    // (0: no cap).  Defaults: 4-row strips with 6 forward and 4 backward
+    // workgroups per CU (1024x320 x 64 shots: forward 64.1 -> 61.6 us, backward
+    // 151.5 -> 148.0 us against 8-row strips at 3; 4096x1024 x 16 shots:
+    // forward 199.8 -> 184.7 us, backward unchanged)
+    int int_wgs_fwd = 6, int_wgs_bwd = 4;
     int lds_cu = 0;  // LDS bytes per CU of the device
     // backward pass: adjoint step k on st beside imaging step k+1 on st2
     // (independent: the adjoint writes only the mu buffer the imaging step
@@ -474,8 +491,9 @@ static std::vector<std::pair<void **, size_t>> create_table(fwi_hip_ctx *c)
     const FwiGeomDev &g = c->g;
     const long long S = desc->max_shots, NT = desc->max_nt, NR = desc->max_nr;
     const int maxgroups = (int)S;  // image accumulators: at most one group per shot, whatever the grouping
-    const size_t strips = sizeof(int) * S * g.ntx * ((g.pnz + FWI_STRIP - 1) / FWI_STRIP);
-    const size_t items = sizeof(int) * S * g.bntx * ((g.nz + FWI_REC_ROWS - 1) / FWI_REC_ROWS);
+    // list capacity for the shortest strips (4 rows)
+    const size_t strips = sizeof(int) * S * g.ntx * ((g.pnz + 3) / 4);
+    const size_t items = sizeof(int) * S * g.bntx * ((g.nz + 3) / 4);
     return {
         {(void **)&c->ekp, sizeof(float) * c->fs},
         {(void **)&c->vel, sizeof(float) * gr.nx * gr.nz},
@@ -683,6 +701,45 @@ extern "C" int fwi_hip_set_model(fwi_hip_ctx *c, const float *vel)
     return FWI_HIP_OK;
 }
 
+// strip lists of the forward (strip_fwd rows) and adjoint (strip_bwd rows)
+// tile kernels, logical strip (s nsz + sz) ntx + tx, for the context's shots;
+// the backward pass builds its reconstruction and fused lists itself
+static int build_strip_lists(fwi_hip_ctx *c)
+{
+    const FwiGeomDev &g = c->g;
+    const int nshots = c->nshots;
+    std::vector<int> lf[2], la[2];
+    for (int q = 0; q < 2; q++) c->adj_pre[q].assign(nshots + 1, 0);
+    const int fr = c->strip_fwd, br = c->strip_bwd;
+    const int nszf = (g.pnz + fr - 1) / fr, nszb = (g.pnz + br - 1) / br;
+    for (int s = 0; s < nshots; s++) {
+        for (int q = 0; q < 2; q++) c->adj_pre[q][s] = (int)la[q].size();
+        for (int sz = 0; sz < nszf; sz++)
+            for (int tx = 0; tx < g.ntx; tx++)
+                lf[fwi_fwd_strip_interior(g, c->shots_h[s], tx * FWI_TX, sz * fr, fr) ? 0 : 1].push_back(
+                    (s * nszf + sz) * g.ntx + tx);
+        for (int sz = 0; sz < nszb; sz++)
+            for (int tx = 0; tx < g.ntx; tx++)
+                la[fwi_adj_strip_interior(g, c->shots_h[s], tx * FWI_TX, sz * br, br) ? 0 : 1].push_back(
+                    (s * nszb + sz) * g.ntx + tx);
+    }
+    for (int q = 0; q < 2; q++) c->adj_pre[q][nshots] = (int)la[q].size();
+    for (int q = 0; q < 2; q++) {
+        c->n_fwd[q] = (int)lf[q].size();
+        c->n_adj[q] = (int)la[q].size();
+        if (c->n_fwd[q])
+            HIPCHK(c, hipMemcpyAsync(c->tl_fwd[q], lf[q].data(), sizeof(int) * lf[q].size(), hipMemcpyHostToDevice,
+                                     c->st));
+        if (c->n_adj[q])
+            HIPCHK(c, hipMemcpyAsync(c->tl_adj[q], la[q].data(), sizeof(int) * la[q].size(), hipMemcpyHostToDevice,
+                                     c->st));
+    }
+    c->rec_lists_for = -1;
+    c->bwd_lists_for = -1;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return FWI_HIP_OK;
+}
+
 extern "C" int fwi_hip_set_shots(fwi_hip_ctx *c, int nshots, const fwi_hip_shot *shots)
 {
     if (!c || !shots) return set_err(c, FWI_HIP_EINVAL, "set_shots: NULL argument");
@@ -773,34 +830,6 @@ extern "C" int fwi_hip_set_shots(fwi_hip_ctx *c, int nshots, const fwi_hip_shot 
     HIPCHK(c, hipMemcpyAsync(c->rmap, rmap.data(), sizeof(int) * rmap.size(), hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipMemcpyAsync(c->rrow, rrow.data(), sizeof(int) * rrow.size(), hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipMemcpyAsync(c->shots_d, sd.data(), sizeof(FwiShotDev) * nshots, hipMemcpyHostToDevice, c->st));
-    // strip lists of the forward and adjoint tile kernels (logical strip (s nsz + sz) ntx + tx)
-    {
-        const int nsz = (g.pnz + FWI_STRIP - 1) / FWI_STRIP;
-        std::vector<int> lf[2], la[2];
-        for (int q = 0; q < 2; q++) c->adj_pre[q].assign(nshots + 1, 0);
-        for (int s = 0; s < nshots; s++) {
-            for (int q = 0; q < 2; q++) c->adj_pre[q][s] = (int)la[q].size();
-            for (int sz = 0; sz < nsz; sz++)
-                for (int tx = 0; tx < g.ntx; tx++) {
-                    const int x0 = tx * FWI_TX, r0 = sz * FWI_STRIP, l = (s * nsz + sz) * g.ntx + tx;
-                    lf[fwi_fwd_strip_interior(g, sd[s], x0, r0) ? 0 : 1].push_back(l);
-                    la[fwi_adj_strip_interior(g, sd[s], x0, r0) ? 0 : 1].push_back(l);
-                }
-        }
-        for (int q = 0; q < 2; q++) c->adj_pre[q][nshots] = (int)la[q].size();
-        for (int q = 0; q < 2; q++) {
-            c->n_fwd[q] = (int)lf[q].size();
-            c->n_adj[q] = (int)la[q].size();
-            if (c->n_fwd[q])
-                HIPCHK(c, hipMemcpyAsync(c->tl_fwd[q], lf[q].data(), sizeof(int) * lf[q].size(), hipMemcpyHostToDevice,
-                                         c->st));
-            if (c->n_adj[q])
-                HIPCHK(c, hipMemcpyAsync(c->tl_adj[q], la[q].data(), sizeof(int) * la[q].size(), hipMemcpyHostToDevice,
-                                         c->st));
-        }
-        c->rec_lists_for = -1;
-        c->bwd_lists_for = -1;
-    }
     HIPCHK(c, hipStreamSynchronize(c->st));
     if (c->gx_fwd) (void)hipGraphExecDestroy(c->gx_fwd);
     c->gx_fwd = nullptr;
@@ -811,33 +840,62 @@ extern "C" int fwi_hip_set_shots(fwi_hip_ctx *c, int nshots, const fwi_hip_shot 
     c->ntrace = toff;
     c->ngroups = (nshots + c->sg - 1) / c->sg;
     c->have_fwd = c->have_obs = c->have_res = false;
-    return FWI_HIP_OK;
+    return build_strip_lists(c);
 }
 
 // dynamic LDS per interior workgroup so that int_wgs of them, and no more,
 // fit on a CU beside one general workgroup (0 when not capped)
-static int interior_lds(const fwi_hip_ctx *c)
+static int interior_lds(const fwi_hip_ctx *c, int fr, int n)
 {
-    if (!c->concurrent || c->int_wgs <= 0 || c->lds_cu <= 0) return 0;
+    if (!c->concurrent || n <= 0 || c->lds_cu <= 0) return 0;
     // the least LDS per interior workgroup that keeps n + 1 of them off a CU,
     // so that the rest holds as many general workgroups as it can
-    const int n = c->int_wgs;
     const int total = (c->lds_cu / (n + 1) + 256) & ~255;
-    if (total * n + FWI_LDS_GEN > c->lds_cu || total * (n + 1) <= c->lds_cu || total <= FWI_LDS_TILE) return 0;
-    return total - FWI_LDS_TILE;
+    if (total * n + fwi_lds_gen(fr) > c->lds_cu || total * (n + 1) <= c->lds_cu || total <= fwi_lds_tile(fr))
+        return 0;
+    return total - fwi_lds_tile(fr);
 }
 
-static FwiPair pair_of(const fwi_hip_ctx *c)
+// streams and occupancy cap of a pass (0 forward, 1 backward)
+static FwiPair pair_of(const fwi_hip_ctx *c, int pass)
 {
+    const int fr = pass ? c->strip_bwd : c->strip_fwd;
     FwiPair p;
     p.st = c->st;
     p.st2 = c->concurrent ? c->st2 : nullptr;
     p.fork = c->ev_fork;
     p.join = c->ev_join;
-    p.lds_int = interior_lds(c);
+    p.lds_int = interior_lds(c, fr, pass ? c->int_wgs_bwd : c->int_wgs_fwd);
     p.st3 = c->concurrent ? c->st3 : nullptr;
     p.join3 = c->ev_join3;
     return p;
+}
+
+// the tile kernels of the strip height of the pass (fwi_hip_set_pass_shape)
+static hipError_t fwd_tile(const fwi_hip_ctx *c, const FwiFwdArgs *a, const FwiFwdArgs *a_dev, const int *const *tl,
+                           const int *ntl, const int *kbase, int koff, const FwiPair &p, int mode)
+{
+    return c->strip_fwd == 4 ? fwi_launch_fwd_tile_s4(a, a_dev, tl, ntl, kbase, koff, p, mode)
+                             : fwi_launch_fwd_tile(a, a_dev, tl, ntl, kbase, koff, p, mode);
+}
+static hipError_t adj_tile(const fwi_hip_ctx *c, const FwiAdjArgs *a, const FwiAdjArgs *a_dev, const int *const *tl,
+                           const int *ntl, const int *kbase, int koff, const FwiPair &p)
+{
+    return c->strip_bwd == 4 ? fwi_launch_adj_tile_s4(a, a_dev, tl, ntl, kbase, koff, p)
+                             : fwi_launch_adj_tile(a, a_dev, tl, ntl, kbase, koff, p);
+}
+static hipError_t rec_tile(const fwi_hip_ctx *c, const FwiRecArgs *a, const FwiRecArgs *a_dev, const int *const *tl,
+                           const int *ntl, const int *kbase, int koff, int gsize, const FwiPair &p)
+{
+    return c->strip_bwd == 4 ? fwi_launch_rec_tile_s4(a, a_dev, tl, ntl, kbase, koff, gsize, p)
+                             : fwi_launch_rec_tile(a, a_dev, tl, ntl, kbase, koff, gsize, p);
+}
+static hipError_t bwd_tile(const fwi_hip_ctx *c, const FwiAdjArgs *a, const FwiAdjArgs *a_dev, const FwiRecArgs *r_dev,
+                           const int *const *tl, const int *ntl, const int *kbase, int koff, int gsize,
+                           const FwiPair &p)
+{
+    return c->strip_bwd == 4 ? fwi_launch_bwd_tile_s4(a, a_dev, r_dev, tl, ntl, kbase, koff, gsize, p)
+                             : fwi_launch_bwd_tile(a, a_dev, r_dev, tl, ntl, kbase, koff, gsize, p);
 }
 
 static FwiBatchDev batch_dev(fwi_hip_ctx *c)
@@ -1248,13 +1306,28 @@ extern "C" int fwi_hip_set_concurrent(fwi_hip_ctx *c, int on)
     return FWI_HIP_OK;
 }
 
+extern "C" int fwi_hip_set_pass_shape(fwi_hip_ctx *c, int pass, int rows, int interior_wgs)
+{
+    if (!c || (pass != 0 && pass != 1) || (rows != 4 && rows != 8) || interior_wgs < 0)
+        return set_err(c, FWI_HIP_EINVAL, "set_pass_shape: need pass 0 or 1, rows 4 or 8, interior_wgs >= 0 (got %d, %d, %d)",
+                       pass, rows, interior_wgs);
+    int &r = pass ? c->strip_bwd : c->strip_fwd, &w = pass ? c->int_wgs_bwd : c->int_wgs_fwd;
+    if (rows == r && interior_wgs == w) return FWI_HIP_OK;
+    if (c->gx_fwd) (void)hipGraphExecDestroy(c->gx_fwd);
+    c->gx_fwd = nullptr;
+    drop_bwd(c);  // captured launches hold the kernels, lists and LDS size
+    w = interior_wgs;
+    if (rows == r) return FWI_HIP_OK;
+    r = rows;
+    return c->nshots > 0 ? build_strip_lists(c) : FWI_HIP_OK;
+}
 extern "C" int fwi_hip_set_interior_wgs(fwi_hip_ctx *c, int wgs)
 {
     if (!c || wgs < 0) return set_err(c, FWI_HIP_EINVAL, "set_interior_wgs: need wgs >= 0");
     if (c->gx_fwd) (void)hipGraphExecDestroy(c->gx_fwd);
     c->gx_fwd = nullptr;
     drop_bwd(c);  // captured launches hold the LDS size
-    c->int_wgs = wgs;
+    c->int_wgs_fwd = c->int_wgs_bwd = wgs;
     return FWI_HIP_OK;
 }
 
@@ -1462,7 +1535,7 @@ static int forward_impl(fwi_hip_ctx *c, float *traces, bool tm, fwi_hip_block_fn
         for (long long j = 0; j * K < c->kmax; j++) {
             if ((rc = ck_save(c, j))) return rc;
             for (int k = (int)(j * K); k < std::min((long long)c->kmax, (j + 1) * K); k++)
-                HIPCHK(c, fwi_launch_fwd_tile(&a, ad, c->tl_fwd, c->n_fwd, nullptr, k, pair_of(c), fmode));
+                HIPCHK(c, fwd_tile(c, &a, ad, c->tl_fwd, c->n_fwd, nullptr, k, pair_of(c, 0), fmode));
         }
     } else if (method == 2) {
         // the binomial schedule's first sweep: S_0 in slot 0, its snapshots in
@@ -1478,7 +1551,7 @@ static int forward_impl(fwi_hip_ctx *c, float *traces, bool tm, fwi_hip_block_fn
                 if ((rc = ck_save_at(c, x.slot, k, false))) return rc;
             }
             if (k == c->kmax - 1 && (rc = ck_save_at(c, snaps, k, false))) return rc;
-            HIPCHK(c, fwi_launch_fwd_tile(&a, ad, c->tl_fwd, c->n_fwd, nullptr, k, pair_of(c), fmode));
+            HIPCHK(c, fwd_tile(c, &a, ad, c->tl_fwd, c->n_fwd, nullptr, k, pair_of(c, 0), fmode));
         }
     } else if (B > 0 && c->timing != 1) {
         if (!c->gx_fwd) {
@@ -1486,7 +1559,7 @@ static int forward_impl(fwi_hip_ctx *c, float *traces, bool tm, fwi_hip_block_fn
                 hipError_t e = hipSuccess;
                 for (int j = 0; j < B && e == hipSuccess; j++)
                     e = c->fwd_kind ? fwi_launch_fwd(&a, c->kbase, j, nblocks, c->st)
-                                    : fwi_launch_fwd_tile(&a, ad, c->tl_fwd, c->n_fwd, c->kbase, j, pair_of(c), fmode);
+                                    : fwd_tile(c, &a, ad, c->tl_fwd, c->n_fwd, c->kbase, j, pair_of(c, 0), fmode);
                 if (e == hipSuccess) e = fwi_launch_advance(c->kbase, B, c->st);
                 return e;
             });
@@ -1510,7 +1583,7 @@ static int forward_impl(fwi_hip_ctx *c, float *traces, bool tm, fwi_hip_block_fn
             hipEvent_t e0, e1;
             if ((rc = tick(c, &e0))) return rc;
             HIPCHK(c, c->fwd_kind ? fwi_launch_fwd(&a, nullptr, k, nblocks, c->st)
-                                  : fwi_launch_fwd_tile(&a, ad, c->tl_fwd, c->n_fwd, nullptr, k, pair_of(c), fmode));
+                                  : fwd_tile(c, &a, ad, c->tl_fwd, c->n_fwd, nullptr, k, pair_of(c, 0), fmode));
             if ((rc = tick(c, &e1))) return rc;
             if (c->timing == 1) marks.push_back({0, {e0, e1}});
             if (marks.size() >= 512 && (rc = collect_timing(c, marks))) return rc;
@@ -1664,16 +1737,17 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
     if (tile && c->rec_lists_for != rgs) {
         // items (group, tile) of the reconstruction kernel; interior when the tile is
         // interior for every shot of the group
-        const int ntz = (g.nz + FWI_REC_ROWS - 1) / FWI_REC_ROWS, ntiles = g.bntx * ntz;
+        const int rr = c->strip_bwd;  // reconstruction rows per wave = the strip height
+        const int ntz = (g.nz + rr - 1) / rr, ntiles = g.bntx * ntz;
         std::vector<int> lr[2];
         for (int q = 0; q < 2; q++) c->rec_pre[q].assign(ngr + 1, 0);
         for (int grp = 0; grp < ngr; grp++) {
             for (int q = 0; q < 2; q++) c->rec_pre[q][grp] = (int)lr[q].size();
             for (int t = 0; t < ntiles; t++) {
-                const int bx0 = (t % g.bntx) * FWI_TX, z0 = (t / g.bntx) * FWI_REC_ROWS;
+                const int bx0 = (t % g.bntx) * FWI_TX, z0 = (t / g.bntx) * rr;
                 bool in = true;
                 for (int s = grp * rgs; s < std::min((grp + 1) * rgs, c->nshots) && in; s++)
-                    in = fwi_rec_tile_interior(g, c->shots_h[s], bx0, z0);
+                    in = fwi_rec_tile_interior(g, c->shots_h[s], bx0, z0, rr);
                 lr[in ? 0 : 1].push_back(grp * ntiles + t);
             }
         }
@@ -1692,7 +1766,7 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
         // a tile of the padded grid is fused for a shot group when it is
         // interior for the adjoint and for the reconstruction of every shot of
         // the group; the other tiles run as adjoint strips + reconstruction items
-        const int nsz = (g.pnz + FWI_STRIP - 1) / FWI_STRIP, ntiles = g.ntx * nsz;
+        const int br = c->strip_bwd, nsz = (g.pnz + br - 1) / br, ntiles = g.ntx * nsz;
         std::vector<char> fz((size_t)ngr * ntiles);
         std::vector<int> lb[4];
         for (int q = 0; q < 4; q++) c->bwd_pre[q].assign((q == 1 || q == 2 ? c->nshots : ngr) + 1, 0);
@@ -1700,11 +1774,11 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
             c->bwd_pre[0][grp] = (int)lb[0].size();
             c->bwd_pre[3][grp] = (int)lb[3].size();
             for (int t = 0; t < ntiles; t++) {
-                const int x0 = (t % g.ntx) * FWI_TX, r0 = (t / g.ntx) * FWI_STRIP;
+                const int x0 = (t % g.ntx) * FWI_TX, r0 = (t / g.ntx) * br;
                 bool in = true;
                 for (int s = grp * rgs; s < std::min((grp + 1) * rgs, c->nshots) && in; s++)
-                    in = fwi_adj_strip_interior(g, c->shots_h[s], x0, r0) &&
-                         fwi_rec_tile_interior(g, c->shots_h[s], x0 - g.npml, r0);
+                    in = fwi_adj_strip_interior(g, c->shots_h[s], x0, r0, br) &&
+                         fwi_rec_tile_interior(g, c->shots_h[s], x0 - g.npml, r0, br);
                 fz[(size_t)grp * ntiles + t] = in;
                 if (in) lb[0].push_back(grp * ntiles + t);
                 else if (r0 < g.nz) lb[3].push_back(grp * ntiles + t);
@@ -1722,7 +1796,7 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
             for (int t = 0; t < ntiles; t++) {
                 if (fz[(size_t)(s / rgs) * ntiles + t]) continue;
                 const int x0 = (t % g.ntx) * FWI_TX, sz = t / g.ntx;
-                const int cls = fwi_adj_strip_class(g, c->shots_h[s], x0, sz * FWI_STRIP);
+                const int cls = fwi_adj_strip_class(g, c->shots_h[s], x0, sz * br, br);
                 const bool corner = (cls & FWI_CLS_X) && (cls & (FWI_CLS_Z | FWI_CLS_RS));
                 lb[corner ? 2 : 1].push_back((s * nsz + sz) * g.ntx + t % g.ntx);
             }
@@ -1771,7 +1845,7 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
                 const RvAct &x = c->rv_sched[i];
                 if (x.op == RV_ADVANCE) {
                     for (int k = x.a; k < x.b; k++)
-                        HIPCHK(c, fwi_launch_fwd_tile(&f, c->fwd_rc_d, c->tl_fwd, c->n_fwd, nullptr, k, pair_of(c),
+                        HIPCHK(c, fwd_tile(c, &f, c->fwd_rc_d, c->tl_fwd, c->n_fwd, nullptr, k, pair_of(c, 0),
                                                       FWI_FWD_PLAIN));
                     rec += x.b - x.a;
                 } else if (x.op == RV_TAKESHOT) {
@@ -1781,9 +1855,9 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
                 } else {  // turn: X^k from S_k, then the adjoint step k and its image
                     const int k = x.a;
                     HIPCHK(c, fwi_launch_set_int(k0_dev, k, c->st));
-                    HIPCHK(c, fwi_launch_fwd_tile(&f, c->fwd_rc_d, c->tl_fwd, c->n_fwd, nullptr, k, pair_of(c),
+                    HIPCHK(c, fwd_tile(c, &f, c->fwd_rc_d, c->tl_fwd, c->n_fwd, nullptr, k, pair_of(c, 0),
                                                   FWI_FWD_XSEG));
-                    HIPCHK(c, fwi_launch_adj_tile(&a, ad, c->tl_adj, c->n_adj, nullptr, k, pair_of(c)));
+                    HIPCHK(c, adj_tile(c, &a, ad, c->tl_adj, c->n_adj, nullptr, k, pair_of(c, 1)));
                     HIPCHK(c, fwi_launch_img_seg(&r, rd, c->ck_xseg, k, rgs, ngr, c->st));
                     rec++;
                 }
@@ -1795,10 +1869,10 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
             rec += k1 - k0;
             HIPCHK(c, fwi_launch_set_int(k0_dev, k0, c->st));
             for (int k = k0; k < k1; k++)
-                HIPCHK(c, fwi_launch_fwd_tile(&f, c->fwd_rc_d, c->tl_fwd, c->n_fwd, nullptr, k, pair_of(c),
+                HIPCHK(c, fwd_tile(c, &f, c->fwd_rc_d, c->tl_fwd, c->n_fwd, nullptr, k, pair_of(c, 0),
                                               FWI_FWD_XSEG));
             for (int k = k1 - 1; k >= k0; k--) {
-                HIPCHK(c, fwi_launch_adj_tile(&a, ad, c->tl_adj, c->n_adj, nullptr, k, pair_of(c)));
+                HIPCHK(c, adj_tile(c, &a, ad, c->tl_adj, c->n_adj, nullptr, k, pair_of(c, 1)));
                 HIPCHK(c, fwi_launch_img_seg(&r, rd, c->ck_xseg + (long long)(k - k0) * S * nbb, k, rgs, ngr, c->st));
             }
         }
@@ -1848,15 +1922,15 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
             }
         }
         auto launch_adj = [&](const int *kb, int koff) {
-            return tile ? fwi_launch_adj_tile(&a, ad, ta, na, kb, koff, pair_of(c))
+            return tile ? adj_tile(c, &a, ad, ta, na, kb, koff, pair_of(c, 1))
                         : fwi_launch_adj(&a, kb, koff, nb_adj, c->st);
         };
         auto launch_rec = [&](const int *kb, int koff) {
-            return tile ? fwi_launch_rec_tile(&r, rd, tr, nr, kb, koff, rgs, pair_of(c))
+            return tile ? rec_tile(c, &r, rd, tr, nr, kb, koff, rgs, pair_of(c, 1))
                         : fwi_launch_rec(&r, kb, koff, nb_rec, c->st);
         };
         auto launch_bwd = [&](const int *kb, int koff) {
-            return fwi_launch_bwd_tile(&a, ad, rd, tb, nbw, kb, koff, rgs, pair_of(c));
+            return bwd_tile(c, &a, ad, rd, tb, nbw, kb, koff, rgs, pair_of(c, 1));
         };
         // overlapped form: adjoint k on st, imaging k+1 on st2, joined per step;
         // k runs down to -1 so that the last pair images step 0 (kernels
@@ -1867,8 +1941,8 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
                     p2 = {c->st2, nullptr, c->ev_fork, c->ev_join, 0, nullptr, nullptr};
             hipError_t e = hipEventRecord(c->ev_fork2, c->st);
             if (e == hipSuccess) e = hipStreamWaitEvent(c->st2, c->ev_fork2, 0);
-            if (e == hipSuccess) e = fwi_launch_rec_tile(&r, rd, tr, nr, kb, koff + 1, rgs, p2);
-            if (e == hipSuccess) e = fwi_launch_adj_tile(&a, ad, ta, na, kb, koff, p1);
+            if (e == hipSuccess) e = rec_tile(c, &r, rd, tr, nr, kb, koff + 1, rgs, p2);
+            if (e == hipSuccess) e = adj_tile(c, &a, ad, ta, na, kb, koff, p1);
             if (e == hipSuccess) e = hipEventRecord(c->ev_join2, c->st2);
             if (e == hipSuccess) e = hipStreamWaitEvent(c->st, c->ev_join2, 0);
             return e;

@@ -26,6 +26,21 @@
 #include <type_traits>
 #include "fwi_dev.h"
 
+// This file is compiled twice: with the default 8-row strips, and as the
+// 4-row variant (FWI_TZ2 = 16, FWI_REC_ROWS = 4, FWI_VARIANT = _s4,
+// FWI_SECONDARY).  Each build keeps its kernels in a namespace of its own and
+// suffixes the launchers of the strip-height-dependent tile kernels; the
+// other launchers come from the default build only.
+#ifndef FWI_VARIANT
+#define FWI_VARIANT
+#endif
+#define FWI_CAT2(a, b) a##b
+#define FWI_CAT(a, b) FWI_CAT2(a, b)
+#define FWI_VNAME(name) FWI_CAT(name, FWI_VARIANT)
+#define FWI_KNS FWI_CAT(fwik, FWI_VARIANT)
+
+namespace FWI_KNS {
+
 #define TX FWI_TX
 
 __device__ __forceinline__ void wave_sync()
@@ -1437,6 +1452,9 @@ __device__ __forceinline__ void adj_tile_rows(const FwiAdjArgs &A, const FwiShot
         if (FUSED) mk[d] = ek[d];
         else mk[d] = EDGE ? buf_ld_row(rk, vo, i, pnz, rowb) : buf_ld(rk, vo, i * rowb);
     }
+    float m0 = 0.f;  // row 0 (strips shorter than 6 rows: the free-surface rule below)
+    if constexpr (FR <= 5)
+        if (EDGE) m0 = buf_ld(rnow, vo, 0);
 
     // chi after step k goes to buffer (k & 1); it was read from buffer ((k+1) & 1).
     // One buffer resource per array over both buffers of the shot: the lane's
@@ -1622,10 +1640,14 @@ __device__ __forceinline__ void adj_tile_rows(const FwiAdjArgs &A, const FwiShot
                 if (jn >= 1 && jn < pnz - 1) t -= c0z * a[n + 2];
                 if (jn - 2 >= 1 && jn - 2 < pnz - 1) t += c1z * a[n];
                 if (jn + 1 >= 1 && jn + 1 < pnz - 1) t -= c1z * a[n + 3];
-                if (r0 == 0 && n >= 1 && n <= 5) {  // vz row 0 (free surface) reads v nodes 0..4
+                // vz row 0 (free surface) reads v nodes 0..4, so those nodes
+                // take a term of row 0: on the strip at r0 = 0, and with
+                // strips of fewer than 6 rows on the next one too (row 0 is
+                // then outside a[], read as m0: no z CPML at row 0)
+                if (r0 <= 5 && jn >= 0 && jn <= 4) {
                     const double i24 = (double)g.iz24;
-                    const double w0 = (n == 1) ? -22. : (n == 2) ? 17. : (n == 3) ? 9. : (n == 4) ? -5. : 1.;
-                    t += (float)(w0 * i24) * a[3];
+                    const double w0 = (jn == 0) ? -22. : (jn == 1) ? 17. : (jn == 2) ? 9. : (jn == 3) ? -5. : 1.;
+                    t += (float)(w0 * i24) * (r0 == 0 ? a[3] : m0);
                 }
                 if (jn == pnz) t += izf * a[n + 1];
                 if (jn == pnz - 1) t -= izf * a[n + 2];
@@ -2719,8 +2741,12 @@ static hipError_t launch_fwd_tile_mode(const FwiFwdArgs *a, const FwiFwdArgs *a_
         });
 }
 
+}  // namespace FWI_KNS
+using namespace FWI_KNS;
+
 extern "C" {
 
+#ifndef FWI_SECONDARY
 hipError_t fwi_launch_fwd(const FwiFwdArgs *a, const int *kbase, int koff, int nblocks, hipStream_t st)
 {
     hipLaunchKernelGGL(k_fwd_step, dim3(((nblocks + 7) / 8) * 8), dim3(FWI_TX * a->b.sg), 0, st, *a, kbase, koff);
@@ -2728,8 +2754,10 @@ hipError_t fwi_launch_fwd(const FwiFwdArgs *a, const int *kbase, int koff, int n
 }
 
 // tile lists: [0] interior tiles, [1] general tiles (device arrays and sizes)
-hipError_t fwi_launch_fwd_tile(const FwiFwdArgs *a, const FwiFwdArgs *a_dev, const int *const *tl, const int *ntl,
-                               const int *kbase, int koff, const FwiPair &p, int mode)
+#endif
+
+hipError_t FWI_VNAME(fwi_launch_fwd_tile)(const FwiFwdArgs *a, const FwiFwdArgs *a_dev, const int *const *tl,
+                                          const int *ntl, const int *kbase, int koff, const FwiPair &p, int mode)
 {
     if (mode == FWI_FWD_TRACES) return launch_fwd_tile_mode<FWI_FWD_TRACES>(a, a_dev, tl, ntl, kbase, koff, p);
     if (mode == FWI_FWD_XSEG) return launch_fwd_tile_mode<FWI_FWD_XSEG>(a, a_dev, tl, ntl, kbase, koff, p);
@@ -2737,8 +2765,8 @@ hipError_t fwi_launch_fwd_tile(const FwiFwdArgs *a, const FwiFwdArgs *a_dev, con
     return launch_fwd_tile_mode<FWI_FWD_FRAMES>(a, a_dev, tl, ntl, kbase, koff, p);
 }
 
-hipError_t fwi_launch_adj_tile(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, const int *const *tl, const int *ntl,
-                               const int *kbase, int koff, const FwiPair &p)
+hipError_t FWI_VNAME(fwi_launch_adj_tile)(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, const int *const *tl,
+                                          const int *ntl, const int *kbase, int koff, const FwiPair &p)
 {
     return launch_pair(
         p, ntl[0] > 0, ntl[1] > 0,
@@ -2752,8 +2780,8 @@ hipError_t fwi_launch_adj_tile(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, con
         });
 }
 
-hipError_t fwi_launch_rec_tile(const FwiRecArgs *a, const FwiRecArgs *a_dev, const int *const *tl, const int *ntl,
-                               const int *kbase, int koff, int gsize, const FwiPair &p)
+hipError_t FWI_VNAME(fwi_launch_rec_tile)(const FwiRecArgs *a, const FwiRecArgs *a_dev, const int *const *tl,
+                                          const int *ntl, const int *kbase, int koff, int gsize, const FwiPair &p)
 {
     return launch_pair(
         p, ntl[0] > 0, ntl[1] > 0,
@@ -2775,9 +2803,9 @@ hipError_t fwi_launch_rec_tile(const FwiRecArgs *a, const FwiRecArgs *a_dev, con
 // p^{k-1} and the image and read only the other side's previous-step fields.
 // With p.st3 as well, the corner strips run on st3 beside the other adjoint
 // strips, joined into st2 before the reconstruction.
-hipError_t fwi_launch_bwd_tile(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, const FwiRecArgs *r_dev,
-                               const int *const *tl, const int *ntl, const int *kbase, int koff, int gsize,
-                               const FwiPair &p)
+hipError_t FWI_VNAME(fwi_launch_bwd_tile)(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, const FwiRecArgs *r_dev,
+                                          const int *const *tl, const int *ntl, const int *kbase, int koff,
+                                          int gsize, const FwiPair &p)
 {
     auto launch_int = [&](hipStream_t st) {
         hipLaunchKernelGGL(k_bwd_tile_int, dim3(((ntl[0] + 7) / 8) * 8), dim3(256), p.lds_int, st, a_dev, r_dev,
@@ -2821,6 +2849,7 @@ hipError_t fwi_launch_bwd_tile(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, con
     });
 }
 
+#ifndef FWI_SECONDARY
 hipError_t fwi_launch_adj(const FwiAdjArgs *a, const int *kbase, int koff, int nblocks, hipStream_t st)
 {
     hipLaunchKernelGGL(k_adj_step, dim3(((nblocks + 7) / 8) * 8), dim3(FWI_TX * a->b.sg), 0, st, *a, kbase, koff);
@@ -2958,5 +2987,7 @@ hipError_t fwi_launch_img_seg(const FwiRecArgs *a, const FwiRecArgs *a_dev, cons
                        ngr);
     return hipGetLastError();
 }
+
+#endif  // FWI_SECONDARY
 
 }  // extern "C"

@@ -218,8 +218,9 @@ int fwi_hip_set_concurrent(fwi_hip_ctx *ctx, int on);
 
 /* With the general strips on the second stream: at most `wgs` workgroups of
  * the interior launches (forward, fused backward) per CU, by reserving unused
- * LDS in each, so that a general workgroup fits beside them (default 3; 0:
- * no cap).  Results are bit-identical.                                      */
+ * LDS in each, so that a general workgroup fits beside them (0: no cap), in
+ * both passes (defaults 6 forward, 4 backward; set_pass_shape sets one pass).
+ * Results are bit-identical.                                                */
 int fwi_hip_set_interior_wgs(fwi_hip_ctx *ctx, int wgs);
 
 /* Backward step of the tile kernels fused into one kernel (on = 1, the
@@ -251,6 +252,17 @@ int fwi_hip_set_bwd_chunk(fwi_hip_ctx *ctx, int shots);
 /* Shots per workgroup of the tile reconstruction + imaging kernel (default
  * 8): the group's image sum meets the FP64 accumulator once per step.     */
 int fwi_hip_set_rec_group(fwi_hip_ctx *ctx, int shots);
+
+/* Tile shape of one pass (0: the forward pass and the recompute of the
+ * checkpointed backward pass; 1: the backward pass): rows per wave of the
+ * tile kernels, 4 or 8, and the interior workgroups per CU beside the
+ * general strips (as set_interior_wgs, which sets both passes).  Defaults:
+ * 4 rows in both passes, 6 forward and 4 backward workgroups per CU.
+ * A workgroup of 4 waves covers 64 x 4*rows cells; shorter strips take fewer
+ * registers and re-read more halo rows.  Traces, adjoint fields and images
+ * are bit-identical across the shapes.  No counterpart in the reference (its
+ * loops have no tiles); the autotuner picks it per workload shape.          */
+int fwi_hip_set_pass_shape(fwi_hip_ctx *ctx, int pass, int rows, int interior_wgs);
 
 /* Wait for all work of the context. */
 int fwi_hip_synchronize(fwi_hip_ctx *ctx);

@@ -14,9 +14,9 @@ Tolerances (north star): seismograms rel-L2 <= 1e-5 and gradients rel-L2 <=
     residual that is nonzero for those four shots only (the other 60 shots
     back-propagate zero and image exactly zero).
   * the same workload at its full nt=4000: the default form equals the
-    one-stream separate kernels bit for bit, the z-marching kernels up to the
-    FP64 order of the image sum, and the batch equals the sum of its 64
-    single-shot runs.
+    one-stream separate kernels and the 4-row strips bit for bit, the
+    z-marching kernels up to the FP64 order of the image sum, and the batch
+    equals the sum of its 64 single-shot runs.
   * 1536x400 shots with receiver apertures and uneven nt through the drop-in
     grad_all_shots with the host Butterworth band-pass.
   * 16-bit checkpointed snapshots at 1024x320 (2 shots) against FP64.
@@ -128,6 +128,16 @@ def test_config2_full_nt_forms_and_single_shots(lib):
     np.testing.assert_array_equal(tr1, tr)
     np.testing.assert_array_equal(g1, grad)
     assert J1 == J
+    # the tile kernels with 4-row strips in both passes (the default form otherwise)
+    ctx.set_concurrent(True)
+    ctx.set_bwd_fused(True)
+    ctx.set_pass_shape(0, 4, 3)
+    ctx.set_pass_shape(1, 4, 3)
+    tr4, J4, g4 = gradient(ctx, obs)
+    np.testing.assert_array_equal(tr4, tr)
+    np.testing.assert_array_equal(g4, grad)
+    ctx.set_pass_shape(0, 8, 3)
+    ctx.set_pass_shape(1, 8, 3)
     # z-marching kernels: the same traces, the image summed over shots in another order
     ctx.set_kernels(1)
     tr2, J2, g2 = gradient(ctx, obs)

@@ -329,6 +329,74 @@ def test_concurrent_general_strips_give_the_same_results(lib, graph):
         ctx.close()
 
 
+@pytest.mark.parametrize("graph", [100, 0])
+def test_strip_rows_give_the_same_results(lib, graph):
+    """The tile kernels built with 4-row strips, in either pass or both, give
+    the 8-row traces and image bit for bit: fused and separate backward steps,
+    one and several streams, graph replays, and the checkpointed backward pass
+    (its recompute runs the forward pass's strips).  The shallow sources and
+    receivers put the free-surface rows, whose transposed rule reaches row 0
+    from the second 4-row strip, into general strips of every kind."""
+    g, vel, shots = small_case(nshots=3, nt=230, seed=41)
+    rng = np.random.default_rng(42)
+    res = [rng.standard_normal((s.nr, s.nt)).astype(np.float32) for s in shots]
+    ctx = lib.Context(g, max_shots=3, max_nt=230, max_nr=100)
+    ctx.set_graph(graph)
+    ctx.set_rec_group(2)
+    ctx.set_model(vel)
+    ctx.set_shots(shots)
+
+    def run():
+        out = []
+        for _ in range(2):                      # a second pass replays the captured graphs
+            tr = ctx.forward()
+            ctx.set_residual(ctx.pack(res))
+            out.append((tr, ctx.adjoint()[1]))
+        return out
+
+    try:
+        for fused in (True, False):
+            for concurrent in (True, False):
+                ctx.set_bwd_fused(fused)
+                ctx.set_concurrent(concurrent)
+                ctx.set_pass_shape(0, 8, 3)
+                ctx.set_pass_shape(1, 8, 3)
+                ref = run()
+                np.testing.assert_array_equal(ref[1][0], ref[0][0])
+                np.testing.assert_array_equal(ref[1][1], ref[0][1])
+                for rows in ((4, 4), (4, 8), (8, 4)):
+                    ctx.set_pass_shape(0, rows[0], 5 if rows[0] == 4 else 3)
+                    ctx.set_pass_shape(1, rows[1], 4 if rows[1] == 4 else 3)
+                    for tr, gk in run():
+                        np.testing.assert_array_equal(tr, ref[0][0], err_msg=f"{rows} fused={fused}")
+                        np.testing.assert_array_equal(gk, ref[0][1], err_msg=f"{rows} fused={fused}")
+        ctx.set_bwd_fused(True)
+        ctx.set_concurrent(True)
+        ctx.set_checkpointing(50)
+        ctx.set_pass_shape(0, 8, 3)
+        ctx.set_pass_shape(1, 8, 3)
+        ref = run()
+        for rows in ((4, 4), (4, 8), (8, 4)):
+            ctx.set_pass_shape(0, rows[0], 3)
+            ctx.set_pass_shape(1, rows[1], 3)
+            for tr, gk in run():
+                np.testing.assert_array_equal(tr, ref[0][0], err_msg=f"checkpointed {rows}")
+                np.testing.assert_array_equal(gk, ref[0][1], err_msg=f"checkpointed {rows}")
+    finally:
+        ctx.close()
+
+
+def test_pass_shape_refuses_other_shapes(lib):
+    g, vel, shots = small_case(nshots=1, nt=50, seed=43)
+    ctx = lib.Context(g, max_shots=1, max_nt=50, max_nr=100)
+    try:
+        for args in ((0, 2, 3), (1, 16, 3), (0, 6, 3), (2, 8, 3), (0, 8, -1)):
+            with pytest.raises(lib.FwiError):
+                ctx.set_pass_shape(*args)
+    finally:
+        ctx.close()
+
+
 @pytest.mark.parametrize("kind", [0, 1])
 def test_imaging_window_is_a_partition_of_the_image(lib, kind):
     """The image over every step equals the sum of the images over the even
