@@ -47,7 +47,8 @@ def parse():
     ap.add_argument("--group", type=int, default=0, help="shots per workgroup of the z-marching kernels")
     ap.add_argument("--kernels", type=int, default=0, help="0: 2-D tile kernels, 1: z-marching kernels")
     ap.add_argument("--rec-group", type=int, default=0, help="shots per workgroup of the tile imaging kernel")
-    ap.add_argument("--bwd-chunk", type=int, default=-1, help="shots per backward pass (0: all; -1: the default)")
+    ap.add_argument("--bwd-chunk", type=int, default=-1,
+                    help="shots per backward pass (0: all; -1: the library's choice by cache fit)")
     ap.add_argument("--bwd-overlap", type=int, default=-1, help="adjoint beside imaging on two streams (1/0; -1: default)")
     ap.add_argument("--concurrent", type=int, default=-1,
                     help="general strips on a second stream beside the interior launch (1/0; -1: default)")
@@ -704,7 +705,8 @@ def main():
                                f"nt={args.nt}, {nshots} shots x {nr} receivers, npml={g.npml}, free surface",
                    "nx": g.nx, "nz": g.nz, "nt": args.nt, "nshots": nshots, "nr": nr, "npml": g.npml,
                    "launch": launch or {"kernels": args.kernels, "rec_group": args.rec_group or 8,
-                                        "bwd_chunk": max(args.bwd_chunk, 0), "graph": 100,
+                                        "bwd_chunk": "auto" if args.bwd_chunk < 0 else args.bwd_chunk,
+                                        "graph": 100,
                                         "fwd_shape": list(args.fwd_shape or (4, 6)),
                                         "bwd_shape": list(args.bwd_shape or (4, 4))}},
         "gcell_updates_per_s": gcell,

@@ -259,7 +259,7 @@ struct fwi_hip_ctx {
     std::vector<int> bwd_pre[4];  // list entries before group g ([0], [3]) or shot s ([1], [2])
     hipGraphExec_t gx_fwd = nullptr;
     std::vector<hipGraphExec_t> gx_bwd; // one captured block of backward steps per shot chunk
-    int bwd_chunk = 0;                  // shots per backward pass (0: all in one)
+    int bwd_chunk = -1;                 // shots per backward pass (0: all in one, -1: auto_bwd_chunk)
     std::vector<int> adj_pre[2], rec_pre[2]; // list entries before shot s / before group g
     // timing: 0 off, 1 per launch (plain launches), 2 per pass (graphs as usual)
     int timing = 0;
@@ -1460,7 +1460,7 @@ extern "C" int fwi_hip_set_bwd_overlap(fwi_hip_ctx *c, int on)
 
 extern "C" int fwi_hip_set_bwd_chunk(fwi_hip_ctx *c, int shots)
 {
-    if (!c || shots < 0) return set_err(c, FWI_HIP_EINVAL, "set_bwd_chunk: need shots >= 0");
+    if (!c || shots < -1) return set_err(c, FWI_HIP_EINVAL, "set_bwd_chunk: need shots >= -1");
     drop_bwd(c);
     c->bwd_chunk = shots;
     return FWI_HIP_OK;
@@ -1675,6 +1675,24 @@ extern "C" int fwi_hip_set_residual_tm(fwi_hip_ctx *c, const float *res_tm)
     return FWI_HIP_OK;
 }
 
+// Shots per backward pass when the caller leaves it to the library
+// (set_bwd_chunk -1, the default): a step of the backward pass reads the
+// fields the step before wrote, so when the four wavefields of a chunk (mu and
+// p, two buffers each) fit in the device's last-level cache (256 MB Infinity
+// Cache on MI355X; 200 MB of it counted) those reads come from it.  A chunk
+// must still give a step at least 10M cell-updates to fill the GPU.  At
+// 1024x320 x 64 shots that is 32 shots per pass: backward 148.1 -> 132.8 us
+// per step (16 shots per pass: 182 us); at 4096x1024 x 16 shots no chunk fits.
+#define FWI_LLC_USABLE 200.0e6
+static int auto_bwd_chunk(const fwi_hip_ctx *c, int rgs)
+{
+    const double shot_bytes = 4.0 * (double)c->fs * sizeof(float);
+    const int n = (int)(FWI_LLC_USABLE / shot_bytes) / rgs * rgs;
+    if (n <= 0 || n >= c->nshots) return 0;
+    if ((double)n * c->g.pnx * c->g.pnz < 1.0e7) return 0;
+    return n;
+}
+
 extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
 {
     if (!c) return set_err(c, FWI_HIP_EINVAL, "adjoint: NULL context");
@@ -1694,8 +1712,7 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
     // block beside the steps in the graph form of a one-chunk boundary-saving
     // pass, else whole before them; a receiver-major one is transposed here
     const bool tm_res = c->res_host != nullptr;
-    const bool one_chunk = !(tile && c->bwd_chunk > 0 && c->bwd_chunk < c->nshots);
-    const bool pipe_res = tm_res && c->bwd_method == 0 && c->graph_steps > 0 && c->timing != 1 && one_chunk;
+    const bool pipe_res = tm_res && c->bwd_method == 0 && c->graph_steps > 0 && c->timing != 1;
     if (tm_res && !pipe_res)
         HIPCHK(c, hipMemcpyAsync(c->res_tm, c->res_host, sizeof(float) * c->ntrace, hipMemcpyDefault, c->st));
     for (int s = 0; s < c->nshots && !tm_res; s++) {
@@ -1889,8 +1906,8 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
     // shot chunks, each a whole backward pass over its own slice of the
     // strip and item lists (shot-major and group-major), so that a chunk's
     // working set (mu, p and the FP64 images of its shots) can stay cached
-    const int cs = (tile && c->bwd_chunk > 0 && c->bwd_chunk < c->nshots) ? ((c->bwd_chunk + rgs - 1) / rgs) * rgs
-                                                                          : c->nshots;
+    const int chunk = c->bwd_chunk < 0 ? auto_bwd_chunk(c, rgs) : c->bwd_chunk;
+    const int cs = (tile && chunk > 0 && chunk < c->nshots) ? ((chunk + rgs - 1) / rgs) * rgs : c->nshots;
     const int nch = (c->nshots + cs - 1) / cs;
     if ((int)c->gx_bwd.size() != nch) {
         drop_bwd(c);
@@ -1968,9 +1985,18 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
             }
             HIPCHK(c, fwi_launch_set_int(c->kbase, kmax - 1, c->st));
             const int nblk = ovl ? (kmax + B) / B : (kmax + B - 1) / B;
-            if (pipe_res && (rc = ensure_copy(c, nblk))) return rc;
+            // the residual goes up beside the first chunk's blocks (every
+            // shot's rows, so the later chunks find theirs in place); rows of
+            // steps past the first chunk's last one first
+            const bool up = pipe_res && ch == 0;
+            if (up && (rc = ensure_copy(c, nblk + 1))) return rc;
+            if (up && kmax < c->kmax) {
+                HIPCHK(c, copy_rows(c, const_cast<float *>(c->res_host), c->res_tm, kmax, c->kmax, false, c->stc));
+                HIPCHK(c, hipEventRecord(c->blk_copy[nblk], c->stc));
+                HIPCHK(c, hipStreamWaitEvent(c->st, c->blk_copy[nblk], 0));
+            }
             for (int blk = 0; blk < nblk; blk++) {
-                if (pipe_res) {
+                if (up) {
                     // residual rows of the block's steps kmax-1-blk*B down to kmax-(blk+1)*B
                     const int hi = kmax - blk * B, lo = std::max(0, hi - B);
                     if (hi > 0)

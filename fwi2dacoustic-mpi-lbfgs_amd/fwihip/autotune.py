@@ -11,7 +11,8 @@ forward + adjoint passes of each candidate setting and keeps the fastest:
     tile_z       rows per workgroup tile of the z-marching kernels
     group        shots per workgroup of the z-marching kernels
     rec_group    shots per workgroup of the tile imaging kernel
-    bwd_chunk    shots per backward pass (0: all in one pass)
+    bwd_chunk    shots per backward pass (0: all in one pass; -1: the library's
+                 choice by cache fit)
     graph        time steps per captured HIP graph block
 
 The probe runs the workload's own shots truncated to a few hundred steps, so
@@ -32,7 +33,7 @@ from . import lib as L
 from .workloads import Shot
 
 DEFAULT = dict(kernels=0, strip_fwd=4, wgs_fwd=6, strip_bwd=4, wgs_bwd=4, tile_z=0, group=0, rec_group=8,
-               bwd_chunk=0, graph=100)
+               bwd_chunk=-1, graph=100)
 # stage 1: (rows, interior workgroups per CU) of a pass
 PASS_SHAPES = ((8, 3), (8, 4), (4, 4), (4, 5), (4, 6))
 
@@ -46,7 +47,7 @@ def launch_candidates(nshots, best_shape):
     """Stage 2: imaging group, backward chunk and graph block for the stage-1
     choice; the z-marching kernels over tile height and shot group."""
     out = []
-    chunks = sorted({0} | {c for c in (16, 32) if c < nshots})
+    chunks = sorted({-1, 0} | {c for c in (16, 32) if c < nshots})
     for rg in (4, 8, 16):
         for bc in chunks:
             for gs in (50, 100):

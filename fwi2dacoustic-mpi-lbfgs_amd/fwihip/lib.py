@@ -258,7 +258,7 @@ class Context:
         self._chk(self.lib.fwi_hip_set_bwd_overlap(self.h, int(bool(on))))
 
     def set_bwd_chunk(self, shots):
-        """Shots per backward pass (0: all shots in one pass)."""
+        """Shots per backward pass (0: all shots in one pass; -1: the library's choice by cache fit)."""
         self._chk(self.lib.fwi_hip_set_bwd_chunk(self.h, int(shots)))
 
     def set_imaging(self, jump=1, k0=0, k1=2**31 - 1):

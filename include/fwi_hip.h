@@ -242,11 +242,13 @@ int fwi_hip_set_bwd_fused(fwi_hip_ctx *ctx, int on);
  * on = 0: one stream, adjoint then imaging per step.                        */
 int fwi_hip_set_bwd_overlap(fwi_hip_ctx *ctx, int on);
 
-/* Shots per backward pass of the tile kernels (0, the default: all shots in
- * one pass).  With n > 0 the adjoint pass runs the backward time loop once
- * per chunk of n shots (rounded up to whole reconstruction groups), so that
- * a chunk's working set can stay in the device's last-level cache; the
- * image is bit-identical to the single pass.                               */
+/* Shots per backward pass of the tile kernels.  With n > 0 the adjoint pass
+ * runs the backward time loop once per chunk of n shots (rounded up to whole
+ * reconstruction groups), so that a chunk's wavefields stay in the device's
+ * last-level cache from one step to the next; 0: all shots in one pass;
+ * -1 (the default): chunks when the four wavefields of a chunk fit in 200 MB
+ * and a step still has 10M cell-updates or more (32 of 64 shots at
+ * 1024x320), else one pass.  The image is bit-identical to the single pass. */
 int fwi_hip_set_bwd_chunk(fwi_hip_ctx *ctx, int shots);
 
 /* Shots per workgroup of the tile reconstruction + imaging kernel (default
