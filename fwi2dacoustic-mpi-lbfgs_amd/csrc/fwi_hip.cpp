@@ -38,6 +38,12 @@ hipError_t fwi_launch_rec(const FwiRecArgs *a, const int *kbase, int koff, int n
 hipError_t fwi_launch_bwd_tile(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, const FwiRecArgs *r_dev,
                                const int *const *tl, const int *ntl, const int *kbase, int koff, int gsize,
                                const FwiPair &p);
+hipError_t fwi_launch_fwd_mixed(const FwiFwdArgs *a, const FwiFwdArgs *a_dev, const int *work, int nwork,
+                                const int *const *tl, const int *ntl, const int *kbase, int koff, hipStream_t st,
+                                int mode);
+hipError_t fwi_launch_fwd_mixed_s4(const FwiFwdArgs *a, const FwiFwdArgs *a_dev, const int *work, int nwork,
+                                   const int *const *tl, const int *ntl, const int *kbase, int koff, hipStream_t st,
+                                   int mode);
 // the same tile kernels built with 4-row strips
 hipError_t fwi_launch_fwd_tile_s4(const FwiFwdArgs *a, const FwiFwdArgs *a_dev, const int *const *tl,
                                   const int *ntl, const int *kbase, int koff, const FwiPair &p, int mode);
@@ -161,6 +167,12 @@ struct fwi_hip_ctx {
     // recompute of the checkpointed backward pass) and in the backward pass:
     // 8 or 4 (fwi_hip_set_pass_shape); the strip lists are built for them
     int strip_fwd = 4, strip_bwd = 4;
+    // the forward step as one launch of interior and general strips
+    // (k_fwd_mixed, fwi_hip_set_fwd_mixed) instead of two streams joined per
+    // step; fwd_work holds its workgroup order (n_fwd_work, a multiple of 8)
+    bool fwd_mixed = true;
+    int *fwd_work = nullptr;
+    int n_fwd_work = 0;
     // With the second stream, the interior launches of the forward and fused
     // backward steps reserve unused LDS so that at most int_wgs_* of their
     // workgroups share a CU, leaving room for a general workgroup beside them
@@ -521,6 +533,7 @@ static std::vector<std::pair<void **, size_t>> create_table(fwi_hip_ctx *c)
         {(void **)&c->fwd_args_d, sizeof(FwiFwdArgs)},
         {(void **)&c->tl_fwd[0], strips},
         {(void **)&c->tl_fwd[1], strips},
+        {(void **)&c->fwd_work, strips + sizeof(int) * 64},
         {(void **)&c->tl_adj[0], strips},
         {(void **)&c->tl_adj[1], strips},
         {(void **)&c->tl_rec[0], items},
@@ -701,9 +714,36 @@ extern "C" int fwi_hip_set_model(fwi_hip_ctx *c, const float *vel)
     return FWI_HIP_OK;
 }
 
+// The workgroup order of a one-launch step over two lists of n_int and n_gen
+// workgroups: block b runs on XCD b % 8, which takes a contiguous share of
+// each list (neighbouring tiles share its L2, as xcd_remap arranges for the
+// separate launches); within an XCD the general workgroups come first, since
+// they live longest (1024x320 x 64 shots, forward step: 58.5 us against 64.1
+// with the general ones spread evenly through the interior ones, and 59.4 on
+// two streams).  Entry (role << 28) | index, role 1 for the general list; -1
+// pads the XCDs to one length.
+static std::vector<int> work_order(int n_int, int n_gen)
+{
+    const int pi = (n_int + 7) / 8, pg = (n_gen + 7) / 8;
+    std::vector<std::vector<int>> seq(8);
+    size_t len = 0;
+    for (int x = 0; x < 8; x++) {
+        const int i0 = std::min(x * pi, n_int), i1 = std::min(i0 + pi, n_int);
+        const int g0 = std::min(x * pg, n_gen), g1 = std::min(g0 + pg, n_gen);
+        for (int i = g0; i < g1; i++) seq[x].push_back((1 << 28) | i);
+        for (int i = i0; i < i1; i++) seq[x].push_back(i);
+        len = std::max(len, seq[x].size());
+    }
+    std::vector<int> w(8 * len, -1);
+    for (int x = 0; x < 8; x++)
+        for (size_t j = 0; j < seq[x].size(); j++) w[j * 8 + x] = seq[x][j];
+    return w;
+}
+
 // strip lists of the forward (strip_fwd rows) and adjoint (strip_bwd rows)
-// tile kernels, logical strip (s nsz + sz) ntx + tx, for the context's shots;
-// the backward pass builds its reconstruction and fused lists itself
+// tile kernels, logical strip (s nsz + sz) ntx + tx, for the context's shots,
+// and the workgroup order of the one-launch forward step; the backward pass
+// builds its reconstruction and fused lists itself
 static int build_strip_lists(fwi_hip_ctx *c)
 {
     const FwiGeomDev &g = c->g;
@@ -734,6 +774,10 @@ static int build_strip_lists(fwi_hip_ctx *c)
             HIPCHK(c, hipMemcpyAsync(c->tl_adj[q], la[q].data(), sizeof(int) * la[q].size(), hipMemcpyHostToDevice,
                                      c->st));
     }
+    const std::vector<int> work = work_order((c->n_fwd[0] + 3) / 4, (c->n_fwd[1] + 3) / 4);
+    c->n_fwd_work = (int)work.size();
+    if (!work.empty())
+        HIPCHK(c, hipMemcpyAsync(c->fwd_work, work.data(), sizeof(int) * work.size(), hipMemcpyHostToDevice, c->st));
     c->rec_lists_for = -1;
     c->bwd_lists_for = -1;
     HIPCHK(c, hipStreamSynchronize(c->st));
@@ -875,6 +919,10 @@ static FwiPair pair_of(const fwi_hip_ctx *c, int pass)
 static hipError_t fwd_tile(const fwi_hip_ctx *c, const FwiFwdArgs *a, const FwiFwdArgs *a_dev, const int *const *tl,
                            const int *ntl, const int *kbase, int koff, const FwiPair &p, int mode)
 {
+    if (c->fwd_mixed && tl == c->tl_fwd)
+        return c->strip_fwd == 4
+                   ? fwi_launch_fwd_mixed_s4(a, a_dev, c->fwd_work, c->n_fwd_work, tl, ntl, kbase, koff, p.st, mode)
+                   : fwi_launch_fwd_mixed(a, a_dev, c->fwd_work, c->n_fwd_work, tl, ntl, kbase, koff, p.st, mode);
     return c->strip_fwd == 4 ? fwi_launch_fwd_tile_s4(a, a_dev, tl, ntl, kbase, koff, p, mode)
                              : fwi_launch_fwd_tile(a, a_dev, tl, ntl, kbase, koff, p, mode);
 }
@@ -1320,6 +1368,15 @@ extern "C" int fwi_hip_set_pass_shape(fwi_hip_ctx *c, int pass, int rows, int in
     if (rows == r) return FWI_HIP_OK;
     r = rows;
     return c->nshots > 0 ? build_strip_lists(c) : FWI_HIP_OK;
+}
+extern "C" int fwi_hip_set_fwd_mixed(fwi_hip_ctx *c, int on)
+{
+    if (!c) return set_err(c, FWI_HIP_EINVAL, "set_fwd_mixed: NULL context");
+    if (c->gx_fwd) (void)hipGraphExecDestroy(c->gx_fwd);
+    c->gx_fwd = nullptr;
+    drop_bwd(c);  // the checkpointed backward pass recomputes with the forward kernels
+    c->fwd_mixed = on != 0;
+    return FWI_HIP_OK;
 }
 extern "C" int fwi_hip_set_interior_wgs(fwi_hip_ctx *c, int wgs)
 {

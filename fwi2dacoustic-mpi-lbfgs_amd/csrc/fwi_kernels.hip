@@ -1011,6 +1011,51 @@ __global__ __launch_bounds__(256) void k_fwd_tile(const FwiFwdArgs *__restrict__
     }
 }
 
+// The forward step as one launch: the interior and general strips' workgroups
+// in one grid, each running its own body, so that no step waits on another
+// stream.  work[b] = (role << 28) | workgroup index in the role's list (role
+// 0: interior strips s_int, 1: general strips s_gen), -1: idle; the host
+// orders the workgroups per XCD, the general ones spread through the
+// interior ones (fwi_work_order).  The grid takes the general body's
+// registers, so every strip runs at its occupancy.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_fwd_mixed(const FwiFwdArgs *__restrict__ Ap,
+                                                   const FwiShotDev *__restrict__ shots, const int *__restrict__ work,
+                                                   const int *__restrict__ s_int, int n_int,
+                                                   const int *__restrict__ s_gen, int n_gen, const int *kbase, int koff)
+{
+    const int k = (kbase ? *kbase : 0) + koff;
+    if (k < 0) return;
+    const int wv = work[blockIdx.x];
+    if (wv < 0) return;
+    const FwiFwdArgs &A = *Ap;
+    const FwiGeomDev &g = A.g;
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const bool gen = (wv >> 28) != 0;
+    const int lb = wv & 0x0FFFFFFF;
+    __shared__ float Psh[4][FR][HC];
+    __shared__ float Ush[4][FR][HC];
+    __shared__ float stab[FWI_TABN];
+    if (gen) stage_tables_256(stab, A.b.tab, g.npml);  // uniform per workgroup: its only barrier
+    const int e = 4 * lb + w;
+    if (e >= (gen ? n_gen : n_int)) return;
+    const int nsz = (g.pnz + FR - 1) / FR, l = (gen ? s_gen : s_int)[e];
+    const int tx = l % g.ntx, rest = l / g.ntx, sz = rest % nsz, s = rest / nsz;
+    const FwiShotDev sh = shots[s];
+    if (k >= sh.nt) return;
+    if (gen) {
+        const int c = fwi_fwd_strip_class(g, sh, tx * TX, sz * FR);
+        if ((c & ~(FWI_CLS_X | FWI_CLS_FR)) == 0)
+            fwd_tile_rows<true, MODE, FWI_CLS_X | FWI_CLS_FR>(A, sh, stab, Psh[w], Ush[w], s, k, tx * TX, sz * FR, lane);
+        else if ((c & ~(FWI_CLS_Z | FWI_CLS_FR)) == 0)
+            fwd_tile_rows<true, MODE, FWI_CLS_Z | FWI_CLS_FR>(A, sh, stab, Psh[w], Ush[w], s, k, tx * TX, sz * FR, lane);
+        else
+            fwd_tile_rows<true, MODE>(A, sh, stab, Psh[w], Ush[w], s, k, tx * TX, sz * FR, lane);
+    } else {
+        fwd_tile_rows<false, MODE>(A, sh, stab, Psh[w], Ush[w], s, k, tx * TX, sz * FR, lane);
+    }
+}
+
 // ============================================================================
 // adjoint: exact transpose, x chain
 // ============================================================================
@@ -2763,6 +2808,24 @@ hipError_t FWI_VNAME(fwi_launch_fwd_tile)(const FwiFwdArgs *a, const FwiFwdArgs 
     if (mode == FWI_FWD_XSEG) return launch_fwd_tile_mode<FWI_FWD_XSEG>(a, a_dev, tl, ntl, kbase, koff, p);
     if (mode == FWI_FWD_PLAIN) return launch_fwd_tile_mode<FWI_FWD_PLAIN>(a, a_dev, tl, ntl, kbase, koff, p);
     return launch_fwd_tile_mode<FWI_FWD_FRAMES>(a, a_dev, tl, ntl, kbase, koff, p);
+}
+
+// the forward step as one launch (k_fwd_mixed) over the work order `work` of
+// nwork workgroups (a multiple of 8)
+hipError_t FWI_VNAME(fwi_launch_fwd_mixed)(const FwiFwdArgs *a, const FwiFwdArgs *a_dev, const int *work, int nwork,
+                                           const int *const *tl, const int *ntl, const int *kbase, int koff,
+                                           hipStream_t st, int mode)
+{
+    if (nwork <= 0) return hipSuccess;
+#define FWI_MIXED(M)                                                                                            \
+    hipLaunchKernelGGL(k_fwd_mixed<M>, dim3(nwork), dim3(256), 0, st, a_dev, a->b.shots, work, tl[0], ntl[0], \
+                       tl[1], ntl[1], kbase, koff)
+    if (mode == FWI_FWD_TRACES) FWI_MIXED(FWI_FWD_TRACES);
+    else if (mode == FWI_FWD_XSEG) FWI_MIXED(FWI_FWD_XSEG);
+    else if (mode == FWI_FWD_PLAIN) FWI_MIXED(FWI_FWD_PLAIN);
+    else FWI_MIXED(FWI_FWD_FRAMES);
+#undef FWI_MIXED
+    return hipGetLastError();
 }
 
 hipError_t FWI_VNAME(fwi_launch_adj_tile)(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, const int *const *tl,

@@ -266,6 +266,13 @@ int fwi_hip_set_rec_group(fwi_hip_ctx *ctx, int shots);
  * loops have no tiles); the autotuner picks it per workload shape.          */
 int fwi_hip_set_pass_shape(fwi_hip_ctx *ctx, int pass, int rows, int interior_wgs);
 
+/* The forward step of the tile kernels as one launch (on = 1, the default):
+ * the interior and general strips' workgroups in one grid, the general ones
+ * first on each XCD, instead of two launches on two streams joined every
+ * step (on = 0).  Results are bit-identical.  No counterpart in the
+ * reference (its loops run one after another).                             */
+int fwi_hip_set_fwd_mixed(fwi_hip_ctx *ctx, int on);
+
 /* Wait for all work of the context. */
 int fwi_hip_synchronize(fwi_hip_ctx *ctx);
 

@@ -386,6 +386,43 @@ def test_strip_rows_give_the_same_results(lib, graph):
         ctx.close()
 
 
+@pytest.mark.parametrize("graph", [100, 0])
+def test_fwd_mixed_gives_the_same_results(lib, graph):
+    """The forward step as one launch of interior and general strips gives the
+    two-launch traces and image bit for bit, with either strip height, over
+    graph replays, and in the checkpointed backward pass's recompute."""
+    g, vel, shots = small_case(nshots=3, nt=230, seed=45)
+    rng = np.random.default_rng(46)
+    res = [rng.standard_normal((s.nr, s.nt)).astype(np.float32) for s in shots]
+    ctx = lib.Context(g, max_shots=3, max_nt=230, max_nr=100)
+    ctx.set_graph(graph)
+    ctx.set_rec_group(2)
+    ctx.set_model(vel)
+    ctx.set_shots(shots)
+
+    def run():
+        out = []
+        for _ in range(2):                      # a second pass replays the captured graphs
+            tr = ctx.forward()
+            ctx.set_residual(ctx.pack(res))
+            out.append((tr, ctx.adjoint()[1]))
+        return out
+
+    try:
+        for ckpt in (0, 50):
+            ctx.set_checkpointing(ckpt)
+            for rows in (4, 8):
+                ctx.set_pass_shape(0, rows, 3)
+                ctx.set_fwd_mixed(False)
+                ref = run()
+                ctx.set_fwd_mixed(True)
+                for tr, gk in run():
+                    np.testing.assert_array_equal(tr, ref[0][0], err_msg=f"rows {rows} ckpt {ckpt}")
+                    np.testing.assert_array_equal(gk, ref[0][1], err_msg=f"rows {rows} ckpt {ckpt}")
+    finally:
+        ctx.close()
+
+
 def test_pass_shape_refuses_other_shapes(lib):
     g, vel, shots = small_case(nshots=1, nt=50, seed=43)
     ctx = lib.Context(g, max_shots=1, max_nt=50, max_nr=100)
