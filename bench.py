@@ -61,6 +61,8 @@ def parse():
                     help="backward pass: rows per wave of the tile kernels (4 or 8) and interior workgroups per CU")
     ap.add_argument("--fwd-mixed", type=int, default=-1,
                     help="the forward step as one launch of interior and general strips (1/0; -1: default)")
+    ap.add_argument("--bwd-mixed", type=int, default=-1,
+                    help="the fused backward step as one launch (1/0; -1: default)")
     ap.add_argument("--bwd-fused", type=int, default=-1,
                     help="one fused adjoint + reconstruction + imaging kernel per backward step (1/0; -1: default 1)")
     ap.add_argument("--ckpt-every", type=int, default=100, help="snapshot interval of --workload checkpoint")
@@ -573,6 +575,8 @@ def main():
             ctx.set_interior_wgs(args.interior_wgs)
         if args.fwd_mixed >= 0:
             ctx.set_fwd_mixed(args.fwd_mixed)
+        if args.bwd_mixed >= 0:
+            ctx.set_bwd_mixed(args.bwd_mixed)
         if args.fwd_shape:
             ctx.set_pass_shape(0, *args.fwd_shape)
         if args.bwd_shape:

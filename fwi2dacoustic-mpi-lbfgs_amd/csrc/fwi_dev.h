@@ -190,6 +190,11 @@ struct FwiRecArgs {
     const float *pstrip, *xstrip;
     double *gacc;        // [ngroups][nz][nx]
     int img_jump, img_k0, img_k1;  // steps imaged: k0 <= k <= k1, (k - k0) % jump == 0
+    int pad_;
+    // X^k of the general tiles of the one-launch backward step, which image
+    // one step late: xg[(k & 1) * xg_b + s nx nz + i nx + j]
+    float *xg;
+    long long xg_b;
 };
 
 // whether reconstruction step k adds to the image

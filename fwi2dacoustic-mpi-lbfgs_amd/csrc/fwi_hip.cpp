@@ -44,6 +44,12 @@ hipError_t fwi_launch_fwd_mixed(const FwiFwdArgs *a, const FwiFwdArgs *a_dev, co
 hipError_t fwi_launch_fwd_mixed_s4(const FwiFwdArgs *a, const FwiFwdArgs *a_dev, const int *work, int nwork,
                                    const int *const *tl, const int *ntl, const int *kbase, int koff, hipStream_t st,
                                    int mode);
+hipError_t fwi_launch_bwd_mixed(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, const FwiRecArgs *r_dev,
+                                const int *work, int nwork, const int *const *tl, const int *ntl, const int *kbase,
+                                int koff, int gsize, hipStream_t st);
+hipError_t fwi_launch_bwd_mixed_s4(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, const FwiRecArgs *r_dev,
+                                   const int *work, int nwork, const int *const *tl, const int *ntl, const int *kbase,
+                                   int koff, int gsize, hipStream_t st);
 // the same tile kernels built with 4-row strips
 hipError_t fwi_launch_fwd_tile_s4(const FwiFwdArgs *a, const FwiFwdArgs *a_dev, const int *const *tl,
                                   const int *ntl, const int *kbase, int koff, const FwiPair &p, int mode);
@@ -173,6 +179,15 @@ struct fwi_hip_ctx {
     bool fwd_mixed = true;
     int *fwd_work = nullptr;
     int n_fwd_work = 0;
+    // the fused backward step as one launch (k_bwd_mixed, fwi_hip_set_bwd_mixed);
+    // bwd_work holds the workgroup order of every shot chunk, chunk ch at
+    // bwd_work_off[ch] with bwd_work_n[ch] entries, built for bwd_work_for
+    bool bwd_mixed = false;
+    int *bwd_work = nullptr;
+    long long bwd_work_cap = 0;
+    std::vector<long long> bwd_work_off;
+    std::vector<int> bwd_work_n;
+    long long bwd_work_for = -1;
     // With the second stream, the interior launches of the forward and fused
     // backward steps reserve unused LDS so that at most int_wgs_* of their
     // workgroups share a CU, leaving room for a general workgroup beside them
@@ -206,6 +221,10 @@ struct fwi_hip_ctx {
     float *tab = nullptr;
     // frames and traces
     float *pstrip = nullptr, *xstrip = nullptr;  // boundary frames, allocated by the first boundary-saving forward
+    // X^k of the general tiles of the one-launch backward step, kept one step
+    // for their late imaging (2 x max_shots x nx nz), allocated by the first
+    // such backward pass
+    float *xg = nullptr;
     // checkpointed backward pass (ckpt_every > 0): snapshots of p every
     // ckpt_every steps in FP32/FP16/BF16 (per-shot max-abs scale for 16 bit)
     // with the CPML memory in FP32, a second CPML set for the recompute,
@@ -714,24 +733,24 @@ extern "C" int fwi_hip_set_model(fwi_hip_ctx *c, const float *vel)
     return FWI_HIP_OK;
 }
 
-// The workgroup order of a one-launch step over two lists of n_int and n_gen
-// workgroups: block b runs on XCD b % 8, which takes a contiguous share of
-// each list (neighbouring tiles share its L2, as xcd_remap arranges for the
-// separate launches); within an XCD the general workgroups come first, since
-// they live longest (1024x320 x 64 shots, forward step: 58.5 us against 64.1
-// with the general ones spread evenly through the interior ones, and 59.4 on
-// two streams).  Entry (role << 28) | index, role 1 for the general list; -1
-// pads the XCDs to one length.
-static std::vector<int> work_order(int n_int, int n_gen)
+// The workgroup order of a one-launch step over the lists of its roles
+// (count[role] workgroups each): block b runs on XCD b % 8, which takes a
+// contiguous share of each list (neighbouring tiles share its L2, as
+// xcd_remap arranges for the separate launches); within an XCD the roles come
+// in `order`, the general ones first, since they live longest (1024x320 x 64
+// shots, forward step: 58.5 us against 64.1 with the general workgroups spread
+// evenly through the interior ones, and 59.4 on two streams).  Entry
+// (role << 28) | index; -1 pads the XCDs to one length.
+static std::vector<int> work_order(const std::vector<int> &count, const std::vector<int> &order)
 {
-    const int pi = (n_int + 7) / 8, pg = (n_gen + 7) / 8;
     std::vector<std::vector<int>> seq(8);
     size_t len = 0;
     for (int x = 0; x < 8; x++) {
-        const int i0 = std::min(x * pi, n_int), i1 = std::min(i0 + pi, n_int);
-        const int g0 = std::min(x * pg, n_gen), g1 = std::min(g0 + pg, n_gen);
-        for (int i = g0; i < g1; i++) seq[x].push_back((1 << 28) | i);
-        for (int i = i0; i < i1; i++) seq[x].push_back(i);
+        for (int role : order) {
+            const int n = count[role], per = (n + 7) / 8;
+            const int i0 = std::min(x * per, n), i1 = std::min(i0 + per, n);
+            for (int i = i0; i < i1; i++) seq[x].push_back((role << 28) | i);
+        }
         len = std::max(len, seq[x].size());
     }
     std::vector<int> w(8 * len, -1);
@@ -774,7 +793,7 @@ static int build_strip_lists(fwi_hip_ctx *c)
             HIPCHK(c, hipMemcpyAsync(c->tl_adj[q], la[q].data(), sizeof(int) * la[q].size(), hipMemcpyHostToDevice,
                                      c->st));
     }
-    const std::vector<int> work = work_order((c->n_fwd[0] + 3) / 4, (c->n_fwd[1] + 3) / 4);
+    const std::vector<int> work = work_order({(c->n_fwd[0] + 3) / 4, (c->n_fwd[1] + 3) / 4}, {1, 0});
     c->n_fwd_work = (int)work.size();
     if (!work.empty())
         HIPCHK(c, hipMemcpyAsync(c->fwd_work, work.data(), sizeof(int) * work.size(), hipMemcpyHostToDevice, c->st));
@@ -1116,13 +1135,15 @@ static void free_frames(fwi_hip_ctx *c)
 {
     dfree(c, c->pstrip);
     dfree(c, c->xstrip);
-    c->pstrip = c->xstrip = nullptr;
+    dfree(c, c->xg);
+    c->pstrip = c->xstrip = c->xg = nullptr;
 }
 
 // bytes of each backward method's buffers, at the context's capacity
 static size_t frames_bytes(const fwi_hip_ctx *c)
 {
-    return 2 * sizeof(float) * (size_t)c->d.max_shots * (size_t)c->d.max_nt * (size_t)c->g.sl;
+    return 2 * sizeof(float) * (size_t)c->d.max_shots * (size_t)c->d.max_nt * (size_t)c->g.sl +
+           (c->bwd_mixed ? 2 * sizeof(float) * (size_t)c->d.max_shots * (size_t)c->g.nx * (size_t)c->g.nz : 0);
 }
 static size_t rv_slot_bytes(const fwi_hip_ctx *c, int storage)
 {
@@ -1165,7 +1186,7 @@ static size_t base_bytes(const fwi_hip_ctx *c)
 {
     size_t held = c->bytes;
     for (const auto &a : c->allocs)
-        if (a.first == c->pstrip || a.first == c->xstrip || a.first == c->ck_store || a.first == c->ck_scale ||
+        if (a.first == c->pstrip || a.first == c->xstrip || a.first == c->xg || a.first == c->ck_store || a.first == c->ck_scale ||
             a.first == c->ck_cpml || a.first == c->ck_xseg || a.first == c->xh2 || a.first == c->xd2 ||
             a.first == c->zh2 || a.first == c->zd2)
             held -= a.second;
@@ -1368,6 +1389,13 @@ extern "C" int fwi_hip_set_pass_shape(fwi_hip_ctx *c, int pass, int rows, int in
     if (rows == r) return FWI_HIP_OK;
     r = rows;
     return c->nshots > 0 ? build_strip_lists(c) : FWI_HIP_OK;
+}
+extern "C" int fwi_hip_set_bwd_mixed(fwi_hip_ctx *c, int on)
+{
+    if (!c) return set_err(c, FWI_HIP_EINVAL, "set_bwd_mixed: NULL context");
+    drop_bwd(c);
+    c->bwd_mixed = on != 0;
+    return FWI_HIP_OK;
 }
 extern "C" int fwi_hip_set_fwd_mixed(fwi_hip_ctx *c, int on)
 {
@@ -1802,6 +1830,12 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
     r.img_jump = c->img_jump;
     r.img_k0 = c->img_k0;
     r.img_k1 = c->img_k1;
+    const bool fused_pass = tile && c->bwd_fused && c->ckpt_every == 0;
+    if (fused_pass && c->bwd_mixed && !c->xg &&
+        (rc = dalloc(c, (void **)&c->xg, sizeof(float) * 2 * (size_t)c->d.max_shots * nb)))
+        return rc;
+    r.xg = c->xg;
+    r.xg_b = (long long)c->d.max_shots * nb;
     c->adj_args_h = a;
     c->rec_args_h = r;
     HIPCHK(c, hipMemcpyAsync(c->adj_args_d, &c->adj_args_h, sizeof(FwiAdjArgs), hipMemcpyHostToDevice, c->st));
@@ -1885,6 +1919,7 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
         }
         HIPCHK(c, hipStreamSynchronize(c->st));
         c->bwd_lists_for = rgs;
+        c->bwd_work_for = -1;
     }
     const int nb_adj = g.ntx * g.ntz * c->ngroups;
     const int nb_rec = g.bntx * g.bntz * c->ngroups;
@@ -1970,6 +2005,38 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
         drop_bwd(c);
         c->gx_bwd.assign(nch, nullptr);
     }
+    // the one-launch backward step: the workgroup order of every chunk, the
+    // general roles first (reconstruction, corners, x/z-side strips)
+    const bool mixed = fused && c->bwd_mixed && c->xg;
+    const long long work_key = (long long)rgs * 1000003LL + cs;
+    if (mixed && c->bwd_work_for != work_key) {
+        std::vector<int> all;
+        c->bwd_work_off.assign(nch, 0);
+        c->bwd_work_n.assign(nch, 0);
+        for (int ch = 0; ch < nch; ch++) {
+            const int s0 = ch * cs, s1 = std::min(s0 + cs, c->nshots), g0 = s0 / rgs, g1 = (s1 + rgs - 1) / rgs;
+            int n[4];
+            for (int q = 0; q < 4; q++) {
+                const bool per_shot = q == 1 || q == 2;
+                n[q] = c->bwd_pre[q][per_shot ? s1 : g1] - c->bwd_pre[q][per_shot ? s0 : g0];
+            }
+            const std::vector<int> w = work_order({n[0], (n[1] + 3) / 4, (n[2] + 3) / 4, n[3]}, {3, 2, 1, 0});
+            c->bwd_work_off[ch] = (long long)all.size();
+            c->bwd_work_n[ch] = (int)w.size();
+            all.insert(all.end(), w.begin(), w.end());
+        }
+        if ((long long)all.size() > c->bwd_work_cap) {
+            dfree(c, c->bwd_work);
+            c->bwd_work = nullptr;
+            c->bwd_work_cap = 0;
+            if ((rc = dalloc(c, (void **)&c->bwd_work, sizeof(int) * all.size()))) return rc;
+            c->bwd_work_cap = (long long)all.size();
+        }
+        if (!all.empty())
+            HIPCHK(c, hipMemcpyAsync(c->bwd_work, all.data(), sizeof(int) * all.size(), hipMemcpyHostToDevice, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        c->bwd_work_for = work_key;
+    }
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
     const int B = c->graph_steps;
     for (int ch = 0; ch < nch; ch++) {
@@ -2004,6 +2071,12 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
                         : fwi_launch_rec(&r, kb, koff, nb_rec, c->st);
         };
         auto launch_bwd = [&](const int *kb, int koff) {
+            if (mixed) {
+                const int *w = c->bwd_work + c->bwd_work_off[ch];
+                return c->strip_bwd == 4
+                           ? fwi_launch_bwd_mixed_s4(&a, ad, rd, w, c->bwd_work_n[ch], tb, nbw, kb, koff, rgs, c->st)
+                           : fwi_launch_bwd_mixed(&a, ad, rd, w, c->bwd_work_n[ch], tb, nbw, kb, koff, rgs, c->st);
+            }
             return bwd_tile(c, &a, ad, rd, tb, nbw, kb, koff, rgs, pair_of(c, 1));
         };
         // overlapped form: adjoint k on st, imaging k+1 on st2, joined per step;
@@ -2063,6 +2136,10 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
                 }
                 HIPCHK(c, hipGraphLaunch(c->gx_bwd[ch], c->st));
             }
+            // the one-launch step images its general tiles one step late:
+            // the image of step 0 follows the last step (the last block ran
+            // k = -1 itself when it went past step 0)
+            if (mixed && kmax - nblk * B >= 0) HIPCHK(c, launch_bwd(nullptr, -1));
         } else if (ovl) {
             for (int k = kmax - 1; k >= -1; k--) HIPCHK(c, pair_step(nullptr, k));
         } else if (fused) {
@@ -2074,6 +2151,7 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
                 if (c->timing == 1) marks.push_back({1, {e0, e1}});
                 if (marks.size() >= 512 && (rc = collect_timing(c, marks))) return rc;
             }
+            if (mixed) HIPCHK(c, launch_bwd(nullptr, -1));  // the image of step 0
         } else {
             for (int k = kmax - 1; k >= 0; k--) {
                 hipEvent_t e0, e1, e2;

@@ -1796,6 +1796,29 @@ __global__ __launch_bounds__(256) void k_adj_tile(const FwiAdjArgs *__restrict__
 // side with the receiver rows (96 VGPRs).  Otherwise: the corner strips, every
 // rule compiled in.  The host puts each strip in the list of its kernel with
 // fwi_adj_strip_class.
+// One adjoint strip l of k_adj_tile_split's lists at step k (CPML tables in stab)
+template <bool LIGHT>
+__device__ __forceinline__ void adj_split_strip(const FwiAdjArgs &A, const FwiShotDev *__restrict__ shots, int l,
+                                                int k, int lane, float (*Ws)[HC], float (*Us)[HC], const float *stab)
+{
+    const FwiGeomDev &g = A.g;
+    const int nsz = (g.pnz + FR - 1) / FR;
+    const int tx = l % g.ntx, rest = l / g.ntx, sz = rest % nsz, s = rest / nsz;
+    const FwiShotDev sh = shots[s];
+    if (k >= sh.nt) return;
+    if constexpr (LIGHT) {
+        const int c = fwi_adj_strip_class(g, sh, tx * TX, sz * FR);
+        if ((c & ~FWI_CLS_X) == 0)
+            adj_tile_rows<true, false, FWI_CLS_X>(A, sh, stab, Ws, Us, s, k, tx * TX, sz * FR, lane);
+        else if ((c & ~FWI_CLS_Z) == 0)
+            adj_tile_rows<true, false, FWI_CLS_Z>(A, sh, stab, Ws, Us, s, k, tx * TX, sz * FR, lane);
+        else
+            adj_tile_rows<true, false, FWI_CLS_Z | FWI_CLS_RS>(A, sh, stab, Ws, Us, s, k, tx * TX, sz * FR, lane);
+    } else {
+        adj_tile_rows<true>(A, sh, stab, Ws, Us, s, k, tx * TX, sz * FR, lane);
+    }
+}
+
 template <bool LIGHT>
 __global__ __launch_bounds__(256) void k_adj_tile_split(const FwiAdjArgs *__restrict__ Ap,
                                                         const FwiShotDev *__restrict__ shots,
@@ -1805,32 +1828,16 @@ __global__ __launch_bounds__(256) void k_adj_tile_split(const FwiAdjArgs *__rest
     const int k = (kbase ? *kbase : 0) + koff;
     if (k < 0) return;
     const FwiAdjArgs &A = *Ap;
-    const FwiGeomDev &g = A.g;
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lb = xcd_remap(blockIdx.x, (nst + 3) >> 2);
     if (lb < 0) return;
     __shared__ float Wsh[4][FR][HC];
     __shared__ float Ush[4][FR][HC];
     __shared__ float stab[FWI_TABN];
-    stage_tables_256(stab, A.b.tab, g.npml);  // the only workgroup barrier
+    stage_tables_256(stab, A.b.tab, A.g.npml);  // the only workgroup barrier
     const int e = 4 * lb + w;
     if (e >= nst) return;
-    const int nsz = (g.pnz + FR - 1) / FR, l = strips[e];
-    const int tx = l % g.ntx, rest = l / g.ntx, sz = rest % nsz, s = rest / nsz;
-    const FwiShotDev sh = shots[s];
-    if (k >= sh.nt) return;
-    if constexpr (LIGHT) {
-        const int c = fwi_adj_strip_class(g, sh, tx * TX, sz * FR);
-        if ((c & ~FWI_CLS_X) == 0)
-            adj_tile_rows<true, false, FWI_CLS_X>(A, sh, stab, Wsh[w], Ush[w], s, k, tx * TX, sz * FR, lane);
-        else if ((c & ~FWI_CLS_Z) == 0)
-            adj_tile_rows<true, false, FWI_CLS_Z>(A, sh, stab, Wsh[w], Ush[w], s, k, tx * TX, sz * FR, lane);
-        else
-            adj_tile_rows<true, false, FWI_CLS_Z | FWI_CLS_RS>(A, sh, stab, Wsh[w], Ush[w], s, k, tx * TX, sz * FR,
-                                                              lane);
-    } else {
-        adj_tile_rows<true>(A, sh, stab, Wsh[w], Ush[w], s, k, tx * TX, sz * FR, lane);
-    }
+    adj_split_strip<LIGHT>(A, shots, strips[e], k, lane, Wsh[w], Ush[w], stab);
 }
 
 // ============================================================================
@@ -2110,13 +2117,19 @@ static_assert(RR % 4 == 0 && 3 * RR <= 64, "reconstruction rows per wave");
 // depend on the adjoint step, so the fused backward step issues them before
 // the adjoint's arithmetic.  bx0 may be negative (tiles of the padded grid
 // start npml columns left of the grid).
+// LATE: the tile images step k+1 at step k, mu^{k+2} X^{k+1} (mu^{k+2} is
+// read-only during step k; X^{k+1} was kept in xg by step k+1), so that it
+// does not wait for the adjoint step k of its cells; rec_tile_shot loads
+// those two (late_load), and nothing is loaded here at k = -1 (the image of
+// step 0, after the last step).
 struct RecIn {
     float p[RPR], hv[RHP], pq[RR], pm[RR];
 };
-template <bool EDGE, bool FUSED = false>
+template <bool EDGE, bool FUSED = false, bool LATE = false>
 __device__ __forceinline__ void rec_tile_load(const FwiRecArgs &A, int s, int k, int bx0, int r0, int lane,
                                               RecIn &in)
 {
+    if (LATE && k < 0) return;
     const FwiGeomDev &g = A.g;
     const int npml = g.npml, nx = g.nx, nz = g.nz, ld = g.ld;
     const int bx = bx0 + lane, fx = bx + npml;
@@ -2148,13 +2161,38 @@ __device__ __forceinline__ void rec_tile_load(const FwiRecArgs &A, int s, int k,
     for (int d = 0; d < RR; ++d) {
         const int i = r0 + d;
         in.pq[d] = EDGE ? buf_ld_row_sel(rq, vo, i, nz, rowb) : buf_ld(rq, vo, i * rowb);
-        if (!FUSED) in.pm[d] = EDGE ? buf_ld_row_sel(rm, vo, i, nz, rowb) : buf_ld(rm, vo, i * rowb);
+        if (!FUSED && !LATE) in.pm[d] = EDGE ? buf_ld_row_sel(rm, vo, i, nz, rowb) : buf_ld(rm, vo, i * rowb);
+    }
+}
+
+// mu^{k+2} and X^{k+1} of a tile's rows (LATE imaging)
+template <bool EDGE>
+__device__ __forceinline__ void late_load(const FwiRecArgs &A, int s, int k, int bx0, int r0, int lane, float *mu2,
+                                          float *xq)
+{
+    const FwiGeomDev &g = A.g;
+    const int npml = g.npml, nx = g.nx, nz = g.nz, ld = g.ld;
+    const int bx = bx0 + lane;
+    const bool colok = !EDGE || (bx >= 0 && bx < nx);
+    const float *mu = ((k & 1) ? A.M1 : A.M0) + (long long)s * A.b.fs;  // mu^{k+2}
+    const brsrc rm = buf_rsrc(mu, (unsigned)(A.b.fs * sizeof(float)));
+    const brsrc rx =
+        buf_rsrc(A.xg + ((k + 1) & 1) * A.xg_b + (long long)s * nx * nz, (unsigned)(sizeof(float) * nx * nz));
+    const int vo = (colok ? bx + npml : npml) * 4, rowb = ld * 4, xo = (colok ? bx : 0) * 4;
+#pragma unroll
+    for (int d = 0; d < RR; ++d) {
+        const int i = r0 + d;
+        mu2[d] = EDGE ? buf_ld_row_sel(rm, vo, i, nz, rowb) : buf_ld(rm, vo, i * rowb);
+        xq[d] = buf_ld_row_sel(rx, xo, i, nz, nx * 4);
+        xq[d] = colok ? xq[d] : 0.f;
     }
 }
 
 // One shot's reconstruction + imaging step on a tile from the loads of
 // rec_tile_load.  FUSED: mu^{k+1} of the rows comes from the caller (mu1).
-template <bool EDGE, bool FUSED = false>
+// LATE: the image term is that of step k+1 (when the shot has one), and X^k
+// goes to xg for step k-1.
+template <bool EDGE, bool FUSED = false, bool LATE = false>
 __device__ __forceinline__ void rec_tile_shot(const FwiRecArgs &A, const FwiShotDev &sh, int s, int k, int bx0,
                                               int r0, int lane, float (*Ps)[HC], float (*Us)[HC], const float *pk,
                                               double *acc, const RecIn &in, const float *mu1 = nullptr)
@@ -2163,6 +2201,16 @@ __device__ __forceinline__ void rec_tile_shot(const FwiRecArgs &A, const FwiShot
     const int npml = g.npml, nx = g.nx, nz = g.nz, ld = g.ld;
     const int bx = bx0 + lane, fx = bx + npml;
     const bool colok = !EDGE || (bx >= 0 && bx < nx);
+    // LATE: the term of step k+1, in the order the other tiles add theirs
+    const bool late_term = LATE && k + 1 < sh.nt;
+    float mu2[RR], xq[RR];
+    if (LATE && k < 0) {
+        late_load<EDGE>(A, s, k, bx0, r0, lane, mu2, xq);
+#pragma unroll
+        for (int d = 0; d < RR; ++d)
+            if (colok && r0 + d < nz) acc[d] += (double)mu2[d] * (double)xq[d];
+        return;
+    }
     const long long so = (long long)s * A.b.fs;
     const int cur = k & 1;
     float *pp = (cur ? A.P0 : A.P1) + so;        // p^{k+1} -> p^{k-1}
@@ -2216,6 +2264,9 @@ __device__ __forceinline__ void rec_tile_shot(const FwiRecArgs &A, const FwiShot
         // used only on the source row
         fsrc = gbl(A.b.srcf)[sh.src_off + ((sh.isz >= r0 && sh.isz < r0 + RR) ? k : 0)];
     }
+    if (LATE) late_load<EDGE>(A, s, k, bx0, r0, lane, mu2, xq);
+    const brsrc rxk =
+        buf_rsrc(A.xg + (k & 1) * A.xg_b + (long long)s * nx * nz, (unsigned)(sizeof(float) * nx * nz));
 #pragma unroll
     for (int d = 0; d < RR; ++d) {
         const int i = r0 + d;
@@ -2230,7 +2281,12 @@ __device__ __forceinline__ void rec_tile_shot(const FwiRecArgs &A, const FwiShot
             inner = i >= 3 && i < nz - 3 && bx >= 3 && bx < nx - 3;
             if (!inner && colok) X = xsv[d];
         }
-        if (colok) acc[d] += (double)pm[d] * (double)X;
+        if (LATE) {
+            if (colok && late_term) acc[d] += (double)mu2[d] * (double)xq[d];
+            if (colok) buf_st(X, rxk, bx * 4, i * nx * 4);
+        } else if (colok) {
+            acc[d] += (double)pm[d] * (double)X;
+        }
         if (k >= 1) {
             float pnew;
             if (!EDGE || inner) pnew = leapfrog(p[d + 3], pq[d], pk[d] * X);
@@ -2245,28 +2301,23 @@ __device__ __forceinline__ void rec_tile_shot(const FwiRecArgs &A, const FwiShot
 // is interior when the tile is interior for every shot of the group).
 // PADDED: the tiles are those of the padded grid (64 x FR from column 0, as
 // the fused backward step's), so bx0 = the tile's first column - npml.
-template <bool EDGE, bool PADDED = false>
-__device__ __forceinline__ void rec_tile_kernel(const FwiRecArgs *__restrict__ Ap, const FwiShotDev *__restrict__ shots,
-                                                const int *__restrict__ items, int nit, const int *kbase, int koff,
-                                                int gsize)
+// LDS of a reconstruction item: per wave, staged p rows and u rows while
+// propagating, then its partial image (FP64)
+typedef float RecLds[4][2][RR][HC];
+static_assert(sizeof(float[2][RR][HC]) >= RR * TX * sizeof(double), "partial image fits the wave's rows");
+
+// One reconstruction item l = (group, tile) at step k (LATE: see rec_tile_load)
+template <bool EDGE, bool PADDED, bool LATE>
+__device__ __forceinline__ void rec_item(const FwiRecArgs &A, const FwiShotDev *__restrict__ shots, int l, int k,
+                                         int gsize, int lane, int w, RecLds &lds)
 {
-    const int k = (kbase ? *kbase : 0) + koff;
-    if (k < 0) return;
-    const FwiRecArgs &A = *Ap;
     const FwiGeomDev &g = A.g;
-    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int ncol = PADDED ? g.ntx : g.bntx;
     const int ntz = PADDED ? (g.pnz + RR - 1) / RR : (g.nz + RR - 1) / RR;
     const int ntiles = ncol * ntz;
-    const int lb = xcd_remap(blockIdx.x, nit);
-    if (lb < 0) return;
-    const int l = items[lb];
     const int grp = l / ntiles, t = l - grp * ntiles;
     const int tzi = t / ncol, txi = t - tzi * ncol;
     const int bx0 = txi * TX - (PADDED ? g.npml : 0), r0 = tzi * RR;
-    // per wave: staged p rows and u rows while propagating, then its partial image (FP64)
-    __shared__ __attribute__((aligned(16))) float lds[4][2][RR][HC];
-    static_assert(sizeof(lds[0]) >= RR * TX * sizeof(double), "partial image fits the wave's rows");
     const int nx = g.nx, nz = g.nz, ld = g.ld, npml = g.npml;
     const int bx = bx0 + lane;
     const bool colok = bx >= 0 && bx < nx;
@@ -2286,11 +2337,11 @@ __device__ __forceinline__ void rec_tile_kernel(const FwiRecArgs *__restrict__ A
         const FwiShotDev sh = shots[s];
         if (k >= sh.nt) continue;
         RecIn rin;
-        rec_tile_load<EDGE>(A, s, k, bx0, r0, lane, rin);
-        rec_tile_shot<EDGE>(A, sh, s, k, bx0, r0, lane, lds[w][0], lds[w][1], pk, acc, rin);
+        rec_tile_load<EDGE, false, LATE>(A, s, k, bx0, r0, lane, rin);
+        rec_tile_shot<EDGE, false, LATE>(A, sh, s, k, bx0, r0, lane, lds[w][0], lds[w][1], pk, acc, rin);
     }
     // steps outside the imaging window only reconstruct (uniform per launch)
-    if (!fwi_image_step(A.img_jump, A.img_k0, A.img_k1, k)) return;
+    if (!fwi_image_step(A.img_jump, A.img_k0, A.img_k1, LATE ? k + 1 : k)) return;
     // partial images of the four waves -> LDS; wave w then finishes rows 2w, 2w+1
     wave_sync();
     double(*red)[TX] = reinterpret_cast<double(*)[TX]>(&lds[w][0][0][0]);
@@ -2306,6 +2357,20 @@ __device__ __forceinline__ void rec_tile_kernel(const FwiRecArgs *__restrict__ A
         for (int q = 0; q < 4; ++q) sum += reinterpret_cast<double(*)[TX]>(&lds[q][0][0][0])[d][lane];
         if (colok && i < nz) gacc[(long long)i * nx + bx] += sum;
     }
+}
+
+template <bool EDGE, bool PADDED = false>
+__device__ __forceinline__ void rec_tile_kernel(const FwiRecArgs *__restrict__ Ap, const FwiShotDev *__restrict__ shots,
+                                                const int *__restrict__ items, int nit, const int *kbase, int koff,
+                                                int gsize)
+{
+    const int k = (kbase ? *kbase : 0) + koff;
+    if (k < 0) return;
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lb = xcd_remap(blockIdx.x, nit);
+    if (lb < 0) return;
+    __shared__ __attribute__((aligned(16))) RecLds lds;
+    rec_item<EDGE, PADDED, false>(*Ap, shots, items[lb], k, gsize, lane, w, lds);
 }
 
 // Occupancy targets: the interior items fit 80 VGPRs without spilling when the
@@ -2349,25 +2414,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 // Occupancy: 123 VGPRs without spilling at 4 waves per SIMD.  A cap of 112
 // (so that a 160-VGPR general adjoint wave fits beside three of these waves)
 // spills 8 VGPRs and costs more than the co-residency gains.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_bwd_tile_int(
-    const FwiAdjArgs *__restrict__ Aa, const FwiRecArgs *__restrict__ Ar, const FwiShotDev *__restrict__ shots,
-    const int *__restrict__ items, int nit, const int *kbase, int koff, int gsize)
+// One fused item l = (group, tile) at step k; lds per wave: the adjoint's
+// w' / u~ rows, then the reconstruction's p / u rows, then its partial image
+// (FP64)
+__device__ __forceinline__ void bwd_int_item(const FwiAdjArgs &A, const FwiRecArgs &R,
+                                             const FwiShotDev *__restrict__ shots, int l, int k, int gsize, int lane,
+                                             int w, RecLds &lds)
 {
-    const int k = (kbase ? *kbase : 0) + koff;
-    if (k < 0) return;
-    const FwiAdjArgs &A = *Aa;
-    const FwiRecArgs &R = *Ar;
-    const FwiGeomDev &g = A.g;
-    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int lb = xcd_remap(blockIdx.x, nit);
-    if (lb < 0) return;
-    // per wave: the adjoint's w' / u~ rows, then the reconstruction's p / u rows,
-    // then its partial image (FP64)
-    __shared__ __attribute__((aligned(16))) float lds[4][2][FR][HC];
-    static_assert(sizeof(lds[0]) >= FR * TX * sizeof(double), "partial image fits the wave's rows");
     static_assert(FR == RR, "the adjoint and reconstruction rows of a tile coincide");
+    const FwiGeomDev &g = A.g;
     const int nsz = (g.pnz + FR - 1) / FR, ntiles = g.ntx * nsz;
-    const int l = items[lb];
     const int grp = l / ntiles, t = l - grp * ntiles;
     const int sz = t / g.ntx, tx = t - sz * g.ntx;
     const int x0 = tx * TX, r0 = sz * FR;
@@ -2411,6 +2467,62 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 #pragma unroll
         for (int q = 0; q < 4; ++q) sum += reinterpret_cast<double(*)[TX]>(&lds[q][0][0][0])[d][lane];
         gacc[(long long)i * nx + bx] += sum;
+    }
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_bwd_tile_int(
+    const FwiAdjArgs *__restrict__ Aa, const FwiRecArgs *__restrict__ Ar, const FwiShotDev *__restrict__ shots,
+    const int *__restrict__ items, int nit, const int *kbase, int koff, int gsize)
+{
+    const int k = (kbase ? *kbase : 0) + koff;
+    if (k < 0) return;
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lb = xcd_remap(blockIdx.x, nit);
+    if (lb < 0) return;
+    __shared__ __attribute__((aligned(16))) RecLds lds;
+    bwd_int_item(*Aa, *Ar, shots, items[lb], k, gsize, lane, w, lds);
+}
+
+// The fused backward step as one launch, so that no step waits on another
+// stream: role 0 the interior items (l0, one per workgroup), 1 the other
+// tiles' x-side and z-side adjoint strips (l1, four per workgroup), 2 their
+// corner strips (l2), 3 their reconstruction items (l3), which image one step
+// late (rec_tile_load LATE) and so read nothing the same launch writes.  At
+// k = -1 only role 3 works: the image of step 0.  work[b] = (role << 28) |
+// workgroup index in the role's list, -1 idle, as in k_fwd_mixed.  The roles
+// share one LDS block; a workgroup runs one role, so the workgroup barriers
+// inside each are reached by all its waves.
+#define FWI_LDS_MIXED (2 * 4 * FR * HC + FWI_TABN)
+static_assert(sizeof(RecLds) <= FWI_LDS_MIXED * sizeof(float), "every role fits the shared block");
+__global__ __launch_bounds__(256) void k_bwd_mixed(const FwiAdjArgs *__restrict__ Aa, const FwiRecArgs *__restrict__ Ar,
+                                                   const FwiShotDev *__restrict__ shots, const int *__restrict__ work,
+                                                   const int *__restrict__ l0, const int *__restrict__ l1, int n1,
+                                                   const int *__restrict__ l2, int n2, const int *__restrict__ l3,
+                                                   const int *kbase, int koff, int gsize)
+{
+    const int k = (kbase ? *kbase : 0) + koff;
+    if (k < -1) return;
+    const int wv = work[blockIdx.x];
+    if (wv < 0) return;
+    const int role = wv >> 28, lb = wv & 0x0FFFFFFF;
+    if (k < 0 && role != 3) return;
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    __shared__ __attribute__((aligned(16))) float smem[FWI_LDS_MIXED];
+    if (role == 0) {
+        bwd_int_item(*Aa, *Ar, shots, l0[lb], k, gsize, lane, w, *reinterpret_cast<RecLds *>(smem));
+    } else if (role == 3) {
+        rec_item<true, true, true>(*Ar, shots, l3[lb], k, gsize, lane, w, *reinterpret_cast<RecLds *>(smem));
+    } else {
+        float(*Ws)[FR][HC] = reinterpret_cast<float(*)[FR][HC]>(smem);
+        float(*Us)[FR][HC] = Ws + 4;
+        float *stab = smem + 2 * 4 * FR * HC;
+        stage_tables_256(stab, Aa->b.tab, Aa->g.npml);
+        const int e = 4 * lb + w;
+        if (role == 1) {
+            if (e < n1) adj_split_strip<true>(*Aa, shots, l1[e], k, lane, Ws[w], Us[w], stab);
+        } else {
+            if (e < n2) adj_split_strip<false>(*Aa, shots, l2[e], k, lane, Ws[w], Us[w], stab);
+        }
     }
 }
 
@@ -2825,6 +2937,19 @@ hipError_t FWI_VNAME(fwi_launch_fwd_mixed)(const FwiFwdArgs *a, const FwiFwdArgs
     else if (mode == FWI_FWD_PLAIN) FWI_MIXED(FWI_FWD_PLAIN);
     else FWI_MIXED(FWI_FWD_FRAMES);
 #undef FWI_MIXED
+    return hipGetLastError();
+}
+
+// the fused backward step as one launch (k_bwd_mixed) over the work order
+// `work` of nwork workgroups (a multiple of 8); tl[0..3] as for
+// fwi_launch_bwd_tile, the reconstruction items imaging one step late
+hipError_t FWI_VNAME(fwi_launch_bwd_mixed)(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, const FwiRecArgs *r_dev,
+                                           const int *work, int nwork, const int *const *tl, const int *ntl,
+                                           const int *kbase, int koff, int gsize, hipStream_t st)
+{
+    if (nwork <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bwd_mixed, dim3(nwork), dim3(256), 0, st, a_dev, r_dev, a->b.shots, work, tl[0], tl[1],
+                       ntl[1], tl[2], ntl[2], tl[3], kbase, koff, gsize);
     return hipGetLastError();
 }
 

@@ -25,7 +25,7 @@ EXPORTS = (
     "fwi_hip_set_residual", "fwi_hip_adjoint", "fwi_hip_synchronize", "fwi_hip_set_timing",
     "fwi_hip_kernel_time", "fwi_hip_read_state", "fwi_hip_check_model", "fwi_hip_op_pml", "fwi_hip_op_interior",
     "fwi_hip_time_integral", "fwi_hip_op_pml_noalloc", "fwi_hip_write_bdr", "fwi_hip_write_bdr_fs_no", "fwi_hip_model_ext_fs_no", "fwi_hip_read_bdr", "fwi_hip_cpml_coe",
-    "fwi_hip_set_graph", "fwi_hip_set_kernels", "fwi_hip_set_rec_group", "fwi_hip_set_imaging", "fwi_hip_set_bwd_chunk", "fwi_hip_set_checkpointing", "fwi_hip_set_revolve", "fwi_hip_backward_plan", "fwi_hip_plan_memory", "fwi_hip_revolve_schedule", "fwi_hip_set_bwd_overlap", "fwi_hip_set_bwd_fused", "fwi_hip_set_concurrent", "fwi_hip_set_interior_wgs", "fwi_hip_set_pass_shape", "fwi_hip_set_fwd_mixed", "fwi_hip_device_count", "fwi_hip_dmalloc", "fwi_hip_dfree", "fwi_hip_host_alloc", "fwi_hip_host_free", "fwi_hip_memcpy",
+    "fwi_hip_set_graph", "fwi_hip_set_kernels", "fwi_hip_set_rec_group", "fwi_hip_set_imaging", "fwi_hip_set_bwd_chunk", "fwi_hip_set_checkpointing", "fwi_hip_set_revolve", "fwi_hip_backward_plan", "fwi_hip_plan_memory", "fwi_hip_revolve_schedule", "fwi_hip_set_bwd_overlap", "fwi_hip_set_bwd_fused", "fwi_hip_set_concurrent", "fwi_hip_set_interior_wgs", "fwi_hip_set_pass_shape", "fwi_hip_set_fwd_mixed", "fwi_hip_set_bwd_mixed", "fwi_hip_device_count", "fwi_hip_dmalloc", "fwi_hip_dfree", "fwi_hip_host_alloc", "fwi_hip_host_free", "fwi_hip_memcpy",
 )
 
 
@@ -252,6 +252,10 @@ class Context:
     def set_fwd_mixed(self, on):
         """The forward step as one launch of interior and general strips."""
         self._chk(self.lib.fwi_hip_set_fwd_mixed(self.h, int(bool(on))))
+
+    def set_bwd_mixed(self, on):
+        """The fused backward step as one launch (general tiles imaged one step late)."""
+        self._chk(self.lib.fwi_hip_set_bwd_mixed(self.h, int(bool(on))))
 
     def set_bwd_fused(self, on):
         """One fused kernel per backward step (adjoint + reconstruction + imaging)."""

@@ -273,6 +273,15 @@ int fwi_hip_set_pass_shape(fwi_hip_ctx *ctx, int pass, int rows, int interior_wg
  * reference (its loops run one after another).                             */
 int fwi_hip_set_fwd_mixed(fwi_hip_ctx *ctx, int on);
 
+/* The fused backward step of the tile kernels as one launch (on = 1): the
+ * interior items, the other tiles' adjoint strips and their reconstruction
+ * items in one grid, the general ones first on each XCD, instead of three
+ * streams joined every step (on = 0).  The reconstruction of the other tiles
+ * then images one step late (mu^{k+2} X^{k+1} at step k, X kept one step in
+ * device memory), so that it does not wait for the adjoint of its cells; the
+ * image is bit-identical.  No counterpart in the reference.                 */
+int fwi_hip_set_bwd_mixed(fwi_hip_ctx *ctx, int on);
+
 /* Wait for all work of the context. */
 int fwi_hip_synchronize(fwi_hip_ctx *ctx);
 
