@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--rec-group", type=int, default=0, help="shots per workgroup of the tile imaging kernel")
     ap.add_argument("--bwd-chunk", type=int, default=-1,
                     help="shots per backward pass (0: all; -1: the library's choice by cache fit)")
+    ap.add_argument("--bwd-lanes", type=int, default=-1,
+                    help="backward chunks in flight (2: in pairs on two stream sets, 1: one after another; -1: default)")
     ap.add_argument("--bwd-overlap", type=int, default=-1, help="adjoint beside imaging on two streams (1/0; -1: default)")
     ap.add_argument("--concurrent", type=int, default=-1,
                     help="general strips on a second stream beside the interior launch (1/0; -1: default)")
@@ -565,6 +567,8 @@ def main():
             ctx.set_rec_group(args.rec_group)
         if args.bwd_chunk >= 0:
             ctx.set_bwd_chunk(args.bwd_chunk)
+        if args.bwd_lanes >= 0:
+            ctx.set_bwd_lanes(args.bwd_lanes)
         if args.bwd_overlap >= 0:
             ctx.set_bwd_overlap(args.bwd_overlap)
         if args.bwd_fused >= 0:
@@ -714,6 +718,7 @@ def main():
                    "nx": g.nx, "nz": g.nz, "nt": args.nt, "nshots": nshots, "nr": nr, "npml": g.npml,
                    "launch": launch or {"kernels": args.kernels, "rec_group": args.rec_group or 8,
                                         "bwd_chunk": "auto" if args.bwd_chunk < 0 else args.bwd_chunk,
+                                        "bwd_lanes": 2 if args.bwd_lanes < 0 else args.bwd_lanes,
                                         "graph": 100,
                                         "fwd_shape": list(args.fwd_shape or (4, 6)),
                                         "bwd_shape": list(args.bwd_shape or (4, 4))}},

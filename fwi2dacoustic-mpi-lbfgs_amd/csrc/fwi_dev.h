@@ -208,11 +208,9 @@ struct FwiPair {
     hipStream_t st, st2;
     hipEvent_t fork, join;
     int lds_int;  // dynamic LDS bytes reserved per interior workgroup (occupancy cap)
-    // third stream of the backward step (the corner adjoint strips beside the
-    // other general strips), joined into st2 before the reconstruction
-    hipStream_t st3;
-    hipEvent_t join3;
 };
+// workgroups of k_adj_tile_gen for n strips (4 per workgroup), a multiple of 8
+__host__ __device__ inline int fwi_gen_blocks(int n) { return ((((n + 3) >> 2) + 7) >> 3) << 3; }
 // LDS of one workgroup of the tile kernels with fr-row strips: interior (wave
 // rows) and general (wave rows + CPML tables)
 __host__ __device__ constexpr int fwi_lds_tile(int fr) { return 4 * 2 * fr * 72 * 4; }

@@ -163,11 +163,6 @@ struct fwi_hip_ctx {
     // and tails (596 -> 621 shot-gradients/s at 1024x320 x 64 shots, nt=400)
     hipStream_t st2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-    // third stream of the fused backward step: its corner adjoint strips (the
-    // only general strips whose body does not fit beside three interior waves)
-    // run beside the other general strips instead of after them
-    hipStream_t st3 = nullptr;
-    hipEvent_t ev_join3 = nullptr;
     bool concurrent = true;
     // rows per wave of the tile kernels in the forward pass (and the
     // recompute of the checkpointed backward pass) and in the backward pass:
@@ -202,6 +197,12 @@ struct fwi_hip_ctx {
     // does not read, and never touches p or the images)
     bool bwd_overlap = false;
     hipEvent_t ev_fork2 = nullptr, ev_join2 = nullptr;
+    // the second chunk in flight of the backward pass (bwd_lanes 2): its
+    // interior and general streams, their fork/join events, and the events
+    // that hand it over from st at the start of a graph block and back at
+    // the end
+    hipStream_t stB = nullptr, stB2 = nullptr;
+    hipEvent_t evB_fork = nullptr, evB_join = nullptr, evL_fork = nullptr, evL_join = nullptr;
     // host <-> device trace copies beside the steps (forward_tm, a residual
     // from set_residual_tm): a copy stream and one event pair per graph block
     hipStream_t stc = nullptr;
@@ -291,6 +292,7 @@ struct fwi_hip_ctx {
     hipGraphExec_t gx_fwd = nullptr;
     std::vector<hipGraphExec_t> gx_bwd; // one captured block of backward steps per shot chunk
     int bwd_chunk = -1;                 // shots per backward pass (0: all in one, -1: auto_bwd_chunk)
+    int bwd_lanes = 2;                  // backward chunks in flight (1 or 2; fwi_hip_set_bwd_lanes)
     std::vector<int> adj_pre[2], rec_pre[2]; // list entries before shot s / before group g
     // timing: 0 off, 1 per launch (plain launches), 2 per pass (graphs as usual)
     int timing = 0;
@@ -446,12 +448,14 @@ extern "C" void fwi_hip_destroy(fwi_hip_ctx *c)
     for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
     if (c->own_stream && c->st) (void)hipStreamDestroy(c->st);
     if (c->st2) (void)hipStreamDestroy(c->st2);
-    if (c->st3) (void)hipStreamDestroy(c->st3);
-    if (c->ev_join3) (void)hipEventDestroy(c->ev_join3);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     if (c->ev_fork2) (void)hipEventDestroy(c->ev_fork2);
     if (c->ev_join2) (void)hipEventDestroy(c->ev_join2);
+    if (c->stB) (void)hipStreamDestroy(c->stB);
+    if (c->stB2) (void)hipStreamDestroy(c->stB2);
+    for (hipEvent_t e : {c->evB_fork, c->evB_join, c->evL_fork, c->evL_join})
+        if (e) (void)hipEventDestroy(e);
     if (c->stc) (void)hipStreamSynchronize(c->stc);
     for (hipEvent_t e : c->blk_done) (void)hipEventDestroy(e);
     for (hipEvent_t e : c->blk_copy) (void)hipEventDestroy(e);
@@ -601,12 +605,14 @@ extern "C" int fwi_hip_create(const fwi_hip_desc *desc, fwi_hip_ctx **out)
             hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) == hipSuccess)
             c->lds_cu = lds;
         hipError_t e = hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking);
-        if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->st3, hipStreamNonBlocking);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join3, hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork2, hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join2, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stB, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stB2, hipStreamNonBlocking);
+        for (hipEvent_t *ev : {&c->evB_fork, &c->evB_join, &c->evL_fork, &c->evL_join})
+            if (e == hipSuccess) e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
         if (e != hipSuccess) {
             rc = set_err(c, FWI_HIP_EHIP, "second stream: %s", hipGetErrorString(e));
             fwi_hip_destroy(c);
@@ -929,8 +935,6 @@ static FwiPair pair_of(const fwi_hip_ctx *c, int pass)
     p.fork = c->ev_fork;
     p.join = c->ev_join;
     p.lds_int = interior_lds(c, fr, pass ? c->int_wgs_bwd : c->int_wgs_fwd);
-    p.st3 = c->concurrent ? c->st3 : nullptr;
-    p.join3 = c->ev_join3;
     return p;
 }
 
@@ -1551,6 +1555,14 @@ extern "C" int fwi_hip_set_bwd_chunk(fwi_hip_ctx *c, int shots)
     return FWI_HIP_OK;
 }
 
+extern "C" int fwi_hip_set_bwd_lanes(fwi_hip_ctx *c, int lanes)
+{
+    if (!c || lanes < 1 || lanes > 2) return set_err(c, FWI_HIP_EINVAL, "set_bwd_lanes: need 1 or 2");
+    drop_bwd(c);
+    c->bwd_lanes = lanes;
+    return FWI_HIP_OK;
+}
+
 // tm: traces go to the host time-major (as the device records them) and fn,
 // when given, sees each block of steps as soon as its rows have arrived
 static int forward_impl(fwi_hip_ctx *c, float *traces, bool tm, fwi_hip_block_fn fn, void *user)
@@ -1762,19 +1774,21 @@ extern "C" int fwi_hip_set_residual_tm(fwi_hip_ctx *c, const float *res_tm)
 
 // Shots per backward pass when the caller leaves it to the library
 // (set_bwd_chunk -1, the default): a step of the backward pass reads the
-// fields the step before wrote, so when the four wavefields of a chunk (mu and
-// p, two buffers each) fit in the device's last-level cache (256 MB Infinity
-// Cache on MI355X; 200 MB of it counted) those reads come from it.  A chunk
-// must still give a step at least 10M cell-updates to fill the GPU.  At
-// 1024x320 x 64 shots that is 32 shots per pass: backward 148.1 -> 132.8 us
-// per step (16 shots per pass: 182 us); at 4096x1024 x 16 shots no chunk fits.
+// fields the step before wrote, so when the four wavefields (mu and p, two
+// buffers each) of the chunks in flight fit in the device's last-level cache
+// (256 MB Infinity Cache on MI355X; 200 MB of it counted) those reads come
+// from it.  The chunks in flight must still give a step at least 10M
+// cell-updates to fill the GPU.  At 1024x320 x 64 shots: 32 shots per pass
+// with one chunk in flight (backward 148.1 -> 132.8 us per step), 16 with two
+// (130.6 -> 119.6 us); 16 shots alone: 182 us.  At 4096x1024 x 16 shots no
+// chunk fits.
 #define FWI_LLC_USABLE 200.0e6
-static int auto_bwd_chunk(const fwi_hip_ctx *c, int rgs)
+static int auto_bwd_chunk(const fwi_hip_ctx *c, int rgs, int lanes)
 {
     const double shot_bytes = 4.0 * (double)c->fs * sizeof(float);
-    const int n = (int)(FWI_LLC_USABLE / shot_bytes) / rgs * rgs;
-    if (n <= 0 || n >= c->nshots) return 0;
-    if ((double)n * c->g.pnx * c->g.pnz < 1.0e7) return 0;
+    const int n = (int)(FWI_LLC_USABLE / shot_bytes) / (rgs * lanes) * rgs;
+    if (n <= 0 || n * lanes >= c->nshots) return 0;
+    if ((double)n * lanes * c->g.pnx * c->g.pnz < 1.0e7) return 0;
     return n;
 }
 
@@ -1896,8 +1910,8 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
         c->bwd_pre[3][ngr] = (int)lb[3].size();
         // adjoint strips of the other tiles: [1] those whose rules are all on
         // the x side (interior strips included) or all on the z side with the
-        // receiver rows, [2] the corner strips that meet both (the bodies of
-        // k_adj_tile_split<true> / <false>)
+        // receiver rows, [2] the corner strips that meet both (the light and
+        // the full body of k_adj_tile_gen)
         for (int s = 0; s < c->nshots; s++) {
             c->bwd_pre[1][s] = (int)lb[1].size();
             c->bwd_pre[2][s] = (int)lb[2].size();
@@ -1998,7 +2012,12 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
     // shot chunks, each a whole backward pass over its own slice of the
     // strip and item lists (shot-major and group-major), so that a chunk's
     // working set (mu, p and the FP64 images of its shots) can stay cached
-    const int chunk = c->bwd_chunk < 0 ? auto_bwd_chunk(c, rgs) : c->bwd_chunk;
+    // Two chunks in flight (bwd_lanes 2, graph form of the fused step): chunk
+    // ch + 1 runs its steps on stB/stB2 beside chunk ch on st/st2, so that each
+    // fills the other's joins and tails; the two touch disjoint shots.
+    const bool lanes_ok = fused && !c->bwd_mixed && c->graph_steps > 0 && c->timing != 1 && c->concurrent;
+    const int lanes = lanes_ok ? c->bwd_lanes : 1;
+    const int chunk = c->bwd_chunk < 0 ? auto_bwd_chunk(c, rgs, lanes) : c->bwd_chunk;
     const int cs = (tile && chunk > 0 && chunk < c->nshots) ? ((chunk + rgs - 1) / rgs) * rgs : c->nshots;
     const int nch = (c->nshots + cs - 1) / cs;
     if ((int)c->gx_bwd.size() != nch) {
@@ -2039,10 +2058,11 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
     }
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
     const int B = c->graph_steps;
-    for (int ch = 0; ch < nch; ch++) {
-        const int s0 = ch * cs, s1 = std::min(s0 + cs, c->nshots);
+    for (int ch = 0; ch < nch;) {
+        const bool pair = lanes == 2 && ch + 1 < nch;  // chunks ch and ch + 1 in flight
+        const int s0 = ch * cs, s1 = std::min(s0 + cs, c->nshots), s2 = pair ? std::min(s1 + cs, c->nshots) : s1;
         int kmax = 0;
-        for (int s = s0; s < s1; s++) kmax = std::max(kmax, c->shots_h[s].nt);
+        for (int s = s0; s < s2; s++) kmax = std::max(kmax, c->shots_h[s].nt);
         const int *ta[2], *tr[2], *tb[4] = {nullptr, nullptr, nullptr, nullptr};
         int na[2], nr[2], nbw[4] = {0, 0, 0, 0};
         if (tile) {
@@ -2060,6 +2080,18 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
                     tb[q] = c->tl_bwd[q] + c->bwd_pre[q][lo];
                     nbw[q] = c->bwd_pre[q][hi] - c->bwd_pre[q][lo];
                 }
+            }
+        }
+        // the second chunk's lists (shots s1 .. s2 - 1)
+        const int *tb2[4] = {nullptr, nullptr, nullptr, nullptr};
+        int nbw2[4] = {0, 0, 0, 0};
+        if (pair) {
+            const int g1 = s1 / rgs, g2 = (s2 + rgs - 1) / rgs;
+            for (int q = 0; q < 4; q++) {
+                const bool per_shot = q == 1 || q == 2;
+                const int lo = per_shot ? s1 : g1, hi = per_shot ? s2 : g2;
+                tb2[q] = c->tl_bwd[q] + c->bwd_pre[q][lo];
+                nbw2[q] = c->bwd_pre[q][hi] - c->bwd_pre[q][lo];
             }
         }
         auto launch_adj = [&](const int *kb, int koff) {
@@ -2084,8 +2116,8 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
         // return for k < 0 and for k >= a shot's nt)
         const bool ovl = tile && c->bwd_overlap && c->timing != 1 && !fused;
         auto pair_step = [&](const int *kb, int koff) {
-            FwiPair p1 = {c->st, nullptr, c->ev_fork, c->ev_join, 0, nullptr, nullptr},
-                    p2 = {c->st2, nullptr, c->ev_fork, c->ev_join, 0, nullptr, nullptr};
+            FwiPair p1 = {c->st, nullptr, c->ev_fork, c->ev_join, 0},
+                    p2 = {c->st2, nullptr, c->ev_fork, c->ev_join, 0};
             hipError_t e = hipEventRecord(c->ev_fork2, c->st);
             if (e == hipSuccess) e = hipStreamWaitEvent(c->st2, c->ev_fork2, 0);
             if (e == hipSuccess) e = rec_tile(c, &r, rd, tr, nr, kb, koff + 1, rgs, p2);
@@ -2098,6 +2130,23 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
             if (!c->gx_bwd[ch]) {
                 rc = capture(c, &c->gx_bwd[ch], [&]() {
                     hipError_t e = hipSuccess;
+                    if (pair) {
+                        FwiPair p2 = pair_of(c, 1);
+                        p2.st = c->stB;
+                        p2.st2 = c->stB2;
+                        p2.fork = c->evB_fork;
+                        p2.join = c->evB_join;
+                        e = hipEventRecord(c->evL_fork, c->st);
+                        if (e == hipSuccess) e = hipStreamWaitEvent(c->stB, c->evL_fork, 0);
+                        for (int j = 0; j < B && e == hipSuccess; j++) {
+                            e = launch_bwd(c->kbase, -j);
+                            if (e == hipSuccess) e = bwd_tile(c, &a, ad, rd, tb2, nbw2, c->kbase, -j, rgs, p2);
+                        }
+                        if (e == hipSuccess) e = hipEventRecord(c->evL_join, c->stB);
+                        if (e == hipSuccess) e = hipStreamWaitEvent(c->st, c->evL_join, 0);
+                        if (e == hipSuccess) e = fwi_launch_advance(c->kbase, -B, c->st);
+                        return e;
+                    }
                     for (int j = 0; j < B && e == hipSuccess; j++) {
                         if (fused) {
                             e = launch_bwd(c->kbase, -j);
@@ -2167,6 +2216,7 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
                 if (marks.size() >= 512 && (rc = collect_timing(c, marks))) return rc;
             }
         }
+        ch += pair ? 2 : 1;
     }
     if ((rc = pass_end(c, 1, pe0, c->kmax))) return rc;
     if ((rc = collect_timing(c, marks))) return rc;

@@ -1786,17 +1786,15 @@ __global__ __launch_bounds__(256) void k_adj_tile(const FwiAdjArgs *__restrict__
     }
 }
 
-// The adjoint strips beside the fused backward step's interior launch, split by
-// register need.  One launch holding every body takes the VGPRs of its largest
-// body and more (154 in one kernel, against 85 for the x-side body, 97 for the
-// z-side one and 133 for a strip that meets x-side and z-side rules at once),
-// and a 154-VGPR wave does not fit on a SIMD beside three 128-VGPR interior
-// waves.  LIGHT: strips whose rules are all on the x side (interior strips
-// included: every x rule then resolves to the compact form) or all on the z
-// side with the receiver rows (96 VGPRs).  Otherwise: the corner strips, every
-// rule compiled in.  The host puts each strip in the list of its kernel with
-// fwi_adj_strip_class.
-// One adjoint strip l of k_adj_tile_split's lists at step k (CPML tables in stab)
+// The adjoint strips beside the fused backward step's interior launch.  A body
+// compiled for every rule takes the most registers (86 VGPRs with 4-row
+// strips, 116 with 8-row ones), so the strips whose rules are all on one side
+// run lighter bodies (63 / 83 VGPRs): LIGHT covers strips whose rules are all
+// on the x side (interior strips included: every x rule then resolves to the
+// compact form) or all on the z side with the receiver rows; the others, the
+// corner strips, run the body with every rule.  The host puts each strip in
+// its list with fwi_adj_strip_class.
+// One adjoint strip l of those lists at step k (CPML tables in stab)
 template <bool LIGHT>
 __device__ __forceinline__ void adj_split_strip(const FwiAdjArgs &A, const FwiShotDev *__restrict__ shots, int l,
                                                 int k, int lane, float (*Ws)[HC], float (*Us)[HC], const float *stab)
@@ -1819,25 +1817,36 @@ __device__ __forceinline__ void adj_split_strip(const FwiAdjArgs &A, const FwiSh
     }
 }
 
-template <bool LIGHT>
-__global__ __launch_bounds__(256) void k_adj_tile_split(const FwiAdjArgs *__restrict__ Ap,
-                                                        const FwiShotDev *__restrict__ shots,
-                                                        const int *__restrict__ strips, int nst, const int *kbase,
-                                                        int koff)
+// Every adjoint strip of the other tiles in one launch, so that the
+// reconstruction after it follows on the same stream: the corner strips (lh,
+// the longer waves) in the first blocks, dealt round the XCDs, then the light
+// strips (ll) in XCD-contiguous order.  Each part's block count is a multiple
+// of 8, so the light part's blocks keep blockIdx & 7 as their XCD.
+__global__ __launch_bounds__(256) void k_adj_tile_gen(const FwiAdjArgs *__restrict__ Ap,
+                                                      const FwiShotDev *__restrict__ shots,
+                                                      const int *__restrict__ lh, int nh,
+                                                      const int *__restrict__ ll, int nl, const int *kbase, int koff)
 {
     const int k = (kbase ? *kbase : 0) + koff;
     if (k < 0) return;
     const FwiAdjArgs &A = *Ap;
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int lb = xcd_remap(blockIdx.x, (nst + 3) >> 2);
-    if (lb < 0) return;
+    const int bh = fwi_gen_blocks(nh), b = blockIdx.x;
     __shared__ float Wsh[4][FR][HC];
     __shared__ float Ush[4][FR][HC];
     __shared__ float stab[FWI_TABN];
-    stage_tables_256(stab, A.b.tab, A.g.npml);  // the only workgroup barrier
-    const int e = 4 * lb + w;
-    if (e >= nst) return;
-    adj_split_strip<LIGHT>(A, shots, strips[e], k, lane, Wsh[w], Ush[w], stab);
+    if (b < bh) {
+        if (4 * b >= nh) return;  // uniform: before the barrier
+        stage_tables_256(stab, A.b.tab, A.g.npml);
+        const int e = 4 * b + w;
+        if (e < nh) adj_split_strip<false>(A, shots, lh[e], k, lane, Wsh[w], Ush[w], stab);
+    } else {
+        const int lb = xcd_remap(b - bh, (nl + 3) >> 2);
+        if (lb < 0) return;
+        stage_tables_256(stab, A.b.tab, A.g.npml);
+        const int e = 4 * lb + w;
+        if (e < nl) adj_split_strip<true>(A, shots, ll[e], k, lane, Wsh[w], Ush[w], stab);
+    }
 }
 
 // ============================================================================
@@ -2984,57 +2993,33 @@ hipError_t FWI_VNAME(fwi_launch_rec_tile)(const FwiRecArgs *a, const FwiRecArgs 
 }
 
 // one backward step: the fused interior items (tl[0]) beside the adjoint's
-// strips of the other tiles (tl[1]: those of k_adj_tile_split<true>, tl[2]:
-// the corner strips), then the reconstruction of the other tiles (tl[3]),
-// which reads their mu^{k+1}.  With p.st2 the other tiles' chain runs on st2
-// beside the interior launch: the two sides write disjoint cells of mu^{k+1},
-// p^{k-1} and the image and read only the other side's previous-step fields.
-// With p.st3 as well, the corner strips run on st3 beside the other adjoint
-// strips, joined into st2 before the reconstruction.
+// strips of the other tiles (tl[1]: the light strips, tl[2]: the corner
+// strips, one launch of k_adj_tile_gen), then the reconstruction of the other
+// tiles (tl[3]), which reads their mu^{k+1}.  With p.st2 the other tiles'
+// chain runs on st2 beside the interior launch: the two sides write disjoint
+// cells of mu^{k+1}, p^{k-1} and the image and read only the other side's
+// previous-step fields.  The adjoint strips and the reconstruction share a
+// stream, so nothing inside the step waits for another stream (a wait across
+// streams costs about 10 us here).
 hipError_t FWI_VNAME(fwi_launch_bwd_tile)(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, const FwiRecArgs *r_dev,
                                           const int *const *tl, const int *ntl, const int *kbase, int koff,
                                           int gsize, const FwiPair &p)
 {
-    auto launch_int = [&](hipStream_t st) {
-        hipLaunchKernelGGL(k_bwd_tile_int, dim3(((ntl[0] + 7) / 8) * 8), dim3(256), p.lds_int, st, a_dev, r_dev,
-                           a->b.shots, tl[0], ntl[0], kbase, koff, gsize);
-    };
-    auto launch_light = [&](hipStream_t st) {
-        hipLaunchKernelGGL(k_adj_tile_split<true>, dim3(((ntl[1] + 31) / 32) * 8), dim3(256), 0, st, a_dev,
-                           a->b.shots, tl[1], ntl[1], kbase, koff);
-    };
-    auto launch_corner = [&](hipStream_t st) {
-        hipLaunchKernelGGL(k_adj_tile_split<false>, dim3(((ntl[2] + 31) / 32) * 8), dim3(256), 0, st, a_dev,
-                           a->b.shots, tl[2], ntl[2], kbase, koff);
-    };
-    auto launch_rec = [&](hipStream_t st) {
-        hipLaunchKernelGGL(k_rec_tile_pad, dim3(((ntl[3] + 7) / 8) * 8), dim3(256), 0, st, r_dev, a->b.shots, tl[3],
-                           ntl[3], kbase, koff, gsize);
-    };
-    if (p.st2 && p.st3 && ntl[0] > 0 && ntl[1] > 0 && ntl[2] > 0) {
-        // one fork, two joins: st3 -> st2 before the reconstruction, st2 -> st
-        hipError_t e = hipEventRecord(p.fork, p.st);
-        if (e == hipSuccess) e = hipStreamWaitEvent(p.st2, p.fork, 0);
-        if (e == hipSuccess) e = hipStreamWaitEvent(p.st3, p.fork, 0);
-        if (e != hipSuccess) return e;
-        launch_corner(p.st3);
-        launch_light(p.st2);
-        launch_int(p.st);
-        e = hipEventRecord(p.join3, p.st3);
-        if (e == hipSuccess) e = hipStreamWaitEvent(p.st2, p.join3, 0);
-        if (e != hipSuccess) return e;
-        if (ntl[3] > 0) launch_rec(p.st2);
-        e = hipEventRecord(p.join, p.st2);
-        if (e == hipSuccess) e = hipStreamWaitEvent(p.st, p.join, 0);
-        if (e != hipSuccess) return e;
-        return hipGetLastError();
-    }
     const bool has_gen = ntl[1] > 0 || ntl[2] > 0 || ntl[3] > 0;
-    return launch_pair(p, ntl[0] > 0, has_gen, launch_int, [&](hipStream_t st) {
-        if (ntl[1] > 0) launch_light(st);
-        if (ntl[2] > 0) launch_corner(st);
-        if (ntl[3] > 0) launch_rec(st);
-    });
+    return launch_pair(
+        p, ntl[0] > 0, has_gen,
+        [&](hipStream_t st) {
+            hipLaunchKernelGGL(k_bwd_tile_int, dim3(((ntl[0] + 7) / 8) * 8), dim3(256), p.lds_int, st, a_dev, r_dev,
+                               a->b.shots, tl[0], ntl[0], kbase, koff, gsize);
+        },
+        [&](hipStream_t st) {
+            if (ntl[1] > 0 || ntl[2] > 0)
+                hipLaunchKernelGGL(k_adj_tile_gen, dim3(fwi_gen_blocks(ntl[2]) + fwi_gen_blocks(ntl[1])), dim3(256),
+                                   0, st, a_dev, a->b.shots, tl[2], ntl[2], tl[1], ntl[1], kbase, koff);
+            if (ntl[3] > 0)
+                hipLaunchKernelGGL(k_rec_tile_pad, dim3(((ntl[3] + 7) / 8) * 8), dim3(256), 0, st, r_dev,
+                                   a->b.shots, tl[3], ntl[3], kbase, koff, gsize);
+        });
 }
 
 #ifndef FWI_SECONDARY

@@ -13,6 +13,7 @@ forward + adjoint passes of each candidate setting and keeps the fastest:
     rec_group    shots per workgroup of the tile imaging kernel
     bwd_chunk    shots per backward pass (0: all in one pass; -1: the library's
                  choice by cache fit)
+    bwd_lanes    backward chunks in flight (2: in pairs on two stream sets; 1)
     graph        time steps per captured HIP graph block
 
 The probe runs the workload's own shots truncated to a few hundred steps, so
@@ -33,7 +34,7 @@ from . import lib as L
 from .workloads import Shot
 
 DEFAULT = dict(kernels=0, strip_fwd=4, wgs_fwd=6, strip_bwd=4, wgs_bwd=4, tile_z=0, group=0, rec_group=8,
-               bwd_chunk=-1, graph=100)
+               bwd_chunk=-1, bwd_lanes=2, graph=100)
 # stage 1: (rows, interior workgroups per CU) of a pass
 PASS_SHAPES = ((8, 3), (8, 4), (4, 4), (4, 5), (4, 6))
 
@@ -44,14 +45,15 @@ def shape_candidates():
 
 
 def launch_candidates(nshots, best_shape):
-    """Stage 2: imaging group, backward chunk and graph block for the stage-1
-    choice; the z-marching kernels over tile height and shot group."""
+    """Stage 2: imaging group, backward chunk (in pairs, and one after another
+    at the library's chunk size) and graph block for the stage-1 choice; the
+    z-marching kernels over tile height and shot group."""
     out = []
     chunks = sorted({-1, 0} | {c for c in (16, 32) if c < nshots})
     for rg in (4, 8, 16):
-        for bc in chunks:
+        for bc, lanes in [(c, 2) for c in chunks] + [(-1, 1)]:
             for gs in (50, 100):
-                out.append(dict(best_shape, rec_group=rg, bwd_chunk=bc, graph=gs))
+                out.append(dict(best_shape, rec_group=rg, bwd_chunk=bc, bwd_lanes=lanes, graph=gs))
     for tz in (16, 32):
         for sg in (2, 4):
             out.append(dict(DEFAULT, kernels=1, tile_z=tz, group=sg, wgs_fwd=0, wgs_bwd=0))
@@ -87,6 +89,7 @@ def apply(ctx, params):
     ctx.set_kernels(params["kernels"])
     ctx.set_rec_group(params["rec_group"])
     ctx.set_bwd_chunk(params["bwd_chunk"])
+    ctx.set_bwd_lanes(params.get("bwd_lanes", DEFAULT["bwd_lanes"]))
     ctx.set_pass_shape(0, params.get("strip_fwd", DEFAULT["strip_fwd"]), params.get("wgs_fwd", DEFAULT["wgs_fwd"]))
     ctx.set_pass_shape(1, params.get("strip_bwd", DEFAULT["strip_bwd"]), params.get("wgs_bwd", DEFAULT["wgs_bwd"]))
     ctx.set_graph(params["graph"])

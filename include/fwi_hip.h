@@ -246,10 +246,20 @@ int fwi_hip_set_bwd_overlap(fwi_hip_ctx *ctx, int on);
  * runs the backward time loop once per chunk of n shots (rounded up to whole
  * reconstruction groups), so that a chunk's wavefields stay in the device's
  * last-level cache from one step to the next; 0: all shots in one pass;
- * -1 (the default): chunks when the four wavefields of a chunk fit in 200 MB
- * and a step still has 10M cell-updates or more (32 of 64 shots at
- * 1024x320), else one pass.  The image is bit-identical to the single pass. */
+ * -1 (the default): chunks when the four wavefields of the chunks in flight
+ * (set_bwd_lanes) fit in 200 MB and a step of them still has 10M
+ * cell-updates or more (16 of 64 shots per chunk at 1024x320 with two in
+ * flight), else one pass.  The image is bit-identical to the single pass.   */
 int fwi_hip_set_bwd_chunk(fwi_hip_ctx *ctx, int shots);
+
+/* Backward chunks in flight: 2 (the default) runs chunks in pairs, the second
+ * chunk's steps on streams of its own beside the first's, so that each fills
+ * the other's waits between steps; 1: one chunk after another.  Pairs run in
+ * the graph form of the fused backward step with concurrent general strips
+ * (set_graph > 0, set_bwd_fused 1, set_concurrent 1, set_bwd_mixed 0);
+ * otherwise chunks run one after another.  The image is bit-identical.
+ * No counterpart in the reference (its shots run one per MPI rank).        */
+int fwi_hip_set_bwd_lanes(fwi_hip_ctx *ctx, int lanes);
 
 /* Shots per workgroup of the tile reconstruction + imaging kernel (default
  * 8): the group's image sum meets the FP64 accumulator once per step.     */

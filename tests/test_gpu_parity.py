@@ -461,6 +461,72 @@ def test_bwd_mixed_gives_the_same_image(lib, graph):
         ctx.close()
 
 
+@pytest.mark.parametrize("graph", [100, 50, 0])
+def test_chunk_pairs_give_the_same_image(lib, graph):
+    """Backward chunks run in pairs (the second chunk's steps on streams of
+    its own beside the first's) give the image of chunks run one after
+    another and of the single pass bit for bit: both strip heights, pairs
+    whose chunks end at different steps (uneven nt), a last chunk without a
+    partner, graph replays."""
+    g = small_geom()
+    rng = np.random.default_rng(61)
+    vel = np.clip(2200 + 300 * rng.standard_normal((g.nz, g.nx)), 1500, 3200).astype(np.float32)
+    shots = []
+    for j in range(8):
+        irx = np.arange(j % 2, g.nx, 2, dtype=np.int32)
+        ntj = 230 - 17 * j
+        shots.append(Shot(nt=ntj, isx=10 + 17 * j, isz=2 + j, wav=ricker(ntj, g.dt, 20.0), irx=irx,
+                          irz=np.full(len(irx), 2 + (j % 3), np.int32)))
+    res = [rng.standard_normal((s.nr, s.nt)).astype(np.float32) for s in shots]
+    ctx = lib.Context(g, max_shots=8, max_nt=230, max_nr=100)
+    ctx.set_graph(graph)
+    ctx.set_rec_group(2)
+    ctx.set_model(vel)
+    ctx.set_shots(shots)
+
+    def run(n=8):
+        out = []
+        for _ in range(2):                      # a second pass replays the captured graphs
+            ctx.forward(want_traces=False)
+            ctx.set_residual(ctx.pack(res[:n]))
+            out.append(ctx.adjoint()[1])
+        return out
+
+    try:
+        for rows in (4, 8):
+            ctx.set_pass_shape(1, rows, 3)
+            ctx.set_bwd_chunk(0)
+            ref = run()[0]
+            assert np.abs(ref).max() > 0
+            for chunk in (2, 4, 6):             # two pairs; one pair; a pair of 6 + 2 shots
+                ctx.set_bwd_chunk(chunk)
+                for lanes in (1, 2):
+                    ctx.set_bwd_lanes(lanes)
+                    for gk in run():
+                        np.testing.assert_array_equal(gk, ref, err_msg=f"rows {rows} chunk {chunk} lanes {lanes}")
+            ctx.set_bwd_lanes(2)
+            ctx.set_shots(shots[:6])             # three chunks: a pair, then one alone
+            ctx.set_bwd_chunk(0)
+            ref6 = run(6)[0]
+            ctx.set_bwd_chunk(2)
+            for gk in run(6):
+                np.testing.assert_array_equal(gk, ref6, err_msg=f"rows {rows} 6 shots chunk 2")
+            ctx.set_shots(shots)
+    finally:
+        ctx.close()
+
+
+def test_bwd_lanes_refuses_other_counts(lib):
+    g, vel, shots = small_case(nshots=1, nt=50, seed=43)
+    ctx = lib.Context(g, max_shots=1, max_nt=50, max_nr=100)
+    try:
+        for bad in (0, 3, -1):
+            with pytest.raises(lib.FwiError):
+                ctx.set_bwd_lanes(bad)
+    finally:
+        ctx.close()
+
+
 def test_pass_shape_refuses_other_shapes(lib):
     g, vel, shots = small_case(nshots=1, nt=50, seed=43)
     ctx = lib.Context(g, max_shots=1, max_nt=50, max_nr=100)
