@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--rec-group", type=int, default=0, help="shots per workgroup of the tile imaging kernel")
     ap.add_argument("--bwd-chunk", type=int, default=-1,
                     help="shots per backward pass (0: all; -1: the library's choice by cache fit)")
+    ap.add_argument("--bwd-gen-rows", type=int, default=-1,
+                    help="rows per wave of the backward step's general side under 4-row strips (4 or 8; -1: default)")
     ap.add_argument("--bwd-lanes", type=int, default=-1,
                     help="backward chunks in flight (2: in pairs on two stream sets, 1: one after another; -1: default)")
     ap.add_argument("--bwd-overlap", type=int, default=-1, help="adjoint beside imaging on two streams (1/0; -1: default)")
@@ -569,6 +571,8 @@ def main():
             ctx.set_bwd_chunk(args.bwd_chunk)
         if args.bwd_lanes >= 0:
             ctx.set_bwd_lanes(args.bwd_lanes)
+        if args.bwd_gen_rows >= 0:
+            ctx.set_bwd_general_rows(args.bwd_gen_rows)
         if args.bwd_overlap >= 0:
             ctx.set_bwd_overlap(args.bwd_overlap)
         if args.bwd_fused >= 0:

@@ -60,6 +60,14 @@ hipError_t fwi_launch_rec_tile_s4(const FwiRecArgs *a, const FwiRecArgs *a_dev, 
 hipError_t fwi_launch_bwd_tile_s4(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, const FwiRecArgs *r_dev,
                                   const int *const *tl, const int *ntl, const int *kbase, int koff, int gsize,
                                   const FwiPair &p);
+// the two sides of a fused backward step (fwi_launch_bwd_tile), apart: the
+// interior side of the 4-row build beside the general side of the 8-row one
+hipError_t fwi_launch_bwd_int_s4(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, const FwiRecArgs *r_dev,
+                                 const int *tl0, int n0, const int *kbase, int koff, int gsize, int lds,
+                                 hipStream_t st);
+hipError_t fwi_launch_bwd_gen(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, const FwiRecArgs *r_dev,
+                              const int *const *tl, const int *ntl, const int *kbase, int koff, int gsize,
+                              hipStream_t st);
 hipError_t fwi_launch_advance(int *kbase, int delta, hipStream_t st);
 hipError_t fwi_launch_set_int(int *p, int v, hipStream_t st);
 hipError_t fwi_launch_fill_f32(float *p, long long n, float v, hipStream_t st);
@@ -293,6 +301,10 @@ struct fwi_hip_ctx {
     std::vector<hipGraphExec_t> gx_bwd; // one captured block of backward steps per shot chunk
     int bwd_chunk = -1;                 // shots per backward pass (0: all in one, -1: auto_bwd_chunk)
     int bwd_lanes = 2;                  // backward chunks in flight (1 or 2; fwi_hip_set_bwd_lanes)
+    // rows per wave of the fused backward step's general side when the
+    // backward pass runs 4-row strips (4, or 8: the 8-row build's general
+    // kernels on 8-row blocks; fwi_hip_set_bwd_general_rows)
+    int gen_rows_bwd = 4;
     std::vector<int> adj_pre[2], rec_pre[2]; // list entries before shot s / before group g
     // timing: 0 off, 1 per launch (plain launches), 2 per pass (graphs as usual)
     int timing = 0;
@@ -939,6 +951,13 @@ static FwiPair pair_of(const fwi_hip_ctx *c, int pass)
 }
 
 // the tile kernels of the strip height of the pass (fwi_hip_set_pass_shape)
+// whether the fused backward lists are built with 8-row blocks on the general
+// side under 4-row interior tiles (the one-launch step needs one strip height)
+static bool tall_general(const fwi_hip_ctx *c)
+{
+    return c->strip_bwd == 4 && c->gen_rows_bwd == 8 && !c->bwd_mixed;
+}
+
 static hipError_t fwd_tile(const fwi_hip_ctx *c, const FwiFwdArgs *a, const FwiFwdArgs *a_dev, const int *const *tl,
                            const int *ntl, const int *kbase, int koff, const FwiPair &p, int mode)
 {
@@ -965,6 +984,24 @@ static hipError_t bwd_tile(const fwi_hip_ctx *c, const FwiAdjArgs *a, const FwiA
                            const int *const *tl, const int *ntl, const int *kbase, int koff, int gsize,
                            const FwiPair &p)
 {
+    if (tall_general(c)) {
+        // the interior side of the 4-row build beside the general side of
+        // the 8-row build, forked and joined as fwi_launch_bwd_tile does
+        const bool has_int = ntl[0] > 0, has_gen = ntl[1] > 0 || ntl[2] > 0 || ntl[3] > 0;
+        if (p.st2 && has_int && has_gen) {
+            hipError_t e = hipEventRecord(p.fork, p.st);
+            if (e == hipSuccess) e = hipStreamWaitEvent(p.st2, p.fork, 0);
+            if (e == hipSuccess) e = fwi_launch_bwd_gen(a, a_dev, r_dev, tl, ntl, kbase, koff, gsize, p.st2);
+            if (e == hipSuccess)
+                e = fwi_launch_bwd_int_s4(a, a_dev, r_dev, tl[0], ntl[0], kbase, koff, gsize, p.lds_int, p.st);
+            if (e == hipSuccess) e = hipEventRecord(p.join, p.st2);
+            if (e == hipSuccess) e = hipStreamWaitEvent(p.st, p.join, 0);
+            return e;
+        }
+        hipError_t e = fwi_launch_bwd_int_s4(a, a_dev, r_dev, tl[0], ntl[0], kbase, koff, gsize, p.lds_int, p.st);
+        if (e == hipSuccess && has_gen) e = fwi_launch_bwd_gen(a, a_dev, r_dev, tl, ntl, kbase, koff, gsize, p.st);
+        return e;
+    }
     return c->strip_bwd == 4 ? fwi_launch_bwd_tile_s4(a, a_dev, r_dev, tl, ntl, kbase, koff, gsize, p)
                              : fwi_launch_bwd_tile(a, a_dev, r_dev, tl, ntl, kbase, koff, gsize, p);
 }
@@ -1555,6 +1592,14 @@ extern "C" int fwi_hip_set_bwd_chunk(fwi_hip_ctx *c, int shots)
     return FWI_HIP_OK;
 }
 
+extern "C" int fwi_hip_set_bwd_general_rows(fwi_hip_ctx *c, int rows)
+{
+    if (!c || (rows != 4 && rows != 8)) return set_err(c, FWI_HIP_EINVAL, "set_bwd_general_rows: need 4 or 8");
+    drop_bwd(c);
+    c->gen_rows_bwd = rows;
+    return FWI_HIP_OK;
+}
+
 extern "C" int fwi_hip_set_bwd_lanes(fwi_hip_ctx *c, int lanes)
 {
     if (!c || lanes < 1 || lanes > 2) return set_err(c, FWI_HIP_EINVAL, "set_bwd_lanes: need 1 or 2");
@@ -1884,43 +1929,54 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
         c->rec_lists_for = rgs;
     }
     const bool fused = tile && c->bwd_fused && c->ckpt_every == 0;
-    if (fused && c->bwd_lists_for != rgs) {
+    const bool tall = tall_general(c);
+    const int lists_key = rgs + (tall ? 1000000 : 0);
+    if (fused && c->bwd_lists_for != lists_key) {
         // a tile of the padded grid is fused for a shot group when it is
         // interior for the adjoint and for the reconstruction of every shot of
-        // the group; the other tiles run as adjoint strips + reconstruction items
-        const int br = c->strip_bwd, nsz = (g.pnz + br - 1) / br, ntiles = g.ntx * nsz;
-        std::vector<char> fz((size_t)ngr * ntiles);
+        // the group; the other tiles run as adjoint strips + reconstruction
+        // items.  The general side works on blocks of gr rows (the interior
+        // tile height br, or 8 over 4-row interior tiles: a block is fused
+        // when both of its 4-row tiles are, and its tiles then go to the
+        // interior list; the fused test of 8 rows is that of its two halves)
+        const int br = c->strip_bwd, gr = tall ? 8 : br;
+        const int nsz = (g.pnz + br - 1) / br, ntiles = g.ntx * nsz;
+        const int nszg = (g.pnz + gr - 1) / gr, ntg = g.ntx * nszg;
+        std::vector<char> fz((size_t)ngr * ntg);
         std::vector<int> lb[4];
         for (int q = 0; q < 4; q++) c->bwd_pre[q].assign((q == 1 || q == 2 ? c->nshots : ngr) + 1, 0);
         for (int grp = 0; grp < ngr; grp++) {
             c->bwd_pre[0][grp] = (int)lb[0].size();
             c->bwd_pre[3][grp] = (int)lb[3].size();
-            for (int t = 0; t < ntiles; t++) {
-                const int x0 = (t % g.ntx) * FWI_TX, r0 = (t / g.ntx) * br;
-                bool in = true;
+            for (int t = 0; t < ntg; t++) {
+                const int x0 = (t % g.ntx) * FWI_TX, r0 = (t / g.ntx) * gr;
+                bool in = r0 + gr <= g.pnz;
                 for (int s = grp * rgs; s < std::min((grp + 1) * rgs, c->nshots) && in; s++)
-                    in = fwi_adj_strip_interior(g, c->shots_h[s], x0, r0, br) &&
-                         fwi_rec_tile_interior(g, c->shots_h[s], x0 - g.npml, r0, br);
-                fz[(size_t)grp * ntiles + t] = in;
-                if (in) lb[0].push_back(grp * ntiles + t);
-                else if (r0 < g.nz) lb[3].push_back(grp * ntiles + t);
+                    in = fwi_adj_strip_interior(g, c->shots_h[s], x0, r0, gr) &&
+                         fwi_rec_tile_interior(g, c->shots_h[s], x0 - g.npml, r0, gr);
+                fz[(size_t)grp * ntg + t] = in;
+                if (!in && r0 < g.nz) lb[3].push_back(grp * ntg + t);
+            }
+            for (int t = 0; t < ntiles; t++) {   // interior tiles in row order
+                const int tg = (t / g.ntx) * br / gr * g.ntx + t % g.ntx;
+                if (fz[(size_t)grp * ntg + tg]) lb[0].push_back(grp * ntiles + t);
             }
         }
         c->bwd_pre[0][ngr] = (int)lb[0].size();
         c->bwd_pre[3][ngr] = (int)lb[3].size();
-        // adjoint strips of the other tiles: [1] those whose rules are all on
+        // adjoint strips of the other blocks: [1] those whose rules are all on
         // the x side (interior strips included) or all on the z side with the
         // receiver rows, [2] the corner strips that meet both (the light and
         // the full body of k_adj_tile_gen)
         for (int s = 0; s < c->nshots; s++) {
             c->bwd_pre[1][s] = (int)lb[1].size();
             c->bwd_pre[2][s] = (int)lb[2].size();
-            for (int t = 0; t < ntiles; t++) {
-                if (fz[(size_t)(s / rgs) * ntiles + t]) continue;
+            for (int t = 0; t < ntg; t++) {
+                if (fz[(size_t)(s / rgs) * ntg + t]) continue;
                 const int x0 = (t % g.ntx) * FWI_TX, sz = t / g.ntx;
-                const int cls = fwi_adj_strip_class(g, c->shots_h[s], x0, sz * br, br);
+                const int cls = fwi_adj_strip_class(g, c->shots_h[s], x0, sz * gr, gr);
                 const bool corner = (cls & FWI_CLS_X) && (cls & (FWI_CLS_Z | FWI_CLS_RS));
-                lb[corner ? 2 : 1].push_back((s * nsz + sz) * g.ntx + t % g.ntx);
+                lb[corner ? 2 : 1].push_back((s * nszg + sz) * g.ntx + t % g.ntx);
             }
         }
         c->bwd_pre[1][c->nshots] = (int)lb[1].size();
@@ -1932,7 +1988,7 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
                                          c->st));
         }
         HIPCHK(c, hipStreamSynchronize(c->st));
-        c->bwd_lists_for = rgs;
+        c->bwd_lists_for = lists_key;
         c->bwd_work_for = -1;
     }
     const int nb_adj = g.ntx * g.ntz * c->ngroups;

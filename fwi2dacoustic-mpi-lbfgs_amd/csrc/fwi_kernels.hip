@@ -2992,13 +2992,40 @@ hipError_t FWI_VNAME(fwi_launch_rec_tile)(const FwiRecArgs *a, const FwiRecArgs 
         });
 }
 
-// one backward step: the fused interior items (tl[0]) beside the adjoint's
-// strips of the other tiles (tl[1]: the light strips, tl[2]: the corner
-// strips, one launch of k_adj_tile_gen), then the reconstruction of the other
-// tiles (tl[3]), which reads their mu^{k+1}.  With p.st2 the other tiles'
-// chain runs on st2 beside the interior launch: the two sides write disjoint
-// cells of mu^{k+1}, p^{k-1} and the image and read only the other side's
-// previous-step fields.  The adjoint strips and the reconstruction share a
+// The two sides of a fused backward step.  The interior side: the fused items
+// (tl[0]) with the occupancy reserve lds.  The general side: every adjoint
+// strip of the other tiles (tl[1]: the light strips, tl[2]: the corner
+// strips, one launch of k_adj_tile_gen), then the reconstruction of those
+// tiles (tl[3]), which reads their mu^{k+1}, on the same stream.  The general
+// side may come from the build with taller strips than the interior side's:
+// the host then builds its lists in that build's strip height.
+hipError_t FWI_VNAME(fwi_launch_bwd_int)(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, const FwiRecArgs *r_dev,
+                                         const int *tl0, int n0, const int *kbase, int koff, int gsize, int lds,
+                                         hipStream_t st)
+{
+    if (n0 > 0)
+        hipLaunchKernelGGL(k_bwd_tile_int, dim3(((n0 + 7) / 8) * 8), dim3(256), lds, st, a_dev, r_dev, a->b.shots,
+                           tl0, n0, kbase, koff, gsize);
+    return hipGetLastError();
+}
+
+hipError_t FWI_VNAME(fwi_launch_bwd_gen)(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, const FwiRecArgs *r_dev,
+                                         const int *const *tl, const int *ntl, const int *kbase, int koff, int gsize,
+                                         hipStream_t st)
+{
+    if (ntl[1] > 0 || ntl[2] > 0)
+        hipLaunchKernelGGL(k_adj_tile_gen, dim3(fwi_gen_blocks(ntl[2]) + fwi_gen_blocks(ntl[1])), dim3(256), 0, st,
+                           a_dev, a->b.shots, tl[2], ntl[2], tl[1], ntl[1], kbase, koff);
+    if (ntl[3] > 0)
+        hipLaunchKernelGGL(k_rec_tile_pad, dim3(((ntl[3] + 7) / 8) * 8), dim3(256), 0, st, r_dev, a->b.shots, tl[3],
+                           ntl[3], kbase, koff, gsize);
+    return hipGetLastError();
+}
+
+// one backward step of this build's strip height: the interior side on p.st;
+// with p.st2 the general side runs on st2 beside it (the two sides write
+// disjoint cells of mu^{k+1}, p^{k-1} and the image and read only the other
+// side's previous-step fields).  The general side's two launches share a
 // stream, so nothing inside the step waits for another stream (a wait across
 // streams costs about 10 us here).
 hipError_t FWI_VNAME(fwi_launch_bwd_tile)(const FwiAdjArgs *a, const FwiAdjArgs *a_dev, const FwiRecArgs *r_dev,
@@ -3009,17 +3036,9 @@ hipError_t FWI_VNAME(fwi_launch_bwd_tile)(const FwiAdjArgs *a, const FwiAdjArgs 
     return launch_pair(
         p, ntl[0] > 0, has_gen,
         [&](hipStream_t st) {
-            hipLaunchKernelGGL(k_bwd_tile_int, dim3(((ntl[0] + 7) / 8) * 8), dim3(256), p.lds_int, st, a_dev, r_dev,
-                               a->b.shots, tl[0], ntl[0], kbase, koff, gsize);
+            FWI_VNAME(fwi_launch_bwd_int)(a, a_dev, r_dev, tl[0], ntl[0], kbase, koff, gsize, p.lds_int, st);
         },
-        [&](hipStream_t st) {
-            if (ntl[1] > 0 || ntl[2] > 0)
-                hipLaunchKernelGGL(k_adj_tile_gen, dim3(fwi_gen_blocks(ntl[2]) + fwi_gen_blocks(ntl[1])), dim3(256),
-                                   0, st, a_dev, a->b.shots, tl[2], ntl[2], tl[1], ntl[1], kbase, koff);
-            if (ntl[3] > 0)
-                hipLaunchKernelGGL(k_rec_tile_pad, dim3(((ntl[3] + 7) / 8) * 8), dim3(256), 0, st, r_dev,
-                                   a->b.shots, tl[3], ntl[3], kbase, koff, gsize);
-        });
+        [&](hipStream_t st) { FWI_VNAME(fwi_launch_bwd_gen)(a, a_dev, r_dev, tl, ntl, kbase, koff, gsize, st); });
 }
 
 #ifndef FWI_SECONDARY

@@ -25,7 +25,7 @@ EXPORTS = (
     "fwi_hip_set_residual", "fwi_hip_adjoint", "fwi_hip_synchronize", "fwi_hip_set_timing",
     "fwi_hip_kernel_time", "fwi_hip_read_state", "fwi_hip_check_model", "fwi_hip_op_pml", "fwi_hip_op_interior",
     "fwi_hip_time_integral", "fwi_hip_op_pml_noalloc", "fwi_hip_write_bdr", "fwi_hip_write_bdr_fs_no", "fwi_hip_model_ext_fs_no", "fwi_hip_read_bdr", "fwi_hip_cpml_coe",
-    "fwi_hip_set_graph", "fwi_hip_set_kernels", "fwi_hip_set_rec_group", "fwi_hip_set_imaging", "fwi_hip_set_bwd_chunk", "fwi_hip_set_bwd_lanes", "fwi_hip_set_checkpointing", "fwi_hip_set_revolve", "fwi_hip_backward_plan", "fwi_hip_plan_memory", "fwi_hip_revolve_schedule", "fwi_hip_set_bwd_overlap", "fwi_hip_set_bwd_fused", "fwi_hip_set_concurrent", "fwi_hip_set_interior_wgs", "fwi_hip_set_pass_shape", "fwi_hip_set_fwd_mixed", "fwi_hip_set_bwd_mixed", "fwi_hip_device_count", "fwi_hip_dmalloc", "fwi_hip_dfree", "fwi_hip_host_alloc", "fwi_hip_host_free", "fwi_hip_memcpy",
+    "fwi_hip_set_graph", "fwi_hip_set_kernels", "fwi_hip_set_rec_group", "fwi_hip_set_imaging", "fwi_hip_set_bwd_chunk", "fwi_hip_set_bwd_lanes", "fwi_hip_set_bwd_general_rows", "fwi_hip_set_checkpointing", "fwi_hip_set_revolve", "fwi_hip_backward_plan", "fwi_hip_plan_memory", "fwi_hip_revolve_schedule", "fwi_hip_set_bwd_overlap", "fwi_hip_set_bwd_fused", "fwi_hip_set_concurrent", "fwi_hip_set_interior_wgs", "fwi_hip_set_pass_shape", "fwi_hip_set_fwd_mixed", "fwi_hip_set_bwd_mixed", "fwi_hip_device_count", "fwi_hip_dmalloc", "fwi_hip_dfree", "fwi_hip_host_alloc", "fwi_hip_host_free", "fwi_hip_memcpy",
 )
 
 
@@ -268,6 +268,10 @@ class Context:
     def set_bwd_chunk(self, shots):
         """Shots per backward pass (0: all shots in one pass; -1: the library's choice by cache fit)."""
         self._chk(self.lib.fwi_hip_set_bwd_chunk(self.h, int(shots)))
+
+    def set_bwd_general_rows(self, rows):
+        """Rows per wave of the fused backward step's general side under 4-row strips (4 or 8)."""
+        self._chk(self.lib.fwi_hip_set_bwd_general_rows(self.h, int(rows)))
 
     def set_bwd_lanes(self, lanes):
         """Backward chunks in flight: 2 runs chunks in pairs on two stream sets, 1 one after another."""

@@ -261,6 +261,16 @@ int fwi_hip_set_bwd_chunk(fwi_hip_ctx *ctx, int shots);
  * No counterpart in the reference (its shots run one per MPI rank).        */
 int fwi_hip_set_bwd_lanes(fwi_hip_ctx *ctx, int lanes);
 
+/* Rows per wave of the fused backward step's general side (the adjoint
+ * strips and reconstruction items of the tiles that are not interior) when
+ * the backward pass runs 4-row strips: 4, or 8 (the general side on blocks
+ * of 8 rows, run by the 8-row kernels; a block's two 4-row tiles stay on the
+ * interior side when both are interior).  Taller general strips pay each
+ * wave's fixed work (class, coefficients, buffer set-up) once per 8 rows.
+ * Ignored with 8-row strips and with set_bwd_mixed 1.  The image is
+ * bit-identical.  No counterpart in the reference.                         */
+int fwi_hip_set_bwd_general_rows(fwi_hip_ctx *ctx, int rows);
+
 /* Shots per workgroup of the tile reconstruction + imaging kernel (default
  * 8): the group's image sum meets the FP64 accumulator once per step.     */
 int fwi_hip_set_rec_group(fwi_hip_ctx *ctx, int shots);

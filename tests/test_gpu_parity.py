@@ -516,6 +516,60 @@ def test_chunk_pairs_give_the_same_image(lib, graph):
         ctx.close()
 
 
+@pytest.mark.parametrize("graph", [100, 0])
+def test_tall_general_side_gives_the_same_image(lib, graph):
+    """The fused backward step's general side on 8-row blocks (the 8-row
+    kernels) beside 4-row interior tiles gives the image of the 4-row general
+    side bit for bit: shots with a deep source and deep receiver rows (general
+    blocks inside the grid, a block's 4-row tile beside an interior one), the
+    grid's last partial block, chunks one at a time and in pairs, concurrent
+    and one stream, and the setting refuses other heights."""
+    g = small_geom()
+    rng = np.random.default_rng(71)
+    vel = np.clip(2200 + 300 * rng.standard_normal((g.nz, g.nx)), 1500, 3200).astype(np.float32)
+    shots = []
+    for j in range(6):
+        irx = np.arange(j % 2, g.nx, 3, dtype=np.int32)
+        ntj = 200 - 13 * j
+        deep = j % 3 == 1
+        shots.append(Shot(nt=ntj, isx=12 + 21 * j, isz=30 + j if deep else 2, wav=ricker(ntj, g.dt, 20.0),
+                          irx=irx, irz=np.full(len(irx), 41 + j if deep else 3, np.int32)))
+    res = [rng.standard_normal((s.nr, s.nt)).astype(np.float32) for s in shots]
+    ctx = lib.Context(g, max_shots=6, max_nt=200, max_nr=100)
+    ctx.set_graph(graph)
+    ctx.set_rec_group(2)
+    ctx.set_model(vel)
+    ctx.set_shots(shots)
+
+    def run():
+        out = []
+        for _ in range(2):
+            ctx.forward(want_traces=False)
+            ctx.set_residual(ctx.pack(res))
+            out.append(ctx.adjoint()[1])
+        return out
+
+    try:
+        for bad in (2, 6, 16):
+            with pytest.raises(lib.FwiError):
+                ctx.set_bwd_general_rows(bad)
+        ctx.set_pass_shape(1, 4, 3)
+        for concurrent in (True, False):
+            ctx.set_concurrent(concurrent)
+            for chunk, lanes in ((0, 2), (2, 2), (2, 1)):
+                ctx.set_bwd_chunk(chunk)
+                ctx.set_bwd_lanes(lanes)
+                ctx.set_bwd_general_rows(4)
+                ref = run()[0]
+                assert np.abs(ref).max() > 0
+                ctx.set_bwd_general_rows(8)
+                for gk in run():
+                    np.testing.assert_array_equal(
+                        gk, ref, err_msg=f"concurrent {concurrent} chunk {chunk} lanes {lanes}")
+    finally:
+        ctx.close()
+
+
 def test_bwd_lanes_refuses_other_counts(lib):
     g, vel, shots = small_case(nshots=1, nt=50, seed=43)
     ctx = lib.Context(g, max_shots=1, max_nt=50, max_nr=100)
