@@ -3036,9 +3036,11 @@ hipError_t FWI_VNAME(fwi_launch_bwd_tile)(const FwiAdjArgs *a, const FwiAdjArgs 
     return launch_pair(
         p, ntl[0] > 0, has_gen,
         [&](hipStream_t st) {
-            FWI_VNAME(fwi_launch_bwd_int)(a, a_dev, r_dev, tl[0], ntl[0], kbase, koff, gsize, p.lds_int, st);
+            (void)FWI_VNAME(fwi_launch_bwd_int)(a, a_dev, r_dev, tl[0], ntl[0], kbase, koff, gsize, p.lds_int, st);
         },
-        [&](hipStream_t st) { FWI_VNAME(fwi_launch_bwd_gen)(a, a_dev, r_dev, tl, ntl, kbase, koff, gsize, st); });
+        [&](hipStream_t st) {
+            (void)FWI_VNAME(fwi_launch_bwd_gen)(a, a_dev, r_dev, tl, ntl, kbase, koff, gsize, st);
+        });  // launch errors: hipGetLastError at the end of launch_pair
 }
 
 #ifndef FWI_SECONDARY
