@@ -92,9 +92,10 @@ __device__ __forceinline__ void buf_st(float v, brsrc r, int voff, int soff)
 // instead keeps one 64-bit lane mask per row alive until the loads return:
 // with every load of a strip issued first, that spilled 70 to 200 SGPRs of
 // the general bodies into VGPR lanes.
+// (n >= 0: one unsigned compare tests 0 <= r < n)
 __device__ __forceinline__ float buf_ld_row(brsrc rs, int voff, int r, int n, int rowb, int base = 0)
 {
-    const bool in = r >= 0 && r < n;
+    const bool in = (unsigned)r < (unsigned)n;
     return buf_ld(rs, in ? voff : FWI_BUF_OUT, base + (in ? r : 0) * rowb);
 }
 // the same with 0 selected after the load: one VGPR for all rows' lane offset
@@ -102,7 +103,7 @@ __device__ __forceinline__ float buf_ld_row(brsrc rs, int voff, int r, int n, in
 // run out of VGPRs with an offset per row)
 __device__ __forceinline__ float buf_ld_row_sel(brsrc rs, int voff, int r, int n, int rowb)
 {
-    const bool in = r >= 0 && r < n;
+    const bool in = (unsigned)r < (unsigned)n;
     const float v = buf_ld(rs, voff, (in ? r : 0) * rowb);
     return in ? v : 0.f;
 }
