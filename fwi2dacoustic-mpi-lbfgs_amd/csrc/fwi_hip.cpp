@@ -1711,14 +1711,20 @@ static int forward_impl(fwi_hip_ctx *c, float *traces, bool tm, fwi_hip_block_fn
         const int nblk = (c->kmax + B - 1) / B;
         pipe = tm && traces;
         if (pipe && (rc = ensure_copy(c, nblk))) return rc;
+        // the copies go on st2 when the forward leaves it idle (one launch per
+        // step, or the z-marching kernels): the copy stream may share a
+        // hardware queue with st once the backward pass's four streams exist,
+        // and copies queued behind the steps would no longer overlap them
+        const bool st2_idle = c->fwd_kind != 0 || c->fwd_mixed || !c->concurrent;
+        hipStream_t cs = st2_idle && c->st2 ? c->st2 : c->stc;
         for (int blk = 0; blk < nblk; blk++) {
             HIPCHK(c, hipGraphLaunch(c->gx_fwd, c->st));
             if (!pipe) continue;
             // the block's trace rows go to the host while the next block runs
             HIPCHK(c, hipEventRecord(c->blk_done[blk], c->st));
-            HIPCHK(c, hipStreamWaitEvent(c->stc, c->blk_done[blk], 0));
-            HIPCHK(c, copy_rows(c, traces, c->syn_tm, blk * B, std::min(c->kmax, (blk + 1) * B), true, c->stc));
-            HIPCHK(c, hipEventRecord(c->blk_copy[blk], c->stc));
+            HIPCHK(c, hipStreamWaitEvent(cs, c->blk_done[blk], 0));
+            HIPCHK(c, copy_rows(c, traces, c->syn_tm, blk * B, std::min(c->kmax, (blk + 1) * B), true, cs));
+            HIPCHK(c, hipEventRecord(c->blk_copy[blk], cs));
         }
     } else {
         for (int k = 0; k < c->kmax; k++) {
