@@ -466,8 +466,8 @@ def test_chunk_pairs_give_the_same_image(lib, graph):
     """Backward chunks run in pairs (the second chunk's steps on streams of
     its own beside the first's) give the image of chunks run one after
     another and of the single pass bit for bit: both strip heights, pairs
-    whose chunks end at different steps (uneven nt), a last chunk without a
-    partner, graph replays."""
+    whose chunks end at different steps (uneven nt), an imaging window, a
+    last chunk without a partner, graph replays."""
     g = small_geom()
     rng = np.random.default_rng(61)
     vel = np.clip(2200 + 300 * rng.standard_normal((g.nz, g.nx)), 1500, 3200).astype(np.float32)
@@ -505,6 +505,13 @@ def test_chunk_pairs_give_the_same_image(lib, graph):
                     for gk in run():
                         np.testing.assert_array_equal(gk, ref, err_msg=f"rows {rows} chunk {chunk} lanes {lanes}")
             ctx.set_bwd_lanes(2)
+            ctx.set_imaging(3, 20, 190)          # an imaging window, pairs against one pass
+            ctx.set_bwd_chunk(0)
+            refw = run()[0]
+            ctx.set_bwd_chunk(2)
+            for gk in run():
+                np.testing.assert_array_equal(gk, refw, err_msg=f"rows {rows} imaging window")
+            ctx.set_imaging(1, 0, 2**31 - 1)
             ctx.set_shots(shots[:6])             # three chunks: a pair, then one alone
             ctx.set_bwd_chunk(0)
             ref6 = run(6)[0]
