@@ -211,6 +211,9 @@ struct fwi_hip_ctx {
     // the end
     hipStream_t stB = nullptr, stB2 = nullptr;
     hipEvent_t evB_fork = nullptr, evB_join = nullptr, evL_fork = nullptr, evL_join = nullptr;
+    // the checkpointed backward pass images step k on stB beside the adjoint
+    // step k - 1: adjoint step k done, imaging steps k (ring of two) done
+    hipEvent_t ev_adjd = nullptr, ev_img[2] = {nullptr, nullptr};
     // host <-> device trace copies beside the steps (forward_tm, a residual
     // from set_residual_tm): a copy stream and one event pair per graph block
     hipStream_t stc = nullptr;
@@ -466,7 +469,7 @@ extern "C" void fwi_hip_destroy(fwi_hip_ctx *c)
     if (c->ev_join2) (void)hipEventDestroy(c->ev_join2);
     if (c->stB) (void)hipStreamDestroy(c->stB);
     if (c->stB2) (void)hipStreamDestroy(c->stB2);
-    for (hipEvent_t e : {c->evB_fork, c->evB_join, c->evL_fork, c->evL_join})
+    for (hipEvent_t e : {c->evB_fork, c->evB_join, c->evL_fork, c->evL_join, c->ev_adjd, c->ev_img[0], c->ev_img[1]})
         if (e) (void)hipEventDestroy(e);
     if (c->stc) (void)hipStreamSynchronize(c->stc);
     for (hipEvent_t e : c->blk_done) (void)hipEventDestroy(e);
@@ -623,7 +626,8 @@ extern "C" int fwi_hip_create(const fwi_hip_desc *desc, fwi_hip_ctx **out)
         if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join2, hipEventDisableTiming);
         if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stB, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stB2, hipStreamNonBlocking);
-        for (hipEvent_t *ev : {&c->evB_fork, &c->evB_join, &c->evL_fork, &c->evL_join})
+        for (hipEvent_t *ev : {&c->evB_fork, &c->evB_join, &c->evL_fork, &c->evL_join, &c->ev_adjd, &c->ev_img[0],
+                               &c->ev_img[1]})
             if (e == hipSuccess) e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
         if (e != hipSuccess) {
             rc = set_err(c, FWI_HIP_EHIP, "second stream: %s", hipGetErrorString(e));
@@ -2048,8 +2052,17 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
                 }
             }
         }
+        // imaging step k (mu^{k+1} X^k, mu^{k+1} in M[(k+1) & 1]) on a stream of
+        // its own beside the adjoint step k - 1, which only reads that buffer;
+        // the adjoint step k - 2 overwrites it, so it waits for the imaging
+        // step k, and a segment's recompute, which overwrites the stored X,
+        // waits for the previous segment's last imaging step.  The images of
+        // a group still add up step by step in the same order.
+        hipStream_t si = c->concurrent && c->stB ? c->stB : c->st;
+        int last_img = -1;  // the last imaging step queued on si
         for (long long j = c->bwd_method == 1 ? (c->kmax + K - 1) / K - 1 : -1; j >= 0; j--) {
             const int k0 = (int)(j * K), k1 = (int)std::min((long long)c->kmax, (j + 1) * K);
+            if (si != c->st && last_img >= 0) HIPCHK(c, hipStreamWaitEvent(c->st, c->ev_img[last_img & 1], 0));
             if ((rc = ck_restore(c, j))) return rc;
             rec += k1 - k0;
             HIPCHK(c, fwi_launch_set_int(k0_dev, k0, c->st));
@@ -2057,10 +2070,20 @@ extern "C" int fwi_hip_adjoint(fwi_hip_ctx *c, double *grad, double *gk)
                 HIPCHK(c, fwd_tile(c, &f, c->fwd_rc_d, c->tl_fwd, c->n_fwd, nullptr, k, pair_of(c, 0),
                                               FWI_FWD_XSEG));
             for (int k = k1 - 1; k >= k0; k--) {
+                // imaging step k + 2 (queued when k + 2 < kmax) reads the buffer
+                // this adjoint step writes; ev_img[k & 1] was last recorded by it
+                if (si != c->st && k + 2 < c->kmax) HIPCHK(c, hipStreamWaitEvent(c->st, c->ev_img[k & 1], 0));
                 HIPCHK(c, adj_tile(c, &a, ad, c->tl_adj, c->n_adj, nullptr, k, pair_of(c, 1)));
-                HIPCHK(c, fwi_launch_img_seg(&r, rd, c->ck_xseg + (long long)(k - k0) * S * nbb, k, rgs, ngr, c->st));
+                if (si != c->st) {
+                    HIPCHK(c, hipEventRecord(c->ev_adjd, c->st));
+                    HIPCHK(c, hipStreamWaitEvent(si, c->ev_adjd, 0));
+                }
+                HIPCHK(c, fwi_launch_img_seg(&r, rd, c->ck_xseg + (long long)(k - k0) * S * nbb, k, rgs, ngr, si));
+                if (si != c->st) HIPCHK(c, hipEventRecord(c->ev_img[k & 1], si));
+                last_img = k;
             }
         }
+        if (si != c->st && last_img >= 0) HIPCHK(c, hipStreamWaitEvent(c->st, c->ev_img[last_img & 1], 0));
         c->recomputed = rec;
         if ((rc = pass_end(c, 1, pe0, c->kmax))) return rc;
         HIPCHK(c, fwi_launch_grad_finalize(c->gacc, ngr, nb, c->vel, c->gk, c->grad, c->st));
