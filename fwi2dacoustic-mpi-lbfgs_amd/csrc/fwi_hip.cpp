@@ -299,7 +299,10 @@ struct fwi_hip_ctx {
     int *tl_bwd[4] = {nullptr, nullptr, nullptr, nullptr};
     int n_bwd[4] = {0, 0, 0, 0};
     int bwd_lists_for = -1;  // rec_group the fused lists were built for (-1: none)
-    std::vector<int> bwd_pre[4];  // list entries before group g ([0], [3]) or shot s ([1], [2])
+    std::vector<int> bwd_pre[4];
+    // host copies of the shot tables last uploaded (set_shots skips an identical batch)
+    std::vector<float> srcf_h;
+    std::vector<int> rrow_h, rmap_h;  // list entries before group g ([0], [3]) or shot s ([1], [2])
     hipGraphExec_t gx_fwd = nullptr;
     std::vector<hipGraphExec_t> gx_bwd; // one captured block of backward steps per shot chunk
     int bwd_chunk = -1;                 // shots per backward pass (0: all in one, -1: auto_bwd_chunk)
@@ -893,6 +896,15 @@ extern "C" int fwi_hip_set_shots(fwi_hip_ctx *c, int nshots, const fwi_hip_shot 
     }
     if (toff > (long long)c->d.max_shots * c->d.max_nt * c->d.max_nr)
         return set_err(c, FWI_HIP_EINVAL, "set_shots: traces exceed the context capacity");
+    if (rmap.empty()) rmap.assign(g.ld, -1);
+    // the same shots as the device holds (a driver that sets its shots before
+    // every gradient): keep the tables, strip lists and captured graphs
+    if (nshots == c->nshots && kmax == c->kmax && srcf == c->srcf_h && rrow == c->rrow_h && rmap == c->rmap_h &&
+        memcmp(sd.data(), c->shots_h.data(), sizeof(FwiShotDev) * nshots) == 0) {
+        c->have_fwd = c->have_obs = c->have_res = false;
+        return FWI_HIP_OK;
+    }
+    c->srcf_h.clear();  // no match until the new tables are on the device
     // (re)allocate the per-batch tables
     if ((long long)srcf.size() > c->srcf_cap) {
         dfree(c, c->srcf);
@@ -902,7 +914,6 @@ extern "C" int fwi_hip_set_shots(fwi_hip_ctx *c, int nshots, const fwi_hip_shot 
         if (rc) return rc;
         c->srcf_cap = (long long)srcf.size();
     }
-    if (rmap.empty()) rmap.assign(g.ld, -1);
     if ((long long)rmap.size() > c->rmap_cap) {
         dfree(c, c->rmap);
         c->rmap = nullptr;
@@ -920,6 +931,9 @@ extern "C" int fwi_hip_set_shots(fwi_hip_ctx *c, int nshots, const fwi_hip_shot 
     c->gx_fwd = nullptr;
     drop_bwd(c);
     c->shots_h = sd;
+    c->srcf_h = std::move(srcf);
+    c->rrow_h = std::move(rrow);
+    c->rmap_h = std::move(rmap);
     c->nshots = nshots;
     c->kmax = kmax;
     c->ntrace = toff;
