@@ -96,7 +96,9 @@ int fwi_hip_check_model(const fwi_hip_grid *g, const float *vel, long long *cell
 
 /* The shots of the next forward/adjoint pair (receiver cells must be unique
  * within a shot).  Traces of a batch are packed shot after shot, each
- * [nr][nt] with time contiguous (fwi_msf.syn, header.h:66).                 */
+ * [nr][nt] with time contiguous (fwi_msf.syn, header.h:66).  A batch equal
+ * to the one the context holds (every nt, cell, receiver and wavelet sample)
+ * keeps the device tables, strip lists and captured graphs.                 */
 int fwi_hip_set_shots(fwi_hip_ctx *ctx, int nshots, const fwi_hip_shot *shots);
 
 /* Forward propagation of every shot in one launch per time step.  Receiver
