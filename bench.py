@@ -23,6 +23,7 @@ import argparse
 import json
 import os
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -81,6 +82,8 @@ def parse():
                          "96 shots with nt 2000-6000, apertures 256-1536, host Butterworth band-pass")
     ap.add_argument("--autotune", action="store_true",
                     help="time short passes of the candidate launch settings first and use the fastest")
+    ap.add_argument("--log-dir", default=None,
+                    help="folder of the per-iteration log of --workload inversion (default: the temporary folder)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-single", action="store_true", help="skip the one-shot-per-launch comparison")
     ap.add_argument("--cpu-nt", type=int, default=0, help="time steps of each CPU sample shot (0: the workload's nt)")
@@ -493,7 +496,7 @@ def run_inversion(args):
     lp = LbfgsParam()
     h.fwi_lbfgs_default_param(ctypes.byref(lp))
     lp.m, lp.max_iterations = 5, args.iters
-    log = os.path.join(ROOT, "gpurun_out", f"inversion_{args.misfit}_{args.storage}.jsonl")
+    log = os.path.join(args.log_dir or tempfile.gettempdir(), f"inversion_{args.misfit}_{args.storage}.jsonl")
     os.makedirs(os.path.dirname(log), exist_ok=True)
     if os.path.exists(log):
         os.remove(log)

@@ -47,7 +47,7 @@ def test_cpml_closed_form_config1(o64, o32):
         assert a[40] == 0.0                      # d = 0 on the inner edge
         assert abs(b[40] - g["b_inner"]) < tol * 10
         assert np.all(a[:40] < 0) and np.all((b > 0) & (b <= 1))
-    # survey-quoted d0 and b0[0] for config 1
+    # d0 and b0[0] for npml = 20, dx = 10 m, dt = 1 ms, to the digits quoted for this geometry
     assert abs(g["d0"] - 310.85) < 0.01 and abs(g["b_outer"] - 0.73282) < 1e-5
 
 
